@@ -1,0 +1,137 @@
+/*
+ * vnlbhip: MI355X (gfx950) VNLB hot path behind a plain C ABI.
+ *
+ * Replaces, for one GPU, the stage pipeline the reference drives from its
+ * VapourSynth plugins: the CPU stage process_basic_anchor_no_flow /
+ * process_final_anchor_no_flow (src/core/core.hpp:107-116) followed by
+ * vnlb.Aggregate (src/vs/plugin.cpp:966-1045), as the CUDA plugin returns it
+ * in one call (render_frame, src/vs/cuda_plugin.cpp:1635-1864, with
+ * StagePipeline::enqueue src/cuda/pipeline.hpp:82-137).
+ *
+ * Video layout: contiguous float32 [frames][channels][height][width],
+ * channels 1 (GrayS) or 3 (YUV444PS).  All functions return 0 on success and
+ * a negative code on failure; vnlbhip_last_error() then holds the message.
+ * No exception crosses this boundary.
+ */
+#ifndef VNLBHIP_H
+#define VNLBHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VNLBHIP_ABI_VERSION 1
+
+enum {
+    VNLBHIP_STAGE_BASIC = 0,
+    VNLBHIP_STAGE_FINAL = 1,
+};
+
+enum {
+    VNLBHIP_OK = 0,
+    VNLBHIP_ERROR_INVALID = -1,  /* parameters, geometry or limits */
+    VNLBHIP_ERROR_DEVICE = -2,   /* HIP runtime failure */
+    VNLBHIP_ERROR_MODEL = -3,    /* a group could not cover the rank */
+};
+
+/* Stage parameters with the meaning and units of StageParameters
+ * (src/core/core.hpp:17-41): sigma and sigma_basic in [0,1] sample units,
+ * tau as a squared distance in those units (the plugin front end divides
+ * 8-bit values by 255 and 255^2, src/vs/plugin.cpp:291-312). */
+typedef struct vnlbhip_params {
+    float sigma;
+    int patch_size;
+    int patch_time;
+    int search_window;
+    int search_bwd;
+    int search_fwd;
+    int similar;
+    int rank;
+    float similar_cap_factor;
+    float model_cap_factor;
+    float beta;
+    float tau;
+    float variance_threshold;
+    float sigma_basic;
+    float gamma;
+    float weight_alpha;
+    float weight_beta;
+    float weight_gamma;
+    float weight_epsilon;
+    float membership_noise_floor;
+    int proc_step;
+    int flat_areas;
+    int couple_channels;
+} vnlbhip_params;
+
+typedef struct vnlbhip_stats {
+    int64_t anchors;              /* grid anchors matched */
+    int64_t groups;               /* anchors that survived the paste mask */
+    int64_t unconverged_groups;   /* eigen problems that hit the sweep limit */
+    int32_t batches;              /* anchor-frame batches of the last call */
+    int32_t reserved;
+} vnlbhip_stats;
+
+typedef struct vnlbhip_context vnlbhip_context;
+
+/* Defaults of vnlb.Basic (stage 0) or vnlb.Final (stage 1): StageParameters
+ * defaults (src/core/core.hpp:17-41) with the Final overrides of
+ * parse_stage_parameters (src/vs/plugin.cpp:331-339); sigma is left 0. */
+int vnlbhip_default_params(int stage, vnlbhip_params* out);
+
+/* Validation of validate_stage_configuration (src/core/core.cpp:1997-2024)
+ * plus this build's kernel limits.  Needs no GPU. */
+int vnlbhip_validate(int stage, const vnlbhip_params* params, int width,
+                     int height, int frames, int channels, char* message,
+                     size_t message_size);
+
+/* One context per stream/thread (the reference's reserve/enqueue objects are
+ * likewise not thread-safe, src/cuda/pipeline.hpp:82-126). */
+int vnlbhip_create(int device, vnlbhip_context** out);
+void vnlbhip_destroy(vnlbhip_context* context);
+const char* vnlbhip_last_error(const vnlbhip_context* context);
+
+/* Whole-clip stage on device memory: every anchor frame's matching, paste
+ * mask, group model and weighted aggregation, then normalisation with the
+ * noisy clip as fallback, written to out.  ref is the Final stage's guide
+ * clip (the Basic output), NULL for Basic.  stream is a hipStream_t or NULL.
+ * Blocks until the stage has finished (group counts are read back). */
+int vnlbhip_stage(vnlbhip_context* context, int stage,
+                  const vnlbhip_params* params, const float* noisy,
+                  const float* ref, float* out, int width, int height,
+                  int frames, int channels, void* stream);
+
+/* Same with host buffers: copies in, runs, copies out. */
+int vnlbhip_stage_host(vnlbhip_context* context, int stage,
+                       const vnlbhip_params* params, const float* noisy,
+                       const float* ref, float* out, int width, int height,
+                       int frames, int channels);
+
+/* Diagnostics for one anchor frame (device input pointers, host outputs):
+ * the grid anchors in raster order and, per anchor, the match list
+ * (positions as linear clip indices ((frame*height)+y)*width+x, distances),
+ * the paste-mask decision, aggregation weights (active anchors), model
+ * eigenvalues (first model, active modelled anchors) and group flags
+ * (bit0 equal, bit1 flat, bit2 modelled).  Per-anchor arrays have
+ * capacity_anchors entries, match arrays capacity_anchors * cap entries
+ * with cap from vnlbhip_retained_cap; optional outputs may be NULL. */
+int vnlbhip_debug_frame(vnlbhip_context* context, int stage,
+                        const vnlbhip_params* params, const float* noisy,
+                        const float* ref, int width, int height, int frames,
+                        int channels, int anchor_frame, int capacity_anchors,
+                        int* num_anchors, int* anchor_x, int* anchor_y,
+                        int* counts, int* positions, float* distances,
+                        unsigned char* active, float* weights,
+                        float* eigenvalues, int* group_flags);
+
+int vnlbhip_retained_cap(const vnlbhip_params* params);
+int vnlbhip_get_stats(const vnlbhip_context* context, vnlbhip_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VNLBHIP_H */

@@ -1,0 +1,203 @@
+"""ctypes wrapper of the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module, always as the checker or the timed CPU baseline, never as the thing
+the product runs.
+"""
+from __future__ import annotations
+
+import ctypes
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ORACLE_DIR = Path(__file__).resolve().parent
+LIBRARY = ORACLE_DIR / "_build" / "libvnlb_oracle.so"
+
+ACCUM_DOUBLE = 1
+SELECT_FAITHFUL = 2
+MODEL_DOUBLE = 4
+
+_FIELDS = [
+    ("sigma", ctypes.c_float), ("patch_size", ctypes.c_int), ("patch_time", ctypes.c_int),
+    ("search_window", ctypes.c_int), ("search_bwd", ctypes.c_int), ("search_fwd", ctypes.c_int),
+    ("similar", ctypes.c_int), ("rank", ctypes.c_int), ("similar_cap_factor", ctypes.c_float),
+    ("model_cap_factor", ctypes.c_float), ("beta", ctypes.c_float), ("tau", ctypes.c_float),
+    ("variance_threshold", ctypes.c_float), ("sigma_basic", ctypes.c_float),
+    ("gamma", ctypes.c_float), ("weight_alpha", ctypes.c_float), ("weight_beta", ctypes.c_float),
+    ("weight_gamma", ctypes.c_float), ("weight_epsilon", ctypes.c_float),
+    ("membership_noise_floor", ctypes.c_float), ("proc_step", ctypes.c_int),
+    ("flat_areas", ctypes.c_int), ("couple_channels", ctypes.c_int),
+]
+
+
+class VoParams(ctypes.Structure):
+    _fields_ = _FIELDS
+
+
+def to_struct(params) -> VoParams:
+    s = VoParams()
+    source = params if isinstance(params, dict) else params.as_dict()
+    for name, _ in _FIELDS:
+        value = source[name]
+        setattr(s, name, int(value) if isinstance(value, bool) else value)
+    return s
+
+
+def build() -> Path:
+    subprocess.run(["make", "-C", str(ORACLE_DIR), "-s"], check=True)
+    return LIBRARY
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIBRARY.exists():
+            build()
+        lib = ctypes.CDLL(str(LIBRARY))
+        P = ctypes.POINTER
+        fp, ip, dp = P(ctypes.c_float), P(ctypes.c_int), P(ctypes.c_double)
+        lib.vo_stage.argtypes = [ctypes.c_int, P(VoParams), fp, fp, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_int, ctypes.c_int, fp, ctypes.c_int, ctypes.c_int,
+                                 P(ctypes.c_int64)]
+        lib.vo_frame_debug.argtypes = [ctypes.c_int, P(VoParams), fp, fp, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, ip, ip, ip, ip, ip, fp, P(ctypes.c_ubyte),
+                                       fp, fp, ip, ctypes.c_int]
+        lib.vo_retained_cap.argtypes = [P(VoParams)]
+        lib.vo_grid_count.argtypes = [P(VoParams), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int]
+        lib.vo_linalg_mean.argtypes = [dp, ctypes.c_int, ctypes.c_int, dp]
+        lib.vo_linalg_gram_centered.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, dp]
+        lib.vo_linalg_covariance_centered.argtypes = [dp, ctypes.c_int, ctypes.c_int,
+                                                      ctypes.c_double, dp]
+        lib.vo_linalg_topk_eigen.argtypes = [dp, ctypes.c_int, ctypes.c_int, dp, dp, ip]
+        lib.vo_linalg_topk_eigen_f32.argtypes = [fp, ctypes.c_int, ctypes.c_int, fp, fp, ip]
+        lib.vo_linalg_dual_basis.argtypes = [dp, ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_int, dp]
+        lib.vo_last_error.restype = ctypes.c_char_p
+        _lib = lib
+    return _lib
+
+
+def _video(a) -> tuple[np.ndarray, tuple[int, int, int, int]]:
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.ndim == 3:
+        a = a[:, None]
+    f, c, h, w = a.shape
+    return a, (w, h, f, c)
+
+
+def _fp(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def stage(stage_id: int, params, noisy, ref=None, flags: int = ACCUM_DOUBLE, threads: int = 1):
+    lib = load()
+    src, (w, h, f, c) = _video(noisy)
+    guide = _video(ref)[0] if ref is not None else None
+    out = np.empty_like(src)
+    groups = ctypes.c_int64(0)
+    p = to_struct(params)
+    rc = lib.vo_stage(stage_id, ctypes.byref(p), _fp(src), _fp(guide), w, h, f, c, _fp(out),
+                      flags, threads, ctypes.byref(groups))
+    if rc != 0:
+        raise ValueError(lib.vo_last_error().decode())
+    return out, int(groups.value)
+
+
+def frame_debug(stage_id: int, params, noisy, ref, anchor_frame: int, flags: int = ACCUM_DOUBLE) -> dict:
+    lib = load()
+    src, (w, h, f, c) = _video(noisy)
+    guide = _video(ref)[0] if ref is not None else None
+    p = to_struct(params)
+    cap = lib.vo_retained_cap(ctypes.byref(p))
+    capacity = w * h
+    rank = max(int(p.rank), 1)
+    num = ctypes.c_int(0)
+    ax = np.zeros(capacity, np.int32)
+    ay = np.zeros(capacity, np.int32)
+    counts = np.zeros(capacity, np.int32)
+    positions = np.full(capacity * cap, -1, np.int32)
+    distances = np.full(capacity * cap, np.nan, np.float32)
+    active = np.zeros(capacity, np.uint8)
+    weights = np.full(capacity * cap, np.nan, np.float32)
+    eig = np.full(capacity * rank, np.nan, np.float32)
+    gflags = np.zeros(capacity, np.int32)
+    ip = ctypes.POINTER(ctypes.c_int)
+    rc = lib.vo_frame_debug(stage_id, ctypes.byref(p), _fp(src), _fp(guide), w, h, f, c,
+                            anchor_frame, capacity, ctypes.byref(num), ax.ctypes.data_as(ip),
+                            ay.ctypes.data_as(ip), counts.ctypes.data_as(ip),
+                            positions.ctypes.data_as(ip), _fp(distances),
+                            active.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), _fp(weights),
+                            _fp(eig), gflags.ctypes.data_as(ip), flags)
+    if rc != 0:
+        raise ValueError(lib.vo_last_error().decode())
+    n = num.value
+    return {
+        "cap": cap, "anchor_x": ax[:n], "anchor_y": ay[:n], "counts": counts[:n],
+        "positions": positions[: n * cap].reshape(n, cap),
+        "distances": distances[: n * cap].reshape(n, cap), "active": active[:n],
+        "weights": weights[: n * cap].reshape(n, cap),
+        "eigenvalues": eig[: n * rank].reshape(n, rank), "group_flags": gflags[:n],
+    }
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def linalg_mean(samples):
+    lib = load()
+    s = np.ascontiguousarray(samples, np.float64)
+    mean = np.zeros(s.shape[1])
+    assert lib.vo_linalg_mean(_dp(s), s.shape[0], s.shape[1], _dp(mean)) == 0
+    return mean
+
+
+def linalg_gram_centered(centered, scale=1.0):
+    lib = load()
+    s = np.ascontiguousarray(centered, np.float64)
+    gram = np.zeros((s.shape[0], s.shape[0]))
+    assert lib.vo_linalg_gram_centered(_dp(s), s.shape[0], s.shape[1], scale, _dp(gram)) == 0
+    return gram
+
+
+def linalg_covariance_centered(centered, scale=1.0):
+    lib = load()
+    s = np.ascontiguousarray(centered, np.float64)
+    cov = np.zeros((s.shape[1], s.shape[1]))
+    assert lib.vo_linalg_covariance_centered(_dp(s), s.shape[0], s.shape[1], scale, _dp(cov)) == 0
+    return cov
+
+
+def linalg_topk_eigen(matrix, rank, dtype=np.float64):
+    lib = load()
+    m = np.ascontiguousarray(matrix, dtype)
+    n = m.shape[0]
+    values = np.zeros(rank, dtype)
+    vectors = np.zeros((rank, n), dtype)
+    converged = ctypes.c_int(0)
+    if dtype == np.float64:
+        sweeps = lib.vo_linalg_topk_eigen(_dp(m), n, rank, _dp(values), _dp(vectors),
+                                          ctypes.byref(converged))
+    else:
+        sweeps = lib.vo_linalg_topk_eigen_f32(_fp(m), n, rank, _fp(values), _fp(vectors),
+                                              ctypes.byref(converged))
+    assert sweeps >= 0
+    return values, vectors, bool(converged.value), sweeps
+
+
+def linalg_dual_basis(centered, gram_values, gram_vectors):
+    lib = load()
+    s = np.ascontiguousarray(centered, np.float64)
+    values = np.ascontiguousarray(gram_values, np.float64)
+    vectors = np.ascontiguousarray(gram_vectors, np.float64)
+    rank = values.shape[0]
+    basis = np.zeros((rank, s.shape[1]))
+    assert lib.vo_linalg_dual_basis(_dp(s), s.shape[0], s.shape[1], _dp(values), _dp(vectors),
+                                    rank, _dp(basis)) == 0
+    return basis

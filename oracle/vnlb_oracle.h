@@ -1,0 +1,126 @@
+/*
+ * VNLB CPU oracle: C interface.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library restates the CPU hot path of
+ * yuygfgg/Vapoursynth-VNLB (src/core/core.cpp, src/linalg/linalg.cpp,
+ * src/aggregate/aggregate.cpp, src/vs/plugin.cpp) so that the HIP path can be
+ * checked against it.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it; the product never does.
+ *
+ * Video layout everywhere: contiguous float32 [frames][channels][height][width].
+ * Patch positions are linear clip indices: ((frame * height) + y) * width + x.
+ */
+#ifndef VNLB_ORACLE_H
+#define VNLB_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Field meaning and defaults follow StageParameters (src/core/core.hpp:17-41).
+ * sigma, sigma_basic are in [0,1] sample units; tau is a squared distance in
+ * the same units (the plugin divides the 8-bit values by 255 and 255^2). */
+typedef struct vo_params {
+    float sigma;
+    int patch_size;
+    int patch_time;
+    int search_window;
+    int search_bwd;
+    int search_fwd;
+    int similar;
+    int rank;
+    float similar_cap_factor;
+    float model_cap_factor;
+    float beta;
+    float tau;
+    float variance_threshold;
+    float sigma_basic;
+    float gamma;
+    float weight_alpha;
+    float weight_beta;
+    float weight_gamma;
+    float weight_epsilon;
+    float membership_noise_floor;
+    int proc_step;
+    int flat_areas;
+    int couple_channels;
+} vo_params;
+
+enum {
+    VO_STAGE_BASIC = 0,
+    VO_STAGE_FINAL = 1,
+};
+
+enum {
+    /* Accumulate contribution stacks in double and round once per pixel
+     * (the order-independent form the HIP path uses).  Without it the stacks
+     * are float, accumulated in raster order as the reference does. */
+    VO_ACCUM_DOUBLE = 1,
+    /* Select matches with the reference's sequential heap / bounded buffer /
+     * std::partial_sort procedure instead of the equivalent set rule. */
+    VO_SELECT_FAITHFUL = 2,
+    /* Run the group model (mean .. projection, weights) in double. */
+    VO_MODEL_DOUBLE = 4,
+};
+
+/* Whole-clip stage: every anchor frame, then the vnlb.Aggregate
+ * normalisation with the noisy clip as fallback.  ref is the Final stage's
+ * matching/model clip (ignored for Basic).  Returns 0 or -1 (see
+ * vo_last_error). */
+int vo_stage(int stage, const vo_params* params, const float* noisy,
+             const float* ref, int width, int height, int frames,
+             int channels, float* out, int flags, int threads,
+             int64_t* groups_out);
+
+/* Per anchor frame diagnostics.  Anchors are the staggered grid in raster
+ * order; arrays are sized [capacity_anchors] and [capacity_anchors * cap]
+ * where cap = retained group cap.  Optional outputs may be NULL.
+ *   counts[a], positions[a*cap + i], distances[a*cap + i]: match lists of
+ *     every grid anchor (matching does not depend on the paste mask);
+ *   active[a]: 1 if the anchor survives the ordered paste mask;
+ *   weights[a*cap + i]: final aggregation weight of each retained patch
+ *     (active anchors only);
+ *   eigenvalues[a*rank + j]: model eigenvalues (active, modelled groups);
+ *   group_flags[a]: bit0 equal group, bit1 flat group, bit2 modelled. */
+int vo_frame_debug(int stage, const vo_params* params, const float* noisy,
+                   const float* ref, int width, int height, int frames,
+                   int channels, int anchor_frame, int capacity_anchors,
+                   int* num_anchors, int* anchor_x, int* anchor_y,
+                   int* counts, int* positions, float* distances,
+                   unsigned char* active, float* weights, float* eigenvalues,
+                   int* group_flags, int flags);
+
+/* Retained group cap (max(K, ceil(K * cap_factor)) limited by the window). */
+int vo_retained_cap(const vo_params* params);
+
+/* Staggered grid anchor count of one anchor frame. */
+int vo_grid_count(const vo_params* params, int width, int height, int frames,
+                  int anchor_frame);
+
+/* Dense helpers exposed for the reference's linalg known-answer tests.
+ * All matrices are row-major double. */
+int vo_linalg_mean(const double* samples, int rows, int cols, double* mean);
+int vo_linalg_gram_centered(const double* centered, int rows, int cols,
+                            double scale, double* gram);
+int vo_linalg_covariance_centered(const double* centered, int rows, int cols,
+                                  double scale, double* covariance);
+/* Top-rank eigenpairs of a symmetric dim x dim matrix by cyclic Jacobi.
+ * Returns the sweep count (>= 0) and sets *converged, or -1 on error. */
+int vo_linalg_topk_eigen(const double* matrix, int dim, int rank,
+                         double* values, double* vectors, int* converged);
+int vo_linalg_topk_eigen_f32(const float* matrix, int dim, int rank,
+                             float* values, float* vectors, int* converged);
+int vo_linalg_dual_basis(const double* centered, int rows, int cols,
+                         const double* gram_values,
+                         const double* gram_vectors, int rank,
+                         double* basis);
+
+const char* vo_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VNLB_ORACLE_H */

@@ -1,0 +1,105 @@
+// Accumulator clear and normalisation.
+//
+// The stage accumulates every weighted contribution straight into one
+// double-precision (numerator per channel, weight) set per output frame.
+// That is the sum the reference forms in two steps -- per anchor-frame
+// contribution stacks (src/aggregate/aggregate.hpp:18-71) summed by
+// vnlb.Aggregate over the anchors [t - fwd - (Pt - 1), t + bwd]
+// (src/vs/plugin.cpp:940-1045) -- with the slot filter applied at scatter
+// time.  Normalisation: numerator / weight, or the source sample where no
+// weight landed.
+#include "common.h"
+
+namespace vnlbhip {
+namespace {
+
+__global__ __launch_bounds__(256) void fill_zero_kernel(double* __restrict__ data, size_t count) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+         i += stride) {
+        data[i] = 0.0;
+    }
+}
+
+__global__ __launch_bounds__(256) void normalize_kernel(StageDesc d,
+                                                        const double* __restrict__ acc,
+                                                        const float* __restrict__ source,
+                                                        float* __restrict__ out) {
+    const size_t plane = static_cast<size_t>(d.width) * d.height;
+    const size_t total = plane * d.frames;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+         i += stride) {
+        const size_t frame = i / plane;
+        const size_t pixel = i - frame * plane;
+        const double* base = acc + frame * (d.channels + 1) * plane;
+        const float weight = static_cast<float>(base[static_cast<size_t>(d.channels) * plane + pixel]);
+        for (int c = 0; c < d.channels; ++c) {
+            const size_t index = (frame * d.channels + c) * plane + pixel;
+            const float numerator = static_cast<float>(base[static_cast<size_t>(c) * plane + pixel]);
+            out[index] = weight > 0.0F ? numerator / weight : source[index];
+        }
+    }
+}
+
+
+// Stage health counters: summary[0] counts groups whose model rows cannot
+// cover the rank (the reference throws for those, core.cpp:645-648),
+// summary[1] eigen problems that reached the sweep limit without meeting the
+// convergence test.
+__global__ __launch_bounds__(256) void summarize_kernel(const int* __restrict__ flags,
+                                                        const int* __restrict__ status,
+                                                        int groups, int models,
+                                                        int* __restrict__ summary) {
+    const int problem = blockIdx.x * blockDim.x + threadIdx.x;
+    if (problem >= groups * models) {
+        return;
+    }
+    const int group = problem / models;
+    const int flag = flags[group];
+    if ((problem % models) == 0 && (flag & 8) != 0) {
+        atomicAdd(&summary[0], 1);
+    }
+    if (status != nullptr && (flag & kFlagModelled) != 0 && (status[problem] >> 16) != 0) {
+        atomicAdd(&summary[1], 1);
+    }
+}
+
+unsigned grid_for(size_t count) {
+    size_t blocks = (count + 255) / 256;
+    if (blocks > 65536) {
+        blocks = 65536;
+    }
+    return static_cast<unsigned>(blocks == 0 ? 1 : blocks);
+}
+
+}  // namespace
+
+hipError_t launch_summarize(const int* flags, const int* status, int groups, int models,
+                            int* summary, hipStream_t stream) {
+    const int problems = groups * models;
+    if (problems <= 0) {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(summarize_kernel, dim3(static_cast<unsigned>((problems + 255) / 256)),
+                       dim3(256), 0, stream, flags, status, groups, models, summary);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_zero(double* data, size_t count, hipStream_t stream) {
+    if (count == 0) {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(grid_for(count)), dim3(256), 0, stream, data, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_normalize(const StageDesc& d, const double* acc, const float* source,
+                            float* out, hipStream_t stream) {
+    const size_t total = static_cast<size_t>(d.width) * d.height * d.frames;
+    hipLaunchKernelGGL(normalize_kernel, dim3(grid_for(total)), dim3(256), 0, stream, d, acc,
+                       source, out);
+    return hipGetLastError();
+}
+
+}  // namespace vnlbhip

@@ -1,0 +1,126 @@
+// Device helpers shared by the stage kernels: geometry rules of the
+// reference CPU path and wave/block reductions for 64-lane wavefronts.
+#pragma once
+
+#include "common.h"
+
+namespace vnlbhip {
+
+// compute_temporal_range (reference src/core/core.cpp:873-896).
+__host__ __device__ inline void temporal_range(const StageDesc& d, int anchor,
+                                               int& low, int& high) {
+    const int max_origin = d.frames - d.patch_time;
+    const int start = anchor - d.search_bwd;
+    const int end = anchor + d.search_fwd;
+    const int shift = (start < 0 ? start : 0) + (end - max_origin > 0 ? end - max_origin : 0);
+    low = start - shift > 0 ? start - shift : 0;
+    high = end - shift < max_origin ? end - shift : max_origin;
+}
+
+// compute_spatial_range (reference src/core/core.cpp:910-932).
+__host__ __device__ inline void spatial_range(const StageDesc& d, int center,
+                                              int axis, int& low, int& high) {
+    const int max_origin = axis - d.patch;
+    const int half = (d.search_window - 1) / 2;
+    const int start = center - half;
+    const int end = center + half;
+    const int shift = (start < 0 ? start : 0) + (end - max_origin > 0 ? end - max_origin : 0);
+    low = start - shift > 0 ? start - shift : 0;
+    high = end - shift < max_origin ? end - shift : max_origin;
+}
+
+// schedule_temporal_range (reference src/core/core.cpp:898-908): the anchor
+// frame, the future frames ascending, then the past frames descending.
+__host__ __device__ inline int scheduled_frame(int anchor, int low, int high,
+                                               int index) {
+    if (index == 0) {
+        return anchor;
+    }
+    const int future = high - anchor;
+    if (index <= future) {
+        return anchor + index;
+    }
+    return anchor - (index - future);
+}
+
+__device__ inline float video_at(const float* __restrict__ video,
+                                 const StageDesc& d, int x, int y, int frame,
+                                 int channel) {
+    return video[((static_cast<size_t>(frame) * d.channels + channel) * d.height +
+                  y) *
+                     d.width +
+                 x];
+}
+
+// Halving-tree sum over a 16-lane segment (the ReduceSum order of the CPU
+// restatement); every lane of the segment ends with the total.
+__device__ inline float segment16_sum(float v) {
+    v = v + __shfl_xor(v, 8, 16);
+    v = v + __shfl_xor(v, 4, 16);
+    v = v + __shfl_xor(v, 2, 16);
+    v = v + __shfl_xor(v, 1, 16);
+    return v;
+}
+
+__device__ inline int wave_inclusive_scan(int value) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int offset = 1; offset < 64; offset <<= 1) {
+        const int other = __shfl_up(value, offset, 64);
+        if (lane >= offset) {
+            value += other;
+        }
+    }
+    return value;
+}
+
+// Exclusive prefix sum over the block in thread order.  scratch holds one
+// int per wave.  All threads must call it.
+__device__ inline int block_exclusive_scan(int value, int& total, int* scratch) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int waves = (blockDim.x + 63) >> 6;
+    const int inclusive = wave_inclusive_scan(value);
+    if (lane == 63) {
+        scratch[wave] = inclusive;
+    }
+    __syncthreads();
+    int base = 0;
+    int sum = 0;
+    for (int w = 0; w < waves; ++w) {
+        const int v = scratch[w];
+        if (w < wave) {
+            base += v;
+        }
+        sum += v;
+    }
+    __syncthreads();
+    total = sum;
+    return base + inclusive - value;
+}
+
+__device__ inline float wave_max(float v) {
+#pragma unroll
+    for (int offset = 32; offset >= 1; offset >>= 1) {
+        v = fmaxf(v, __shfl_xor(v, offset, 64));
+    }
+    return v;
+}
+
+__device__ inline float wave_min(float v) {
+#pragma unroll
+    for (int offset = 32; offset >= 1; offset >>= 1) {
+        v = fminf(v, __shfl_xor(v, offset, 64));
+    }
+    return v;
+}
+
+__device__ inline double wave_sum_double(double v) {
+#pragma unroll
+    for (int offset = 32; offset >= 1; offset >>= 1) {
+        v = v + __shfl_xor(v, offset, 64);
+    }
+    return v;
+}
+
+}  // namespace vnlbhip

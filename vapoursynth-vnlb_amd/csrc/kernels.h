@@ -1,0 +1,43 @@
+// Launch entry points of the stage kernels (defined in the .hip files).
+#pragma once
+
+#include "common.h"
+
+#include <stddef.h>
+
+namespace vnlbhip {
+
+hipError_t launch_match(const StageDesc& d, const float* source, const uint32_t* anchors,
+                        int anchor_count, int* counts, uint32_t* positions, float* distances,
+                        hipStream_t stream);
+
+size_t paste_shared_bytes(const StageDesc& d, int max_anchors);
+hipError_t launch_paste(const StageDesc& d, int frame_count, const int* frame_offsets,
+                        const int* frame_ids, const int* row_tables, const int* counts,
+                        const uint32_t* positions, unsigned char* active, int max_anchors,
+                        hipStream_t stream);
+hipError_t launch_compact(const unsigned char* active, int anchor_count, int* group_list,
+                          int* group_count, hipStream_t stream);
+
+size_t prepare_shared_bytes(const StageDesc& d);
+size_t filter_shared_bytes(const StageDesc& d);
+hipError_t launch_prepare(const StageDesc& d, const float* noisy, const float* basic,
+                          const uint32_t* anchors, const int* counts, const uint32_t* positions,
+                          const int* group_list, int group_count, int problem_stride,
+                          int* group_flags, int* problem_dims, float* means, float* problems,
+                          hipStream_t stream);
+hipError_t launch_jacobi(const StageDesc& d, int problem_count, int problem_stride,
+                         const int* group_flags, const int* problem_dims, const float* problems,
+                         float* eigenvalues, float* vectors, int* status, hipStream_t stream);
+hipError_t launch_filter(const StageDesc& d, const float* noisy, const float* basic,
+                         const uint32_t* anchors, const int* counts, const uint32_t* positions,
+                         const int* group_list, int group_count, int problem_stride,
+                         const int* group_flags, const int* problem_dims, const float* means,
+                         const float* eigenvalues, const float* vectors, double* accumulators,
+                         float* weights_out, hipStream_t stream);
+
+hipError_t launch_fill_zero(double* data, size_t count, hipStream_t stream);
+hipError_t launch_normalize(const StageDesc& d, const double* acc, const float* source, float* out,
+                            hipStream_t stream);
+
+}  // namespace vnlbhip

@@ -1,0 +1,787 @@
+// Host side of the vnlbhip C ABI: parameter validation, the staggered anchor
+// grid, anchor-frame batching and the kernel sequence of one stage.
+//
+// Per batch of anchor frames (whole clip when it fits the budget):
+//   match -> paste mask -> compact active anchors -> prepare -> jacobi ->
+//   filter/scatter
+// then one normalisation over all output frames.  The reference runs the
+// same stage per anchor frame on the CPU (process_anchor_impl,
+// src/core/core.cpp:1820-1906) or per chunk of groups on CUDA
+// (StagePipeline::enqueue, src/cuda/pipeline.cu:554-667).
+#include "vnlbhip.h"
+
+#include "kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+using vnlbhip::StageDesc;
+
+struct InvalidArgument : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+struct DeviceFailure : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+struct ModelFailure : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void require(bool condition, const char* message) {
+    if (!condition) {
+        throw InvalidArgument(message);
+    }
+}
+
+void check(hipError_t status, const char* what) {
+    if (status != hipSuccess) {
+        throw DeviceFailure(std::string(what) + ": " + hipGetErrorString(status));
+    }
+}
+
+struct Geometry {
+    int width = 0;
+    int height = 0;
+    int frames = 0;
+    int channels = 0;
+};
+
+// soft_similar_cap / model_similar_cap (src/core/core.cpp:139-172).
+int capped_similar(int similar, float factor) {
+    if (factor == 0.0F) {
+        return INT_MAX;
+    }
+    const float max_factor = static_cast<float>(INT_MAX) / static_cast<float>(similar);
+    if (factor >= max_factor) {
+        return INT_MAX;
+    }
+    const float cap = std::ceil(static_cast<float>(similar) * factor);
+    return std::max(similar, static_cast<int>(cap));
+}
+
+int retained_cap(const vnlbhip_params& p) {
+    const long long raw = static_cast<long long>(p.search_window) * p.search_window *
+                          (static_cast<long long>(p.search_bwd) + p.search_fwd + 1);
+    return static_cast<int>(std::min<long long>(capped_similar(p.similar, p.similar_cap_factor), raw));
+}
+
+// validate_stage_parameters / validate_stage_configuration
+// (src/core/core.cpp:1948-2024), same messages.
+void validate_reference(const vnlbhip_params& p, const Geometry& g) {
+    require(std::isfinite(p.sigma) && p.sigma > 0.0F, "sigma must be finite and positive");
+    require(p.patch_size > 0, "block_size must be positive");
+    require(p.patch_time > 0, "patch_time must be positive");
+    require(p.search_window > 0 && p.search_window % 2 == 1,
+            "bm_range must describe a positive odd search window");
+    require(p.search_bwd >= 0 && p.search_fwd >= 0,
+            "search_bwd and search_fwd must be non-negative");
+    require(p.similar > 0, "group_size must be positive");
+    require(p.rank >= 0, "rank must be non-negative");
+    require(std::isfinite(p.similar_cap_factor) &&
+                (p.similar_cap_factor == 0.0F || p.similar_cap_factor >= 1.0F),
+            "cap_factor must be zero or at least 1.0");
+    require(std::isfinite(p.model_cap_factor) &&
+                (p.model_cap_factor == 0.0F || p.model_cap_factor >= 1.0F),
+            "model_cap_factor must be zero or at least 1.0");
+    require(std::isfinite(p.beta) && p.beta > 0.0F, "beta must be finite and positive");
+    require(std::isfinite(p.tau) && p.tau >= 0.0F, "tau must be finite and non-negative");
+    require(std::isfinite(p.variance_threshold), "variance_threshold must be finite");
+    require(std::isfinite(p.sigma_basic) && p.sigma_basic >= 0.0F,
+            "sigma_basic must be finite and non-negative");
+    require(std::isfinite(p.gamma) && p.gamma > 0.0F, "gamma must be finite and positive");
+    require(std::isfinite(p.weight_alpha) && p.weight_alpha >= 0.0F,
+            "weight_alpha must be finite and non-negative");
+    require(std::isfinite(p.weight_beta) && p.weight_beta >= 0.0F,
+            "weight_beta must be finite and non-negative");
+    require(std::isfinite(p.weight_gamma) && p.weight_gamma >= 0.0F,
+            "weight_gamma must be finite and non-negative");
+    require(std::isfinite(p.weight_epsilon) && p.weight_epsilon > 0.0F,
+            "weight_epsilon must be finite and positive");
+    require(std::isfinite(p.membership_noise_floor) && p.membership_noise_floor > 0.0F,
+            "membership_noise_floor must be finite and positive");
+    require(p.proc_step >= 0, "block_step must be non-negative; use 0 for auto");
+    require(g.width > 0 && g.height > 0 && g.frames > 0 && g.channels > 0,
+            "video geometry must be non-empty");
+    require(p.patch_size <= g.width && p.patch_size <= g.height,
+            "block_size must fit inside the frame");
+    require(p.patch_time <= g.frames, "patch_time must fit inside the clip");
+    const long long estimator_dim = static_cast<long long>(p.patch_size) * p.patch_size *
+                                    p.patch_time * (p.couple_channels != 0 ? g.channels : 1);
+    require(estimator_dim <= INT_MAX, "estimator dimension overflows int");
+    require(p.rank <= estimator_dim, "rank must not exceed the effective patch dimension");
+    require(p.similar >= std::min<long long>(estimator_dim, p.rank),
+            "group_size must cover the effective VNLB rank");
+}
+
+// prepare_aggregation_window (src/core/core.cpp:296-324).
+void aggregation_window(const vnlbhip_params& p, float* window) {
+    const int n = p.patch_size;
+    for (int i = 0; i < n * n; ++i) {
+        window[i] = 1.0F;
+    }
+    const float gamma = p.weight_gamma;
+    if (gamma == 0.0F || n <= 1) {
+        return;
+    }
+    const float center = 0.5F * static_cast<float>(n - 1);
+    const float radius = std::max(center, 0.5F);
+    float max_weight = 0.0F;
+    for (int y = 0; y < n; ++y) {
+        const float dy = (static_cast<float>(y) - center) / radius;
+        for (int x = 0; x < n; ++x) {
+            const float dx = (static_cast<float>(x) - center) / radius;
+            const float raw = std::exp(-0.5F * ((dx * dx) + (dy * dy)));
+            window[y * n + x] = raw;
+            max_weight = std::max(max_weight, raw);
+        }
+    }
+    const float inv_max = max_weight > 0.0F ? 1.0F / max_weight : 1.0F;
+    for (int i = 0; i < n * n; ++i) {
+        window[i] = std::pow(window[i] * inv_max, gamma);
+    }
+}
+
+StageDesc make_desc(int stage, const vnlbhip_params& p, const Geometry& g) {
+    require(stage == VNLBHIP_STAGE_BASIC || stage == VNLBHIP_STAGE_FINAL,
+            "stage must be 0 (Basic) or 1 (Final)");
+    validate_reference(p, g);
+
+    StageDesc d{};
+    d.stage = stage;
+    d.width = g.width;
+    d.height = g.height;
+    d.frames = g.frames;
+    d.channels = g.channels;
+    d.patch = p.patch_size;
+    d.patch_time = p.patch_time;
+    d.patch_area = p.patch_size * p.patch_size;
+    d.patch_dim = d.patch_area * p.patch_time;
+    d.dim = d.patch_dim * g.channels;
+    d.coupled = (p.couple_channels != 0 && g.channels > 1) ? 1 : 0;
+    d.models = d.coupled ? 1 : g.channels;
+    d.model_dim = d.coupled ? d.dim : d.patch_dim;
+    d.search_window = p.search_window;
+    d.search_bwd = p.search_bwd;
+    d.search_fwd = p.search_fwd;
+    d.similar = p.similar;
+    d.cap = retained_cap(p);
+    d.soft_cap = capped_similar(p.similar, p.similar_cap_factor);
+    d.model_cap = capped_similar(p.similar, p.model_cap_factor);
+    d.rank = p.rank;
+    d.slots = p.search_bwd + p.search_fwd + p.patch_time;
+    d.step = p.proc_step > 0 ? p.proc_step : (p.patch_size <= 1 ? 1 : p.patch_size);
+    d.flat_areas = p.flat_areas != 0 ? 1 : 0;
+    d.weights_enabled =
+        (p.weight_alpha != 0.0F || p.weight_beta != 0.0F || p.weight_gamma != 0.0F) ? 1 : 0;
+    d.windowed = p.weight_gamma != 0.0F ? 1 : 0;
+    d.fused_width = (d.coupled && !d.windowed) ? (p.patch_size / vnlbhip::kLanes) * vnlbhip::kLanes : 0;
+    d.max_rows = std::min(d.cap, d.model_cap);
+    d.use_dual = (d.model_dim >= 64 && d.max_rows < d.model_dim) ? 1 : 0;
+    d.jacobi_dim = d.use_dual ? d.max_rows : d.model_dim;
+    d.max_sweeps = 64;
+    d.tau = p.tau;
+    d.sigma = p.sigma;
+    d.sigma_basic = p.sigma_basic;
+    d.beta = p.beta;
+    d.variance_threshold = p.variance_threshold;
+    d.gamma = p.gamma;
+    d.weight_alpha = p.weight_alpha;
+    d.weight_beta = p.weight_beta;
+    d.weight_gamma = p.weight_gamma;
+    d.weight_epsilon = p.weight_epsilon;
+    d.membership_noise_floor = p.membership_noise_floor;
+
+    // Kernel limits of this build.
+    require(g.width <= vnlbhip::kMaxSide && g.height <= vnlbhip::kMaxSide,
+            "frame width and height must be at most 4096");
+    require(g.frames <= vnlbhip::kMaxFrames, "at most 256 frames per stage call");
+    require(g.channels == 1 || g.channels == 3, "only GrayS and YUV444PS clips are supported");
+    require(p.patch_size <= vnlbhip::kMaxPatch, "block_size above 16 is not supported");
+    require(d.dim <= vnlbhip::kMaxDim, "patch dimension above 1024 is not supported");
+    require(d.cap <= vnlbhip::kMaxCap, "retained group cap above 256 is not supported");
+    require(d.rank <= vnlbhip::kMaxRank, "rank above 64 is not supported");
+    require(d.jacobi_dim <= 64,
+            "eigen problems above 64x64 are not supported (covariance path with a large patch)");
+    require(d.max_rows <= vnlbhip::kMaxModelRows, "more than 64 model rows is not supported");
+    const long long candidates = static_cast<long long>(p.search_window) * p.search_window *
+                                 (static_cast<long long>(p.search_bwd) + p.search_fwd + 1);
+    require(candidates <= vnlbhip::kMaxCandidates,
+            "search window times temporal slots above 8192 candidates is not supported");
+    require(p.search_bwd + p.search_fwd + 1 <= vnlbhip::kMaxTemporal,
+            "temporal search radius too large");
+    require(static_cast<long long>(d.max_rows) * d.model_dim * 4 <= 96 * 1024,
+            "model rows times model dimension exceed the prepare kernel's LDS budget");
+    require(static_cast<long long>(d.rank) * d.dim * 4 + 8LL * d.dim <= 96 * 1024,
+            "rank times patch dimension exceed the filter kernel's LDS budget");
+    aggregation_window(p, d.window);
+    return d;
+}
+
+// reference_processing_mask_selected / initialize_frame_processing_mask
+// (src/core/core.cpp:934-967): anchors in raster order plus, per row, the
+// first grid index, the column phase and the anchor count.
+struct Grid {
+    std::vector<uint32_t> anchors;
+    std::vector<int> offsets;  // per processed frame, into anchors
+    std::vector<int> ids;      // anchor frame of each processed frame
+    std::vector<int> rows;     // [frame][3][height]
+};
+
+void append_grid_frame(const StageDesc& d, int anchor_frame, Grid& grid) {
+    const int step = d.step;
+    const int max_x = d.width - d.patch;
+    const int max_y = d.height - d.patch;
+    const int max_frame = d.frames - d.patch_time;
+    const int phase_y = anchor_frame == max_frame ? 0 : anchor_frame;
+    const size_t row_base = grid.rows.size();
+    grid.rows.resize(row_base + static_cast<size_t>(3) * d.height, -1);
+    int* first = grid.rows.data() + row_base;
+    int* phase = first + d.height;
+    int* count = phase + d.height;
+    const int frame_begin = static_cast<int>(grid.anchors.size());
+    grid.offsets.push_back(frame_begin);
+    grid.ids.push_back(anchor_frame);
+    for (int y = 0; y <= max_y; ++y) {
+        if (y != 0 && y != max_y && y % step != phase_y % step) {
+            count[y] = 0;
+            continue;
+        }
+        const int phase_x = y == max_y ? 0 : phase_y + (y / step);
+        const int ph = phase_x % step;
+        first[y] = static_cast<int>(grid.anchors.size()) - frame_begin;
+        phase[y] = ph;
+        int n = 0;
+        for (int x = 0; x <= max_x; ++x) {
+            if (x == 0 || x == max_x || x % step == ph) {
+                grid.anchors.push_back(vnlbhip::pack_position(x, y, anchor_frame));
+                ++n;
+            }
+        }
+        count[y] = n;
+    }
+    for (int y = max_y + 1; y < d.height; ++y) {
+        count[y] = 0;
+    }
+}
+
+template <class T> struct DeviceBuffer {
+    T* ptr = nullptr;
+    size_t capacity = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer() { release(); }
+    void release() {
+        if (ptr != nullptr) {
+            (void)hipFree(ptr);
+            ptr = nullptr;
+        }
+        capacity = 0;
+    }
+    T* reserve(size_t count) {
+        if (count == 0) {
+            count = 1;
+        }
+        if (count > capacity) {
+            release();
+            check(hipMalloc(reinterpret_cast<void**>(&ptr), count * sizeof(T)), "hipMalloc");
+            capacity = count;
+        }
+        return ptr;
+    }
+};
+
+struct DebugOutput {
+    int capacity_anchors = 0;
+    int* num_anchors = nullptr;
+    int* anchor_x = nullptr;
+    int* anchor_y = nullptr;
+    int* counts = nullptr;
+    int* positions = nullptr;
+    float* distances = nullptr;
+    unsigned char* active = nullptr;
+    float* weights = nullptr;
+    float* eigenvalues = nullptr;
+    int* group_flags = nullptr;
+};
+
+}  // namespace
+
+struct vnlbhip_context {
+    int device = 0;
+    std::string error;
+    vnlbhip_stats stats{};
+    DeviceBuffer<uint32_t> anchors;
+    DeviceBuffer<int> offsets;
+    DeviceBuffer<int> ids;
+    DeviceBuffer<int> rows;
+    DeviceBuffer<int> counts;
+    DeviceBuffer<uint32_t> positions;
+    DeviceBuffer<float> distances;
+    DeviceBuffer<unsigned char> active;
+    DeviceBuffer<int> group_list;
+    DeviceBuffer<int> group_count;
+    DeviceBuffer<int> group_flags;
+    DeviceBuffer<int> problem_dims;
+    DeviceBuffer<float> means;
+    DeviceBuffer<float> problems;
+    DeviceBuffer<float> eigenvalues;
+    DeviceBuffer<float> vectors;
+    DeviceBuffer<int> status;
+    DeviceBuffer<float> weights;
+    DeviceBuffer<double> accumulators;
+    DeviceBuffer<int> summary;
+    DeviceBuffer<float> host_in;
+    DeviceBuffer<float> host_ref;
+    DeviceBuffer<float> host_out;
+    size_t batch_entry_budget = static_cast<size_t>(1) << 28;  // anchors * cap
+};
+
+namespace vnlbhip {
+hipError_t launch_summarize(const int* flags, const int* status, int groups, int models,
+                            int* summary, hipStream_t stream);
+}
+
+namespace {
+
+void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const float* noisy,
+               const float* ref, float* out, const Geometry& g, hipStream_t stream,
+               int only_frame, DebugOutput* debug) {
+    const StageDesc d = make_desc(stage, p, g);
+    require(noisy != nullptr, "video data pointer must not be null");
+    if (stage == VNLBHIP_STAGE_FINAL) {
+        require(ref != nullptr, "Final needs the ref (basic) clip");
+    }
+    check(hipSetDevice(ctx.device), "hipSetDevice");
+    ctx.stats = vnlbhip_stats{};
+
+    const int last_anchor_frame = g.frames - p.patch_time;
+    std::vector<int> frames;
+    if (only_frame >= 0) {
+        if (only_frame <= last_anchor_frame) {
+            frames.push_back(only_frame);
+        }
+    } else {
+        for (int f = 0; f <= last_anchor_frame; ++f) {
+            frames.push_back(f);
+        }
+    }
+
+    const size_t plane = static_cast<size_t>(g.width) * g.height;
+    const size_t acc_count = plane * (g.channels + 1) * g.frames;
+    double* acc = ctx.accumulators.reserve(acc_count);
+    check(vnlbhip::launch_fill_zero(acc, acc_count, stream), "fill accumulators");
+    int* summary = ctx.summary.reserve(4);
+    check(hipMemsetAsync(summary, 0, 4 * sizeof(int), stream), "hipMemsetAsync");
+
+    const float* source = stage == VNLBHIP_STAGE_FINAL ? ref : noisy;
+    const int cap = d.cap;
+    const int n_stride = d.jacobi_dim;
+    size_t next = 0;
+    while (next < frames.size()) {
+        // Gather frames into a batch within the entry budget.
+        Grid grid;
+        int max_anchors = 0;
+        while (next < frames.size()) {
+            const size_t before = grid.anchors.size();
+            append_grid_frame(d, frames[next], grid);
+            const int added = static_cast<int>(grid.anchors.size() - before);
+            const bool over = grid.anchors.size() * static_cast<size_t>(cap) > ctx.batch_entry_budget;
+            if (over && grid.offsets.size() > 1) {
+                // Roll back this frame; it starts the next batch.
+                grid.anchors.resize(before);
+                grid.offsets.pop_back();
+                grid.ids.pop_back();
+                grid.rows.resize(grid.rows.size() - static_cast<size_t>(3) * g.height);
+                break;
+            }
+            max_anchors = std::max(max_anchors, added);
+            ++next;
+        }
+        const int frame_count = static_cast<int>(grid.offsets.size());
+        grid.offsets.push_back(static_cast<int>(grid.anchors.size()));
+        const int anchor_count = static_cast<int>(grid.anchors.size());
+        ctx.stats.anchors += anchor_count;
+        ++ctx.stats.batches;
+
+        uint32_t* d_anchors = ctx.anchors.reserve(grid.anchors.size());
+        int* d_offsets = ctx.offsets.reserve(grid.offsets.size());
+        int* d_ids = ctx.ids.reserve(grid.ids.size());
+        int* d_rows = ctx.rows.reserve(grid.rows.size());
+        check(hipMemcpyAsync(d_anchors, grid.anchors.data(), grid.anchors.size() * sizeof(uint32_t),
+                             hipMemcpyHostToDevice, stream), "upload anchors");
+        check(hipMemcpyAsync(d_offsets, grid.offsets.data(), grid.offsets.size() * sizeof(int),
+                             hipMemcpyHostToDevice, stream), "upload offsets");
+        check(hipMemcpyAsync(d_ids, grid.ids.data(), grid.ids.size() * sizeof(int),
+                             hipMemcpyHostToDevice, stream), "upload frame ids");
+        check(hipMemcpyAsync(d_rows, grid.rows.data(), grid.rows.size() * sizeof(int),
+                             hipMemcpyHostToDevice, stream), "upload row tables");
+
+        int* d_counts = ctx.counts.reserve(anchor_count);
+        uint32_t* d_positions = ctx.positions.reserve(static_cast<size_t>(anchor_count) * cap);
+        float* d_distances = ctx.distances.reserve(static_cast<size_t>(anchor_count) * cap);
+        unsigned char* d_active = ctx.active.reserve(anchor_count);
+        int* d_list = ctx.group_list.reserve(anchor_count);
+        int* d_group_count = ctx.group_count.reserve(1);
+
+        check(vnlbhip::launch_match(d, source, d_anchors, anchor_count, d_counts, d_positions,
+                                    d_distances, stream), "match kernel");
+        check(vnlbhip::launch_paste(d, frame_count, d_offsets, d_ids, d_rows, d_counts,
+                                    d_positions, d_active, max_anchors, stream), "paste kernel");
+        check(hipMemsetAsync(d_group_count, 0, sizeof(int), stream), "hipMemsetAsync");
+        check(vnlbhip::launch_compact(d_active, anchor_count, d_list, d_group_count, stream),
+              "compact kernel");
+        int groups = 0;
+        check(hipMemcpyAsync(&groups, d_group_count, sizeof(int), hipMemcpyDeviceToHost, stream),
+              "read group count");
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        ctx.stats.groups += groups;
+
+        const int problems = groups * d.models;
+        int* d_flags = ctx.group_flags.reserve(groups);
+        int* d_dims = ctx.problem_dims.reserve(groups);
+        float* d_means = ctx.means.reserve(static_cast<size_t>(groups) * 2 * d.dim);
+        float* d_problems =
+            ctx.problems.reserve(static_cast<size_t>(problems) * n_stride * n_stride);
+        float* d_eigen = ctx.eigenvalues.reserve(static_cast<size_t>(problems) * std::max(d.rank, 1));
+        float* d_vectors =
+            ctx.vectors.reserve(static_cast<size_t>(problems) * std::max(d.rank, 1) * n_stride);
+        int* d_status = ctx.status.reserve(problems);
+        float* d_weights = nullptr;
+        if (debug != nullptr) {
+            d_weights = ctx.weights.reserve(static_cast<size_t>(anchor_count) * cap);
+        }
+        const float* basic = stage == VNLBHIP_STAGE_FINAL ? ref : noisy;
+        check(vnlbhip::launch_prepare(d, noisy, basic, d_anchors, d_counts, d_positions, d_list,
+                                      groups, n_stride, d_flags, d_dims, d_means, d_problems,
+                                      stream), "prepare kernel");
+        if (d.rank > 0) {
+            check(vnlbhip::launch_jacobi(d, problems, n_stride, d_flags, d_dims, d_problems,
+                                         d_eigen, d_vectors, d_status, stream), "jacobi kernel");
+        }
+        check(vnlbhip::launch_filter(d, noisy, basic, d_anchors, d_counts, d_positions, d_list,
+                                     groups, n_stride, d_flags, d_dims, d_means, d_eigen,
+                                     d_vectors, acc, d_weights, stream), "filter kernel");
+        check(vnlbhip::launch_summarize(d_flags, d.rank > 0 ? d_status : nullptr, groups,
+                                        d.models, summary, stream), "summarize kernel");
+
+        if (debug != nullptr) {
+            require(anchor_count <= debug->capacity_anchors, "debug capacity too small");
+            std::vector<int> counts(anchor_count);
+            std::vector<uint32_t> packed(static_cast<size_t>(anchor_count) * cap);
+            std::vector<float> dist(static_cast<size_t>(anchor_count) * cap);
+            std::vector<unsigned char> act(anchor_count);
+            std::vector<float> wts(static_cast<size_t>(anchor_count) * cap);
+            std::vector<int> list(groups);
+            std::vector<int> flags(groups);
+            std::vector<float> eig(static_cast<size_t>(problems) * std::max(d.rank, 1));
+            check(hipMemcpy(counts.data(), d_counts, counts.size() * sizeof(int),
+                            hipMemcpyDeviceToHost), "read counts");
+            check(hipMemcpy(packed.data(), d_positions, packed.size() * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost), "read positions");
+            check(hipMemcpy(dist.data(), d_distances, dist.size() * sizeof(float),
+                            hipMemcpyDeviceToHost), "read distances");
+            check(hipMemcpy(act.data(), d_active, act.size(), hipMemcpyDeviceToHost), "read active");
+            check(hipMemcpy(wts.data(), d_weights, wts.size() * sizeof(float),
+                            hipMemcpyDeviceToHost), "read weights");
+            if (groups > 0) {
+                check(hipMemcpy(list.data(), d_list, list.size() * sizeof(int),
+                                hipMemcpyDeviceToHost), "read group list");
+                check(hipMemcpy(flags.data(), d_flags, flags.size() * sizeof(int),
+                                hipMemcpyDeviceToHost), "read group flags");
+                if (d.rank > 0) {
+                    check(hipMemcpy(eig.data(), d_eigen, eig.size() * sizeof(float),
+                                    hipMemcpyDeviceToHost), "read eigenvalues");
+                }
+            }
+            if (debug->num_anchors != nullptr) {
+                *debug->num_anchors = anchor_count;
+            }
+            for (int a = 0; a < anchor_count; ++a) {
+                const uint32_t anchor = grid.anchors[static_cast<size_t>(a)];
+                if (debug->anchor_x != nullptr) {
+                    debug->anchor_x[a] = vnlbhip::position_x(anchor);
+                }
+                if (debug->anchor_y != nullptr) {
+                    debug->anchor_y[a] = vnlbhip::position_y(anchor);
+                }
+                if (debug->counts != nullptr) {
+                    debug->counts[a] = counts[static_cast<size_t>(a)];
+                }
+                for (int i = 0; i < counts[static_cast<size_t>(a)] && i < cap; ++i) {
+                    const size_t index = static_cast<size_t>(a) * cap + i;
+                    const uint32_t pk = packed[index];
+                    if (debug->positions != nullptr) {
+                        debug->positions[index] =
+                            (vnlbhip::position_frame(pk) * g.height + vnlbhip::position_y(pk)) *
+                                g.width +
+                            vnlbhip::position_x(pk);
+                    }
+                    if (debug->distances != nullptr) {
+                        debug->distances[index] = dist[index];
+                    }
+                    if (debug->weights != nullptr && act[static_cast<size_t>(a)] != 0) {
+                        debug->weights[index] = wts[index];
+                    }
+                }
+                if (debug->active != nullptr) {
+                    debug->active[a] = act[static_cast<size_t>(a)];
+                }
+                if (debug->group_flags != nullptr) {
+                    debug->group_flags[a] = 0;
+                }
+            }
+            for (int gi = 0; gi < groups; ++gi) {
+                const int a = list[static_cast<size_t>(gi)];
+                if (debug->group_flags != nullptr) {
+                    debug->group_flags[a] = flags[static_cast<size_t>(gi)] & 7;
+                }
+                if (debug->eigenvalues != nullptr && (flags[static_cast<size_t>(gi)] & 4) != 0) {
+                    for (int j = 0; j < d.rank; ++j) {
+                        debug->eigenvalues[static_cast<size_t>(a) * d.rank + j] =
+                            eig[static_cast<size_t>(gi) * d.models * d.rank + j];
+                    }
+                }
+            }
+        }
+    }
+
+    if (out != nullptr) {
+        check(vnlbhip::launch_normalize(d, acc, noisy, out, stream), "normalize kernel");
+    }
+    int host_summary[4] = {0, 0, 0, 0};
+    check(hipMemcpyAsync(host_summary, summary, sizeof(host_summary), hipMemcpyDeviceToHost, stream),
+          "read summary");
+    check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    ctx.stats.unconverged_groups = host_summary[1];
+    if (host_summary[0] != 0) {
+        throw ModelFailure("group_size must cover the effective VNLB rank");
+    }
+}
+
+int fail(vnlbhip_context* ctx, int code, const char* message) {
+    if (ctx != nullptr) {
+        ctx->error = message;
+    }
+    return code;
+}
+
+template <class Fn> int guarded(vnlbhip_context* ctx, Fn&& fn) {
+    try {
+        fn();
+        if (ctx != nullptr) {
+            ctx->error.clear();
+        }
+        return VNLBHIP_OK;
+    } catch (const InvalidArgument& e) {
+        return fail(ctx, VNLBHIP_ERROR_INVALID, e.what());
+    } catch (const ModelFailure& e) {
+        return fail(ctx, VNLBHIP_ERROR_MODEL, e.what());
+    } catch (const DeviceFailure& e) {
+        return fail(ctx, VNLBHIP_ERROR_DEVICE, e.what());
+    } catch (const std::exception& e) {
+        return fail(ctx, VNLBHIP_ERROR_DEVICE, e.what());
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int vnlbhip_default_params(int stage, vnlbhip_params* out) {
+    if (out == nullptr || (stage != VNLBHIP_STAGE_BASIC && stage != VNLBHIP_STAGE_FINAL)) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    vnlbhip_params p{};
+    p.sigma = 0.0F;
+    p.patch_size = 8;
+    p.patch_time = 1;
+    p.search_window = 19;
+    p.search_bwd = 1;
+    p.search_fwd = 1;
+    p.similar = 8;
+    p.rank = 8;
+    p.similar_cap_factor = 4.0F;
+    p.model_cap_factor = 1.0F;
+    p.beta = 1.0F;
+    p.tau = 0.0F;
+    p.variance_threshold = 1.1F;
+    p.sigma_basic = 0.0F;
+    p.gamma = 0.95F;
+    p.weight_alpha = 0.75F;
+    p.weight_beta = 0.35F;
+    p.weight_gamma = 1.0F;
+    p.weight_epsilon = 1.0e-6F;
+    p.membership_noise_floor = 0.25F;
+    p.proc_step = 8;
+    p.flat_areas = 0;
+    p.couple_channels = 1;
+    if (stage == VNLBHIP_STAGE_FINAL) {
+        p.tau = 400.0F / (255.0F * 255.0F);
+        p.variance_threshold = 1.7F;
+        p.flat_areas = 1;
+        p.weight_alpha = 1.0F;
+        p.weight_beta = 0.5F;
+        p.weight_gamma = 1.0F;
+    }
+    *out = p;
+    return VNLBHIP_OK;
+}
+
+int vnlbhip_validate(int stage, const vnlbhip_params* params, int width, int height, int frames,
+                     int channels, char* message, size_t message_size) {
+    try {
+        if (params == nullptr) {
+            throw InvalidArgument("params must not be null");
+        }
+        (void)make_desc(stage, *params, Geometry{width, height, frames, channels});
+        if (message != nullptr && message_size > 0) {
+            message[0] = '\0';
+        }
+        return VNLBHIP_OK;
+    } catch (const std::exception& e) {
+        if (message != nullptr && message_size > 0) {
+            std::snprintf(message, message_size, "%s", e.what());
+        }
+        return VNLBHIP_ERROR_INVALID;
+    }
+}
+
+int vnlbhip_retained_cap(const vnlbhip_params* params) {
+    if (params == nullptr || params->similar <= 0) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    return retained_cap(*params);
+}
+
+int vnlbhip_create(int device, vnlbhip_context** out) {
+    if (out == nullptr) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        return VNLBHIP_ERROR_DEVICE;
+    }
+    if (device < 0 || device >= count) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    if (hipSetDevice(device) != hipSuccess) {
+        return VNLBHIP_ERROR_DEVICE;
+    }
+    auto* ctx = new (std::nothrow) vnlbhip_context();
+    if (ctx == nullptr) {
+        return VNLBHIP_ERROR_DEVICE;
+    }
+    ctx->device = device;
+    *out = ctx;
+    return VNLBHIP_OK;
+}
+
+void vnlbhip_destroy(vnlbhip_context* context) {
+    if (context != nullptr) {
+        (void)hipSetDevice(context->device);
+        delete context;
+    }
+}
+
+const char* vnlbhip_last_error(const vnlbhip_context* context) {
+    return context == nullptr ? "null context" : context->error.c_str();
+}
+
+int vnlbhip_get_stats(const vnlbhip_context* context, vnlbhip_stats* out) {
+    if (context == nullptr || out == nullptr) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    *out = context->stats;
+    return VNLBHIP_OK;
+}
+
+int vnlbhip_stage(vnlbhip_context* context, int stage, const vnlbhip_params* params,
+                  const float* noisy, const float* ref, float* out, int width, int height,
+                  int frames, int channels, void* stream) {
+    if (context == nullptr) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    return guarded(context, [&]() {
+        require(params != nullptr && out != nullptr, "params and out must not be null");
+        run_stage(*context, stage, *params, noisy, ref, out,
+                  Geometry{width, height, frames, channels},
+                  reinterpret_cast<hipStream_t>(stream), -1, nullptr);
+    });
+}
+
+int vnlbhip_stage_host(vnlbhip_context* context, int stage, const vnlbhip_params* params,
+                       const float* noisy, const float* ref, float* out, int width, int height,
+                       int frames, int channels) {
+    if (context == nullptr) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    return guarded(context, [&]() {
+        require(params != nullptr && noisy != nullptr && out != nullptr,
+                "params, noisy and out must not be null");
+        require(stage != VNLBHIP_STAGE_FINAL || ref != nullptr, "Final needs the ref (basic) clip");
+        require(width > 0 && height > 0 && frames > 0 && channels > 0,
+                "video geometry must be non-empty");
+        check(hipSetDevice(context->device), "hipSetDevice");
+        const size_t count = static_cast<size_t>(width) * height * frames * channels;
+        float* d_in = context->host_in.reserve(count);
+        float* d_out = context->host_out.reserve(count);
+        float* d_ref = nullptr;
+        check(hipMemcpy(d_in, noisy, count * sizeof(float), hipMemcpyHostToDevice), "upload clip");
+        if (stage == VNLBHIP_STAGE_FINAL) {
+            d_ref = context->host_ref.reserve(count);
+            check(hipMemcpy(d_ref, ref, count * sizeof(float), hipMemcpyHostToDevice), "upload ref");
+        }
+        run_stage(*context, stage, *params, d_in, d_ref, d_out,
+                  Geometry{width, height, frames, channels}, nullptr, -1, nullptr);
+        check(hipMemcpy(out, d_out, count * sizeof(float), hipMemcpyDeviceToHost), "download output");
+    });
+}
+
+int vnlbhip_debug_frame(vnlbhip_context* context, int stage, const vnlbhip_params* params,
+                        const float* noisy, const float* ref, int width, int height, int frames,
+                        int channels, int anchor_frame, int capacity_anchors, int* num_anchors,
+                        int* anchor_x, int* anchor_y, int* counts, int* positions,
+                        float* distances, unsigned char* active, float* weights,
+                        float* eigenvalues, int* group_flags) {
+    if (context == nullptr) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    return guarded(context, [&]() {
+        require(params != nullptr, "params must not be null");
+        require(anchor_frame >= 0, "anchor_frame must be non-negative");
+        DebugOutput debug;
+        debug.capacity_anchors = capacity_anchors;
+        debug.num_anchors = num_anchors;
+        debug.anchor_x = anchor_x;
+        debug.anchor_y = anchor_y;
+        debug.counts = counts;
+        debug.positions = positions;
+        debug.distances = distances;
+        debug.active = active;
+        debug.weights = weights;
+        debug.eigenvalues = eigenvalues;
+        debug.group_flags = group_flags;
+        if (num_anchors != nullptr) {
+            *num_anchors = 0;
+        }
+        run_stage(*context, stage, *params, noisy, ref, nullptr,
+                  Geometry{width, height, frames, channels}, nullptr, anchor_frame, &debug);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,197 @@
+"""vnlbhip: the VNLB Basic / Final stages on an MI355X.
+
+Python mirror of the reference's VapourSynth interface for the hot path:
+``Basic(clip, sigma, ...)`` and ``Final(clip, ref, sigma, ...)`` accept the
+argument names and defaults of ``vnlb.Basic`` / ``vnlb.Final``
+(src/vs/plugin.cpp:1162-1185) and, like the CUDA plugin ``vnlbcu``, return
+the aggregated frames (``vnlb.Aggregate(vnlb.Basic(...), src)``).
+
+A clip is a float32 array shaped (frames, channels, height, width) with
+channels 1 (GrayS) or 3 (YUV444PS), or (frames, height, width) for GrayS.
+numpy arrays are processed through host buffers; torch tensors on a ROCm
+device stay in device memory and run on the current stream.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+
+import numpy as np
+
+from . import _abi
+from .params import BASIC, FINAL, StageParameters, VNLBError, parse_stage_parameters
+
+__all__ = ["Basic", "Final", "Context", "StageParameters", "VNLBError", "parse_stage_parameters",
+           "BASIC", "FINAL"]
+
+_ERROR_NAMES = {-1: "invalid argument", -2: "device error", -3: "model error"}
+
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+def _canonical_shape(shape) -> tuple[int, int, int, int]:
+    if len(shape) == 3:
+        return int(shape[0]), 1, int(shape[1]), int(shape[2])
+    if len(shape) == 4:
+        return tuple(int(s) for s in shape)  # type: ignore[return-value]
+    raise VNLBError("VNLB HIP: only constant GrayS and YUV444PS clips are supported")
+
+
+class Context:
+    """One device context (not thread-safe; see the module-level helpers)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = _abi.load()
+        self._handle = ctypes.c_void_p()
+        rc = self._lib.vnlbhip_create(int(device), ctypes.byref(self._handle))
+        if rc != 0:
+            raise RuntimeError(f"VNLB HIP: cannot create a context on device {device} ({_ERROR_NAMES.get(rc, rc)})")
+        self.device = int(device)
+
+    def close(self) -> None:
+        if self._handle:
+            self._lib.vnlbhip_destroy(self._handle)
+            self._handle = ctypes.c_void_p()
+
+    def __del__(self):  # pragma: no cover - best effort
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _raise(self, rc: int) -> None:
+        message = self._lib.vnlbhip_last_error(self._handle).decode()
+        raise VNLBError(f"VNLB HIP: {message}") if rc in (-1, -3) else RuntimeError(f"VNLB HIP: {message}")
+
+    def stats(self) -> dict:
+        s = _abi.Stats()
+        self._lib.vnlbhip_get_stats(self._handle, ctypes.byref(s))
+        return {"anchors": s.anchors, "groups": s.groups,
+                "unconverged_groups": s.unconverged_groups, "batches": s.batches}
+
+    def stage(self, stage: int, params: StageParameters, noisy, ref=None):
+        """Run one stage over a whole clip and return the aggregated frames."""
+        frames, channels, height, width = _canonical_shape(tuple(noisy.shape))
+        if channels not in (1, 3):
+            raise VNLBError("VNLB HIP: only constant GrayS and YUV444PS clips are supported")
+        if stage == FINAL:
+            if ref is None:
+                raise VNLBError("VNLB HIP: ref must be a video node")
+            if tuple(ref.shape) != tuple(noisy.shape):
+                raise VNLBError("VNLB HIP: ref clip properties must match input clip")
+        p = _abi.to_struct(params)
+        if _is_torch(noisy):
+            import torch
+
+            if noisy.dtype != torch.float32 or not noisy.is_cuda:
+                raise VNLBError("VNLB HIP: device clips must be float32 tensors on the GPU")
+            src = noisy.contiguous()
+            guide = ref.contiguous() if ref is not None else None
+            out = torch.empty_like(src)
+            stream = torch.cuda.current_stream(src.device).cuda_stream
+            rc = self._lib.vnlbhip_stage(
+                self._handle, stage, ctypes.byref(p), ctypes.c_void_p(src.data_ptr()),
+                ctypes.c_void_p(guide.data_ptr()) if guide is not None else None,
+                ctypes.c_void_p(out.data_ptr()), width, height, frames, channels,
+                ctypes.c_void_p(stream))
+            if rc != 0:
+                self._raise(rc)
+            return out
+        src = np.ascontiguousarray(noisy, dtype=np.float32)
+        guide = np.ascontiguousarray(ref, dtype=np.float32) if ref is not None else None
+        out = np.empty_like(src)
+        fptr = ctypes.POINTER(ctypes.c_float)
+        rc = self._lib.vnlbhip_stage_host(
+            self._handle, stage, ctypes.byref(p), src.ctypes.data_as(fptr),
+            guide.ctypes.data_as(fptr) if guide is not None else None,
+            out.ctypes.data_as(fptr), width, height, frames, channels)
+        if rc != 0:
+            self._raise(rc)
+        return out
+
+    def debug_frame(self, stage: int, params: StageParameters, noisy, ref, anchor_frame: int) -> dict:
+        """Per-anchor diagnostics of one anchor frame (device tensors in)."""
+        import torch
+
+        frames, channels, height, width = _canonical_shape(tuple(noisy.shape))
+        p = _abi.to_struct(params)
+        cap = self._lib.vnlbhip_retained_cap(ctypes.byref(p))
+        if cap <= 0:
+            raise VNLBError("VNLB HIP: invalid parameters")
+        capacity = width * height
+        ints = lambda n: np.zeros(n, dtype=np.int32)  # noqa: E731
+        num = ctypes.c_int(0)
+        ax, ay, counts = ints(capacity), ints(capacity), ints(capacity)
+        positions = np.full(capacity * cap, -1, dtype=np.int32)
+        distances = np.full(capacity * cap, np.nan, dtype=np.float32)
+        active = np.zeros(capacity, dtype=np.uint8)
+        weights = np.full(capacity * cap, np.nan, dtype=np.float32)
+        eig = np.full(capacity * max(params.rank, 1), np.nan, dtype=np.float32)
+        flags = ints(capacity)
+        src = noisy.contiguous()
+        guide = ref.contiguous() if ref is not None else None
+        ip = ctypes.POINTER(ctypes.c_int)
+        fp = ctypes.POINTER(ctypes.c_float)
+        rc = self._lib.vnlbhip_debug_frame(
+            self._handle, stage, ctypes.byref(p), ctypes.c_void_p(src.data_ptr()),
+            ctypes.c_void_p(guide.data_ptr()) if guide is not None else None,
+            width, height, frames, channels, anchor_frame, capacity, ctypes.byref(num),
+            ax.ctypes.data_as(ip), ay.ctypes.data_as(ip), counts.ctypes.data_as(ip),
+            positions.ctypes.data_as(ip), distances.ctypes.data_as(fp),
+            active.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), weights.ctypes.data_as(fp),
+            eig.ctypes.data_as(fp), flags.ctypes.data_as(ip))
+        if rc != 0:
+            self._raise(rc)
+        torch.cuda.synchronize()
+        n = num.value
+        return {
+            "cap": cap,
+            "anchor_x": ax[:n], "anchor_y": ay[:n], "counts": counts[:n],
+            "positions": positions[: n * cap].reshape(n, cap),
+            "distances": distances[: n * cap].reshape(n, cap),
+            "active": active[:n], "weights": weights[: n * cap].reshape(n, cap),
+            "eigenvalues": eig[: n * max(params.rank, 1)].reshape(n, max(params.rank, 1)),
+            "group_flags": flags[:n],
+        }
+
+
+_local = threading.local()
+
+
+def _context(device: int) -> Context:
+    contexts = getattr(_local, "contexts", None)
+    if contexts is None:
+        contexts = _local.contexts = {}
+    if device not in contexts:
+        contexts[device] = Context(device)
+    return contexts[device]
+
+
+def Basic(clip, sigma, block_size=None, block_step=None, group_size=None, bm_range=None,
+          patch_time=None, radius=None, search_bwd=None, search_fwd=None, rank=None,
+          cap_factor=None, model_cap_factor=None, beta=None, tau=None,
+          variance_threshold=None, weight_alpha=None, weight_beta=None, weight_gamma=None,
+          weight_epsilon=None, membership_noise_floor=None, chroma=None, mvfw=None,
+          mvbw=None, device_id=None, num_streams=None, chunk_size=None):
+    """vnlb.Basic followed by vnlb.Aggregate(src=clip), on the GPU."""
+    args = dict(locals())
+    clip = args.pop("clip")
+    params = parse_stage_parameters(BASIC, args)
+    return _context(int(device_id or 0)).stage(BASIC, params, clip)
+
+
+def Final(clip, ref, sigma, block_size=None, block_step=None, group_size=None, bm_range=None,
+          patch_time=None, radius=None, search_bwd=None, search_fwd=None, rank=None,
+          cap_factor=None, model_cap_factor=None, beta=None, tau=None,
+          variance_threshold=None, sigma_basic=None, gamma=None, flat_areas=None,
+          weight_alpha=None, weight_beta=None, weight_gamma=None, weight_epsilon=None,
+          membership_noise_floor=None, chroma=None, mvfw=None, mvbw=None, device_id=None,
+          num_streams=None, chunk_size=None):
+    """vnlb.Final(ref=basic) followed by vnlb.Aggregate(src=clip), on the GPU."""
+    args = dict(locals())
+    clip = args.pop("clip")
+    ref = args.pop("ref")
+    params = parse_stage_parameters(FINAL, args)
+    return _context(int(device_id or 0)).stage(FINAL, params, clip, ref)

@@ -1,0 +1,119 @@
+"""ctypes binding of the vnlbhip C ABI (include/vnlbhip.h).
+
+The library is the product: there is no CPU fallback.  Loading fails loudly
+when libvnlbhip.so has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+from .params import StageParameters
+
+LIBRARY_PATH = Path(__file__).resolve().parent.parent / "_build" / "libvnlbhip.so"
+
+EXPORTED_SYMBOLS = (
+    "vnlbhip_default_params",
+    "vnlbhip_validate",
+    "vnlbhip_create",
+    "vnlbhip_destroy",
+    "vnlbhip_last_error",
+    "vnlbhip_stage",
+    "vnlbhip_stage_host",
+    "vnlbhip_debug_frame",
+    "vnlbhip_retained_cap",
+    "vnlbhip_get_stats",
+)
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("sigma", ctypes.c_float),
+        ("patch_size", ctypes.c_int),
+        ("patch_time", ctypes.c_int),
+        ("search_window", ctypes.c_int),
+        ("search_bwd", ctypes.c_int),
+        ("search_fwd", ctypes.c_int),
+        ("similar", ctypes.c_int),
+        ("rank", ctypes.c_int),
+        ("similar_cap_factor", ctypes.c_float),
+        ("model_cap_factor", ctypes.c_float),
+        ("beta", ctypes.c_float),
+        ("tau", ctypes.c_float),
+        ("variance_threshold", ctypes.c_float),
+        ("sigma_basic", ctypes.c_float),
+        ("gamma", ctypes.c_float),
+        ("weight_alpha", ctypes.c_float),
+        ("weight_beta", ctypes.c_float),
+        ("weight_gamma", ctypes.c_float),
+        ("weight_epsilon", ctypes.c_float),
+        ("membership_noise_floor", ctypes.c_float),
+        ("proc_step", ctypes.c_int),
+        ("flat_areas", ctypes.c_int),
+        ("couple_channels", ctypes.c_int),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("anchors", ctypes.c_int64),
+        ("groups", ctypes.c_int64),
+        ("unconverged_groups", ctypes.c_int64),
+        ("batches", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+def to_struct(p: StageParameters) -> Params:
+    s = Params()
+    for name, _ in Params._fields_:
+        value = getattr(p, name)
+        setattr(s, name, int(value) if isinstance(value, bool) else value)
+    return s
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIBRARY_PATH.exists():
+        raise RuntimeError(
+            f"vnlbhip: {LIBRARY_PATH} is missing; build it with "
+            "`python vapoursynth-vnlb_amd/build.py` (there is no CPU fallback)"
+        )
+    lib = ctypes.CDLL(str(LIBRARY_PATH))
+    P = ctypes.POINTER
+    vp = ctypes.c_void_p
+    fp = P(ctypes.c_float)
+    ip = P(ctypes.c_int)
+    lib.vnlbhip_default_params.argtypes = [ctypes.c_int, P(Params)]
+    lib.vnlbhip_default_params.restype = ctypes.c_int
+    lib.vnlbhip_validate.argtypes = [ctypes.c_int, P(Params), ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
+    lib.vnlbhip_validate.restype = ctypes.c_int
+    lib.vnlbhip_create.argtypes = [ctypes.c_int, P(vp)]
+    lib.vnlbhip_create.restype = ctypes.c_int
+    lib.vnlbhip_destroy.argtypes = [vp]
+    lib.vnlbhip_destroy.restype = None
+    lib.vnlbhip_last_error.argtypes = [vp]
+    lib.vnlbhip_last_error.restype = ctypes.c_char_p
+    lib.vnlbhip_stage.argtypes = [vp, ctypes.c_int, P(Params), vp, vp, vp, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
+    lib.vnlbhip_stage.restype = ctypes.c_int
+    lib.vnlbhip_stage_host.argtypes = [vp, ctypes.c_int, P(Params), fp, fp, fp, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.vnlbhip_stage_host.restype = ctypes.c_int
+    lib.vnlbhip_debug_frame.argtypes = [vp, ctypes.c_int, P(Params), vp, vp, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, ip, ip, ip, ip, ip, fp,
+                                        P(ctypes.c_ubyte), fp, fp, ip]
+    lib.vnlbhip_debug_frame.restype = ctypes.c_int
+    lib.vnlbhip_retained_cap.argtypes = [P(Params)]
+    lib.vnlbhip_retained_cap.restype = ctypes.c_int
+    lib.vnlbhip_get_stats.argtypes = [vp, P(Stats)]
+    lib.vnlbhip_get_stats.restype = ctypes.c_int
+    _lib = lib
+    return lib
