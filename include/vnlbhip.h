@@ -130,6 +130,29 @@ int vnlbhip_debug_frame(vnlbhip_context* context, int stage,
 int vnlbhip_retained_cap(const vnlbhip_params* params);
 int vnlbhip_get_stats(const vnlbhip_context* context, vnlbhip_stats* out);
 
+/* Per-kernel device time, measured with hipEvents recorded on the stage's
+ * stream around every launch while profiling is enabled (no counterpart in
+ * the reference, whose benchmarks time with cudaEvents,
+ * tools/benchmark_cuda_block_match.cpp:512-526).  Enabling resets the
+ * totals. */
+enum {
+    VNLBHIP_KERNEL_MATCH = 0,
+    VNLBHIP_KERNEL_PASTE = 1,
+    VNLBHIP_KERNEL_PREPARE = 2,
+    VNLBHIP_KERNEL_JACOBI = 3,
+    VNLBHIP_KERNEL_FILTER = 4,
+    VNLBHIP_KERNEL_NORMALIZE = 5,
+    VNLBHIP_KERNEL_COUNT = 6,
+};
+
+typedef struct vnlbhip_kernel_times {
+    double milliseconds[VNLBHIP_KERNEL_COUNT];
+    int64_t launches[VNLBHIP_KERNEL_COUNT];
+} vnlbhip_kernel_times;
+
+int vnlbhip_set_profiling(vnlbhip_context* context, int enable);
+int vnlbhip_get_kernel_times(const vnlbhip_context* context, vnlbhip_kernel_times* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -783,8 +783,9 @@ int select_set_rule(const StageSetup& s, int ax, int ay, int af,
 }
 
 // The reference's sequential procedure, stated literally
-// (find_similar_patches core.cpp:1245-1362, select_similar_patches
-// core.cpp:1200-1243), with libstdc++'s heap and partial_sort.
+// (find_similar_patches core.cpp:1245-1362 with the early-exit scan of
+// scan_*_spatial_distances :1033-1150, select_similar_patches
+// :1200-1243), with libstdc++'s heap and partial_sort.
 int select_faithful(const StageSetup& s, int ax, int ay, int af,
                     Workspace& ws) {
     const vo_params& p = s.p;
@@ -793,6 +794,33 @@ int select_faithful(const StageSetup& s, int ax, int ay, int af,
         ws.matches.push_back(Match{0.0F, ax, ay, af, 0});
         return 1;
     }
+    const int P = p.patch_size;
+    const bool final_stage = s.stage == VO_STAGE_FINAL;
+    const float* source = final_stage ? s.ref : s.noisy;
+    const int match_channels = final_stage ? s.g.channels : 1;
+    ws.reference_patch.resize(static_cast<std::size_t>(match_channels) * s.patch_dim);
+    for (int c = 0; c < match_channels; ++c) {
+        for (int t = 0; t < p.patch_time; ++t) {
+            for (int y = 0; y < P; ++y) {
+                for (int x = 0; x < P; ++x) {
+                    ws.reference_patch[static_cast<std::size_t>(c) * s.patch_dim +
+                                       (t * P + y) * P + x] =
+                        s.sample(source, ax + x, ay + y, af + t, c);
+                }
+            }
+        }
+    }
+    int low = 0;
+    int high = 0;
+    temporal_range(p, s.g, af, low, high);
+    schedule_frames(af, low, high, ws.scheduled);
+    int x_low = 0;
+    int x_high = 0;
+    int y_low = 0;
+    int y_high = 0;
+    spatial_range(p, ax, s.g.width, x_low, x_high);
+    spatial_range(p, ay, s.g.height, y_low, y_high);
+
     const long long window_cap = static_cast<long long>(p.search_window) *
                                  p.search_window *
                                  static_cast<long long>(ws.scheduled.size());
@@ -812,32 +840,60 @@ int select_faithful(const StageSetup& s, int ax, int ay, int af,
         soft_worst_index = static_cast<std::size_t>(worst - ws.matches.begin());
         soft_worst = worst->distance;
     };
-    for (const Match& candidate : ws.candidates) {
-        float limit = static_cast<int>(top.size()) >= p.similar
-                          ? std::max(p.tau, top.front())
-                          : std::numeric_limits<float>::infinity();
-        if (static_cast<int>(ws.matches.size()) >= output_cap) {
-            limit = std::min(limit, soft_worst);
-        }
-        if (!(candidate.distance <= limit)) {
-            continue;
-        }
-        if (static_cast<int>(ws.matches.size()) < output_cap) {
-            ws.matches.push_back(candidate);
-            if (static_cast<int>(ws.matches.size()) == output_cap) {
-                refresh();
+    int scan = 0;
+    for (int frame : ws.scheduled) {
+        for (int cy = y_low; cy <= y_high; ++cy) {
+            for (int cx = x_low; cx <= x_high; ++cx, ++scan) {
+                float limit = static_cast<int>(top.size()) >= p.similar
+                                  ? std::max(p.tau, top.front())
+                                  : std::numeric_limits<float>::infinity();
+                if (static_cast<int>(ws.matches.size()) >= output_cap) {
+                    limit = std::min(limit, soft_worst);
+                }
+                float distance = 0.0F;
+                for (int c = 0; c < match_channels && !(distance > limit); ++c) {
+                    for (int t = 0; t < p.patch_time && !(distance > limit); ++t) {
+                        for (int y = 0; y < P; ++y) {
+                            const float* reference_row =
+                                ws.reference_patch.data() +
+                                static_cast<std::size_t>(c) * s.patch_dim + (t * P + y) * P;
+                            const float* candidate_row =
+                                source +
+                                ((static_cast<std::size_t>(frame + t) * s.g.channels + c) *
+                                     s.g.height +
+                                 cy + y) *
+                                    s.g.width +
+                                cx;
+                            distance = add_row_distance(distance, reference_row,
+                                                        candidate_row, P);
+                            if (distance > limit) {
+                                break;
+                            }
+                        }
+                    }
+                }
+                if (!(distance <= limit)) {
+                    continue;
+                }
+                const Match candidate{distance, cx, cy, frame, scan};
+                if (static_cast<int>(ws.matches.size()) < output_cap) {
+                    ws.matches.push_back(candidate);
+                    if (static_cast<int>(ws.matches.size()) == output_cap) {
+                        refresh();
+                    }
+                } else if (candidate.distance < soft_worst) {
+                    ws.matches[soft_worst_index] = candidate;
+                    refresh();
+                }
+                if (static_cast<int>(top.size()) < p.similar) {
+                    top.push_back(candidate.distance);
+                    std::push_heap(top.begin(), top.end(), farther);
+                } else if (candidate.distance < top.front()) {
+                    std::pop_heap(top.begin(), top.end(), farther);
+                    top.back() = candidate.distance;
+                    std::push_heap(top.begin(), top.end(), farther);
+                }
             }
-        } else if (candidate.distance < soft_worst) {
-            ws.matches[soft_worst_index] = candidate;
-            refresh();
-        }
-        if (static_cast<int>(top.size()) < p.similar) {
-            top.push_back(candidate.distance);
-            std::push_heap(top.begin(), top.end(), farther);
-        } else if (candidate.distance < top.front()) {
-            std::pop_heap(top.begin(), top.end(), farther);
-            top.back() = candidate.distance;
-            std::push_heap(top.begin(), top.end(), farther);
         }
     }
     if (ws.matches.empty()) {
@@ -871,10 +927,10 @@ int find_matches(const StageSetup& s, int ax, int ay, int af, Workspace& ws) {
         ws.matches.assign(1, Match{0.0F, ax, ay, af, 0});
         return 1;
     }
-    enumerate_candidates(s, ax, ay, af, ws);
     if ((s.flags & VO_SELECT_FAITHFUL) != 0) {
         return select_faithful(s, ax, ay, af, ws);
     }
+    enumerate_candidates(s, ax, ay, af, ws);
     return select_set_rule(s, ax, ay, af, ws);
 }
 

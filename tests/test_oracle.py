@@ -1,0 +1,235 @@
+"""CPU oracle pinned against the reference's own tests and fixtures and
+against mathematics that fixes the answer independently.
+
+Reference anchors:
+  tests/cpp/test_linalg.cpp:74-234   mean/center/Gram/covariance and eigen KATs
+  tests/cpp/test_core.cpp:150-387    constant-video and gradient pipelines
+  tests/reference_cases/*, tests/compare_reference.py:43-136  pariasm/vnlb outputs
+"""
+import numpy as np
+import pytest
+
+import workloads
+from vnlbhip import synth
+from vnlbhip.params import StageParameters
+
+
+# --- linalg known-answer tests (test_linalg.cpp) ---------------------------
+
+def test_mean_center_gram_covariance(oracle_lib):
+    samples = np.array([[1.0, 2.0], [3.0, 4.0], [5.0, 6.0]])
+    mean = oracle_lib.linalg_mean(samples)
+    np.testing.assert_allclose(mean, [3.0, 4.0], atol=1e-12)
+    centered = samples - mean
+    np.testing.assert_allclose(centered, [[-2, -2], [0, 0], [2, 2]], atol=1e-12)
+    gram = oracle_lib.linalg_gram_centered(centered)
+    np.testing.assert_allclose(gram, [[8, 0, -8], [0, 0, 0], [-8, 0, 8]], atol=1e-12)
+    cov = oracle_lib.linalg_covariance_centered(centered)
+    np.testing.assert_allclose(cov, [[8, 8], [8, 8]], atol=1e-12)
+
+
+def _residual(matrix, values, vectors):
+    return max(np.abs(matrix @ v - lam * v).max() for lam, v in zip(values, vectors))
+
+
+def test_eigen_diagonal(oracle_lib):
+    m = np.diag([4.0, 1.0, 9.0])
+    values, vectors, converged, _ = oracle_lib.linalg_topk_eigen(m, 2)
+    np.testing.assert_allclose(values, [9.0, 4.0], atol=1e-12)
+    assert _residual(m, values, vectors) < 1e-12
+    assert converged
+
+
+def test_eigen_dense_2x2(oracle_lib):
+    m = np.array([[2.0, 1.0], [1.0, 2.0]])
+    values, vectors, converged, _ = oracle_lib.linalg_topk_eigen(m, 2)
+    np.testing.assert_allclose(values, [3.0, 1.0], atol=1e-12)
+    assert _residual(m, values, vectors) < 1e-12
+    assert abs(vectors[0] @ vectors[1]) < 1e-12
+    np.testing.assert_allclose(np.linalg.norm(vectors, axis=1), 1.0, atol=1e-12)
+    assert converged
+
+
+def test_eigen_large_diagonal(oracle_lib):
+    m = np.diag(np.arange(1.0, 65.0))
+    values, vectors, converged, _ = oracle_lib.linalg_topk_eigen(m, 3)
+    np.testing.assert_allclose(values, [64.0, 63.0, 62.0], atol=1e-12)
+    assert _residual(m, values, vectors) < 1e-12
+    assert converged
+
+
+def test_dual_basis_and_projection(oracle_lib):
+    centered = np.array([[1.0, 0.0], [-1.0, 0.0]])
+    basis = oracle_lib.linalg_dual_basis(centered, [2.0], [[1 / np.sqrt(2), -1 / np.sqrt(2)]])
+    np.testing.assert_allclose(basis, [[1.0, 0.0]], atol=1e-12)
+
+
+@pytest.mark.parametrize("n", [3, 8, 16, 33])
+def test_eigen_matches_library_routine(oracle_lib, n):
+    """Cyclic Jacobi against LAPACK's symmetric solver (numpy.linalg.eigh):
+    eigenvalues, and eigenvectors in the canonical sign (largest-magnitude
+    entry positive, linalg.cpp:204-233)."""
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal((n, n + 5))
+    m = a @ a.T / (n + 5)
+    rank = min(n, 6)
+    values, vectors, converged, sweeps = oracle_lib.linalg_topk_eigen(m, rank)
+    ref_values, ref_vectors = np.linalg.eigh(m)
+    order = np.argsort(ref_values)[::-1][:rank]
+    np.testing.assert_allclose(values, ref_values[order], rtol=1e-11, atol=1e-12)
+    for j, k in enumerate(order):
+        v = ref_vectors[:, k]
+        v = v * np.sign(v[np.argmax(np.abs(v))])
+        np.testing.assert_allclose(vectors[j], v, atol=1e-8)
+    assert converged and 1 <= sweeps <= 64
+    # float32 instantiation: the same algorithm at single precision.
+    values32, _, converged32, _ = oracle_lib.linalg_topk_eigen(m.astype(np.float32), rank, np.float32)
+    np.testing.assert_allclose(values32, ref_values[order], rtol=2e-5, atol=2e-6)
+    assert converged32
+
+
+def test_gram_is_xxt(oracle_lib):
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((9, 40))
+    np.testing.assert_allclose(oracle_lib.linalg_gram_centered(x, 1 / 9), x @ x.T / 9, rtol=1e-13)
+    np.testing.assert_allclose(oracle_lib.linalg_covariance_centered(x[:, :20], 1 / 9),
+                               x[:, :20].T @ x[:, :20] / 9, rtol=1e-13)
+
+
+# --- pipeline tests (test_core.cpp) -----------------------------------------
+
+def constant_test_parameters():
+    # tests/cpp/test_core.cpp:150-164 (tau is passed raw, not in 8-bit units)
+    return StageParameters(sigma=0.1, patch_size=2, patch_time=1, search_window=3, search_bwd=1,
+                           search_fwd=1, similar=4, rank=4, beta=2.0, tau=1000.0, proc_step=2)
+
+
+@pytest.mark.parametrize("flags", [0, 1])
+def test_constant_video(oracle_lib, flags):
+    noisy = np.full((3, 1, 5, 5), 0.25, np.float32)
+    p = constant_test_parameters()
+    basic, groups = oracle_lib.stage(0, p, noisy, flags=flags)
+    np.testing.assert_allclose(basic, 0.25, atol=1e-5)
+    final, _ = oracle_lib.stage(1, p, noisy, noisy, flags=flags)
+    np.testing.assert_allclose(final, 0.25, atol=1e-5)
+    assert 0 < groups <= 9 * 3
+
+
+def test_gradient_video_is_finite(oracle_lib):
+    t, y, x = np.mgrid[0:3, 0:5, 0:5].astype(np.float32)
+    noisy = (0.05 * x + 0.03 * y + 0.01 * t)[:, None].astype(np.float32)
+    out, _ = oracle_lib.stage(0, constant_test_parameters(), noisy)
+    assert np.isfinite(out).all()
+
+
+def test_uncovered_pixels_fall_back_to_source(oracle_lib):
+    """With K=1 and rank 0 each group reconstructs its own patch; a grid
+    step wider than the patch leaves pixels with zero weight, which take
+    the source sample (aggregate.cpp:95-163, plugin.cpp:1035-1036)."""
+    noisy, _ = synth.make_clip(40, 32, 3, 1, sigma8=10, seed=2)
+    p = StageParameters(sigma=10 / 255, patch_size=4, patch_time=1, search_window=5, similar=1,
+                        rank=0, proc_step=9)
+    out, _ = oracle_lib.stage(0, p, noisy)
+    np.testing.assert_allclose(out, noisy, atol=1e-6)
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_pariasm_fixtures(oracle_lib, case):
+    name, noisy, basic_ref, final_ref, bp, fp, tol = workloads.pariasm_cases()[case]
+    basic, _ = oracle_lib.stage(0, bp, noisy, flags=0, threads=4)
+    final, _ = oracle_lib.stage(1, fp, noisy, basic, flags=0, threads=4)
+    for got, ref in ((basic[:, 0], basic_ref), (final[:, 0], final_ref)):
+        diff = np.abs(got - ref)
+        assert diff.max() <= tol["max_abs"], name
+        assert diff.mean() <= tol["mean_abs"], name
+        assert np.sqrt((diff ** 2).mean()) <= tol["rmse"], name
+
+
+# --- selection: set rule against the literal heap / buffer procedure --------
+
+@pytest.mark.parametrize("name", ["gray_p8_step8", "yuv_pt2_tau200"])
+def test_set_rule_equals_reference_procedure(oracle_lib, name):
+    noisy, _, bp, fp = workloads.make(name)
+    noisy = noisy[:4, :, :48, :56] if name == "gray_p8_step8" else noisy[:5, :, :40, :48]
+    basic, _ = oracle_lib.stage(0, bp, noisy, flags=1, threads=4)
+    for stage, params, ref in ((0, bp, None), (1, fp, basic)):
+        a = oracle_lib.frame_debug(stage, params, noisy, ref, 1, flags=1)
+        b = oracle_lib.frame_debug(stage, params, noisy, ref, 1, flags=1 | oracle_lib.SELECT_FAITHFUL)
+        np.testing.assert_array_equal(a["counts"], b["counts"])
+        k = params.similar
+        for i, n in enumerate(a["counts"]):
+            assert set(a["positions"][i, :n]) == set(b["positions"][i, :n])
+            np.testing.assert_array_equal(np.sort(a["distances"][i, :n]), np.sort(b["distances"][i, :n]))
+            # the K nearest lead, ascending, in both
+            np.testing.assert_array_equal(a["distances"][i, :min(k, n)], b["distances"][i, :min(k, n)])
+        np.testing.assert_array_equal(a["active"], b["active"])
+    out_a, _ = oracle_lib.stage(1, fp, noisy, basic, flags=1)
+    out_b, _ = oracle_lib.stage(1, fp, noisy, basic, flags=1 | oracle_lib.SELECT_FAITHFUL)
+    np.testing.assert_allclose(out_a, out_b, atol=1e-6)
+
+
+def test_match_distances_brute_force(oracle_lib):
+    """Distances and k-NN sets against a float64 numpy brute force over the
+    temporal window and the boundary-shifted spatial window."""
+    noisy, _, bp, _ = workloads.make("gray_pt2_step4")
+    noisy = noisy[:, :, :48, :64]
+    f, c, h, w = noisy.shape
+    dbg = oracle_lib.frame_debug(0, bp, noisy, None, 2)
+    P, Pt, half = bp.patch_size, bp.patch_time, (bp.search_window - 1) // 2
+    video = noisy[:, 0].astype(np.float64)
+
+    def window(center, axis):
+        lo, hi = center - half, center + half
+        if lo < 0:
+            lo, hi = 0, min(axis - P, hi - lo)
+        if hi > axis - P:
+            lo, hi = max(0, lo - (hi - (axis - P))), axis - P
+        return lo, hi
+
+    af = 2
+    lo_t, hi_t = af - bp.search_bwd, af + bp.search_fwd
+    if lo_t < 0:
+        lo_t, hi_t = 0, hi_t - lo_t
+    if hi_t > f - Pt:
+        lo_t, hi_t = max(0, lo_t - (hi_t - (f - Pt))), f - Pt
+    rng = np.random.default_rng(0)
+    for a in rng.choice(len(dbg["counts"]), size=12, replace=False):
+        ax, ay = dbg["anchor_x"][a], dbg["anchor_y"][a]
+        ref = video[af:af + Pt, ay:ay + P, ax:ax + P]
+        x0, x1 = window(ax, w)
+        y0, y1 = window(ay, h)
+        cand = {}
+        for t in range(lo_t, hi_t + 1):
+            for y in range(y0, y1 + 1):
+                for x in range(x0, x1 + 1):
+                    cand[(t * h + y) * w + x] = ((video[t:t + Pt, y:y + P, x:x + P] - ref) ** 2).sum()
+        n = dbg["counts"][a]
+        got = dict(zip(dbg["positions"][a, :n], dbg["distances"][a, :n]))
+        for pos, dist in got.items():
+            assert abs(dist - cand[pos]) <= 1e-5 * max(1.0, cand[pos])
+        ranked = sorted(cand.values())
+        kth = ranked[bp.similar - 1]
+        expected = {p for p, v in cand.items() if v <= kth * (1 - 1e-6)}
+        assert expected <= set(got)
+        assert all(cand[p] <= kth * (1 + 1e-6) for p in got)
+
+
+# --- documented deviations stay small ----------------------------------------
+
+def test_double_accumulation_matches_reference_float_order(oracle_lib):
+    noisy, _, bp, fp = workloads.make("yuv_pt2_tau200")
+    noisy = noisy[:5, :, :40, :48]
+    b32, _ = oracle_lib.stage(0, bp, noisy, flags=0, threads=4)
+    b64, _ = oracle_lib.stage(0, bp, noisy, flags=1, threads=4)
+    assert np.abs(b32 - b64).max() <= 2e-6
+    f32, _ = oracle_lib.stage(1, fp, noisy, b32, flags=0, threads=4)
+    f64, _ = oracle_lib.stage(1, fp, noisy, b32, flags=1, threads=4)
+    assert np.abs(f32 - f64).max() <= 2e-6
+
+
+def test_fp64_model_close_to_fp32(oracle_lib):
+    noisy, clean, bp, fp = workloads.make("gray_pt2_step4")
+    noisy, clean = noisy[:, :, :48, :64], clean[:, :, :48, :64]
+    b32, _ = oracle_lib.stage(0, bp, noisy, flags=1, threads=4)
+    b64, _ = oracle_lib.stage(0, bp, noisy, flags=1 | oracle_lib.MODEL_DOUBLE, threads=4)
+    assert abs(synth.psnr(b32, clean) - synth.psnr(b64, clean)) < 0.02

@@ -348,6 +348,19 @@ struct vnlbhip_context {
     DeviceBuffer<float> host_ref;
     DeviceBuffer<float> host_out;
     size_t batch_entry_budget = static_cast<size_t>(1) << 28;  // anchors * cap
+    bool profiling = false;
+    vnlbhip_kernel_times times{};
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<hipEvent_t> event_pool;
+    ~vnlbhip_context() {
+        for (auto& p : pending) {
+            event_pool.push_back(p.second.first);
+            event_pool.push_back(p.second.second);
+        }
+        for (hipEvent_t e : event_pool) {
+            (void)hipEventDestroy(e);
+        }
+    }
 };
 
 namespace vnlbhip {
@@ -356,6 +369,45 @@ hipError_t launch_summarize(const int* flags, const int* status, int groups, int
 }
 
 namespace {
+
+hipEvent_t take_event(vnlbhip_context& ctx) {
+    if (!ctx.event_pool.empty()) {
+        hipEvent_t e = ctx.event_pool.back();
+        ctx.event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    check(hipEventCreate(&e), "hipEventCreate");
+    return e;
+}
+
+// Records events around one launch when profiling is on.
+template <class Fn>
+void timed(vnlbhip_context& ctx, int kernel, hipStream_t stream, const char* what, Fn&& launch) {
+    if (!ctx.profiling) {
+        check(launch(), what);
+        return;
+    }
+    hipEvent_t start = take_event(ctx);
+    hipEvent_t stop = take_event(ctx);
+    check(hipEventRecord(start, stream), "hipEventRecord");
+    check(launch(), what);
+    check(hipEventRecord(stop, stream), "hipEventRecord");
+    ctx.pending.push_back({kernel, {start, stop}});
+}
+
+// Folds completed launch timings into the totals (stream synchronised).
+void collect_times(vnlbhip_context& ctx) {
+    for (auto& p : ctx.pending) {
+        float ms = 0.0F;
+        check(hipEventElapsedTime(&ms, p.second.first, p.second.second), "hipEventElapsedTime");
+        ctx.times.milliseconds[p.first] += ms;
+        ctx.times.launches[p.first] += 1;
+        ctx.event_pool.push_back(p.second.first);
+        ctx.event_pool.push_back(p.second.second);
+    }
+    ctx.pending.clear();
+}
 
 void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const float* noisy,
                const float* ref, float* out, const Geometry& g, hipStream_t stream,
@@ -437,10 +489,14 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
         int* d_list = ctx.group_list.reserve(anchor_count);
         int* d_group_count = ctx.group_count.reserve(1);
 
-        check(vnlbhip::launch_match(d, source, d_anchors, anchor_count, d_counts, d_positions,
-                                    d_distances, stream), "match kernel");
-        check(vnlbhip::launch_paste(d, frame_count, d_offsets, d_ids, d_rows, d_counts,
-                                    d_positions, d_active, max_anchors, stream), "paste kernel");
+        timed(ctx, VNLBHIP_KERNEL_MATCH, stream, "match kernel", [&]() {
+            return vnlbhip::launch_match(d, source, d_anchors, anchor_count, d_counts, d_positions,
+                                         d_distances, stream);
+        });
+        timed(ctx, VNLBHIP_KERNEL_PASTE, stream, "paste kernel", [&]() {
+            return vnlbhip::launch_paste(d, frame_count, d_offsets, d_ids, d_rows, d_counts,
+                                         d_positions, d_active, max_anchors, stream);
+        });
         check(hipMemsetAsync(d_group_count, 0, sizeof(int), stream), "hipMemsetAsync");
         check(vnlbhip::launch_compact(d_active, anchor_count, d_list, d_group_count, stream),
               "compact kernel");
@@ -448,6 +504,7 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
         check(hipMemcpyAsync(&groups, d_group_count, sizeof(int), hipMemcpyDeviceToHost, stream),
               "read group count");
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        collect_times(ctx);
         ctx.stats.groups += groups;
 
         const int problems = groups * d.models;
@@ -465,16 +522,22 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
             d_weights = ctx.weights.reserve(static_cast<size_t>(anchor_count) * cap);
         }
         const float* basic = stage == VNLBHIP_STAGE_FINAL ? ref : noisy;
-        check(vnlbhip::launch_prepare(d, noisy, basic, d_anchors, d_counts, d_positions, d_list,
-                                      groups, n_stride, d_flags, d_dims, d_means, d_problems,
-                                      stream), "prepare kernel");
+        timed(ctx, VNLBHIP_KERNEL_PREPARE, stream, "prepare kernel", [&]() {
+            return vnlbhip::launch_prepare(d, noisy, basic, d_anchors, d_counts, d_positions,
+                                           d_list, groups, n_stride, d_flags, d_dims, d_means,
+                                           d_problems, stream);
+        });
         if (d.rank > 0) {
-            check(vnlbhip::launch_jacobi(d, problems, n_stride, d_flags, d_dims, d_problems,
-                                         d_eigen, d_vectors, d_status, stream), "jacobi kernel");
+            timed(ctx, VNLBHIP_KERNEL_JACOBI, stream, "jacobi kernel", [&]() {
+                return vnlbhip::launch_jacobi(d, problems, n_stride, d_flags, d_dims, d_problems,
+                                              d_eigen, d_vectors, d_status, stream);
+            });
         }
-        check(vnlbhip::launch_filter(d, noisy, basic, d_anchors, d_counts, d_positions, d_list,
-                                     groups, n_stride, d_flags, d_dims, d_means, d_eigen,
-                                     d_vectors, acc, d_weights, stream), "filter kernel");
+        timed(ctx, VNLBHIP_KERNEL_FILTER, stream, "filter kernel", [&]() {
+            return vnlbhip::launch_filter(d, noisy, basic, d_anchors, d_counts, d_positions,
+                                          d_list, groups, n_stride, d_flags, d_dims, d_means,
+                                          d_eigen, d_vectors, acc, d_weights, stream);
+        });
         check(vnlbhip::launch_summarize(d_flags, d.rank > 0 ? d_status : nullptr, groups,
                                         d.models, summary, stream), "summarize kernel");
 
@@ -560,12 +623,14 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
     }
 
     if (out != nullptr) {
-        check(vnlbhip::launch_normalize(d, acc, noisy, out, stream), "normalize kernel");
+        timed(ctx, VNLBHIP_KERNEL_NORMALIZE, stream, "normalize kernel",
+              [&]() { return vnlbhip::launch_normalize(d, acc, noisy, out, stream); });
     }
     int host_summary[4] = {0, 0, 0, 0};
     check(hipMemcpyAsync(host_summary, summary, sizeof(host_summary), hipMemcpyDeviceToHost, stream),
           "read summary");
     check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    collect_times(ctx);
     ctx.stats.unconverged_groups = host_summary[1];
     if (host_summary[0] != 0) {
         throw ModelFailure("group_size must cover the effective VNLB rank");
@@ -707,6 +772,23 @@ int vnlbhip_get_stats(const vnlbhip_context* context, vnlbhip_stats* out) {
         return VNLBHIP_ERROR_INVALID;
     }
     *out = context->stats;
+    return VNLBHIP_OK;
+}
+
+int vnlbhip_set_profiling(vnlbhip_context* context, int enable) {
+    if (context == nullptr) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    context->profiling = enable != 0;
+    context->times = vnlbhip_kernel_times{};
+    return VNLBHIP_OK;
+}
+
+int vnlbhip_get_kernel_times(const vnlbhip_context* context, vnlbhip_kernel_times* out) {
+    if (context == nullptr || out == nullptr) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    *out = context->times;
     return VNLBHIP_OK;
 }
 

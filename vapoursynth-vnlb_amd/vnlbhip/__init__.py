@@ -71,6 +71,15 @@ class Context:
         return {"anchors": s.anchors, "groups": s.groups,
                 "unconverged_groups": s.unconverged_groups, "batches": s.batches}
 
+    def set_profiling(self, enable: bool) -> None:
+        self._lib.vnlbhip_set_profiling(self._handle, 1 if enable else 0)
+
+    def kernel_times(self) -> dict:
+        t = _abi.KernelTimes()
+        self._lib.vnlbhip_get_kernel_times(self._handle, ctypes.byref(t))
+        return {name: {"ms": t.milliseconds[i], "launches": t.launches[i]}
+                for i, name in enumerate(_abi.KERNEL_NAMES)}
+
     def stage(self, stage: int, params: StageParameters, noisy, ref=None):
         """Run one stage over a whole clip and return the aggregated frames."""
         frames, channels, height, width = _canonical_shape(tuple(noisy.shape))

@@ -23,7 +23,11 @@ EXPORTED_SYMBOLS = (
     "vnlbhip_debug_frame",
     "vnlbhip_retained_cap",
     "vnlbhip_get_stats",
+    "vnlbhip_set_profiling",
+    "vnlbhip_get_kernel_times",
 )
+
+KERNEL_NAMES = ("match", "paste", "prepare", "jacobi", "filter", "normalize")
 
 
 class Params(ctypes.Structure):
@@ -61,6 +65,13 @@ class Stats(ctypes.Structure):
         ("unconverged_groups", ctypes.c_int64),
         ("batches", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
+    ]
+
+
+class KernelTimes(ctypes.Structure):
+    _fields_ = [
+        ("milliseconds", ctypes.c_double * 6),
+        ("launches", ctypes.c_int64 * 6),
     ]
 
 
@@ -115,5 +126,9 @@ def load() -> ctypes.CDLL:
     lib.vnlbhip_retained_cap.restype = ctypes.c_int
     lib.vnlbhip_get_stats.argtypes = [vp, P(Stats)]
     lib.vnlbhip_get_stats.restype = ctypes.c_int
+    lib.vnlbhip_set_profiling.argtypes = [vp, ctypes.c_int]
+    lib.vnlbhip_set_profiling.restype = ctypes.c_int
+    lib.vnlbhip_get_kernel_times.argtypes = [vp, P(KernelTimes)]
+    lib.vnlbhip_get_kernel_times.restype = ctypes.c_int
     _lib = lib
     return lib
