@@ -65,7 +65,7 @@ def cpu_baseline(workload: dict, seconds_target: float = 20.0) -> dict:
 
     clip = dict(workload["clip"])
     threads = min(16, os.cpu_count() or 1)
-    crop_w, crop_h = 320, 180
+    crop_w, crop_h = 480, 270
     clip.update(width=crop_w, height=crop_h)
     noisy, _ = synth.make_clip(**clip)
     bp = parse_stage_parameters(BASIC, workload["args"])

@@ -61,6 +61,7 @@ struct StageDesc {
     int rank;
     int slots;          // bwd + fwd + Pt
     int step;           // anchor grid step
+    uint32_t step_magic;  // floor(2^18 / step) + 1: x / step == (x * magic) >> 18 for x < 4096
     int flat_areas;
     int weights_enabled;
     int windowed;       // weight_gamma != 0

@@ -7,6 +7,7 @@
 
 namespace vnlbhip {
 
+size_t match_shared_bytes(const StageDesc& d);
 hipError_t launch_match(const StageDesc& d, const float* source, const uint32_t* anchors,
                         int anchor_count, int* counts, uint32_t* positions, float* distances,
                         hipStream_t stream);

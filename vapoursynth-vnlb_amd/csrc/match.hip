@@ -22,46 +22,6 @@ __device__ inline uint32_t distance_key(float d) {
     return d != d ? 0xFFFFFFFFu : __float_as_uint(d);
 }
 
-// Squared distance of one row of P samples added to `distance`
-// (add_squared_row_distance_highway, distance_highway.hpp:11-58, 16 lanes).
-template <int kP>
-__device__ inline float add_row_distance(float distance, const float* ref_row,
-                                         const float* __restrict__ cand_row,
-                                         int runtime_p) {
-    const int p = kP > 0 ? kP : runtime_p;
-    if (p == kLanes) {
-        float sq[kLanes];
-#pragma unroll
-        for (int i = 0; i < kLanes; ++i) {
-            const float diff = ref_row[i] - cand_row[i];
-            sq[i] = diff * diff;
-        }
-#pragma unroll
-        for (int width = kLanes / 2; width >= 1; width >>= 1) {
-#pragma unroll
-            for (int i = 0; i < width; ++i) {
-                sq[i] = sq[i] + sq[i + width];
-            }
-        }
-        return distance + sq[0];
-    }
-    if (kP > 0) {
-#pragma unroll
-        for (int i = 0; i < kP; ++i) {
-            const float diff = ref_row[i] - cand_row[i];
-            const float square = diff * diff;
-            distance = distance + square;
-        }
-    } else {
-        for (int i = 0; i < p; ++i) {
-            const float diff = ref_row[i] - cand_row[i];
-            const float square = diff * diff;
-            distance = distance + square;
-        }
-    }
-    return distance;
-}
-
 // k-th smallest key (1-based rank) among n distances by MSB-first radix
 // select over 8-bit digits.
 __device__ uint32_t radix_select(const float* dist, int n, int rank,
@@ -114,98 +74,64 @@ __device__ uint32_t radix_select(const float* dist, int n, int rank,
     return prefix;
 }
 
+// Row distance of 4 horizontally adjacent candidates against one reference
+// row, each added to its running distance in the reference's element order.
 template <int kP>
-__global__ __launch_bounds__(kMatchThreads) void match_kernel(
-    StageDesc d, const float* __restrict__ source,
-    const uint32_t* __restrict__ anchors, int anchor_count,
-    int* __restrict__ counts, uint32_t* __restrict__ positions,
-    float* __restrict__ distances) {
-    __shared__ float s_dist[kMaxCandidates];
-    __shared__ float s_ref[kMaxDim];
-    __shared__ unsigned s_hist[256];
-    __shared__ int s_int[16];
-    __shared__ int s_scan[8];
-    __shared__ float s_sel_dist[kMaxCap];
-    __shared__ int s_sel_index[kMaxCap];
-    __shared__ int s_sel_rank[kMaxCap];
-
-    const int a = blockIdx.x;
-    if (a >= anchor_count) {
-        return;
+__device__ inline void add_row_distance4(float (&distance)[4], const float* ref_row,
+                                         const float* cand_row) {
+    float values[kP + 3];
+#pragma unroll
+    for (int i = 0; i < kP + 3; ++i) {
+        values[i] = cand_row[i];
     }
-    const uint32_t anchor = anchors[a];
-    const int ax = position_x(anchor);
-    const int ay = position_y(anchor);
-    const int af = position_frame(anchor);
-    const size_t out_base = static_cast<size_t>(a) * d.cap;
-
-    if (d.similar <= 1) {
-        if (threadIdx.x == 0) {
-            counts[a] = 1;
-            positions[out_base] = anchor;
-            distances[out_base] = 0.0F;
-        }
-        return;
+    float ref[kP];
+#pragma unroll
+    for (int i = 0; i < kP; ++i) {
+        ref[i] = ref_row[i];
     }
-
-    const bool final_stage = d.stage == kFinal;
-    const int match_channels = final_stage ? d.channels : 1;
-    const int P = kP > 0 ? kP : d.patch;
-    const int area = P * P;
-
-    int t_low = 0;
-    int t_high = 0;
-    temporal_range(d, af, t_low, t_high);
-    const int nt = t_high - t_low + 1;
-    int x_low = 0;
-    int x_high = 0;
-    int y_low = 0;
-    int y_high = 0;
-    spatial_range(d, ax, d.width, x_low, x_high);
-    spatial_range(d, ay, d.height, y_low, y_high);
-    const int nx = x_high - x_low + 1;
-    const int ny = y_high - y_low + 1;
-    const int per_frame = nx * ny;
-    const int n = nt * per_frame;
-
-    // Reference patch [c][t][y][x] (load_reference_patch_* core.cpp:1152-1198).
-    const int ref_len = match_channels * d.patch_time * area;
-    for (int i = threadIdx.x; i < ref_len; i += blockDim.x) {
-        const int c = i / (d.patch_time * area);
-        const int rem = i - c * d.patch_time * area;
-        const int t = rem / area;
-        const int yx = rem - t * area;
-        const int y = yx / P;
-        const int x = yx - y * P;
-        s_ref[i] = video_at(source, d, ax + x, ay + y, af + t, c);
-    }
-    __syncthreads();
-
-    // Distances in scan order.
-    int valid_local = 0;
-    for (int c = threadIdx.x; c < n; c += blockDim.x) {
-        const int ti = c / per_frame;
-        const int rem = c - ti * per_frame;
-        const int yy = rem / nx;
-        const int cx = x_low + (rem - yy * nx);
-        const int cy = y_low + yy;
-        const int frame = scheduled_frame(af, t_low, t_high, ti);
-        float distance = 0.0F;
-        for (int ch = 0; ch < match_channels; ++ch) {
-            for (int t = 0; t < d.patch_time; ++t) {
-                const float* plane =
-                    source + (static_cast<size_t>(frame + t) * d.channels + ch) *
-                                 d.height * d.width;
-                const float* ref_plane = s_ref + (ch * d.patch_time + t) * area;
-                for (int y = 0; y < P; ++y) {
-                    distance = add_row_distance<kP>(
-                        distance, ref_plane + y * P,
-                        plane + static_cast<size_t>(cy + y) * d.width + cx, P);
+    if (kP == kLanes) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float sq[kLanes];
+#pragma unroll
+            for (int i = 0; i < kLanes; ++i) {
+                const float diff = ref[i] - values[j + i];
+                sq[i] = diff * diff;
+            }
+#pragma unroll
+            for (int width = kLanes / 2; width >= 1; width >>= 1) {
+#pragma unroll
+                for (int i = 0; i < width; ++i) {
+                    sq[i] = sq[i] + sq[i + width];
                 }
             }
+            distance[j] = distance[j] + sq[0];
         }
-        s_dist[c] = distance;
-        valid_local += distance == distance ? 1 : 0;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+            for (int i = 0; i < kP; ++i) {
+                const float diff = ref[i] - values[j + i];
+                const float square = diff * diff;
+                distance[j] = distance[j] + square;
+            }
+        }
+    }
+}
+
+// Selection over s_dist (scan order) into the anchor's output list.
+__device__ void select_and_write(const StageDesc& d, const float* s_dist, int n, int af,
+                                 int t_low, int t_high, int x_low, int y_low, int nx,
+                                 int per_frame, int nt, unsigned* s_hist, int* s_int, int* s_scan,
+                                 float* s_sel_dist, int* s_sel_index, int* s_sel_rank, int a,
+                                 size_t out_base, int* __restrict__ counts,
+                                 uint32_t* __restrict__ positions,
+                                 float* __restrict__ distances) {
+    int valid_local = 0;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+        const float v = s_dist[c];
+        valid_local += v == v ? 1 : 0;
     }
     if (threadIdx.x == 0) {
         s_int[2] = 0;
@@ -238,10 +164,8 @@ __global__ __launch_bounds__(kMatchThreads) void match_kernel(
     const int within = s_int[3];
 
     // Buffer capacity of find_similar_patches (core.cpp:1279-1287).
-    const long long window_cap =
-        static_cast<long long>(d.search_window) * d.search_window * nt;
-    const int buffer_cap =
-        static_cast<int>(window_cap < d.soft_cap ? window_cap : d.soft_cap);
+    const long long window_cap = static_cast<long long>(d.search_window) * d.search_window * nt;
+    const int buffer_cap = static_cast<int>(window_cap < d.soft_cap ? window_cap : d.soft_cap);
 
     // Selection predicate: d < limit, or d == limit within the first
     // `need_equal` ties in scan order.
@@ -338,7 +262,159 @@ __global__ __launch_bounds__(kMatchThreads) void match_kernel(
     }
 }
 
+// Geometry of the LDS search region of one anchor.
+struct Region {
+    int rows;      // ny + P - 1
+    int cols;      // nx + P - 1
+    int frames;    // nt + Pt - 1
+    int floats;    // frames * rows * cols (+ padding)
+};
+
+__host__ __device__ inline int region_floats(int window, int patch, int temporal, int patch_time) {
+    const int side = window + patch - 1;
+    return (temporal + patch_time - 1) * side * side + 8;
+}
+
+// Distances (scan order) with the search region staged in LDS one channel
+// at a time; each thread owns 4 horizontally adjacent candidates of one
+// row so every loaded sample feeds 4 distances.  The per-candidate order of
+// additions is exactly that of the CPU scan: channel, frame offset, row,
+// column.
+template <int kP>
+__global__ __launch_bounds__(kMatchThreads) void match_kernel(
+    StageDesc d, const float* __restrict__ source, const uint32_t* __restrict__ anchors,
+    int anchor_count, int* __restrict__ counts, uint32_t* __restrict__ positions,
+    float* __restrict__ distances) {
+    extern __shared__ float s_dyn[];
+    __shared__ float s_ref[kMaxDim];
+    __shared__ unsigned s_hist[256];
+    __shared__ int s_int[16];
+    __shared__ int s_scan[8];
+    __shared__ float s_sel_dist[kMaxCap];
+    __shared__ int s_sel_index[kMaxCap];
+    __shared__ int s_sel_rank[kMaxCap];
+
+    const int a = blockIdx.x;
+    if (a >= anchor_count) {
+        return;
+    }
+    const uint32_t anchor = anchors[a];
+    const int ax = position_x(anchor);
+    const int ay = position_y(anchor);
+    const int af = position_frame(anchor);
+    const size_t out_base = static_cast<size_t>(a) * d.cap;
+
+    if (d.similar <= 1) {
+        if (threadIdx.x == 0) {
+            counts[a] = 1;
+            positions[out_base] = anchor;
+            distances[out_base] = 0.0F;
+        }
+        return;
+    }
+
+    const bool final_stage = d.stage == kFinal;
+    const int match_channels = final_stage ? d.channels : 1;
+    constexpr int P = kP;
+    constexpr int area = kP * kP;
+
+    int t_low = 0;
+    int t_high = 0;
+    temporal_range(d, af, t_low, t_high);
+    const int nt = t_high - t_low + 1;
+    int x_low = 0;
+    int x_high = 0;
+    int y_low = 0;
+    int y_high = 0;
+    spatial_range(d, ax, d.width, x_low, x_high);
+    spatial_range(d, ay, d.height, y_low, y_high);
+    const int nx = x_high - x_low + 1;
+    const int ny = y_high - y_low + 1;
+    const int per_frame = nx * ny;
+    const int n = nt * per_frame;
+    const int rcols = nx + P - 1;
+    const int rrows = ny + P - 1;
+    const int rframes = nt + d.patch_time - 1;
+    const int plane = rrows * rcols;
+    const int region_size = rframes * plane;
+
+    float* s_region = s_dyn;
+    float* s_dist = s_dyn + region_floats(d.search_window, P, d.search_bwd + d.search_fwd + 1,
+                                          d.patch_time);
+
+    // Reference patch [c][t][y][x] (load_reference_patch_* core.cpp:1152-1198).
+    const int ref_len = match_channels * d.patch_time * area;
+    for (int i = threadIdx.x; i < ref_len; i += blockDim.x) {
+        const int c = i / (d.patch_time * area);
+        const int rem = i - c * d.patch_time * area;
+        const int t = rem / area;
+        const int yx = rem - t * area;
+        const int y = yx / P;
+        const int x = yx - y * P;
+        s_ref[i] = video_at(source, d, ax + x, ay + y, af + t, c);
+    }
+    for (int i = region_size + threadIdx.x; i < region_size + 8; i += blockDim.x) {
+        s_region[i] = 0.0F;
+    }
+
+    const int groups_x = (nx + 3) >> 2;
+    const int items = nt * ny * groups_x;
+    for (int ch = 0; ch < match_channels; ++ch) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < region_size; i += blockDim.x) {
+            const int fr = i / plane;
+            const int rem = i - fr * plane;
+            const int ry = rem / rcols;
+            const int rx = rem - ry * rcols;
+            s_region[i] = video_at(source, d, x_low + rx, y_low + ry, t_low + fr, ch);
+        }
+        __syncthreads();
+        for (int item = threadIdx.x; item < items; item += blockDim.x) {
+            const int gx = item % groups_x;
+            const int rest = item / groups_x;
+            const int yy = rest % ny;
+            const int ti = rest / ny;
+            const int xx0 = gx * 4;
+            const int frame = scheduled_frame(af, t_low, t_high, ti);
+            const int base_c = ti * per_frame + yy * nx + xx0;
+            const int valid = nx - xx0 < 4 ? nx - xx0 : 4;
+            float dist[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                dist[j] = (ch == 0 || j >= valid) ? 0.0F : s_dist[base_c + j];
+            }
+            for (int t = 0; t < d.patch_time; ++t) {
+                const float* region_plane = s_region + (frame - t_low + t) * plane;
+                const float* ref_plane = s_ref + (ch * d.patch_time + t) * area;
+#pragma unroll
+                for (int y = 0; y < P; ++y) {
+                    add_row_distance4<kP>(dist, ref_plane + y * P,
+                                          region_plane + (yy + y) * rcols + xx0);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (j < valid) {
+                    s_dist[base_c + j] = dist[j];
+                }
+            }
+        }
+    }
+    __syncthreads();
+    select_and_write(d, s_dist, n, af, t_low, t_high, x_low, y_low, nx, per_frame, nt, s_hist,
+                     s_int, s_scan, s_sel_dist, s_sel_index, s_sel_rank, a, out_base, counts,
+                     positions, distances);
+}
+
 }  // namespace
+
+size_t match_shared_bytes(const StageDesc& d) {
+    const int temporal = d.search_bwd + d.search_fwd + 1;
+    const long long candidates = static_cast<long long>(d.search_window) * d.search_window * temporal;
+    return (static_cast<size_t>(region_floats(d.search_window, d.patch, temporal, d.patch_time)) +
+            static_cast<size_t>(candidates)) *
+           sizeof(float);
+}
 
 hipError_t launch_match(const StageDesc& d, const float* source,
                         const uint32_t* anchors, int anchor_count, int* counts,
@@ -349,15 +425,32 @@ hipError_t launch_match(const StageDesc& d, const float* source,
     }
     const dim3 grid(static_cast<unsigned>(anchor_count));
     const dim3 block(kMatchThreads);
-    if (d.patch == 8) {
-        hipLaunchKernelGGL(match_kernel<8>, grid, block, 0, stream, d, source,
-                           anchors, anchor_count, counts, positions, distances);
-    } else if (d.patch == 4) {
-        hipLaunchKernelGGL(match_kernel<4>, grid, block, 0, stream, d, source,
-                           anchors, anchor_count, counts, positions, distances);
-    } else {
-        hipLaunchKernelGGL(match_kernel<0>, grid, block, 0, stream, d, source,
-                           anchors, anchor_count, counts, positions, distances);
+    const size_t shared = match_shared_bytes(d);
+    switch (d.patch) {
+#define VNLBHIP_MATCH_CASE(PV)                                                                  \
+    case PV:                                                                                    \
+        hipLaunchKernelGGL(match_kernel<PV>, grid, block, shared, stream, d, source, anchors,  \
+                           anchor_count, counts, positions, distances);                         \
+        break;
+        VNLBHIP_MATCH_CASE(1)
+        VNLBHIP_MATCH_CASE(2)
+        VNLBHIP_MATCH_CASE(3)
+        VNLBHIP_MATCH_CASE(4)
+        VNLBHIP_MATCH_CASE(5)
+        VNLBHIP_MATCH_CASE(6)
+        VNLBHIP_MATCH_CASE(7)
+        VNLBHIP_MATCH_CASE(8)
+        VNLBHIP_MATCH_CASE(9)
+        VNLBHIP_MATCH_CASE(10)
+        VNLBHIP_MATCH_CASE(11)
+        VNLBHIP_MATCH_CASE(12)
+        VNLBHIP_MATCH_CASE(13)
+        VNLBHIP_MATCH_CASE(14)
+        VNLBHIP_MATCH_CASE(15)
+        VNLBHIP_MATCH_CASE(16)
+#undef VNLBHIP_MATCH_CASE
+    default:
+        return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

@@ -527,7 +527,8 @@ __global__ __launch_bounds__(kModelThreads) void filter_kernel(
     GroupRefs refs, int group_count, int problem_stride,
     const int* __restrict__ group_flags, const int* __restrict__ problem_dims,
     const float* __restrict__ means, const float* __restrict__ eigenvalues,
-    const float* __restrict__ vectors, Accumulators acc, float* __restrict__ weights_out) {
+    const float* __restrict__ vectors, Accumulators acc, float* __restrict__ weights_out,
+    unsigned group_stride) {
     extern __shared__ float s_dyn[];
     __shared__ int s_x[kMaxCap];
     __shared__ int s_y[kMaxCap];
@@ -541,10 +542,14 @@ __global__ __launch_bounds__(kModelThreads) void filter_kernel(
     __shared__ float s_red[8];
     __shared__ float s_scalar[4];
 
-    const int g = blockIdx.x;
-    if (g >= group_count) {
+    // Workgroups visit the groups with a stride coprime with their count so
+    // that concurrently running groups scatter into distant parts of the
+    // clip instead of one band of rows (which serialises the atomics).
+    if (static_cast<int>(blockIdx.x) >= group_count) {
         return;
     }
+    const int g = static_cast<int>((static_cast<unsigned long long>(blockIdx.x) * group_stride) %
+                                   static_cast<unsigned>(group_count));
     const int a = refs.group_list[g];
     const int af = position_frame(refs.anchors[a]);
     const int similar = refs.counts[a];
@@ -993,10 +998,26 @@ hipError_t launch_filter(const StageDesc& d, const float* noisy, const float* ba
     const size_t shared = filter_shared_bytes(d);
     const dim3 grid(static_cast<unsigned>(group_count));
     const dim3 block(kModelThreads);
+    // Largest stride near 0.618 * count that is coprime with the count.
+    unsigned stride = static_cast<unsigned>(static_cast<double>(group_count) * 0.6180339887) | 1u;
+    const auto gcd = [](unsigned x, unsigned y) {
+        while (y != 0) {
+            const unsigned r = x % y;
+            x = y;
+            y = r;
+        }
+        return x;
+    };
+    while (stride > 1 && gcd(stride, static_cast<unsigned>(group_count)) != 1) {
+        stride -= 2;
+    }
+    if (stride == 0) {
+        stride = 1;
+    }
 #define VNLBHIP_FILTER(KR)                                                                  \
     hipLaunchKernelGGL(filter_kernel<KR>, grid, block, shared, stream, d, noisy, basic,     \
                        refs, group_count, problem_stride, group_flags, problem_dims, means, \
-                       eigenvalues, vectors, acc, weights_out)
+                       eigenvalues, vectors, acc, weights_out, stride)
     if (chunks <= 4) {
         VNLBHIP_FILTER(4);
     } else if (chunks <= 8) {

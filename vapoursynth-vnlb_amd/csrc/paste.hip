@@ -6,8 +6,7 @@
 // an anchor whose bit is still set is processed and clears the bits of every
 // match in the anchor frame (plus its 4-neighbours away from the border).
 // One wavefront owns one anchor frame; the mask holds one bit per grid
-// anchor in LDS, and match lists stream through a double-buffered LDS stage
-// so global latency overlaps the serial walk.
+// anchor in LDS.
 #include "common.h"
 #include "device_util.h"
 
@@ -21,6 +20,10 @@ struct GridRows {
     const int* phase;   // [height] column phase (mod step)
     const int* count;   // [height] anchors in the row
 };
+
+__device__ inline int div_step(const StageDesc& d, int x) {
+    return static_cast<int>((static_cast<uint32_t>(x) * d.step_magic) >> 18);
+}
 
 // Grid index of (x, y) in the anchor frame's staggered grid, or -1.
 __device__ inline int grid_index(const StageDesc& d, const int* s_first,
@@ -44,41 +47,72 @@ __device__ inline int grid_index(const StageDesc& d, const int* s_first,
         return first + s_count[y] - 1;
     }
     const int phase = s_phase[y];
-    if (x % d.step != phase) {
+    const int q = div_step(d, x);
+    if (x - q * d.step != phase) {
         return -1;
     }
-    const int first_column = phase > 0 ? phase : d.step;
-    return first + 1 + (x - first_column) / d.step;
+    // Columns phase (or step when phase is 0), +step, ... between 0 and
+    // max_x follow the x = 0 anchor.
+    return first + q + (phase > 0 ? 1 : 0);
 }
 
-__device__ inline void clear_bit(unsigned* mask, int index) {
-    if (index >= 0) {
-        atomicAnd(&mask[index >> 5], ~(1u << (index & 31)));
+// Visits the grid indices an anchor's matches clear (the match position and
+// its 4-neighbours away from the border, apply_frame_paste_mask
+// core.cpp:980-1008).  Only matches in the anchor frame clear anything.
+template <class Fn>
+__device__ inline void for_each_clear(const StageDesc& d, const int* s_first, const int* s_phase,
+                                      const int* s_count, uint32_t p, int anchor_frame, Fn&& fn) {
+    if (position_frame(p) != anchor_frame) {
+        return;
+    }
+    const int P = d.patch;
+    const int mx = position_x(p);
+    const int my = position_y(p);
+    fn(grid_index(d, s_first, s_phase, s_count, mx, my));
+    if (my > 2 * P) {
+        fn(grid_index(d, s_first, s_phase, s_count, mx, my - 1));
+    }
+    if (my < d.height - 2 * P) {
+        fn(grid_index(d, s_first, s_phase, s_count, mx, my + 1));
+    }
+    if (mx > 2 * P) {
+        fn(grid_index(d, s_first, s_phase, s_count, mx - 1, my));
+    }
+    if (mx < d.width - 2 * P) {
+        fn(grid_index(d, s_first, s_phase, s_count, mx + 1, my));
     }
 }
 
-template <int kRounds>  // ceil(cap / 64)
+// The serial raster walk, 64 anchors at a time.  Clearing an anchor that is
+// already decided has no effect, so within a chunk only clears of later
+// anchors of the same chunk matter: each lane records, for the anchors its
+// matches clear, an in-chunk predecessor bit.  A scalar scan over the 64
+// lanes then makes the sequential decisions exactly, and the survivors'
+// clears of anchors beyond the chunk are applied to the mask before the next
+// chunk starts.
 __global__ __launch_bounds__(kPasteThreads) void paste_kernel(
-    StageDesc d, const int* __restrict__ frame_offsets,
-    const int* __restrict__ frame_ids, const int* __restrict__ row_tables,
-    const int* __restrict__ counts, const uint32_t* __restrict__ positions,
-    unsigned char* __restrict__ active) {
-    extern __shared__ unsigned s_mem[];
-    constexpr int kBatch = 32 / kRounds;
-
+    StageDesc d, const int* __restrict__ frame_offsets, const int* __restrict__ frame_ids,
+    const int* __restrict__ row_tables, const int* __restrict__ counts,
+    const uint32_t* __restrict__ positions, unsigned char* __restrict__ active) {
+    extern __shared__ unsigned long long s_mem64[];
+    unsigned* s_mem = reinterpret_cast<unsigned*>(s_mem64);
     const int slot = blockIdx.x;
     const int first_anchor = frame_offsets[slot];
     const int anchor_count = frame_offsets[slot + 1] - first_anchor;
     const int anchor_frame = frame_ids[slot];
     const int words = (anchor_count + 31) >> 5;
     const int lane = threadIdx.x;
+    const int cap = d.cap;
+    const int stride = cap + 1;  // padded: lanes read their own list conflict-free
 
-    unsigned* s_mask = s_mem;
+    // 8-byte aligned predecessor masks first, then the mask bits, row tables
+    // and the chunk's match lists.
+    unsigned long long* s_dep = reinterpret_cast<unsigned long long*>(s_mem);
+    unsigned* s_mask = s_mem + 2 * 64;
     int* s_first = reinterpret_cast<int*>(s_mask + words);
     int* s_phase = s_first + d.height;
     int* s_count = s_phase + d.height;
-    uint32_t* s_stage = reinterpret_cast<uint32_t*>(s_count + d.height);
-    int* s_stage_count = reinterpret_cast<int*>(s_stage + 2 * kBatch * 64 * kRounds);
+    uint32_t* s_lists = reinterpret_cast<uint32_t*>(s_count + d.height);
 
     const int* rows = row_tables + static_cast<size_t>(slot) * 3 * d.height;
     for (int y = lane; y < d.height; y += kPasteThreads) {
@@ -90,85 +124,64 @@ __global__ __launch_bounds__(kPasteThreads) void paste_kernel(
         const int remaining = anchor_count - w * 32;
         s_mask[w] = remaining >= 32 ? 0xFFFFFFFFu : ((1u << remaining) - 1u);
     }
+    __syncthreads();
 
-    const int cap = d.cap;
-    const int P = d.patch;
-    uint32_t staged[kBatch * kRounds];
-    int staged_count = 0;
-    const auto load_batch = [&](int batch_begin) {
-        const int batch_n = anchor_count - batch_begin < kBatch ? anchor_count - batch_begin : kBatch;
-        staged_count = lane < batch_n ? counts[first_anchor + batch_begin + lane] : 0;
-#pragma unroll
-        for (int i = 0; i < kBatch; ++i) {
-#pragma unroll
-            for (int r = 0; r < kRounds; ++r) {
-                const int entry = r * 64 + lane;
-                uint32_t value = 0;
-                if (i < batch_n && entry < cap) {
-                    value = positions[static_cast<size_t>(first_anchor + batch_begin + i) * cap +
-                                      entry];
+    for (int base = 0; base < anchor_count; base += 64) {
+        const int n = anchor_count - base < 64 ? anchor_count - base : 64;
+        const size_t list_base = static_cast<size_t>(first_anchor + base) * cap;
+        for (int i = lane; i < n * cap; i += kPasteThreads) {
+            const int j = i / cap;
+            s_lists[j * stride + (i - j * cap)] = positions[list_base + i];
+        }
+        const int a = base + lane;
+        const int similar = lane < n ? counts[first_anchor + a] : 0;
+        s_dep[lane] = 0ull;
+        __syncthreads();
+
+        // In-chunk predecessors.
+        const uint32_t* list = s_lists + lane * stride;
+        const int chunk_end = base + n;
+        for (int k = 0; k < similar; ++k) {
+            for_each_clear(d, s_first, s_phase, s_count, list[k], anchor_frame, [&](int t) {
+                if (t > a && t < chunk_end) {
+                    atomicOr(&s_dep[t - base], 1ull << lane);
                 }
-                staged[i * kRounds + r] = value;
-            }
-        }
-    };
-
-    load_batch(0);
-    int buffer = 0;
-    for (int batch_begin = 0; batch_begin < anchor_count; batch_begin += kBatch) {
-        uint32_t* stage = s_stage + buffer * kBatch * 64 * kRounds;
-        int* stage_count = s_stage_count + buffer * kBatch;
-#pragma unroll
-        for (int i = 0; i < kBatch; ++i) {
-#pragma unroll
-            for (int r = 0; r < kRounds; ++r) {
-                stage[(i * kRounds + r) * 64 + lane] = staged[i * kRounds + r];
-            }
-        }
-        if (lane < kBatch) {
-            stage_count[lane] = staged_count;
+            });
         }
         __syncthreads();
-        if (batch_begin + kBatch < anchor_count) {
-            load_batch(batch_begin + kBatch);
-        }
-        const int batch_n = anchor_count - batch_begin < kBatch ? anchor_count - batch_begin : kBatch;
-        for (int i = 0; i < batch_n; ++i) {
-            const int a = batch_begin + i;
-            const bool set = ((s_mask[a >> 5] >> (a & 31)) & 1u) != 0;
-            if (lane == 0) {
-                active[first_anchor + a] = set ? 1 : 0;
+
+        // Sequential decisions of the chunk (uniform scalar loop).
+        const bool start = lane < n && ((s_mask[a >> 5] >> (a & 31)) & 1u) != 0;
+        const unsigned long long start_bits = __ballot(start);
+        const unsigned long long my_dep = s_dep[lane];
+        const unsigned dep_lo = static_cast<unsigned>(my_dep);
+        const unsigned dep_hi = static_cast<unsigned>(my_dep >> 32);
+        unsigned long long active_bits = 0ull;
+        for (int i = 0; i < n; ++i) {
+            const unsigned long long dep_i =
+                (static_cast<unsigned long long>(
+                     static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(dep_hi), i)))
+                 << 32) |
+                static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(dep_lo), i));
+            if (((start_bits >> i) & 1ull) != 0 && (dep_i & active_bits) == 0) {
+                active_bits |= 1ull << i;
             }
-            if (set) {
-                const int similar = stage_count[i];
-#pragma unroll
-                for (int r = 0; r < kRounds; ++r) {
-                    const int entry = r * 64 + lane;
-                    if (entry < similar) {
-                        const uint32_t p = stage[(i * kRounds + r) * 64 + lane];
-                        if (position_frame(p) == anchor_frame) {
-                            const int mx = position_x(p);
-                            const int my = position_y(p);
-                            clear_bit(s_mask, grid_index(d, s_first, s_phase, s_count, mx, my));
-                            if (my > 2 * P) {
-                                clear_bit(s_mask, grid_index(d, s_first, s_phase, s_count, mx, my - 1));
-                            }
-                            if (my < d.height - 2 * P) {
-                                clear_bit(s_mask, grid_index(d, s_first, s_phase, s_count, mx, my + 1));
-                            }
-                            if (mx > 2 * P) {
-                                clear_bit(s_mask, grid_index(d, s_first, s_phase, s_count, mx - 1, my));
-                            }
-                            if (mx < d.width - 2 * P) {
-                                clear_bit(s_mask, grid_index(d, s_first, s_phase, s_count, mx + 1, my));
-                            }
-                        }
+        }
+        const bool is_active = ((active_bits >> lane) & 1ull) != 0;
+        if (lane < n) {
+            active[first_anchor + a] = is_active ? 1 : 0;
+        }
+
+        // Survivors clear anchors beyond the chunk.
+        if (is_active) {
+            for (int k = 0; k < similar; ++k) {
+                for_each_clear(d, s_first, s_phase, s_count, list[k], anchor_frame, [&](int t) {
+                    if (t >= chunk_end) {
+                        atomicAnd(&s_mask[t >> 5], ~(1u << (t & 31)));
                     }
-                }
-                __syncthreads();
+                });
             }
         }
-        buffer ^= 1;
         __syncthreads();
     }
 }
@@ -197,12 +210,9 @@ __global__ __launch_bounds__(256) void compact_kernel(
 }  // namespace
 
 size_t paste_shared_bytes(const StageDesc& d, int max_anchors) {
-    const int rounds = (d.cap + 63) / 64;
-    const int batch = 32 / (rounds <= 1 ? 1 : (rounds <= 2 ? 2 : 4));
     const size_t words = static_cast<size_t>((max_anchors + 31) / 32);
-    return words * 4 + static_cast<size_t>(3) * d.height * 4 +
-           static_cast<size_t>(2) * batch * 64 * (rounds <= 1 ? 1 : (rounds <= 2 ? 2 : 4)) * 4 +
-           static_cast<size_t>(2) * batch * 4;
+    const size_t list_words = static_cast<size_t>(64) * (d.cap + 1);
+    return 64 * 8 + words * 4 + static_cast<size_t>(3) * d.height * 4 + list_words * 4;
 }
 
 hipError_t launch_paste(const StageDesc& d, int frame_count,
@@ -214,18 +224,9 @@ hipError_t launch_paste(const StageDesc& d, int frame_count,
         return hipSuccess;
     }
     const size_t shared = paste_shared_bytes(d, max_anchors);
-    const dim3 grid(static_cast<unsigned>(frame_count));
-    const dim3 block(kPasteThreads);
-    if (d.cap <= 64) {
-        hipLaunchKernelGGL(paste_kernel<1>, grid, block, shared, stream, d, frame_offsets,
-                           frame_ids, row_tables, counts, positions, active);
-    } else if (d.cap <= 128) {
-        hipLaunchKernelGGL(paste_kernel<2>, grid, block, shared, stream, d, frame_offsets,
-                           frame_ids, row_tables, counts, positions, active);
-    } else {
-        hipLaunchKernelGGL(paste_kernel<4>, grid, block, shared, stream, d, frame_offsets,
-                           frame_ids, row_tables, counts, positions, active);
-    }
+    hipLaunchKernelGGL(paste_kernel, dim3(static_cast<unsigned>(frame_count)), dim3(kPasteThreads),
+                       shared, stream, d, frame_offsets, frame_ids, row_tables, counts, positions,
+                       active);
     return hipGetLastError();
 }
 

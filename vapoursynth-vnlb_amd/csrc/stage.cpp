@@ -183,6 +183,8 @@ StageDesc make_desc(int stage, const vnlbhip_params& p, const Geometry& g) {
     d.rank = p.rank;
     d.slots = p.search_bwd + p.search_fwd + p.patch_time;
     d.step = p.proc_step > 0 ? p.proc_step : (p.patch_size <= 1 ? 1 : p.patch_size);
+    d.step_magic = (1u << 18) / static_cast<uint32_t>(d.step) + 1u;
+    require(d.step <= 64, "block_step above 64 is not supported");
     d.flat_areas = p.flat_areas != 0 ? 1 : 0;
     d.weights_enabled =
         (p.weight_alpha != 0.0F || p.weight_beta != 0.0F || p.weight_gamma != 0.0F) ? 1 : 0;
@@ -226,6 +228,8 @@ StageDesc make_desc(int stage, const vnlbhip_params& p, const Geometry& g) {
             "model rows times model dimension exceed the prepare kernel's LDS budget");
     require(static_cast<long long>(d.rank) * d.dim * 4 + 8LL * d.dim <= 96 * 1024,
             "rank times patch dimension exceed the filter kernel's LDS budget");
+    require(vnlbhip::match_shared_bytes(d) <= 96 * 1024,
+            "search window too large for the matcher's LDS budget");
     aggregation_window(p, d.window);
     return d;
 }
