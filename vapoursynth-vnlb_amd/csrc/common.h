@@ -69,6 +69,7 @@ struct StageDesc {
     int max_rows;       // max model rows over groups (<= model_cap, <= cap)
     int jacobi_dim;     // n of the eigen problems (dual: model rows, else model_dim)
     int max_sweeps;     // Jacobi sweep limit (64 in the reference)
+    int row_chunk;      // retained rows staged in LDS at a time (>= max_rows)
     float tau;
     float sigma;
     float sigma_basic;

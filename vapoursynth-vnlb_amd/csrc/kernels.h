@@ -13,10 +13,13 @@ hipError_t launch_match(const StageDesc& d, const float* source, const uint32_t*
                         hipStream_t stream);
 
 size_t paste_shared_bytes(const StageDesc& d, int max_anchors);
+int paste_target_stride(const StageDesc& d);
+// dep ([anchor_count] 64-bit) must be zero on entry.
 hipError_t launch_paste(const StageDesc& d, int frame_count, const int* frame_offsets,
                         const int* frame_ids, const int* row_tables, const int* counts,
                         const uint32_t* positions, unsigned char* active, int max_anchors,
-                        hipStream_t stream);
+                        int anchor_count, unsigned long long* dep, int* target_counts,
+                        int* targets, hipStream_t stream);
 hipError_t launch_compact(const unsigned char* active, int anchor_count, int* group_list,
                           int* group_count, hipStream_t stream);
 

@@ -29,16 +29,23 @@ struct GroupRefs {
     const int* group_list;
 };
 
-// Value of element e of a sample row (layout [c][t][y][x]) of a patch.
-__device__ inline float row_value(const float* __restrict__ video, const StageDesc& d,
-                                  int px, int py, int pf, int e) {
-    const int c = e / d.patch_dim;
-    const int w = e - c * d.patch_dim;
-    const int t = w / d.patch_area;
-    const int yx = w - t * d.patch_area;
-    const int y = yx / d.patch;
-    const int x = yx - y * d.patch;
-    return video_at(video, d, px + x, py + y, pf + t, c);
+// Per-element offset of a sample row element (layout [c][t][y][x]) from the
+// patch origin in a [F][C][H][W] video, built once per workgroup so that an
+// element read is base + offset.
+__device__ inline void build_input_offsets(const StageDesc& d, int* s_off) {
+    for (int e = threadIdx.x; e < d.dim; e += blockDim.x) {
+        const int c = e / d.patch_dim;
+        const int w = e - c * d.patch_dim;
+        const int t = w / d.patch_area;
+        const int yx = w - t * d.patch_area;
+        const int y = yx / d.patch;
+        const int x = yx - y * d.patch;
+        s_off[e] = ((t * d.channels + c) * d.height + y) * d.width + x;
+    }
+}
+
+__device__ inline size_t patch_base(const StageDesc& d, int x, int y, int frame) {
+    return (static_cast<size_t>(frame) * d.channels * d.height + y) * d.width + x;
 }
 
 __device__ inline float log_via_double(float x) {
@@ -63,17 +70,31 @@ __device__ inline bool group_uses_dual(const StageDesc& d, int rows) {
 // prepare
 // ---------------------------------------------------------------------------
 
+// Loads rows [first, first + n) of a group's samples into s_rows
+// ([n][dim]) with consecutive threads on consecutive elements.
+__device__ inline void load_rows(const StageDesc& d, const float* __restrict__ video,
+                                 const size_t* s_base, const int* s_off, int first, int n,
+                                 float* s_rows) {
+    const int total = n * d.dim;
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+        const int r = i / d.dim;
+        const int e = i - r * d.dim;
+        s_rows[i] = video[s_base[first + r] + s_off[e]];
+    }
+}
+
+// Flags (equal / flat), model means and the Gram (dual PCA) or covariance
+// matrix of one group.  The retained rows stream through LDS in chunks of
+// d.row_chunk rows; each element's sums run over the rows in order.
 __global__ __launch_bounds__(kModelThreads) void prepare_kernel(
     StageDesc d, const float* __restrict__ noisy, const float* __restrict__ basic,
     GroupRefs refs, int group_count, int problem_stride,
     int* __restrict__ group_flags, int* __restrict__ problem_dims,
     float* __restrict__ means, float* __restrict__ problems) {
-    extern __shared__ float s_model[];  // [rows][model_dim]
-    __shared__ int s_x[kMaxCap];
-    __shared__ int s_y[kMaxCap];
-    __shared__ int s_f[kMaxCap];
+    extern __shared__ float s_dyn[];  // [row_chunk][dim] | row0 [dim] | offsets [dim]
+    __shared__ size_t s_base[kMaxCap];
     __shared__ int s_flag[4];
-    __shared__ double s_sum[8][kMaxChannels * 2];
+    __shared__ double s_sum[kModelThreads / 64][kMaxChannels][2];
 
     const int g = blockIdx.x;
     if (g >= group_count) {
@@ -82,101 +103,127 @@ __global__ __launch_bounds__(kModelThreads) void prepare_kernel(
     const int a = refs.group_list[g];
     const int similar = refs.counts[a];
     const size_t list = static_cast<size_t>(a) * d.cap;
+    const int dim = d.dim;
+    const int chunk = d.row_chunk;
+    float* s_rows = s_dyn;
+    float* s_row0 = s_rows + static_cast<size_t>(chunk) * dim;
+    int* s_off = reinterpret_cast<int*>(s_row0 + dim);
     for (int i = threadIdx.x; i < similar; i += blockDim.x) {
         const uint32_t p = refs.positions[list + i];
-        s_x[i] = position_x(p);
-        s_y[i] = position_y(p);
-        s_f[i] = position_frame(p);
+        s_base[i] = patch_base(d, position_x(p), position_y(p), position_frame(p));
     }
+    build_input_offsets(d, s_off);
     if (threadIdx.x == 0) {
         s_flag[0] = 1;
     }
     __syncthreads();
 
     const bool final_stage = d.stage == kFinal;
-    const int dim = d.dim;
     const int rows = model_rows_of(d, similar);
+    const bool want_flat = final_stage && d.flat_areas != 0;
+    constexpr int kPerThread = kMaxDim / kModelThreads;  // elements per thread
 
-    // samples_are_equal / group_is_equal (core.cpp:110-131, 516-531).
-    bool equal = similar > 1;
-    if (equal) {
-        bool local = true;
-        const long long total = static_cast<long long>(similar - 1) * dim;
-        for (long long i = threadIdx.x; i < total && local; i += blockDim.x) {
-            const int r = 1 + static_cast<int>(i / dim);
-            const int e = static_cast<int>(i - static_cast<long long>(r - 1) * dim);
-            const float v0 = row_value(noisy, d, s_x[0], s_y[0], s_f[0], e);
-            const float vr = row_value(noisy, d, s_x[r], s_y[r], s_f[r], e);
-            local = v0 == vr;
+    // Pass over the noisy rows: equality against row 0 (core.cpp:516-531),
+    // per-channel double sums for the flat test (core.cpp:533-559) and the
+    // noisy mean over the first `rows` rows (compute_mean linalg.cpp:290-311).
+    float mean_acc[kPerThread];
+    double sums[kMaxChannels][2];
+#pragma unroll
+    for (int k = 0; k < kPerThread; ++k) {
+        mean_acc[k] = 0.0F;
+    }
+#pragma unroll
+    for (int c = 0; c < kMaxChannels; ++c) {
+        sums[c][0] = 0.0;
+        sums[c][1] = 0.0;
+    }
+    bool equal_local = true;
+    for (int first = 0; first < similar; first += chunk) {
+        const int n = similar - first < chunk ? similar - first : chunk;
+        load_rows(d, noisy, s_base, s_off, first, n, s_rows);
+        __syncthreads();
+        if (first == 0) {
+            for (int e = threadIdx.x; e < dim; e += blockDim.x) {
+                s_row0[e] = s_rows[e];
+            }
+            __syncthreads();
         }
-        if (!local) {
-            s_flag[0] = 0;
+#pragma unroll
+        for (int k = 0; k < kPerThread; ++k) {
+            const int e = threadIdx.x + k * kModelThreads;
+            if (e < dim) {
+                const float v0 = s_row0[e];
+                const int c = e / d.patch_dim;
+                for (int r = 0; r < n; ++r) {
+                    const float v = s_rows[r * dim + e];
+                    equal_local = equal_local && (v == v0);
+                    if (first + r < rows) {
+                        mean_acc[k] = mean_acc[k] + v;
+                    }
+                    if (want_flat) {
+#pragma unroll
+                        for (int cc = 0; cc < kMaxChannels; ++cc) {
+                            if (cc == c) {
+                                sums[cc][0] += static_cast<double>(v);
+                                sums[cc][1] += static_cast<double>(v) * static_cast<double>(v);
+                            }
+                        }
+                    }
+                }
+            }
         }
         __syncthreads();
-        equal = s_flag[0] != 0;
     }
-
-    int flags = equal ? kFlagEqual : 0;
-    if (!equal && final_stage && d.flat_areas != 0) {
-        // Per-channel unbiased variance in double (core.cpp:491-559); the
-        // double partial sums are combined in a tree instead of sequentially.
-        const int per_channel = similar * d.patch_dim;
-        const int lane = threadIdx.x & 63;
-        const int wave = threadIdx.x >> 6;
-        float variance = 0.0F;
-        for (int c = 0; c < d.channels; ++c) {
-            double sum = 0.0;
-            double sum2 = 0.0;
-            for (int i = threadIdx.x; i < per_channel; i += blockDim.x) {
-                const int r = i / d.patch_dim;
-                const int w = i - r * d.patch_dim;
-                const float v = row_value(noisy, d, s_x[r], s_y[r], s_f[r], c * d.patch_dim + w);
-                sum += static_cast<double>(v);
-                sum2 += static_cast<double>(v) * static_cast<double>(v);
-            }
-            sum = wave_sum_double(sum);
-            sum2 = wave_sum_double(sum2);
+    if (!equal_local) {
+        s_flag[0] = 0;
+    }
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    if (want_flat) {
+#pragma unroll
+        for (int c = 0; c < kMaxChannels; ++c) {
+            const double s0 = wave_sum_double(sums[c][0]);
+            const double s1 = wave_sum_double(sums[c][1]);
             if (lane == 0) {
-                s_sum[wave][0] = sum;
-                s_sum[wave][1] = sum2;
+                s_sum[wave][c][0] = s0;
+                s_sum[wave][c][1] = s1;
             }
-            __syncthreads();
-            double total = 0.0;
-            double total2 = 0.0;
-            for (int w = 0; w < (kModelThreads >> 6); ++w) {
-                total += s_sum[w][0];
-                total2 += s_sum[w][1];
-            }
-            __syncthreads();
-            if (per_channel > 1) {
-                const double count = static_cast<double>(per_channel);
+        }
+    }
+    __syncthreads();
+    const bool equal = similar > 1 && s_flag[0] != 0;
+    int flags = equal ? kFlagEqual : 0;
+    if (!equal && want_flat) {
+        const int per_channel = similar * d.patch_dim;
+        if (per_channel > 1) {
+            float variance = 0.0F;
+            const double count = static_cast<double>(per_channel);
+            for (int c = 0; c < d.channels; ++c) {
+                double total = 0.0;
+                double total2 = 0.0;
+                for (int w = 0; w < kModelThreads / 64; ++w) {
+                    total += s_sum[w][c][0];
+                    total2 += s_sum[w][c][1];
+                }
                 variance = variance +
                            static_cast<float>((total2 - (total * total / count)) / (count - 1.0));
             }
-        }
-        variance = variance / static_cast<float>(d.channels);
-        const float threshold = (d.sigma * d.sigma) * d.gamma;
-        if (per_channel > 1 && variance < threshold) {
-            flags |= kFlagFlat;
+            variance = variance / static_cast<float>(d.channels);
+            if (variance < (d.sigma * d.sigma) * d.gamma) {
+                flags |= kFlagFlat;
+            }
         }
     }
 
-    // compute_mean (linalg.cpp:290-311) over the first `rows` samples.
     float* mean_noisy = means + static_cast<size_t>(g) * 2 * dim;
     float* mean_basic = mean_noisy + dim;
     const float inv_rows = 1.0F / static_cast<float>(rows);
-    for (int e = threadIdx.x; e < dim; e += blockDim.x) {
-        float acc = 0.0F;
-        for (int r = 0; r < rows; ++r) {
-            acc = acc + row_value(noisy, d, s_x[r], s_y[r], s_f[r], e);
-        }
-        mean_noisy[e] = acc * inv_rows;
-        if (final_stage) {
-            float accb = 0.0F;
-            for (int r = 0; r < rows; ++r) {
-                accb = accb + row_value(basic, d, s_x[r], s_y[r], s_f[r], e);
-            }
-            mean_basic[e] = accb * inv_rows;
+#pragma unroll
+    for (int k = 0; k < kPerThread; ++k) {
+        const int e = threadIdx.x + k * kModelThreads;
+        if (e < dim) {
+            mean_acc[k] = mean_acc[k] * inv_rows;
+            mean_noisy[e] = mean_acc[k];
         }
     }
 
@@ -190,31 +237,61 @@ __global__ __launch_bounds__(kModelThreads) void prepare_kernel(
         group_flags[g] = flags | (modelled ? kFlagModelled : 0);
         problem_dims[g] = n;
     }
+    // The basic mean is needed by the filter even when the group is not
+    // modelled (flat centre, rank 0); the model rows are the first `rows`
+    // basic rows (Final) or noisy rows (Basic), centered.
+    if (final_stage) {
+        load_rows(d, basic, s_base, s_off, 0, rows, s_rows);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kPerThread; ++k) {
+            const int e = threadIdx.x + k * kModelThreads;
+            if (e < dim) {
+                float acc = 0.0F;
+                for (int r = 0; r < rows; ++r) {
+                    acc = acc + s_rows[r * dim + e];
+                }
+                acc = acc * inv_rows;
+                mean_basic[e] = acc;
+                for (int r = 0; r < rows; ++r) {
+                    s_rows[r * dim + e] = s_rows[r * dim + e] - acc;
+                }
+            }
+        }
+    } else {
+        if (!modelled) {
+            return;
+        }
+        if (similar > chunk) {
+            load_rows(d, noisy, s_base, s_off, 0, rows, s_rows);
+            __syncthreads();
+        }
+#pragma unroll
+        for (int k = 0; k < kPerThread; ++k) {
+            const int e = threadIdx.x + k * kModelThreads;
+            if (e < dim) {
+                for (int r = 0; r < rows; ++r) {
+                    s_rows[r * dim + e] = s_rows[r * dim + e] - mean_acc[k];
+                }
+            }
+        }
+    }
     if (!modelled) {
         return;
     }
     __syncthreads();
 
-    const float* model_video = final_stage ? basic : noisy;
-    const float* model_mean = final_stage ? mean_basic : mean_noisy;
     const int md = d.model_dim;
     const float scale = 1.0F / static_cast<float>(rows);
     for (int m = 0; m < d.models; ++m) {
         const int offset = d.coupled ? 0 : m * d.patch_dim;
-        // Centered model rows (center_rows linalg.cpp:313-324).
-        for (int i = threadIdx.x; i < rows * md; i += blockDim.x) {
-            const int r = i / md;
-            const int e = i - r * md;
-            s_model[i] = row_value(model_video, d, s_x[r], s_y[r], s_f[r], offset + e) -
-                         model_mean[offset + e];
-        }
-        __syncthreads();
+        const float* model = s_rows + offset;  // row stride dim
         float* out = problems + (static_cast<size_t>(g) * d.models + m) * problem_stride *
                                     problem_stride;
         if (dual) {
             // compute_gram_centered (linalg.cpp:326-354): scale * dot.
             const int team = threadIdx.x / kLanes;
-            const int lane = threadIdx.x % kLanes;
+            const int tl = threadIdx.x % kLanes;
             const int pairs = rows * (rows + 1) / 2;
             const int chunks = md / kLanes;
             for (int q = team; q < pairs; q += kTeams) {
@@ -223,18 +300,18 @@ __global__ __launch_bounds__(kModelThreads) void prepare_kernel(
                     ++r;
                 }
                 const int c = q - r * (r + 1) / 2;
-                const float* left = s_model + r * md;
-                const float* right = s_model + c * md;
+                const float* left = model + r * dim;
+                const float* right = model + c * dim;
                 float acc = 0.0F;
                 for (int k = 0; k < chunks; ++k) {
-                    acc = fmaf(left[k * kLanes + lane], right[k * kLanes + lane], acc);
+                    acc = fmaf(left[k * kLanes + tl], right[k * kLanes + tl], acc);
                 }
                 float total = segment16_sum(acc);
                 for (int i = chunks * kLanes; i < md; ++i) {
                     const float product = left[i] * right[i];
                     total = total + product;
                 }
-                if (lane == 0) {
+                if (tl == 0) {
                     const float value = scale * total;
                     out[r * problem_stride + c] = value;
                     out[c * problem_stride + r] = value;
@@ -256,9 +333,9 @@ __global__ __launch_bounds__(kModelThreads) void prepare_kernel(
                 const int vector_end = ((md - c0) / kLanes) * kLanes;
                 const bool fused = (c1 - c0) < vector_end;
                 float acc = 0.0F;
-                for (int s = 0; s < rows; ++s) {
-                    const float left = s_model[s * md + c0];
-                    const float right = s_model[s * md + c1];
+                for (int s2 = 0; s2 < rows; ++s2) {
+                    const float left = model[s2 * dim + c0];
+                    const float right = model[s2 * dim + c1];
                     if (fused) {
                         acc = fmaf(left, right, acc);
                     } else {
@@ -271,7 +348,6 @@ __global__ __launch_bounds__(kModelThreads) void prepare_kernel(
                 out[c1 * problem_stride + c0] = value;
             }
         }
-        __syncthreads();
     }
 }
 
@@ -484,34 +560,28 @@ struct Accumulators {
     double* data;  // [frames][channels + 1][height][width]
 };
 
-__device__ inline void scatter_value(const StageDesc& d, Accumulators acc, int px, int py,
-                                     int pf, int anchor_frame, int e, float value, float w) {
-    const int c = e / d.patch_dim;
-    const int wpos = e - c * d.patch_dim;
-    const int t = wpos / d.patch_area;
-    const int yx = wpos - t * d.patch_area;
-    const int y = yx / d.patch;
-    const int x = yx - y * d.patch;
-    const int out_frame = pf + t;
-    const int slot = out_frame - anchor_frame + d.search_bwd;
+__device__ inline void scatter_value(const StageDesc& d, Accumulators acc, size_t acc_base,
+                                     int pf, int anchor_frame, int acc_off, int meta, float win,
+                                     float value, float w) {
+    const int t = meta & 0xFF;
+    const int c = (meta >> 8) & 0xFF;
+    const int slot = pf + t - anchor_frame + d.search_bwd;
     if (slot < 0 || slot >= d.slots) {
         return;
     }
-    const float cw = w * d.window[yx];
+    const float cw = w * win;
     double addend;
-    if (x < d.fused_width) {
+    if ((meta >> 16) != 0) {
         addend = static_cast<double>(w) * static_cast<double>(value);
     } else {
         const float product = cw * value;
         addend = static_cast<double>(product);
     }
-    const size_t plane = static_cast<size_t>(d.width) * d.height;
-    const size_t pixel = static_cast<size_t>(py + y) * d.width + (px + x);
-    double* base = acc.data + static_cast<size_t>(out_frame) * (d.channels + 1) * plane;
-    unsafeAtomicAdd(base + static_cast<size_t>(c) * plane + pixel, addend);
+    double* cell = acc.data + acc_base + acc_off;
+    unsafeAtomicAdd(cell, addend);
     if (c == 0) {
-        unsafeAtomicAdd(base + static_cast<size_t>(d.channels) * plane + pixel,
-                        static_cast<double>(cw));
+        const size_t plane = static_cast<size_t>(d.width) * d.height;
+        unsafeAtomicAdd(cell + static_cast<size_t>(d.channels) * plane, static_cast<double>(cw));
     }
 }
 
@@ -521,6 +591,44 @@ struct RowSlice {
     float v[kR];
 };
 
+// Dot product of a centered LDS row slice with an LDS vector in the 16-lane
+// order (dot_contiguous_highway, kernels_highway.hpp:12-44): lane-wise fma
+// chains, halving tree, then the sequential tail.
+template <int kR>
+__device__ inline float team_dot(const RowSlice<kR>& row, const float* vec, int chunks, int lane,
+                                 const float* row_tail, const float* tail_mean, int md) {
+    float part = 0.0F;
+#pragma unroll
+    for (int k = 0; k < kR; ++k) {
+        if (k < chunks) {
+            part = fmaf(row.v[k], vec[k * kLanes + lane], part);
+        }
+    }
+    float total = segment16_sum(part);
+    for (int i = chunks * kLanes; i < md; ++i) {
+        const float x = row_tail[i] - tail_mean[i];
+        const float product = x * vec[i];
+        total = total + product;
+    }
+    return total;
+}
+
+template <int kR>
+__device__ inline void load_slice(RowSlice<kR>& row, const float* src, const float* mean,
+                                  int chunks, int lane) {
+#pragma unroll
+    for (int k = 0; k < kR; ++k) {
+        row.v[k] = 0.0F;
+        if (k < chunks) {
+            const int e = k * kLanes + lane;
+            row.v[k] = src[e] - mean[e];
+        }
+    }
+}
+
+// Dual-basis map, Wiener coefficients, aggregation weights, reconstruction
+// and the weighted scatter of one group.  Rows stream through LDS in chunks
+// of d.row_chunk.
 template <int kR>
 __global__ __launch_bounds__(kModelThreads) void filter_kernel(
     StageDesc d, const float* __restrict__ noisy, const float* __restrict__ basic,
@@ -530,8 +638,8 @@ __global__ __launch_bounds__(kModelThreads) void filter_kernel(
     const float* __restrict__ vectors, Accumulators acc, float* __restrict__ weights_out,
     unsigned group_stride) {
     extern __shared__ float s_dyn[];
-    __shared__ int s_x[kMaxCap];
-    __shared__ int s_y[kMaxCap];
+    __shared__ size_t s_base[kMaxCap];
+    __shared__ size_t s_acc_base[kMaxCap];
     __shared__ int s_f[kMaxCap];
     __shared__ float s_logw[kMaxCap];
     __shared__ float s_w[kMaxCap];
@@ -540,11 +648,11 @@ __global__ __launch_bounds__(kModelThreads) void filter_kernel(
     __shared__ float s_lhat[kMaxChannels * kMaxRank];
     __shared__ float s_sp[kTeams][kMaxRank];
     __shared__ float s_red[8];
-    __shared__ float s_scalar[4];
+    __shared__ float s_scalar[4 * kMaxChannels];
 
     // Workgroups visit the groups with a stride coprime with their count so
     // that concurrently running groups scatter into distant parts of the
-    // clip instead of one band of rows (which serialises the atomics).
+    // clip instead of one band of rows.
     if (static_cast<int>(blockIdx.x) >= group_count) {
         return;
     }
@@ -554,23 +662,46 @@ __global__ __launch_bounds__(kModelThreads) void filter_kernel(
     const int af = position_frame(refs.anchors[a]);
     const int similar = refs.counts[a];
     const size_t list = static_cast<size_t>(a) * d.cap;
-    for (int i = threadIdx.x; i < similar; i += blockDim.x) {
-        const uint32_t p = refs.positions[list + i];
-        s_x[i] = position_x(p);
-        s_y[i] = position_y(p);
-        s_f[i] = position_frame(p);
-        s_logw[i] = 0.0F;
-        s_w[i] = 1.0F;
-    }
     const int dim = d.dim;
     const int md = d.model_dim;
     const int R = d.rank;
-    float* s_basis = s_dyn;                       // [models][R][md] = R * dim
+    const int chunk = d.row_chunk;
+    float* s_rows = s_dyn;                                      // [chunk][dim]
+    float* s_basis = s_rows + static_cast<size_t>(chunk) * dim;  // [models][R][md]
     float* s_mean_n = s_basis + static_cast<size_t>(R) * dim;
     float* s_mean_b = s_mean_n + dim;
-    const float* g_mean = means + static_cast<size_t>(g) * 2 * dim;
+    int* s_off = reinterpret_cast<int*>(s_mean_b + dim);
+    int* s_acc_off = s_off + dim;
+    int* s_meta = s_acc_off + dim;
+    float* s_win = reinterpret_cast<float*>(s_meta + dim);
+
+    for (int i = threadIdx.x; i < similar; i += blockDim.x) {
+        const uint32_t p = refs.positions[list + i];
+        const int px = position_x(p);
+        const int py = position_y(p);
+        const int pf = position_frame(p);
+        s_base[i] = patch_base(d, px, py, pf);
+        s_acc_base[i] =
+            (static_cast<size_t>(pf) * (d.channels + 1) * d.height + py) * d.width + px;
+        s_f[i] = pf;
+        s_logw[i] = 0.0F;
+        s_w[i] = 1.0F;
+    }
+    build_input_offsets(d, s_off);
+    // Scatter tables: accumulator offset from the patch's accumulator base,
+    // frame offset t, channel, fused flag and window weight per element.
     const bool final_stage = d.stage == kFinal;
+    const float* g_mean = means + static_cast<size_t>(g) * 2 * dim;
     for (int e = threadIdx.x; e < dim; e += blockDim.x) {
+        const int c = e / d.patch_dim;
+        const int w = e - c * d.patch_dim;
+        const int t = w / d.patch_area;
+        const int yx = w - t * d.patch_area;
+        const int y = yx / d.patch;
+        const int x = yx - y * d.patch;
+        s_acc_off[e] = ((t * (d.channels + 1) + c) * d.height + y) * d.width + x;
+        s_meta[e] = t | (c << 8) | ((x < d.fused_width ? 1 : 0) << 16);
+        s_win[e] = d.window[yx];
         s_mean_n[e] = g_mean[e];
         s_mean_b[e] = final_stage ? g_mean[dim + e] : 0.0F;
     }
@@ -579,7 +710,6 @@ __global__ __launch_bounds__(kModelThreads) void filter_kernel(
     const bool flat = (flags & kFlagFlat) != 0;
     const bool modelled = (flags & kFlagModelled) != 0;
     const int rows = model_rows_of(d, similar);
-    const int n = problem_dims[g];
     const bool dual = group_uses_dual(d, rows);
     __syncthreads();
 
@@ -595,9 +725,20 @@ __global__ __launch_bounds__(kModelThreads) void filter_kernel(
     const float model_noise2 =
         final_stage ? (beta * d.sigma_basic) * d.sigma_basic : (beta * sigma) * sigma;
     const float estimate_noise2 = (beta * sigma) * sigma;
+    const bool weights_need_scores = !equal && d.weight_beta != 0.0F;
     int model_count = 0;
+    // Which source rows s_rows holds: 0 none, 1 member rows [0, resident),
+    // 2 noisy rows [0, resident).
+    int resident_kind = 0;
+    int resident = 0;
 
     if (!equal) {
+        if (modelled && dual) {
+            load_rows(d, member_video, s_base, s_off, 0, rows, s_rows);
+            resident_kind = 1;
+            resident = rows;
+            __syncthreads();
+        }
         for (int m = 0; m < d.models; ++m) {
             const int offset = d.coupled ? 0 : m * d.patch_dim;
             float* basis = s_basis + static_cast<size_t>(m) * R * md;
@@ -611,6 +752,7 @@ __global__ __launch_bounds__(kModelThreads) void filter_kernel(
                     const int fused_end = chunks * kLanes;
                     for (int e = threadIdx.x; e < md; e += blockDim.x) {
                         const bool fused = e < fused_end;
+                        const float mean_e = member_mean[offset + e];
                         for (int j0 = 0; j0 < R; j0 += 16) {
                             float accs[16];
 #pragma unroll
@@ -618,9 +760,7 @@ __global__ __launch_bounds__(kModelThreads) void filter_kernel(
                                 accs[jj] = 0.0F;
                             }
                             for (int s = 0; s < rows; ++s) {
-                                const float x =
-                                    row_value(member_video, d, s_x[s], s_y[s], s_f[s], offset + e) -
-                                    member_mean[offset + e];
+                                const float x = s_rows[s * dim + offset + e] - mean_e;
 #pragma unroll
                                 for (int jj = 0; jj < 16; ++jj) {
                                     const int j = j0 + jj;
@@ -698,15 +838,9 @@ __global__ __launch_bounds__(kModelThreads) void filter_kernel(
                     s_lhat[m * kMaxRank + j] = lhat > 0.0F ? lhat : 0.0F;
                 }
             }
-            __syncthreads();
-
-            // accumulate_aggregation_model (core.cpp:326-412).
-            if (d.weight_alpha == 0.0F && d.weight_beta == 0.0F) {
-                ++model_count;
-                continue;
-            }
-            const int rank_used = modelled ? R : 0;
+            // accumulate_aggregation_model (core.cpp:326-412): group terms.
             if (threadIdx.x == 0) {
+                const int rank_used = modelled ? R : 0;
                 float trace = 0.0F;
                 float volume = 0.0F;
                 const float match_sigma = final_stage ? d.sigma_basic : sigma;
@@ -723,98 +857,102 @@ __global__ __launch_bounds__(kModelThreads) void filter_kernel(
                 }
                 const float mean_term = static_cast<float>(md) / static_cast<float>(rows);
                 const float tau = (d.weight_epsilon + mean_term) + trace;
-                s_scalar[0] = d.weight_alpha == 0.0F ? 0.0F : -d.weight_alpha * log_via_double(tau);
-                s_scalar[1] = volume / static_cast<float>(md);
-                s_scalar[2] = membership_noise2;
+                s_scalar[4 * m + 0] =
+                    d.weight_alpha == 0.0F ? 0.0F : -d.weight_alpha * log_via_double(tau);
+                s_scalar[4 * m + 1] = volume / static_cast<float>(md);
+                s_scalar[4 * m + 2] = membership_noise2;
             }
-            __syncthreads();
-            const float log_group = s_scalar[0];
-            if (d.weight_beta == 0.0F) {
-                for (int r = threadIdx.x; r < similar; r += blockDim.x) {
-                    s_logw[r] = s_logw[r] + log_group;
-                }
-                ++model_count;
-                __syncthreads();
-                continue;
-            }
-            const float volume = s_scalar[1];
-            const float membership_noise2 = s_scalar[2];
-            const float score_scale = sqrtf(2.0F * static_cast<float>(md));
-            float local_min = INFINITY;
-            for (int r = team; r < similar; r += kTeams) {
-                RowSlice<kR> row;
-#pragma unroll
-                for (int k = 0; k < kR; ++k) {
-                    row.v[k] = 0.0F;
-                    if (k < chunks) {
-                        const int e = k * kLanes + lane;
-                        row.v[k] = row_value(member_video, d, s_x[r], s_y[r], s_f[r], offset + e) -
-                                   member_mean[offset + e];
-                    }
-                }
-                float part = 0.0F;
-#pragma unroll
-                for (int k = 0; k < kR; ++k) {
-                    if (k < chunks) {
-                        part = fmaf(row.v[k], row.v[k], part);
-                    }
-                }
-                float norm2 = segment16_sum(part);
-                for (int i = chunks * kLanes; i < md; ++i) {
-                    const float x = row_value(member_video, d, s_x[r], s_y[r], s_f[r], offset + i) -
-                                    member_mean[offset + i];
-                    const float product = x * x;
-                    norm2 = norm2 + product;
-                }
-                float distance = 0.0F;
-                float projected = 0.0F;
-                for (int j = 0; j < rank_used; ++j) {
-                    const float* b = basis + static_cast<size_t>(j) * md;
-                    float pj = 0.0F;
-#pragma unroll
-                    for (int k = 0; k < kR; ++k) {
-                        if (k < chunks) {
-                            pj = fmaf(row.v[k], b[k * kLanes + lane], pj);
-                        }
-                    }
-                    float projection = segment16_sum(pj);
-                    for (int i = chunks * kLanes; i < md; ++i) {
-                        const float x =
-                            row_value(member_video, d, s_x[r], s_y[r], s_f[r], offset + i) -
-                            member_mean[offset + i];
-                        const float product = x * b[i];
-                        projection = projection + product;
-                    }
-                    const float projection2 = projection * projection;
-                    distance = distance +
-                               projection2 / (s_lhat[m * kMaxRank + j] + membership_noise2);
-                    projected = projected + projection2;
-                }
-                const float residual = norm2 - projected > 0.0F ? norm2 - projected : 0.0F;
-                distance = distance + residual / membership_noise2;
-                const float score = (distance - static_cast<float>(md)) / score_scale;
-                if (lane == 0) {
-                    s_score[r] = score;
-                }
-                local_min = fminf(local_min, score);
-            }
-            local_min = wave_min(local_min);
-            if ((threadIdx.x & 63) == 0) {
-                s_red[threadIdx.x >> 6] = local_min;
-            }
-            __syncthreads();
-            float min_score = s_red[0];
-            for (int w = 1; w < (kModelThreads >> 6); ++w) {
-                min_score = fminf(min_score, s_red[w]);
-            }
+        }
+        __syncthreads();
+
+        // Membership scores over all rows, all models (the min over rows is
+        // needed before any weight is final).
+        if (d.weight_alpha == 0.0F && d.weight_beta == 0.0F) {
+            model_count = d.models;
+        } else if (!weights_need_scores) {
             for (int r = threadIdx.x; r < similar; r += blockDim.x) {
-                const float excess = s_score[r] - min_score;
-                const float membership = (excess > 0.0F ? excess : 0.0F) + volume;
-                const float delta = log_group - (d.weight_beta * membership);
-                s_logw[r] = s_logw[r] + delta;
+                float lw = s_logw[r];
+                for (int m = 0; m < d.models; ++m) {
+                    lw = lw + s_scalar[4 * m + 0];
+                }
+                s_logw[r] = lw;
             }
-            ++model_count;
-            __syncthreads();
+            model_count = d.models;
+        } else {
+            for (int m = 0; m < d.models; ++m) {
+                const int offset = d.coupled ? 0 : m * d.patch_dim;
+                const float* basis = s_basis + static_cast<size_t>(m) * R * md;
+                const int rank_used = modelled ? R : 0;
+                const float volume = s_scalar[4 * m + 1];
+                const float membership_noise2 = s_scalar[4 * m + 2];
+                const float score_scale = sqrtf(2.0F * static_cast<float>(md));
+                float local_min = INFINITY;
+                for (int first = 0; first < similar; first += chunk) {
+                    const int n = similar - first < chunk ? similar - first : chunk;
+                    if (!(resident_kind == 1 && first == 0 && n <= resident)) {
+                        __syncthreads();
+                        load_rows(d, member_video, s_base, s_off, first, n, s_rows);
+                        resident_kind = first == 0 ? 1 : 0;
+                        resident = first == 0 ? n : 0;
+                        __syncthreads();
+                    }
+                    for (int rr = team; rr < n; rr += kTeams) {
+                        const int r = first + rr;
+                        const float* src = s_rows + rr * dim + offset;
+                        const float* mean = member_mean + offset;
+                        RowSlice<kR> row;
+                        load_slice(row, src, mean, chunks, lane);
+                        float part = 0.0F;
+#pragma unroll
+                        for (int k = 0; k < kR; ++k) {
+                            if (k < chunks) {
+                                part = fmaf(row.v[k], row.v[k], part);
+                            }
+                        }
+                        float norm2 = segment16_sum(part);
+                        for (int i = chunks * kLanes; i < md; ++i) {
+                            const float x = src[i] - mean[i];
+                            const float product = x * x;
+                            norm2 = norm2 + product;
+                        }
+                        float distance = 0.0F;
+                        float projected = 0.0F;
+                        for (int j = 0; j < rank_used; ++j) {
+                            const float projection = team_dot(row, basis + static_cast<size_t>(j) * md,
+                                                              chunks, lane, src, mean, md);
+                            const float projection2 = projection * projection;
+                            distance = distance +
+                                       projection2 / (s_lhat[m * kMaxRank + j] + membership_noise2);
+                            projected = projected + projection2;
+                        }
+                        const float residual = norm2 - projected > 0.0F ? norm2 - projected : 0.0F;
+                        distance = distance + residual / membership_noise2;
+                        const float score = (distance - static_cast<float>(md)) / score_scale;
+                        if (lane == 0) {
+                            s_score[r] = score;
+                        }
+                        local_min = fminf(local_min, score);
+                    }
+                }
+                local_min = wave_min(local_min);
+                if ((threadIdx.x & 63) == 0) {
+                    s_red[threadIdx.x >> 6] = local_min;
+                }
+                __syncthreads();
+                float min_score = s_red[0];
+                for (int w = 1; w < (kModelThreads >> 6); ++w) {
+                    min_score = fminf(min_score, s_red[w]);
+                }
+                const float log_group = s_scalar[4 * m + 0];
+                for (int r = threadIdx.x; r < similar; r += blockDim.x) {
+                    const float excess = s_score[r] - min_score;
+                    const float membership = (excess > 0.0F ? excess : 0.0F) + volume;
+                    const float delta = log_group - (d.weight_beta * membership);
+                    s_logw[r] = s_logw[r] + delta;
+                }
+                __syncthreads();
+            }
+            model_count = d.models;
         }
     }
 
@@ -847,83 +985,78 @@ __global__ __launch_bounds__(kModelThreads) void filter_kernel(
 
     // Reconstruction (project_bayes_estimate core.cpp:578-614) and the
     // weighted scatter of every sample (write_*_contributions).
-    for (int r = team; r < similar; r += kTeams) {
-        const float w = s_w[r];
-        for (int m = 0; m < d.models; ++m) {
-            const int offset = d.coupled ? 0 : m * d.patch_dim;
-            if (equal) {
+    const bool noisy_resident_ok = !final_stage;  // Basic's member rows are the noisy rows
+    for (int first = 0; first < similar; first += chunk) {
+        const int n = similar - first < chunk ? similar - first : chunk;
+        const bool reuse = modelled && !equal && noisy_resident_ok && resident_kind == 1 &&
+                           first == 0 && n <= resident;
+        if (!reuse && (equal || modelled)) {
+            __syncthreads();
+            load_rows(d, noisy, s_base, s_off, first, n, s_rows);
+            __syncthreads();
+        }
+        for (int rr = team; rr < n; rr += kTeams) {
+            const int r = first + rr;
+            const float w = s_w[r];
+            const float* row_n = s_rows + rr * dim;
+            for (int m = 0; m < d.models; ++m) {
+                const int offset = d.coupled ? 0 : m * d.patch_dim;
+                if (equal) {
+                    for (int e = lane; e < md; e += kLanes) {
+                        scatter_value(d, acc, s_acc_base[r], s_f[r], af, s_acc_off[offset + e],
+                                      s_meta[offset + e], s_win[offset + e], row_n[offset + e], w);
+                    }
+                    continue;
+                }
+                if (!modelled) {
+                    for (int e = lane; e < md; e += kLanes) {
+                        scatter_value(d, acc, s_acc_base[r], s_f[r], af, s_acc_off[offset + e],
+                                      s_meta[offset + e], s_win[offset + e], center[offset + e], w);
+                    }
+                    continue;
+                }
+                const float* basis = s_basis + static_cast<size_t>(m) * R * md;
+                const float* src = row_n + offset;
+                const float* cmean = center + offset;
+                RowSlice<kR> row;
+                load_slice(row, src, cmean, chunks, lane);
+                for (int j = 0; j < R; ++j) {
+                    const float coefficient = s_coef[m * kMaxRank + j];
+                    float scaled = 0.0F;
+                    if (coefficient != 0.0F) {
+                        const float projection = team_dot(row, basis + static_cast<size_t>(j) * md,
+                                                          chunks, lane, src, cmean, md);
+                        scaled = coefficient * projection;
+                    }
+                    if (lane == 0) {
+                        s_sp[team][j] = scaled;
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
                 for (int e = lane; e < md; e += kLanes) {
-                    const float value = row_value(noisy, d, s_x[r], s_y[r], s_f[r], offset + e);
-                    scatter_value(d, acc, s_x[r], s_y[r], s_f[r], af, offset + e, value, w);
-                }
-                continue;
-            }
-            if (!modelled) {
-                for (int e = lane; e < md; e += kLanes) {
-                    scatter_value(d, acc, s_x[r], s_y[r], s_f[r], af, offset + e,
-                                  center[offset + e], w);
-                }
-                continue;
-            }
-            const float* basis = s_basis + static_cast<size_t>(m) * R * md;
-            RowSlice<kR> row;
-#pragma unroll
-            for (int k = 0; k < kR; ++k) {
-                row.v[k] = 0.0F;
-                if (k < chunks) {
-                    const int e = k * kLanes + lane;
-                    row.v[k] = row_value(noisy, d, s_x[r], s_y[r], s_f[r], offset + e) -
-                               center[offset + e];
-                }
-            }
-            for (int j = 0; j < R; ++j) {
-                const float coefficient = s_coef[m * kMaxRank + j];
-                float scaled = 0.0F;
-                if (coefficient != 0.0F) {
-                    const float* b = basis + static_cast<size_t>(j) * md;
-                    float pj = 0.0F;
-#pragma unroll
-                    for (int k = 0; k < kR; ++k) {
-                        if (k < chunks) {
-                            pj = fmaf(row.v[k], b[k * kLanes + lane], pj);
+                    const bool fused = e < chunks * kLanes;
+                    float out = 0.0F;
+                    for (int j = 0; j < R; ++j) {
+                        if (s_coef[m * kMaxRank + j] == 0.0F) {
+                            continue;
+                        }
+                        const float sp = s_sp[team][j];
+                        const float b = basis[static_cast<size_t>(j) * md + e];
+                        if (fused) {
+                            out = fmaf(sp, b, out);
+                        } else {
+                            const float product = sp * b;
+                            out = out + product;
                         }
                     }
-                    float projection = segment16_sum(pj);
-                    for (int i = chunks * kLanes; i < md; ++i) {
-                        const float x = row_value(noisy, d, s_x[r], s_y[r], s_f[r], offset + i) -
-                                        center[offset + i];
-                        const float product = x * b[i];
-                        projection = projection + product;
-                    }
-                    scaled = coefficient * projection;
+                    out = out + cmean[e];
+                    scatter_value(d, acc, s_acc_base[r], s_f[r], af, s_acc_off[offset + e],
+                                  s_meta[offset + e], s_win[offset + e], out, w);
                 }
-                if (lane == 0) {
-                    s_sp[team][j] = scaled;
-                }
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             }
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            for (int e = lane; e < md; e += kLanes) {
-                const bool fused = e < chunks * kLanes;
-                float out = 0.0F;
-                for (int j = 0; j < R; ++j) {
-                    if (s_coef[m * kMaxRank + j] == 0.0F) {
-                        continue;
-                    }
-                    const float sp = s_sp[team][j];
-                    const float b = basis[static_cast<size_t>(j) * md + e];
-                    if (fused) {
-                        out = fmaf(sp, b, out);
-                    } else {
-                        const float product = sp * b;
-                        out = out + product;
-                    }
-                }
-                out = out + center[offset + e];
-                scatter_value(d, acc, s_x[r], s_y[r], s_f[r], af, offset + e, out, w);
-            }
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         }
     }
 }
@@ -931,11 +1064,14 @@ __global__ __launch_bounds__(kModelThreads) void filter_kernel(
 }  // namespace
 
 size_t prepare_shared_bytes(const StageDesc& d) {
-    return static_cast<size_t>(d.max_rows) * d.model_dim * sizeof(float);
+    return (static_cast<size_t>(d.row_chunk) * d.dim + 2 * static_cast<size_t>(d.dim)) *
+           sizeof(float);
 }
 
 size_t filter_shared_bytes(const StageDesc& d) {
-    return (static_cast<size_t>(d.rank) * d.dim + 2 * static_cast<size_t>(d.dim)) * sizeof(float);
+    return (static_cast<size_t>(d.row_chunk) * d.dim + static_cast<size_t>(d.rank) * d.dim +
+            6 * static_cast<size_t>(d.dim)) *
+           sizeof(float);
 }
 
 hipError_t launch_prepare(const StageDesc& d, const float* noisy, const float* basic,

@@ -83,77 +83,107 @@ __device__ inline void for_each_clear(const StageDesc& d, const int* s_first, co
     }
 }
 
-// The serial raster walk, 64 anchors at a time.  Clearing an anchor that is
-// already decided has no effect, so within a chunk only clears of later
-// anchors of the same chunk matter: each lane records, for the anchors its
-// matches clear, an in-chunk predecessor bit.  A scalar scan over the 64
-// lanes then makes the sequential decisions exactly, and the survivors'
-// clears of anchors beyond the chunk are applied to the mask before the next
-// chunk starts.
-__global__ __launch_bounds__(kPasteThreads) void paste_kernel(
-    StageDesc d, const int* __restrict__ frame_offsets, const int* __restrict__ frame_ids,
-    const int* __restrict__ row_tables, const int* __restrict__ counts,
-    const uint32_t* __restrict__ positions, unsigned char* __restrict__ active) {
-    extern __shared__ unsigned long long s_mem64[];
-    unsigned* s_mem = reinterpret_cast<unsigned*>(s_mem64);
-    const int slot = blockIdx.x;
-    const int first_anchor = frame_offsets[slot];
-    const int anchor_count = frame_offsets[slot + 1] - first_anchor;
-    const int anchor_frame = frame_ids[slot];
-    const int words = (anchor_count + 31) >> 5;
-    const int lane = threadIdx.x;
-    const int cap = d.cap;
-    const int stride = cap + 1;  // padded: lanes read their own list conflict-free
-
-    // 8-byte aligned predecessor masks first, then the mask bits, row tables
-    // and the chunk's match lists.
-    unsigned long long* s_dep = reinterpret_cast<unsigned long long*>(s_mem);
-    unsigned* s_mask = s_mem + 2 * 64;
-    int* s_first = reinterpret_cast<int*>(s_mask + words);
-    int* s_phase = s_first + d.height;
-    int* s_count = s_phase + d.height;
-    uint32_t* s_lists = reinterpret_cast<uint32_t*>(s_count + d.height);
-
-    const int* rows = row_tables + static_cast<size_t>(slot) * 3 * d.height;
-    for (int y = lane; y < d.height; y += kPasteThreads) {
-        s_first[y] = rows[y];
-        s_phase[y] = rows[d.height + y];
-        s_count[y] = rows[2 * d.height + y];
+// Clear targets of every anchor, computed in parallel (they do not depend
+// on the mask): a clear of a later anchor in the same 64-anchor chunk sets a
+// predecessor bit in that anchor's mask word; clears beyond the chunk are
+// listed.  Clears of earlier anchors cannot change any decision and are
+// dropped.
+__global__ __launch_bounds__(256) void paste_targets_kernel(
+    StageDesc d, int frame_count, const int* __restrict__ frame_offsets,
+    const int* __restrict__ frame_ids, const int* __restrict__ row_tables,
+    const int* __restrict__ counts, const uint32_t* __restrict__ positions,
+    unsigned long long* __restrict__ dep, int* __restrict__ target_counts,
+    int* __restrict__ targets, int target_stride) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int total = frame_offsets[frame_count];
+    if (a >= total) {
+        return;
     }
+    int slot = 0;
+    while (slot + 1 < frame_count && frame_offsets[slot + 1] <= a) {
+        ++slot;
+    }
+    const int first = frame_offsets[slot];
+    const int count = frame_offsets[slot + 1] - first;
+    const int local = a - first;
+    const int chunk_end = min((local & ~63) + 64, count);
+    const int anchor_frame = frame_ids[slot];
+    const int* rows = row_tables + static_cast<size_t>(slot) * 3 * d.height;
+    const int* r_first = rows;
+    const int* r_phase = rows + d.height;
+    const int* r_count = rows + 2 * d.height;
+    const int similar = counts[a];
+    const uint32_t* list = positions + static_cast<size_t>(a) * d.cap;
+    int* out = targets + static_cast<size_t>(a) * target_stride;
+    int n = 0;
+    for (int k = 0; k < similar; ++k) {
+        for_each_clear(d, r_first, r_phase, r_count, list[k], anchor_frame, [&](int t) {
+            if (t <= local) {
+                return;
+            }
+            if (t < chunk_end) {
+                atomicOr(&dep[first + t], 1ull << (local & 63));
+            } else if (n < target_stride) {
+                out[n++] = t;
+            }
+        });
+    }
+    target_counts[a] = n;
+}
+
+// The serial raster walk, 64 anchors at a time: a scalar scan over the
+// chunk's predecessor masks makes the sequential decisions exactly, then the
+// survivors clear their targets beyond the chunk.  The next chunk's inputs
+// are loaded while the current one is decided.
+constexpr int kPrefetchTargets = 24;
+
+__global__ __launch_bounds__(kPasteThreads) void paste_kernel(
+    StageDesc d, const int* __restrict__ frame_offsets,
+    const unsigned long long* __restrict__ dep, const int* __restrict__ target_counts,
+    const int* __restrict__ targets, int target_stride, unsigned char* __restrict__ active) {
+    extern __shared__ unsigned s_mask[];
+    const int slot = blockIdx.x;
+    const int first = frame_offsets[slot];
+    const int count = frame_offsets[slot + 1] - first;
+    const int words = (count + 31) >> 5;
+    const int lane = threadIdx.x;
     for (int w = lane; w < words; w += kPasteThreads) {
-        const int remaining = anchor_count - w * 32;
+        const int remaining = count - w * 32;
         s_mask[w] = remaining >= 32 ? 0xFFFFFFFFu : ((1u << remaining) - 1u);
     }
     __syncthreads();
 
-    for (int base = 0; base < anchor_count; base += 64) {
-        const int n = anchor_count - base < 64 ? anchor_count - base : 64;
-        const size_t list_base = static_cast<size_t>(first_anchor + base) * cap;
-        for (int i = lane; i < n * cap; i += kPasteThreads) {
-            const int j = i / cap;
-            s_lists[j * stride + (i - j * cap)] = positions[list_base + i];
-        }
+    unsigned long long next_dep = 0ull;
+    int next_n = 0;
+    int next_t[kPrefetchTargets];
+    const auto prefetch = [&](int base) {
         const int a = base + lane;
-        const int similar = lane < n ? counts[first_anchor + a] : 0;
-        s_dep[lane] = 0ull;
-        __syncthreads();
-
-        // In-chunk predecessors.
-        const uint32_t* list = s_lists + lane * stride;
-        const int chunk_end = base + n;
-        for (int k = 0; k < similar; ++k) {
-            for_each_clear(d, s_first, s_phase, s_count, list[k], anchor_frame, [&](int t) {
-                if (t > a && t < chunk_end) {
-                    atomicOr(&s_dep[t - base], 1ull << lane);
-                }
-            });
+        const bool valid = a < count;
+        next_dep = valid ? dep[first + a] : 0ull;
+        next_n = valid ? target_counts[first + a] : 0;
+        const int* list = targets + static_cast<size_t>(first + (valid ? a : 0)) * target_stride;
+#pragma unroll
+        for (int i = 0; i < kPrefetchTargets; ++i) {
+            next_t[i] = (valid && i < target_stride) ? list[i] : -1;
         }
-        __syncthreads();
+    };
+    prefetch(0);
+    for (int base = 0; base < count; base += 64) {
+        const int n = count - base < 64 ? count - base : 64;
+        const int a = base + lane;
+        const unsigned long long my_dep = next_dep;
+        const int my_n = next_n;
+        int my_t[kPrefetchTargets];
+#pragma unroll
+        for (int i = 0; i < kPrefetchTargets; ++i) {
+            my_t[i] = next_t[i];
+        }
+        if (base + 64 < count) {
+            prefetch(base + 64);
+        }
 
-        // Sequential decisions of the chunk (uniform scalar loop).
         const bool start = lane < n && ((s_mask[a >> 5] >> (a & 31)) & 1u) != 0;
         const unsigned long long start_bits = __ballot(start);
-        const unsigned long long my_dep = s_dep[lane];
         const unsigned dep_lo = static_cast<unsigned>(my_dep);
         const unsigned dep_hi = static_cast<unsigned>(my_dep >> 32);
         unsigned long long active_bits = 0ull;
@@ -169,17 +199,20 @@ __global__ __launch_bounds__(kPasteThreads) void paste_kernel(
         }
         const bool is_active = ((active_bits >> lane) & 1ull) != 0;
         if (lane < n) {
-            active[first_anchor + a] = is_active ? 1 : 0;
+            active[first + a] = is_active ? 1 : 0;
         }
-
-        // Survivors clear anchors beyond the chunk.
         if (is_active) {
-            for (int k = 0; k < similar; ++k) {
-                for_each_clear(d, s_first, s_phase, s_count, list[k], anchor_frame, [&](int t) {
-                    if (t >= chunk_end) {
-                        atomicAnd(&s_mask[t >> 5], ~(1u << (t & 31)));
-                    }
-                });
+#pragma unroll
+            for (int i = 0; i < kPrefetchTargets; ++i) {
+                if (i < my_n) {
+                    const int t = my_t[i];
+                    atomicAnd(&s_mask[t >> 5], ~(1u << (t & 31)));
+                }
+            }
+            const int* list = targets + static_cast<size_t>(first + a) * target_stride;
+            for (int i = kPrefetchTargets; i < my_n; ++i) {
+                const int t = list[i];
+                atomicAnd(&s_mask[t >> 5], ~(1u << (t & 31)));
             }
         }
         __syncthreads();
@@ -210,23 +243,33 @@ __global__ __launch_bounds__(256) void compact_kernel(
 }  // namespace
 
 size_t paste_shared_bytes(const StageDesc& d, int max_anchors) {
-    const size_t words = static_cast<size_t>((max_anchors + 31) / 32);
-    const size_t list_words = static_cast<size_t>(64) * (d.cap + 1);
-    return 64 * 8 + words * 4 + static_cast<size_t>(3) * d.height * 4 + list_words * 4;
+    (void)d;
+    return static_cast<size_t>((max_anchors + 31) / 32) * 4;
 }
 
-hipError_t launch_paste(const StageDesc& d, int frame_count,
-                        const int* frame_offsets, const int* frame_ids,
-                        const int* row_tables, const int* counts,
-                        const uint32_t* positions, unsigned char* active,
-                        int max_anchors, hipStream_t stream) {
-    if (frame_count <= 0) {
+int paste_target_stride(const StageDesc& d) {
+    return 5 * d.cap;
+}
+
+hipError_t launch_paste(const StageDesc& d, int frame_count, const int* frame_offsets,
+                        const int* frame_ids, const int* row_tables, const int* counts,
+                        const uint32_t* positions, unsigned char* active, int max_anchors,
+                        int anchor_count, unsigned long long* dep, int* target_counts,
+                        int* targets, hipStream_t stream) {
+    if (frame_count <= 0 || anchor_count <= 0) {
         return hipSuccess;
     }
-    const size_t shared = paste_shared_bytes(d, max_anchors);
+    const int stride = paste_target_stride(d);
+    hipLaunchKernelGGL(paste_targets_kernel, dim3(static_cast<unsigned>((anchor_count + 255) / 256)),
+                       dim3(256), 0, stream, d, frame_count, frame_offsets, frame_ids, row_tables,
+                       counts, positions, dep, target_counts, targets, stride);
+    hipError_t status = hipGetLastError();
+    if (status != hipSuccess) {
+        return status;
+    }
     hipLaunchKernelGGL(paste_kernel, dim3(static_cast<unsigned>(frame_count)), dim3(kPasteThreads),
-                       shared, stream, d, frame_offsets, frame_ids, row_tables, counts, positions,
-                       active);
+                       paste_shared_bytes(d, max_anchors), stream, d, frame_offsets, dep,
+                       target_counts, targets, stride, active);
     return hipGetLastError();
 }
 

@@ -194,6 +194,7 @@ StageDesc make_desc(int stage, const vnlbhip_params& p, const Geometry& g) {
     d.use_dual = (d.model_dim >= 64 && d.max_rows < d.model_dim) ? 1 : 0;
     d.jacobi_dim = d.use_dual ? d.max_rows : d.model_dim;
     d.max_sweeps = 64;
+    d.row_chunk = std::max(d.max_rows, std::min(d.cap, 16));
     d.tau = p.tau;
     d.sigma = p.sigma;
     d.sigma_basic = p.sigma_basic;
@@ -224,10 +225,10 @@ StageDesc make_desc(int stage, const vnlbhip_params& p, const Geometry& g) {
             "search window times temporal slots above 8192 candidates is not supported");
     require(p.search_bwd + p.search_fwd + 1 <= vnlbhip::kMaxTemporal,
             "temporal search radius too large");
-    require(static_cast<long long>(d.max_rows) * d.model_dim * 4 <= 96 * 1024,
-            "model rows times model dimension exceed the prepare kernel's LDS budget");
-    require(static_cast<long long>(d.rank) * d.dim * 4 + 8LL * d.dim <= 96 * 1024,
-            "rank times patch dimension exceed the filter kernel's LDS budget");
+    require(vnlbhip::prepare_shared_bytes(d) <= 96 * 1024,
+            "model rows times patch dimension exceed the prepare kernel's LDS budget");
+    require(vnlbhip::filter_shared_bytes(d) <= 112 * 1024,
+            "rank and model rows times patch dimension exceed the filter kernel's LDS budget");
     require(vnlbhip::match_shared_bytes(d) <= 96 * 1024,
             "search window too large for the matcher's LDS budget");
     aggregation_window(p, d.window);
@@ -337,6 +338,9 @@ struct vnlbhip_context {
     DeviceBuffer<float> distances;
     DeviceBuffer<unsigned char> active;
     DeviceBuffer<int> group_list;
+    DeviceBuffer<unsigned long long> paste_dep;
+    DeviceBuffer<int> paste_target_counts;
+    DeviceBuffer<int> paste_targets;
     DeviceBuffer<int> group_count;
     DeviceBuffer<int> group_flags;
     DeviceBuffer<int> problem_dims;
@@ -497,9 +501,16 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
             return vnlbhip::launch_match(d, source, d_anchors, anchor_count, d_counts, d_positions,
                                          d_distances, stream);
         });
+        unsigned long long* d_dep = ctx.paste_dep.reserve(anchor_count);
+        int* d_tcount = ctx.paste_target_counts.reserve(anchor_count);
+        int* d_targets = ctx.paste_targets.reserve(static_cast<size_t>(anchor_count) *
+                                                   vnlbhip::paste_target_stride(d));
+        check(vnlbhip::launch_fill_zero(reinterpret_cast<double*>(d_dep), anchor_count, stream),
+              "clear paste masks");
         timed(ctx, VNLBHIP_KERNEL_PASTE, stream, "paste kernel", [&]() {
             return vnlbhip::launch_paste(d, frame_count, d_offsets, d_ids, d_rows, d_counts,
-                                         d_positions, d_active, max_anchors, stream);
+                                         d_positions, d_active, max_anchors, anchor_count, d_dep,
+                                         d_tcount, d_targets, stream);
         });
         check(hipMemsetAsync(d_group_count, 0, sizeof(int), stream), "hipMemsetAsync");
         check(vnlbhip::launch_compact(d_active, anchor_count, d_list, d_group_count, stream),
