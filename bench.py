@@ -22,7 +22,7 @@ REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO))
 sys.path.insert(0, str(REPO / "vapoursynth-vnlb_amd"))
 
-MI355X_PEAK = {"hbm_gbs": 8000.0, "fp32_tflops": 157.3}
+MI355X_PEAK = {"hbm_gbs": 8000.0, "fp32_tflops": 157.3, "bf16_tflops": 2500.0}
 
 WORKLOADS = {
     "yuv444ps_1080p_16f": dict(
@@ -52,6 +52,32 @@ def algorithmic_match(params, width, height, frames, channels, anchors, stage):
     nt = min(params.search_bwd + params.search_fwd + 1, frames - params.patch_time + 1)
     d = params.patch_size ** 2 * params.patch_time * (channels if stage == 1 else 1)
     return 2.0 * nx * ny * nt * d * anchors
+
+
+def match_roofline(form, times, bp, fp, clip, sb, sf) -> dict:
+    """Roofline of the distance kernel: algorithmic GEMM-form flops (2 N d
+    per anchor, the candidates each anchor needs, not the tile unions or the
+    three bf16 products) over its average launch time from hipEvents."""
+    flops = (algorithmic_match(bp, clip["width"], clip["height"], clip["frames"], clip["channels"], sb["anchors"], 0)
+             + algorithmic_match(fp, clip["width"], clip["height"], clip["frames"], clip["channels"], sf["anchors"], 1))
+    match = times["match"]
+    launches = max(1, match["launches"])
+    avg_s = match["ms"] / launches / 1000.0
+    achieved = flops / launches / avg_s / 1e12 if avg_s > 0 else 0.0
+    if form == "fast":
+        kernel, peak = "match_fast_kernel (bf16x3 MFMA distances)", MI355X_PEAK["bf16_tflops"]
+    else:
+        kernel, peak = "match_kernel (fp32 distances + k-NN select)", MI355X_PEAK["fp32_tflops"]
+    return {
+        "kernel": kernel,
+        "bound": "mfma",
+        "achieved": round(achieved, 3),
+        "peak": peak,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / peak, 5),
+        "traffic": None,
+        "launch_ms": round(avg_s * 1000.0, 4),
+    }
 
 
 def cpu_baseline(workload: dict, seconds_target: float = 20.0) -> dict:
@@ -92,6 +118,8 @@ def main() -> int:
     parser.add_argument("--warmup", type=int, default=1)
     parser.add_argument("--workload", default="yuv444ps_1080p_16f", choices=sorted(WORKLOADS))
     parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--form", default="fast", choices=("fast", "plain"),
+                        help="fast: MFMA candidate distances (the metric's form); plain: exact fp32 order")
     args = parser.parse_args()
 
     if args.gpus != 1:
@@ -99,6 +127,7 @@ def main() -> int:
               "(anchor frames would be sharded across GPUs; see the design notes)", file=sys.stderr)
         return 2
 
+    import numpy as np
     import torch
 
     import vnlbhip
@@ -113,6 +142,7 @@ def main() -> int:
     bp = parse_stage_parameters(BASIC, workload["args"])
     fp = parse_stage_parameters(FINAL, workload["args"])
     ctx = vnlbhip.Context(0)
+    ctx.set_form(args.form)
 
     def step():
         basic = ctx.stage(BASIC, bp, noisy)
@@ -122,10 +152,10 @@ def main() -> int:
         return basic, final, stats_b, stats_f
 
     for i in range(args.warmup):
-        log(f"[bench] warmup {i + 1}/{args.warmup}")
+        log(f"[bench] warmup {i + 1}/{args.warmup} ({args.form})")
         step()
     torch.cuda.synchronize()
-    log(f"[bench] timing {args.steps} steps")
+    log(f"[bench] timing {args.steps} steps ({args.form})")
     t0 = time.perf_counter()
     for i in range(args.steps):
         basic, final, stats_b, stats_f = step()
@@ -143,14 +173,32 @@ def main() -> int:
     ctx.set_profiling(False)
     log("[bench] kernel ms: " + ", ".join(f"{k}={v['ms']:.2f}/{v['launches']}" for k, v in times.items()))
     _, _, sb, sf = step_profile
-    flops = (algorithmic_match(bp, clip["width"], clip["height"], clip["frames"], clip["channels"], sb["anchors"], 0)
-             + algorithmic_match(fp, clip["width"], clip["height"], clip["frames"], clip["channels"], sf["anchors"], 1))
-    match = times["match"]
-    avg_s = match["ms"] / max(1, match["launches"]) / 1000.0
-    per_launch_flops = flops / max(1, match["launches"])
-    achieved = per_launch_flops / avg_s / 1e12 if avg_s > 0 else 0.0
+    roofline = match_roofline(args.form, times, bp, fp, clip, sb, sf)
 
+    # The plain form on the same clip: quality reference of the metric and
+    # its own throughput (one untimed warm step, one timed step).
+    quality = {}
     final_np = final.cpu().numpy()
+    psnr_final = synth.psnr(final_np, clean_np)
+    if args.form != "plain":
+        ctx.set_form("plain")
+        step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, final_plain, _, _ = step()
+        torch.cuda.synchronize()
+        plain_ms = (time.perf_counter() - t0) * 1000.0
+        plain_np = final_plain.cpu().numpy()
+        psnr_plain = synth.psnr(plain_np, clean_np)
+        quality = {
+            "psnr_final_plain_db": round(psnr_plain, 4),
+            "dpsnr_vs_plain_db": round(psnr_final - psnr_plain, 4),
+            "max_abs_vs_plain": float(np.abs(final_np - plain_np).max()),
+            "plain_mpix_s": round(pixels / (plain_ms / 1000.0) / 1e6, 4),
+        }
+        log(f"[bench] plain form: {plain_ms:.1f} ms/step, dPSNR {psnr_final - psnr_plain:+.4f} dB")
+        ctx.set_form(args.form)
+
     result = {
         "metric": "Mpix/s Basic+Final 1080p YUV444PS; dPSNR vs plain GPU form <=0.05 dB",
         "value": round(value, 4),
@@ -162,27 +210,20 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "bf16x3/f32" if args.form == "fast" else "f32",
         "data": "synthetic (seeded gratings + rectangles, sub-pixel motion, Gaussian noise)",
         "config": {
             "workload": args.workload,
-            "form": "plain fp32",
+            "form": args.form,
             "width": clip["width"], "height": clip["height"], "frames": clip["frames"],
             "channels": clip["channels"], **workload["args"],
             "groups_basic": sb["groups"], "groups_final": sf["groups"],
             "psnr_noisy_db": round(synth.psnr(noisy_np, clean_np), 3),
-            "psnr_final_db": round(synth.psnr(final_np, clean_np), 3),
+            "psnr_final_db": round(psnr_final, 4),
+            **quality,
             "kernel_ms": {k: round(v["ms"], 3) for k, v in times.items()},
         },
-        "roofline": {
-            "kernel": "match (distance + k-NN select)",
-            "bound": "mfma",
-            "achieved": round(achieved, 3),
-            "peak": MI355X_PEAK["fp32_tflops"],
-            "unit": "TFLOP/s",
-            "frac": round(achieved / MI355X_PEAK["fp32_tflops"], 4),
-            "traffic": None,
-        },
+        "roofline": roofline,
     }
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")

@@ -142,7 +142,9 @@ enum {
     VNLBHIP_KERNEL_JACOBI = 3,
     VNLBHIP_KERNEL_FILTER = 4,
     VNLBHIP_KERNEL_NORMALIZE = 5,
-    VNLBHIP_KERNEL_COUNT = 6,
+    VNLBHIP_KERNEL_SELECT = 6,  /* FAST form: k-nearest selection (PLAIN selects inside MATCH) */
+    VNLBHIP_KERNEL_ENERGY = 7,  /* FAST form: patch energies, once per stage */
+    VNLBHIP_KERNEL_COUNT = 8,
 };
 
 typedef struct vnlbhip_kernel_times {
@@ -152,6 +154,20 @@ typedef struct vnlbhip_kernel_times {
 
 int vnlbhip_set_profiling(vnlbhip_context* context, int enable);
 int vnlbhip_get_kernel_times(const vnlbhip_context* context, vnlbhip_kernel_times* out);
+
+/* Arithmetic form of the stages run on a context (no counterpart in the
+ * reference).  PLAIN, the default, sums every distance and model product in
+ * fp32 in the CPU path's order.  FAST takes the candidate distances from
+ * bf16x3 products on the matrix cores (8x8 patches; other shapes, and
+ * windows too wide for its tiles, keep the PLAIN matcher); the selection
+ * and the model are the PLAIN ones.  Returns VNLBHIP_ERROR_INVALID for an
+ * unknown form. */
+enum {
+    VNLBHIP_FORM_PLAIN = 0,
+    VNLBHIP_FORM_FAST = 1,
+};
+
+int vnlbhip_set_form(vnlbhip_context* context, int form);
 
 #ifdef __cplusplus
 }

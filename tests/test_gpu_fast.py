@@ -1,0 +1,111 @@
+"""MFMA ("fast") form against the plain form, through the C ABI.
+
+The fast form takes candidate distances from bf16x3 products
+(E(r) + E(c) - 2 <r, c>), so distances agree with the exact fp32 sums to a
+small relative error, the kept candidates can differ where distances are
+within that error of a threshold, and the outputs agree in quality: the
+bound is |PSNR(fast) - PSNR(plain)| <= 0.05 dB against the clean clip.
+Shapes the fast matcher does not take (patch sizes other than 8) run the
+plain matcher and must give bit-identical results.
+"""
+import numpy as np
+import pytest
+import torch
+
+import vnlbhip
+from vnlbhip import synth
+from vnlbhip.params import BASIC, FINAL, parse_stage_parameters
+
+pytestmark = pytest.mark.gpu
+
+PSNR_TOL = 0.05
+DIST_RTOL = 2e-4  # relative to the distance plus the patch energy scale
+
+
+@pytest.fixture(scope="module")
+def contexts():
+    plain = vnlbhip.Context(0)
+    fast = vnlbhip.Context(0)
+    fast.set_form("fast")
+    yield plain, fast
+    plain.close()
+    fast.close()
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).to("cuda:0")
+
+
+ARGS = dict(sigma=15, block_size=8, block_step=4, group_size=16, bm_range=9, patch_time=2,
+            radius=2, rank=16, tau=200, cap_factor=4.0, model_cap_factor=1.0, chroma=1)
+
+
+@pytest.mark.parametrize("stage", [BASIC, FINAL])
+def test_fast_distances_track_exact(contexts, stage):
+    plain, fast = contexts
+    noisy, _ = synth.make_clip(width=96, height=72, frames=6, channels=3, sigma8=15, seed=11,
+                               motion="subpixel")
+    params = parse_stage_parameters(stage, ARGS)
+    ref = None
+    if stage == FINAL:
+        ref = plain.stage(BASIC, parse_stage_parameters(BASIC, ARGS), _dev(noisy))
+    print(f"[fast] distances stage={stage}", flush=True)
+    want = plain.debug_frame(stage, params, _dev(noisy), ref, 2)
+    got = fast.debug_frame(stage, params, _dev(noisy), ref, 2)
+    np.testing.assert_array_equal(got["anchor_x"], want["anchor_x"])
+    same_sets = 0
+    checked = 0
+    for a, n in enumerate(want["counts"]):
+        m = got["counts"][a]
+        assert m >= 1
+        # The anchor itself leads at distance 0.
+        assert got["distances"][a, 0] == 0.0
+        exact = dict(zip(want["positions"][a, :n].tolist(), want["distances"][a, :n].tolist()))
+        scale = float(np.max(want["distances"][a, :n])) + 1.0
+        for pos, dist in zip(got["positions"][a, :m].tolist(), got["distances"][a, :m].tolist()):
+            if pos in exact:
+                assert abs(dist - exact[pos]) <= DIST_RTOL * scale, (a, pos, dist, exact[pos])
+                checked += 1
+        same_sets += set(exact) == set(got["positions"][a, :m].tolist())
+    assert checked > 0
+    assert same_sets >= 0.9 * len(want["counts"]), (same_sets, len(want["counts"]))
+
+
+@pytest.mark.parametrize("channels,width,height", [(3, 128, 96), (1, 112, 80), (3, 40, 36)])
+def test_fast_stage_quality_matches_plain(contexts, channels, width, height):
+    plain, fast = contexts
+    noisy, clean = synth.make_clip(width=width, height=height, frames=6, channels=channels,
+                                   sigma8=15, seed=5, motion="subpixel")
+    bp = parse_stage_parameters(BASIC, ARGS)
+    fp = parse_stage_parameters(FINAL, ARGS)
+    print(f"[fast] stages {channels}x{width}x{height}", flush=True)
+    x = _dev(noisy)
+    b_plain = plain.stage(BASIC, bp, x)
+    f_plain = plain.stage(FINAL, fp, x, b_plain)
+    b_fast = fast.stage(BASIC, bp, x)
+    f_fast = fast.stage(FINAL, fp, x, b_fast)
+    for got, want in ((b_fast, b_plain), (f_fast, f_plain)):
+        g = got.cpu().numpy()
+        w = want.cpu().numpy()
+        assert np.isfinite(g).all()
+        assert abs(synth.psnr(g, clean) - synth.psnr(w, clean)) <= PSNR_TOL
+
+
+def test_fast_form_falls_back_to_plain_matcher(contexts):
+    plain, fast = contexts
+    noisy, _ = synth.make_clip(width=64, height=48, frames=4, channels=1, sigma8=20, seed=3,
+                               motion="integer")
+    args = dict(sigma=20, block_size=5, block_step=3, group_size=8, bm_range=7, patch_time=1,
+                radius=1, rank=6)
+    bp = parse_stage_parameters(BASIC, args)
+    x = _dev(noisy)
+    a = plain.stage(BASIC, bp, x).cpu().numpy()
+    b = fast.stage(BASIC, bp, x).cpu().numpy()
+    np.testing.assert_array_equal(a, b)
+
+
+def test_form_is_validated(contexts):
+    _, fast = contexts
+    with pytest.raises(vnlbhip.VNLBError):
+        fast.set_form("fp8")
+    assert fast.form == "fast"

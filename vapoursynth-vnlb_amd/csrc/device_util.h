@@ -3,45 +3,9 @@
 #pragma once
 
 #include "common.h"
+#include "geometry.h"
 
 namespace vnlbhip {
-
-// compute_temporal_range (reference src/core/core.cpp:873-896).
-__host__ __device__ inline void temporal_range(const StageDesc& d, int anchor,
-                                               int& low, int& high) {
-    const int max_origin = d.frames - d.patch_time;
-    const int start = anchor - d.search_bwd;
-    const int end = anchor + d.search_fwd;
-    const int shift = (start < 0 ? start : 0) + (end - max_origin > 0 ? end - max_origin : 0);
-    low = start - shift > 0 ? start - shift : 0;
-    high = end - shift < max_origin ? end - shift : max_origin;
-}
-
-// compute_spatial_range (reference src/core/core.cpp:910-932).
-__host__ __device__ inline void spatial_range(const StageDesc& d, int center,
-                                              int axis, int& low, int& high) {
-    const int max_origin = axis - d.patch;
-    const int half = (d.search_window - 1) / 2;
-    const int start = center - half;
-    const int end = center + half;
-    const int shift = (start < 0 ? start : 0) + (end - max_origin > 0 ? end - max_origin : 0);
-    low = start - shift > 0 ? start - shift : 0;
-    high = end - shift < max_origin ? end - shift : max_origin;
-}
-
-// schedule_temporal_range (reference src/core/core.cpp:898-908): the anchor
-// frame, the future frames ascending, then the past frames descending.
-__host__ __device__ inline int scheduled_frame(int anchor, int low, int high,
-                                               int index) {
-    if (index == 0) {
-        return anchor;
-    }
-    const int future = high - anchor;
-    if (index <= future) {
-        return anchor + index;
-    }
-    return anchor - (index - future);
-}
 
 __device__ inline float video_at(const float* __restrict__ video,
                                  const StageDesc& d, int x, int y, int frame,

@@ -12,6 +12,24 @@ hipError_t launch_match(const StageDesc& d, const float* source, const uint32_t*
                         int anchor_count, int* counts, uint32_t* positions, float* distances,
                         hipStream_t stream);
 
+// MFMA form of the matcher (match_fast.hip).  Tiles hold 16 anchor indices
+// (-1 for empty slots) of one frame; max_cols / max_rows bound the union of
+// the tiles' search windows.
+bool fast_match_supported(const StageDesc& d, int max_cols, int max_rows);
+size_t fast_match_shared_bytes(const StageDesc& d, int max_cols, int max_rows);
+size_t fast_match_stride(const StageDesc& d);
+// offsets [4], rows [frames][match channels][H][W], energy [frames][H][W].
+hipError_t launch_fast_energy(const StageDesc& d, const float* source, float* offsets, float* rows,
+                              float* energy, hipStream_t stream);
+// Distances of anchor index i land at staged[(i - anchor_base) * stride].
+hipError_t launch_fast_match(const StageDesc& d, int max_cols, int max_rows, const float* source,
+                             const float* offsets, const float* energy, const uint32_t* anchors,
+                             const int* tiles, int tile_count, int anchor_base, float* staged,
+                             hipStream_t stream);
+hipError_t launch_fast_select(const StageDesc& d, const float* staged, const uint32_t* anchors,
+                              int anchor_count, int* counts, uint32_t* positions, float* distances,
+                              hipStream_t stream);
+
 size_t paste_shared_bytes(const StageDesc& d, int max_anchors);
 int paste_target_stride(const StageDesc& d);
 // dep ([anchor_count] 64-bit) must be zero on entry.

@@ -1,0 +1,533 @@
+// Block matching, MFMA form.
+//
+// The squared distance of a candidate patch c to an anchor patch r is
+//
+//     ||r - c||^2 = E(r) + E(c) - 2 <r, c>
+//
+// with E the patch energy (fp32, one value per patch origin) and the cross
+// term accumulated in fp32 on the matrix cores from three bf16 products
+// (x = hi + lo, <r, c> ~ hi.hi + hi.lo + lo.hi).  Samples are offset by a
+// per-channel constant first, which leaves distances unchanged and keeps the
+// energies close to the distances they are subtracted from.  The candidates
+// kept are then chosen by the same k-nearest selection as the exact form
+// (select.h), so only the distance values differ from it.
+//
+// Anchors are matched in tiles of up to 16 anchors of one frame (4 grid rows
+// x 4 anchors): the tile's candidates are the union of the anchors' windows.
+// One v_mfma_f32_16x16x32_bf16 multiplies 16 candidates of a column
+// (consecutive rows) by the 16 anchors over 4 patch rows.  The search region
+// of one (frame, channel) plane is staged in LDS as bf16 hi / lo, four
+// copies shifted by 0..3 samples so that every lane reads its 8 samples with
+// 8-byte aligned loads.  Supported for 8x8 patches.
+#include "common.h"
+#include "device_util.h"
+#include "kernels.h"
+#include "select.h"
+
+#include <hip/hip_bf16.h>
+
+namespace vnlbhip {
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
+typedef __attribute__((__vector_size__(4 * sizeof(unsigned)))) unsigned u32x4;
+
+constexpr int kFastThreads = 256;
+constexpr int kFastWaves = kFastThreads / 64;
+constexpr int kFastPatch = 8;
+constexpr int kTileAnchors = 16;
+constexpr int kCopies = 4;
+
+// Per-channel offset: the mean of a strided sample of the channel.
+__global__ __launch_bounds__(256) void channel_offset_kernel(const float* __restrict__ source,
+                                                             int channels, size_t plane,
+                                                             int frames, float* __restrict__ offsets) {
+    __shared__ double s_part[4];
+    const int c = blockIdx.x;
+    const size_t per_channel = plane * frames;
+    const size_t stride = per_channel / 65536 + 1;
+    double sum = 0.0;
+    size_t count = 0;
+    for (size_t i = threadIdx.x * stride; i < per_channel; i += blockDim.x * stride) {
+        const size_t f = i / plane;
+        const size_t rem = i - f * plane;
+        sum += static_cast<double>(source[(f * channels + c) * plane + rem]);
+        ++count;
+    }
+    sum = wave_sum_double(sum);
+    double n = wave_sum_double(static_cast<double>(count));
+    if ((threadIdx.x & 63) == 0) {
+        s_part[threadIdx.x >> 6] = sum;
+    }
+    __syncthreads();
+    __shared__ double s_count[4];
+    if ((threadIdx.x & 63) == 0) {
+        s_count[threadIdx.x >> 6] = n;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double total = 0.0;
+        double total_n = 0.0;
+        for (int w = 0; w < 4; ++w) {
+            total += s_part[w];
+            total_n += s_count[w];
+        }
+        offsets[c] = total_n > 0.0 ? static_cast<float>(total / total_n) : 0.0F;
+    }
+}
+
+// Horizontal sums of squares: rows[f][c][y][x] = sum_{px<P} (v - offset)^2.
+__global__ __launch_bounds__(256) void row_energy_kernel(StageDesc d, const float* __restrict__ source,
+                                                         const float* __restrict__ offsets,
+                                                         int match_channels, float* __restrict__ rows) {
+    const size_t plane = static_cast<size_t>(d.width) * d.height;
+    const size_t total = plane * match_channels * d.frames;
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= total) {
+        return;
+    }
+    const size_t f = i / (plane * match_channels);
+    const size_t rem = i - f * plane * match_channels;
+    const int c = static_cast<int>(rem / plane);
+    const size_t p = rem - static_cast<size_t>(c) * plane;
+    const int y = static_cast<int>(p / d.width);
+    const int x = static_cast<int>(p - static_cast<size_t>(y) * d.width);
+    float sum = 0.0F;
+    if (x <= d.width - d.patch) {
+        const float mu = offsets[c];
+        const float* src = source + ((f * d.channels + c) * d.height + y) * d.width + x;
+        for (int px = 0; px < d.patch; ++px) {
+            const float v = src[px] - mu;
+            sum = fmaf(v, v, sum);
+        }
+    }
+    rows[i] = sum;
+}
+
+// Patch energies: energy[f][y][x] = sum over channels, frames and rows of
+// the row sums, for every patch origin.
+__global__ __launch_bounds__(256) void patch_energy_kernel(StageDesc d, const float* __restrict__ rows,
+                                                           int match_channels, float* __restrict__ energy) {
+    const size_t plane = static_cast<size_t>(d.width) * d.height;
+    const size_t total = plane * d.frames;
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= total) {
+        return;
+    }
+    const int f = static_cast<int>(i / plane);
+    const size_t p = i - static_cast<size_t>(f) * plane;
+    const int y = static_cast<int>(p / d.width);
+    const int x = static_cast<int>(p - static_cast<size_t>(y) * d.width);
+    float sum = 0.0F;
+    if (x <= d.width - d.patch && y <= d.height - d.patch && f <= d.frames - d.patch_time) {
+        for (int t = 0; t < d.patch_time; ++t) {
+            for (int c = 0; c < match_channels; ++c) {
+                const float* r = rows + ((static_cast<size_t>(f + t) * match_channels + c) * d.height + y) * d.width + x;
+                for (int py = 0; py < d.patch; ++py) {
+                    sum += r[static_cast<size_t>(py) * d.width];
+                }
+            }
+        }
+    }
+    energy[i] = sum;
+}
+
+struct FastShape {
+    int row_tiles;     // 16-row candidate tiles per column
+    int region_rows;   // 16 * row_tiles + P - 1
+    int pitch;         // bf16 samples per LDS copy row (multiple of 4)
+    int copy_elems;    // region_rows * pitch
+    int stride;        // distance slots per anchor (window^2 * temporal)
+    int per_frame;     // window^2
+};
+
+__device__ inline bf16x8 load_bf16x8(const __bf16* p) {
+    // 8-byte aligned: two 64-bit LDS reads.
+    const uint2* q = reinterpret_cast<const uint2*>(p);
+    const uint2 a = q[0];
+    const uint2 b = q[1];
+    const u32x4 v = {a.x, a.y, b.x, b.y};
+    return __builtin_bit_cast(bf16x8, v);
+}
+
+// Cross terms of one anchor tile against its candidates, all candidate
+// frames; distances land in `out` ([anchor][stride], scan order).
+template <int kTilesPerWave>
+__global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
+    StageDesc d, FastShape S, const float* __restrict__ source, const float* __restrict__ offsets,
+    const float* __restrict__ energy, const uint32_t* __restrict__ anchors,
+    const int* __restrict__ tiles, int anchor_base, float* __restrict__ out) {
+    extern __shared__ uint4 s_dyn16[];
+    __bf16* s_copy = reinterpret_cast<__bf16*>(s_dyn16);  // [hi/lo][copy][rows][pitch]
+    float* s_stage = reinterpret_cast<float*>(s_dyn16);   // epilogue staging (aliases the copies)
+    __shared__ int s_index[kTileAnchors];
+    __shared__ int s_ax[kTileAnchors];
+    __shared__ int s_ay[kTileAnchors];
+    __shared__ int s_xl[kTileAnchors];
+    __shared__ int s_yl[kTileAnchors];
+    __shared__ int s_nx[kTileAnchors];
+    __shared__ int s_ny[kTileAnchors];
+    __shared__ float s_energy[kTileAnchors];
+    __shared__ int s_box[5];
+
+    const int tid = threadIdx.x;
+    // Uniform per wavefront: tile indices derived from it stay scalar.
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63;
+    const int* tile = tiles + static_cast<size_t>(blockIdx.x) * kTileAnchors;
+    const bool final_stage = d.stage == kFinal;
+    const int match_channels = final_stage ? d.channels : 1;
+
+    if (tid < kTileAnchors) {
+        const int index = tile[tid];
+        s_index[tid] = index;
+        if (index >= 0) {
+            const uint32_t p = anchors[index];
+            const int ax = position_x(p);
+            const int ay = position_y(p);
+            int xl = 0;
+            int xh = 0;
+            int yl = 0;
+            int yh = 0;
+            spatial_range(d, ax, d.width, xl, xh);
+            spatial_range(d, ay, d.height, yl, yh);
+            s_ax[tid] = ax;
+            s_ay[tid] = ay;
+            s_xl[tid] = xl;
+            s_yl[tid] = yl;
+            s_nx[tid] = xh - xl + 1;
+            s_ny[tid] = yh - yl + 1;
+            s_energy[tid] = energy[(static_cast<size_t>(position_frame(p)) * d.height + ay) * d.width + ax];
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int x0 = 1 << 30;
+        int x1 = -1;
+        int y0 = 1 << 30;
+        int y1 = -1;
+        int first = -1;
+        for (int n = 0; n < kTileAnchors; ++n) {
+            if (s_index[n] >= 0) {
+                first = first < 0 ? s_index[n] : first;
+                x0 = min(x0, s_xl[n]);
+                x1 = max(x1, s_xl[n] + s_nx[n] - 1);
+                y0 = min(y0, s_yl[n]);
+                y1 = max(y1, s_yl[n] + s_ny[n] - 1);
+            }
+        }
+        s_box[0] = x0;
+        s_box[1] = x1;
+        s_box[2] = y0;
+        s_box[3] = y1;
+        s_box[4] = first;
+    }
+    __syncthreads();
+    if (s_box[4] < 0) {
+        return;  // empty tile (the host never emits one)
+    }
+    const int af = position_frame(anchors[s_box[4]]);
+    const int ux0 = s_box[0];
+    const int uy0 = s_box[2];
+    const int ucols = s_box[1] - ux0 + 1;
+    const int urows = s_box[3] - uy0 + 1;
+    const int row_tiles = (urows + 15) / 16;  // <= S.row_tiles (host checked)
+    const int mtiles = ucols * row_tiles;     // <= kTilesPerWave * waves (host checked)
+    // col = mt / row_tiles as (mt * magic) >> 16, exact for mt < 32768.
+    const int tile_magic = (65536 + row_tiles - 1) / row_tiles;
+    int t_low = 0;
+    int t_high = 0;
+    temporal_range(d, af, t_low, t_high);
+    const int nt = t_high - t_low + 1;
+    const size_t plane = static_cast<size_t>(d.width) * d.height;
+
+    const int n_lane = lane & 15;  // anchor (B column) / candidate row (A row)
+    const int kb = lane >> 4;      // 8-sample block of the 32-wide k step
+    const int my_anchor = s_index[n_lane];
+    const int my_ax = my_anchor >= 0 ? s_ax[n_lane] : 0;
+    const int my_ay = my_anchor >= 0 ? s_ay[n_lane] : 0;
+
+    for (int ti = 0; ti < nt; ++ti) {
+        const int frame = scheduled_frame(af, t_low, t_high, ti);
+        floatx4 acc[kTilesPerWave];
+#pragma unroll
+        for (int i = 0; i < kTilesPerWave; ++i) {
+            acc[i] = floatx4{0.0F, 0.0F, 0.0F, 0.0F};
+        }
+        for (int c = 0; c < match_channels; ++c) {
+            const float mu = offsets[c];
+            for (int t = 0; t < d.patch_time; ++t) {
+                __syncthreads();
+                // Stage the plane (frame + t, c) of the union region.
+                const float* src = source + (static_cast<size_t>(frame + t) * d.channels + c) * plane;
+                const int fill_cols = S.pitch + kCopies - 1;
+                const int fill = S.region_rows * fill_cols;
+                for (int i = tid; i < fill; i += kFastThreads) {
+                    const int ry = i / fill_cols;
+                    const int rx = i - ry * fill_cols;
+                    const int y = uy0 + ry;
+                    const int x = ux0 + rx;
+                    float v = 0.0F;
+                    if (y < d.height && x < d.width) {
+                        v = src[static_cast<size_t>(y) * d.width + x] - mu;
+                    }
+                    const __bf16 hi = static_cast<__bf16>(v);
+                    const __bf16 lo = static_cast<__bf16>(v - static_cast<float>(hi));
+#pragma unroll
+                    for (int q = 0; q < kCopies; ++q) {
+                        const int j = rx - q;
+                        if (j >= 0 && j < S.pitch) {
+                            const int at = q * S.copy_elems + ry * S.pitch + j;
+                            s_copy[at] = hi;
+                            s_copy[kCopies * S.copy_elems + at] = lo;
+                        }
+                    }
+                }
+                // Anchor operands: anchor n_lane, patch rows 4 ks + kb.
+                bf16x8 b_hi[2];
+                bf16x8 b_lo[2];
+                const float* anchor_plane = source + (static_cast<size_t>(af + t) * d.channels + c) * plane;
+#pragma unroll
+                for (int ks = 0; ks < 2; ++ks) {
+                    const int py = 4 * ks + kb;
+#pragma unroll
+                    for (int v = 0; v < 8; ++v) {
+                        float x = 0.0F;
+                        if (my_anchor >= 0) {
+                            x = anchor_plane[static_cast<size_t>(my_ay + py) * d.width + my_ax + v] - mu;
+                        }
+                        const __bf16 hi = static_cast<__bf16>(x);
+                        b_hi[ks][v] = hi;
+                        b_lo[ks][v] = static_cast<__bf16>(x - static_cast<float>(hi));
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int i = 0; i < kTilesPerWave; ++i) {
+                    int mt = wave + kFastWaves * i;
+                    // Opaque to loop-invariant motion: the few scalar
+                    // address operations are redone per plane instead of
+                    // holding every tile's addresses in registers.
+                    asm volatile("" : "+s"(mt));
+                    if (mt < mtiles) {
+                        const int col = (mt * tile_magic) >> 16;
+                        const int rt = mt - col * row_tiles;
+                        const int shift = col & (kCopies - 1);
+                        const int j = col - shift;
+#pragma unroll
+                        for (int ks = 0; ks < 2; ++ks) {
+                            const int ry = 16 * rt + n_lane + 4 * ks + kb;
+                            const int at = shift * S.copy_elems + ry * S.pitch + j;
+                            const bf16x8 a_hi = load_bf16x8(s_copy + at);
+                            const bf16x8 a_lo = load_bf16x8(s_copy + kCopies * S.copy_elems + at);
+                            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, b_hi[ks], acc[i], 0, 0, 0);
+                            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, b_lo[ks], acc[i], 0, 0, 0);
+                            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_lo, b_hi[ks], acc[i], 0, 0, 0);
+                        }
+                    }
+                    // Keep one tile's operands live at a time.
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+        }
+        // Epilogue: distances of this candidate frame into LDS per anchor
+        // (window scan order), then out to global memory.
+        __syncthreads();
+        const float* frame_energy = energy + static_cast<size_t>(frame) * plane;
+#pragma unroll
+        for (int i = 0; i < kTilesPerWave; ++i) {
+            int mt = wave + kFastWaves * i;
+            asm volatile("" : "+s"(mt));
+            if (mt < mtiles && my_anchor >= 0) {
+                const int col = (mt * tile_magic) >> 16;
+                const int rt = mt - col * row_tiles;
+                const int cx = ux0 + col;
+                const int rel_x = cx - s_xl[n_lane];
+                if (rel_x >= 0 && rel_x < s_nx[n_lane]) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int cy = uy0 + 16 * rt + 4 * kb + v;
+                        const int rel_y = cy - s_yl[n_lane];
+                        if (rel_y >= 0 && rel_y < s_ny[n_lane]) {
+                            float dist;
+                            if (cx == my_ax && cy == my_ay && frame == af) {
+                                dist = 0.0F;
+                            } else {
+                                const float ec = frame_energy[static_cast<size_t>(cy) * d.width + cx];
+                                dist = (s_energy[n_lane] + ec) - 2.0F * acc[i][v];
+                                dist = dist > 0.0F ? dist : 0.0F;
+                            }
+                            s_stage[n_lane * S.per_frame + rel_y * s_nx[n_lane] + rel_x] = dist;
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        for (int n = 0; n < kTileAnchors; ++n) {
+            const int index = s_index[n];
+            if (index < 0) {
+                continue;
+            }
+            const int count = s_nx[n] * s_ny[n];
+            float* dst = out + static_cast<size_t>(index - anchor_base) * S.stride +
+                         static_cast<size_t>(ti) * count;
+            for (int i = tid; i < count; i += kFastThreads) {
+                dst[i] = s_stage[n * S.per_frame + i];
+            }
+        }
+    }
+}
+
+// Selection of every anchor from its staged distances.
+__global__ __launch_bounds__(256) void select_fast_kernel(
+    StageDesc d, int stride, const float* __restrict__ staged, const uint32_t* __restrict__ anchors,
+    int anchor_count, int* __restrict__ counts, uint32_t* __restrict__ positions,
+    float* __restrict__ distances) {
+    extern __shared__ float s_dist[];
+    __shared__ unsigned s_hist[256];
+    __shared__ int s_int[16];
+    __shared__ int s_scan[8];
+    __shared__ float s_sel_dist[kMaxCap];
+    __shared__ int s_sel_index[kMaxCap];
+    __shared__ int s_sel_rank[kMaxCap];
+    const int a = blockIdx.x;
+    if (a >= anchor_count) {
+        return;
+    }
+    const uint32_t anchor = anchors[a];
+    const int ax = position_x(anchor);
+    const int ay = position_y(anchor);
+    const int af = position_frame(anchor);
+    const size_t out_base = static_cast<size_t>(a) * d.cap;
+    if (d.similar <= 1) {
+        if (threadIdx.x == 0) {
+            counts[a] = 1;
+            positions[out_base] = anchor;
+            distances[out_base] = 0.0F;
+        }
+        return;
+    }
+    int t_low = 0;
+    int t_high = 0;
+    temporal_range(d, af, t_low, t_high);
+    const int nt = t_high - t_low + 1;
+    int x_low = 0;
+    int x_high = 0;
+    int y_low = 0;
+    int y_high = 0;
+    spatial_range(d, ax, d.width, x_low, x_high);
+    spatial_range(d, ay, d.height, y_low, y_high);
+    const int nx = x_high - x_low + 1;
+    const int ny = y_high - y_low + 1;
+    const int per_frame = nx * ny;
+    const int n = nt * per_frame;
+    const float* src = staged + static_cast<size_t>(a) * stride;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        s_dist[i] = src[i];
+    }
+    __syncthreads();
+    select_and_write(d, s_dist, n, af, t_low, t_high, x_low, y_low, nx, per_frame, nt, s_hist, s_int,
+                     s_scan, s_sel_dist, s_sel_index, s_sel_rank, a, out_base, counts, positions,
+                     distances);
+}
+
+FastShape fast_shape(const StageDesc& d, int max_cols, int max_rows) {
+    FastShape S{};
+    S.row_tiles = (max_rows + 15) / 16;
+    S.region_rows = 16 * S.row_tiles + kFastPatch - 1;
+    // Copy rows cover samples j .. j + 7 for j < max_cols rounded to 4.
+    S.pitch = (max_cols + kFastPatch - 1 + 3) / 4 * 4;
+    if ((S.pitch / 4) % 2 == 0) {
+        S.pitch += 4;  // odd number of 8-byte units: rows spread over banks
+    }
+    S.copy_elems = S.region_rows * S.pitch;
+    S.per_frame = d.search_window * d.search_window;
+    S.stride = S.per_frame * (d.search_bwd + d.search_fwd + 1);
+    return S;
+}
+
+constexpr int kMaxTilesPerWave = 24;
+
+}  // namespace
+
+bool fast_match_supported(const StageDesc& d, int max_cols, int max_rows) {
+    if (d.patch != kFastPatch || d.similar <= 1) {
+        return false;
+    }
+    const int row_tiles = (max_rows + 15) / 16;
+    return static_cast<long long>(max_cols) * row_tiles <= static_cast<long long>(kMaxTilesPerWave) * kFastWaves &&
+           fast_match_shared_bytes(d, max_cols, max_rows) <= 64 * 1024;
+}
+
+size_t fast_match_shared_bytes(const StageDesc& d, int max_cols, int max_rows) {
+    const FastShape S = fast_shape(d, max_cols, max_rows);
+    const size_t copies = static_cast<size_t>(2) * kCopies * S.copy_elems * sizeof(__bf16);
+    const size_t staging = static_cast<size_t>(kTileAnchors) * S.per_frame * sizeof(float);
+    return copies > staging ? copies : staging;
+}
+
+size_t fast_match_stride(const StageDesc& d) {
+    return static_cast<size_t>(d.search_window) * d.search_window * (d.search_bwd + d.search_fwd + 1);
+}
+
+hipError_t launch_fast_energy(const StageDesc& d, const float* source, float* offsets, float* rows,
+                              float* energy, hipStream_t stream) {
+    const int match_channels = d.stage == kFinal ? d.channels : 1;
+    const size_t plane = static_cast<size_t>(d.width) * d.height;
+    hipLaunchKernelGGL(channel_offset_kernel, dim3(static_cast<unsigned>(match_channels)), dim3(256), 0,
+                       stream, source, d.channels, plane, d.frames, offsets);
+    const size_t row_total = plane * match_channels * d.frames;
+    hipLaunchKernelGGL(row_energy_kernel, dim3(static_cast<unsigned>((row_total + 255) / 256)), dim3(256), 0,
+                       stream, d, source, offsets, match_channels, rows);
+    const size_t total = plane * d.frames;
+    hipLaunchKernelGGL(patch_energy_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0,
+                       stream, d, rows, match_channels, energy);
+    return hipGetLastError();
+}
+
+hipError_t launch_fast_match(const StageDesc& d, int max_cols, int max_rows, const float* source,
+                             const float* offsets, const float* energy, const uint32_t* anchors,
+                             const int* tiles, int tile_count, int anchor_base, float* staged,
+                             hipStream_t stream) {
+    if (tile_count <= 0) {
+        return hipSuccess;
+    }
+    const FastShape S = fast_shape(d, max_cols, max_rows);
+    const size_t shared = fast_match_shared_bytes(d, max_cols, max_rows);
+    const int mtiles = max_cols * S.row_tiles;
+    const int per_wave = (mtiles + kFastWaves - 1) / kFastWaves;
+#define VNLBHIP_FAST(T)                                                                          \
+    hipLaunchKernelGGL(match_fast_kernel<T>, dim3(static_cast<unsigned>(tile_count)),            \
+                       dim3(kFastThreads), shared, stream, d, S, source, offsets, energy, anchors, \
+                       tiles, anchor_base, staged)
+    if (per_wave <= 8) {
+        VNLBHIP_FAST(8);
+    } else if (per_wave <= 12) {
+        VNLBHIP_FAST(12);
+    } else if (per_wave <= 16) {
+        VNLBHIP_FAST(16);
+    } else if (per_wave <= 20) {
+        VNLBHIP_FAST(20);
+    } else {
+        VNLBHIP_FAST(24);
+    }
+#undef VNLBHIP_FAST
+    return hipGetLastError();
+}
+
+hipError_t launch_fast_select(const StageDesc& d, const float* staged, const uint32_t* anchors,
+                              int anchor_count, int* counts, uint32_t* positions, float* distances,
+                              hipStream_t stream) {
+    if (anchor_count <= 0) {
+        return hipSuccess;
+    }
+    const size_t stride = fast_match_stride(d);
+    hipLaunchKernelGGL(select_fast_kernel, dim3(static_cast<unsigned>(anchor_count)), dim3(256),
+                       stride * sizeof(float), stream, d, static_cast<int>(stride), staged, anchors,
+                       anchor_count, counts, positions, distances);
+    return hipGetLastError();
+}
+
+}  // namespace vnlbhip

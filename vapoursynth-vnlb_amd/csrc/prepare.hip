@@ -1,0 +1,462 @@
+// Per-group model input: group flags (equal / flat), the noisy and basic
+// means, the centered model rows and their Gram (dual PCA) or covariance
+// matrix, for the eigen solver.
+//
+// Semantics follow the reference (equality bypass core.cpp:516-531, flat
+// test :533-559, compute_mean linalg.cpp:290-311, center :313-324,
+// compute_gram_centered :326-354, compute_covariance_centered :377-414) in
+// the 16-lane order of the CPU restatement.
+//
+// Layout as in the filter kernel: model rows live in LDS in slot layout
+// (lane l of a 16-lane team owns elements 16 k + l), one team loads one row
+// with consecutive lanes on consecutive elements, and a Gram entry is a
+// team dot product read with 128-bit LDS loads.  Workgroups are persistent
+// and build the element tables once.
+#include "common.h"
+#include "device_util.h"
+#include "group.h"
+#include "kernels.h"
+
+namespace vnlbhip {
+namespace {
+
+struct PrepareLayout {
+    int pitch;     // words per lane slot row
+    int row;       // 16 * pitch
+    int width;     // models * row: words per sample row
+    int chunks;
+    int tail;
+    int tab_in;    // [models][row] input offsets
+    int tab_c;     // [models][row] channel of the element (-1 outside the row)
+    int rows;      // [max_rows][width] model rows
+    int pos;       // [cap] packed positions
+    int words;
+};
+
+PrepareLayout prepare_layout(const StageDesc& d) {
+    PrepareLayout L{};
+    L.pitch = slot_pitch(d.model_dim);
+    L.row = kLanes * L.pitch;
+    L.width = d.models * L.row;
+    L.chunks = d.model_dim / kLanes;
+    L.tail = d.model_dim % kLanes;
+    int at = 0;
+    const auto take = [&at](int words) {
+        const int start = at;
+        at += (words + 3) / 4 * 4;
+        return start;
+    };
+    L.tab_in = take(L.width);
+    L.tab_c = take(L.width);
+    L.rows = take(d.max_rows * L.width);
+    L.pos = take(d.cap);
+    L.words = at;
+    return L;
+}
+
+// Loads sample row r (all models) into LDS slot layout: team-wide, lane l
+// reads the elements 16 k + l of each model.
+__device__ inline void load_row(const StageDesc& d, const PrepareLayout& L,
+                                const float* __restrict__ video, size_t base, const int* tab_in,
+                                float* dst, int lane) {
+    for (int m = 0; m < d.models; ++m) {
+        const int at = m * L.row + lane * L.pitch;
+        for (int k = 0; k < L.chunks; ++k) {
+            dst[at + k] = video[base + tab_in[at + k]];
+        }
+        if (lane < L.tail) {
+            dst[at + L.chunks] = video[base + tab_in[at + L.chunks]];
+        }
+    }
+}
+
+template <int kS, bool kExact>
+__global__ __launch_bounds__(kModelThreads) void prepare_kernel(
+    StageDesc d, PrepareLayout L, const float* __restrict__ noisy,
+    const float* __restrict__ basic, GroupRefs refs, int group_count, int problem_stride,
+    int* __restrict__ group_flags, int* __restrict__ problem_dims, float* __restrict__ means,
+    float* __restrict__ problems) {
+    extern __shared__ float4 s_dyn4[];
+    float* s_dyn = reinterpret_cast<float*>(s_dyn4);
+    __shared__ int s_equal;
+    __shared__ double s_sum[kModelThreads / 64][kMaxChannels][2];
+
+    int* tab_in = reinterpret_cast<int*>(s_dyn + L.tab_in);
+    int* tab_c = reinterpret_cast<int*>(s_dyn + L.tab_c);
+    float* s_rows = s_dyn + L.rows;
+    uint32_t* s_pos = reinterpret_cast<uint32_t*>(s_dyn + L.pos);
+
+    const int tid = threadIdx.x;
+    const int team = tid / kLanes;
+    const int lane = tid % kLanes;
+    const int width = L.width;
+    const int row = L.row;
+    const int pitch = L.pitch;
+    const int chunks = kExact ? kS : L.chunks;
+    const int tail = kExact ? 0 : L.tail;
+    const int md = d.model_dim;
+    const bool final_stage = d.stage == kFinal;
+    const bool want_flat = final_stage && d.flat_areas != 0;
+
+    for (int i = tid; i < width; i += kModelThreads) {
+        const int m = i / row;
+        const int rem = i - m * row;
+        const int l = rem / pitch;
+        const int k = rem - l * pitch;
+        const bool valid = k < chunks || (k == chunks && l < tail);
+        int in_off = 0;
+        int channel = -1;
+        if (valid) {
+            const int e = (d.coupled ? 0 : m * d.patch_dim) + kLanes * k + l;
+            const int c = e / d.patch_dim;
+            const int wi = e - c * d.patch_dim;
+            const int t = wi / d.patch_area;
+            const int yx = wi - t * d.patch_area;
+            const int y = yx / d.patch;
+            const int x = yx - y * d.patch;
+            in_off = ((t * d.channels + c) * d.height + y) * d.width + x;
+            channel = c;
+        }
+        tab_in[i] = in_off;
+        tab_c[i] = channel;
+    }
+
+    for (int gi = static_cast<int>(blockIdx.x); gi < group_count;
+         gi += static_cast<int>(gridDim.x)) {
+        const int g = gi;
+        __syncthreads();  // the previous group is done with the LDS
+        const int a = refs.group_list[g];
+        const int similar = refs.counts[a];
+        const size_t list = static_cast<size_t>(a) * d.cap;
+        const int rows = model_rows_of(d, similar);
+        for (int r = tid; r < similar; r += kModelThreads) {
+            s_pos[r] = refs.positions[list + r];
+        }
+        if (tid == 0) {
+            s_equal = 1;
+        }
+        __syncthreads();
+
+        // Noisy model rows into LDS; flat sums (double, per channel) over
+        // every retained noisy row.
+        double sums[kMaxChannels][2];
+#pragma unroll
+        for (int c = 0; c < kMaxChannels; ++c) {
+            sums[c][0] = 0.0;
+            sums[c][1] = 0.0;
+        }
+        for (int r = team; r < rows; r += kTeams) {
+            float* dst = s_rows + r * width;
+            load_row(d, L, noisy, patch_base(d, s_pos[r]), tab_in, dst, lane);
+            if (want_flat) {
+                for (int m = 0; m < d.models; ++m) {
+                    const int at = m * row + lane * pitch;
+                    for (int k = 0; k <= chunks; ++k) {
+                        if (k < chunks || lane < tail) {
+                            const int c = tab_c[at + k];
+                            const double v = static_cast<double>(dst[at + k]);
+#pragma unroll
+                            for (int cc = 0; cc < kMaxChannels; ++cc) {
+                                if (cc == c) {
+                                    sums[cc][0] += v;
+                                    sums[cc][1] += v * v;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+
+        // Per element (one thread per slot): equality of the model rows
+        // with row 0 and the noisy mean over the model rows in row order.
+        const float inv_rows = 1.0F / static_cast<float>(rows);
+        float* mean_noisy = means + static_cast<size_t>(g) * 2 * d.dim;
+        float* mean_basic = mean_noisy + d.dim;
+        bool equal_local = true;
+        for (int i = tid; i < width; i += kModelThreads) {
+            if (tab_c[i] < 0) {
+                continue;
+            }
+            const float v0 = s_rows[i];
+            float acc = 0.0F;
+            for (int r = 0; r < rows; ++r) {
+                const float v = s_rows[r * width + i];
+                equal_local = equal_local && (v == v0);
+                acc = acc + v;
+            }
+            const float mean = acc * inv_rows;
+            const int m = i / row;
+            const int rem = i - m * row;
+            const int l = rem / pitch;
+            const int k = rem - l * pitch;
+            mean_noisy[(d.coupled ? 0 : m * d.patch_dim) + kLanes * k + l] = mean;
+            if (!final_stage) {
+                // Basic: the model rows are the noisy rows, centered.
+                for (int r = 0; r < rows; ++r) {
+                    s_rows[r * width + i] = s_rows[r * width + i] - mean;
+                }
+            }
+        }
+        if (!equal_local) {
+            s_equal = 0;
+        }
+        __syncthreads();
+        // Retained rows beyond the model rows: equality (only while all
+        // rows so far equal row 0) and the flat sums.
+        if (similar > rows && (want_flat || s_equal != 0)) {
+            const bool check_equal = s_equal != 0;
+            const size_t base0 = patch_base(d, s_pos[0]);
+            for (int r = rows + team; r < similar; r += kTeams) {
+                const size_t base = patch_base(d, s_pos[r]);
+                for (int m = 0; m < d.models; ++m) {
+                    const int at = m * row + lane * pitch;
+                    bool same = true;
+                    for (int k = 0; k <= chunks; ++k) {
+                        if (k < chunks || lane < tail) {
+                            const float v = noisy[base + tab_in[at + k]];
+                            if (want_flat) {
+                                const int c = tab_c[at + k];
+                                const double dv = static_cast<double>(v);
+#pragma unroll
+                                for (int cc = 0; cc < kMaxChannels; ++cc) {
+                                    if (cc == c) {
+                                        sums[cc][0] += dv;
+                                        sums[cc][1] += dv * dv;
+                                    }
+                                }
+                            }
+                            if (check_equal) {
+                                // Row 0 from the source: Basic rows in LDS
+                                // are centered by now.
+                                same = same && v == noisy[base0 + tab_in[at + k]];
+                            }
+                        }
+                    }
+                    if (check_equal && !same) {
+                        s_equal = 0;
+                    }
+                }
+            }
+        }
+        const int wave = tid >> 6;
+        if (want_flat) {
+#pragma unroll
+            for (int c = 0; c < kMaxChannels; ++c) {
+                const double s0 = wave_sum_double(sums[c][0]);
+                const double s1 = wave_sum_double(sums[c][1]);
+                if ((tid & 63) == 0) {
+                    s_sum[wave][c][0] = s0;
+                    s_sum[wave][c][1] = s1;
+                }
+            }
+        }
+        __syncthreads();
+
+        const bool equal = similar > 1 && s_equal != 0;
+        int flags = equal ? kFlagEqual : 0;
+        if (!equal && want_flat) {
+            const int per_channel = similar * d.patch_dim;
+            if (per_channel > 1) {
+                float variance = 0.0F;
+                const double count = static_cast<double>(per_channel);
+                for (int c = 0; c < d.channels; ++c) {
+                    double total = 0.0;
+                    double total2 = 0.0;
+                    for (int w = 0; w < kModelThreads / 64; ++w) {
+                        total += s_sum[w][c][0];
+                        total2 += s_sum[w][c][1];
+                    }
+                    variance = variance +
+                               static_cast<float>((total2 - (total * total / count)) / (count - 1.0));
+                }
+                variance = variance / static_cast<float>(d.channels);
+                if (variance < (d.sigma * d.sigma) * d.gamma) {
+                    flags |= kFlagFlat;
+                }
+            }
+        }
+        const bool modelled = !equal && d.rank > 0;
+        const bool dual = group_uses_dual(d, rows);
+        if (tid == 0) {
+            if (rows < (md < d.rank ? md : d.rank)) {
+                flags |= 8;  // rows cannot cover the rank: reported by the host
+            }
+            group_flags[g] = flags | (modelled ? kFlagModelled : 0);
+            problem_dims[g] = dual ? rows : md;
+        }
+
+        if (final_stage) {
+            // The basic mean is needed by the filter even when the group is
+            // not modelled (flat centre); the model rows are the basic rows.
+            for (int r = team; r < rows; r += kTeams) {
+                load_row(d, L, basic, patch_base(d, s_pos[r]), tab_in, s_rows + r * width, lane);
+            }
+            __syncthreads();
+            for (int i = tid; i < width; i += kModelThreads) {
+                if (tab_c[i] < 0) {
+                    continue;
+                }
+                float acc = 0.0F;
+                for (int r = 0; r < rows; ++r) {
+                    acc = acc + s_rows[r * width + i];
+                }
+                acc = acc * inv_rows;
+                const int m = i / row;
+                const int rem = i - m * row;
+                const int l = rem / pitch;
+                const int k = rem - l * pitch;
+                mean_basic[(d.coupled ? 0 : m * d.patch_dim) + kLanes * k + l] = acc;
+                for (int r = 0; r < rows; ++r) {
+                    s_rows[r * width + i] = s_rows[r * width + i] - acc;
+                }
+            }
+            __syncthreads();
+        }
+        if (!modelled) {
+            continue;
+        }
+
+        const float scale = 1.0F / static_cast<float>(rows);
+        for (int m = 0; m < d.models; ++m) {
+            float* out = problems + (static_cast<size_t>(g) * d.models + m) * problem_stride * problem_stride;
+            if (dual) {
+                // compute_gram_centered (linalg.cpp:326-354): scale * dot in
+                // the 16-lane order, one team per entry.
+                const int pairs = rows * (rows + 1) / 2;
+                for (int q = team; q < pairs; q += kTeams) {
+                    int r = 0;
+                    while ((r + 1) * (r + 2) / 2 <= q) {
+                        ++r;
+                    }
+                    const int c = q - r * (r + 1) / 2;
+                    const float* left = s_rows + r * width + m * row + lane * pitch;
+                    const float* right = s_rows + c * width + m * row + lane * pitch;
+                    const float4* l4 = reinterpret_cast<const float4*>(left);
+                    const float4* r4 = reinterpret_cast<const float4*>(right);
+                    float part = 0.0F;
+#pragma unroll
+                    for (int h = 0; h < kS / 4; ++h) {
+                        if (4 * h < chunks) {
+                            const float4 x = l4[h];
+                            const float4 y = r4[h];
+                            if (4 * h + 0 < chunks) part = fmaf(x.x, y.x, part);
+                            if (4 * h + 1 < chunks) part = fmaf(x.y, y.y, part);
+                            if (4 * h + 2 < chunks) part = fmaf(x.z, y.z, part);
+                            if (4 * h + 3 < chunks) part = fmaf(x.w, y.w, part);
+                        }
+                    }
+                    float total = segment16_sum(part);
+                    if (tail > 0) {
+                        const float product = lane < tail ? left[chunks] * right[chunks] : 0.0F;
+                        for (int i = 0; i < tail; ++i) {
+                            total = total + __shfl(product, i, kLanes);
+                        }
+                    }
+                    if (lane == 0) {
+                        const float value = scale * total;
+                        out[r * problem_stride + c] = value;
+                        out[c * problem_stride + r] = value;
+                    }
+                }
+            } else {
+                // compute_covariance_centered (linalg.cpp:377-414): per entry
+                // a chain over samples, fused where add_scaled's vector part
+                // covers the column.
+                const int pairs = md * (md + 1) / 2;
+                for (int q = tid; q < pairs; q += kModelThreads) {
+                    int c0 = 0;
+                    int remaining = q;
+                    while (remaining >= md - c0) {
+                        remaining -= md - c0;
+                        ++c0;
+                    }
+                    const int c1 = c0 + remaining;
+                    const int vector_end = ((md - c0) / kLanes) * kLanes;
+                    const bool fused = (c1 - c0) < vector_end;
+                    const int s0 = m * row + (c0 % kLanes) * pitch + c0 / kLanes;
+                    const int s1 = m * row + (c1 % kLanes) * pitch + c1 / kLanes;
+                    float acc = 0.0F;
+                    for (int s2 = 0; s2 < rows; ++s2) {
+                        const float left = s_rows[s2 * width + s0];
+                        const float right = s_rows[s2 * width + s1];
+                        if (fused) {
+                            acc = fmaf(left, right, acc);
+                        } else {
+                            const float product = left * right;
+                            acc = acc + product;
+                        }
+                    }
+                    const float value = acc * scale;
+                    out[c0 * problem_stride + c1] = value;
+                    out[c1 * problem_stride + c0] = value;
+                }
+            }
+        }
+    }
+}
+
+}  // namespace
+
+size_t prepare_shared_bytes(const StageDesc& d) {
+    return static_cast<size_t>(prepare_layout(d).words) * sizeof(float);
+}
+
+hipError_t launch_prepare(const StageDesc& d, const float* noisy, const float* basic,
+                          const uint32_t* anchors, const int* counts, const uint32_t* positions,
+                          const int* group_list, int group_count, int problem_stride,
+                          int* group_flags, int* problem_dims, float* means, float* problems,
+                          hipStream_t stream) {
+    if (group_count <= 0) {
+        return hipSuccess;
+    }
+    const GroupRefs refs{anchors, counts, positions, group_list};
+    const PrepareLayout L = prepare_layout(d);
+    const size_t shared = static_cast<size_t>(L.words) * sizeof(float);
+    int device = 0;
+    int units = 0;
+    hipError_t status = hipGetDevice(&device);
+    if (status == hipSuccess) {
+        status = hipDeviceGetAttribute(&units, hipDeviceAttributeMultiprocessorCount, device);
+    }
+    if (status != hipSuccess) {
+        return status;
+    }
+    const int slots = template_slots(d.model_dim);
+    const bool exact = d.model_dim == kLanes * slots;
+#define VNLBHIP_PREPARE(KS, EXACT)                                                                 \
+    do {                                                                                           \
+        int per_unit = 0;                                                                          \
+        status = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_unit, prepare_kernel<KS, EXACT>, \
+                                                              kModelThreads, shared);              \
+        if (status != hipSuccess) {                                                                \
+            return status;                                                                         \
+        }                                                                                          \
+        const long long resident = static_cast<long long>(per_unit > 0 ? per_unit : 1) * units;    \
+        const unsigned blocks =                                                                    \
+            static_cast<unsigned>(resident < group_count ? resident : group_count);                \
+        hipLaunchKernelGGL((prepare_kernel<KS, EXACT>), dim3(blocks), dim3(kModelThreads), shared, \
+                           stream, d, L, noisy, basic, refs, group_count, problem_stride,          \
+                           group_flags, problem_dims, means, problems);                            \
+    } while (0)
+#define VNLBHIP_PREPARE_SLOTS(KS)      \
+    if (exact) {                       \
+        VNLBHIP_PREPARE(KS, true);     \
+    } else {                           \
+        VNLBHIP_PREPARE(KS, false);    \
+    }
+    switch (slots) {
+        case 4: VNLBHIP_PREPARE_SLOTS(4); break;
+        case 8: VNLBHIP_PREPARE_SLOTS(8); break;
+        case 16: VNLBHIP_PREPARE_SLOTS(16); break;
+        case 24: VNLBHIP_PREPARE_SLOTS(24); break;
+        case 32: VNLBHIP_PREPARE_SLOTS(32); break;
+        default: VNLBHIP_PREPARE_SLOTS(64); break;
+    }
+#undef VNLBHIP_PREPARE_SLOTS
+#undef VNLBHIP_PREPARE
+    return hipGetLastError();
+}
+
+}  // namespace vnlbhip

@@ -10,6 +10,7 @@
 // (StagePipeline::enqueue, src/cuda/pipeline.cu:554-667).
 #include "vnlbhip.h"
 
+#include "geometry.h"
 #include "kernels.h"
 
 #include <hip/hip_runtime.h>
@@ -282,6 +283,65 @@ void append_grid_frame(const StageDesc& d, int anchor_frame, Grid& grid) {
     }
 }
 
+// Anchor tiles of the MFMA matcher: 4 consecutive grid rows x 4 anchors of
+// one frame (slot rr * 4 + ii, -1 when a row is shorter), and the largest
+// union of the tiles' search windows.
+struct Tiles {
+    std::vector<int> slots;
+    int max_cols = 0;
+    int max_rows = 0;
+};
+
+Tiles build_tiles(const StageDesc& d, const Grid& grid, int height) {
+    Tiles tiles;
+    const int frame_count = static_cast<int>(grid.ids.size());
+    for (int fi = 0; fi < frame_count; ++fi) {
+        const int frame_begin = grid.offsets[static_cast<size_t>(fi)];
+        const int* first = grid.rows.data() + static_cast<size_t>(fi) * 3 * height;
+        const int* count = first + 2 * height;
+        std::vector<int> ys;
+        for (int y = 0; y < height; ++y) {
+            if (count[y] > 0) {
+                ys.push_back(y);
+            }
+        }
+        for (size_t r0 = 0; r0 < ys.size(); r0 += 4) {
+            int widest = 0;
+            for (size_t rr = r0; rr < r0 + 4 && rr < ys.size(); ++rr) {
+                widest = std::max(widest, count[ys[rr]]);
+            }
+            for (int i0 = 0; i0 < widest; i0 += 4) {
+                int x0 = INT_MAX;
+                int x1 = INT_MIN;
+                int y0 = INT_MAX;
+                int y1 = INT_MIN;
+                for (int rr = 0; rr < 4; ++rr) {
+                    for (int ii = 0; ii < 4; ++ii) {
+                        int index = -1;
+                        const size_t row = r0 + static_cast<size_t>(rr);
+                        if (row < ys.size() && i0 + ii < count[ys[row]]) {
+                            index = frame_begin + first[ys[row]] + i0 + ii;
+                            const uint32_t p = grid.anchors[static_cast<size_t>(index)];
+                            int lo = 0;
+                            int hi = 0;
+                            vnlbhip::spatial_range(d, vnlbhip::position_x(p), d.width, lo, hi);
+                            x0 = std::min(x0, lo);
+                            x1 = std::max(x1, hi);
+                            vnlbhip::spatial_range(d, vnlbhip::position_y(p), d.height, lo, hi);
+                            y0 = std::min(y0, lo);
+                            y1 = std::max(y1, hi);
+                        }
+                        tiles.slots.push_back(index);
+                    }
+                }
+                tiles.max_cols = std::max(tiles.max_cols, x1 - x0 + 1);
+                tiles.max_rows = std::max(tiles.max_rows, y1 - y0 + 1);
+            }
+        }
+    }
+    return tiles;
+}
+
 template <class T> struct DeviceBuffer {
     T* ptr = nullptr;
     size_t capacity = 0;
@@ -356,6 +416,13 @@ struct vnlbhip_context {
     DeviceBuffer<float> host_ref;
     DeviceBuffer<float> host_out;
     size_t batch_entry_budget = static_cast<size_t>(1) << 28;  // anchors * cap
+    size_t fast_stage_budget = static_cast<size_t>(1) << 30;  // anchors * window slots
+    int form = VNLBHIP_FORM_PLAIN;
+    DeviceBuffer<float> fast_offsets;
+    DeviceBuffer<float> fast_rows;
+    DeviceBuffer<float> fast_energy;
+    DeviceBuffer<int> fast_tiles;
+    DeviceBuffer<float> fast_staged;
     bool profiling = false;
     vnlbhip_kernel_times times{};
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
@@ -448,6 +515,18 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
     check(hipMemsetAsync(summary, 0, 4 * sizeof(int), stream), "hipMemsetAsync");
 
     const float* source = stage == VNLBHIP_STAGE_FINAL ? ref : noisy;
+    // MFMA matcher: patch energies of the matched source, once per stage.
+    float* fast_offsets = nullptr;
+    float* fast_energy = nullptr;
+    if (ctx.form == VNLBHIP_FORM_FAST && d.patch == 8 && d.similar > 1) {
+        const int match_channels = stage == VNLBHIP_STAGE_FINAL ? g.channels : 1;
+        fast_offsets = ctx.fast_offsets.reserve(4);
+        float* rows = ctx.fast_rows.reserve(plane * match_channels * g.frames);
+        fast_energy = ctx.fast_energy.reserve(plane * g.frames);
+        timed(ctx, VNLBHIP_KERNEL_ENERGY, stream, "energy kernels", [&]() {
+            return vnlbhip::launch_fast_energy(d, source, fast_offsets, rows, fast_energy, stream);
+        });
+    }
     const int cap = d.cap;
     const int n_stride = d.jacobi_dim;
     size_t next = 0;
@@ -497,10 +576,70 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
         int* d_list = ctx.group_list.reserve(anchor_count);
         int* d_group_count = ctx.group_count.reserve(1);
 
-        timed(ctx, VNLBHIP_KERNEL_MATCH, stream, "match kernel", [&]() {
-            return vnlbhip::launch_match(d, source, d_anchors, anchor_count, d_counts, d_positions,
-                                         d_distances, stream);
-        });
+        bool matched = false;
+        if (fast_energy != nullptr) {
+            const Tiles tiles = build_tiles(d, grid, g.height);
+            if (vnlbhip::fast_match_supported(d, tiles.max_cols, tiles.max_rows)) {
+                const int tile_count = static_cast<int>(tiles.slots.size() / 16);
+                int* d_tiles = ctx.fast_tiles.reserve(tiles.slots.size());
+                check(hipMemcpyAsync(d_tiles, tiles.slots.data(), tiles.slots.size() * sizeof(int),
+                                     hipMemcpyHostToDevice, stream), "upload tiles");
+                // Distances are staged for a chunk of anchor frames at a time
+                // (tiles and anchors are both in frame order) and selected
+                // before the next chunk reuses the buffer.
+                const size_t stride = vnlbhip::fast_match_stride(d);
+                const int chunk_anchors = static_cast<int>(std::max<size_t>(
+                    static_cast<size_t>(max_anchors), ctx.fast_stage_budget / stride));
+                float* staged = ctx.fast_staged.reserve(static_cast<size_t>(
+                    std::min(chunk_anchors, anchor_count)) * stride);
+                int tile_begin = 0;
+                int frame = 0;
+                while (frame < frame_count) {
+                    const int anchor_begin = grid.offsets[static_cast<size_t>(frame)];
+                    int frame_end = frame + 1;
+                    while (frame_end < frame_count &&
+                           grid.offsets[static_cast<size_t>(frame_end + 1)] - anchor_begin <= chunk_anchors) {
+                        ++frame_end;
+                    }
+                    const int anchor_end = grid.offsets[static_cast<size_t>(frame_end)];
+                    int tile_end = tile_begin;
+                    while (tile_end < tile_count) {
+                        int first = -1;
+                        for (int slot = 0; slot < 16 && first < 0; ++slot) {
+                            first = tiles.slots[static_cast<size_t>(tile_end) * 16 + slot];
+                        }
+                        if (first >= anchor_end) {
+                            break;
+                        }
+                        ++tile_end;
+                    }
+                    timed(ctx, VNLBHIP_KERNEL_MATCH, stream, "fast match kernel", [&]() {
+                        return vnlbhip::launch_fast_match(d, tiles.max_cols, tiles.max_rows, source,
+                                                          fast_offsets, fast_energy, d_anchors,
+                                                          d_tiles + static_cast<size_t>(tile_begin) * 16,
+                                                          tile_end - tile_begin, anchor_begin, staged, stream);
+                    });
+                    timed(ctx, VNLBHIP_KERNEL_SELECT, stream, "select kernel", [&]() {
+                        return vnlbhip::launch_fast_select(d, staged, d_anchors + anchor_begin,
+                                                           anchor_end - anchor_begin, d_counts + anchor_begin,
+                                                           d_positions + static_cast<size_t>(anchor_begin) * cap,
+                                                           d_distances + static_cast<size_t>(anchor_begin) * cap,
+                                                           stream);
+                    });
+                    tile_begin = tile_end;
+                    frame = frame_end;
+                }
+                // The tile list must outlive its upload.
+                check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+                matched = true;
+            }
+        }
+        if (!matched) {
+            timed(ctx, VNLBHIP_KERNEL_MATCH, stream, "match kernel", [&]() {
+                return vnlbhip::launch_match(d, source, d_anchors, anchor_count, d_counts, d_positions,
+                                             d_distances, stream);
+            });
+        }
         unsigned long long* d_dep = ctx.paste_dep.reserve(anchor_count);
         int* d_tcount = ctx.paste_target_counts.reserve(anchor_count);
         int* d_targets = ctx.paste_targets.reserve(static_cast<size_t>(anchor_count) *
@@ -796,6 +935,14 @@ int vnlbhip_set_profiling(vnlbhip_context* context, int enable) {
     }
     context->profiling = enable != 0;
     context->times = vnlbhip_kernel_times{};
+    return VNLBHIP_OK;
+}
+
+int vnlbhip_set_form(vnlbhip_context* context, int form) {
+    if (context == nullptr || (form != VNLBHIP_FORM_PLAIN && form != VNLBHIP_FORM_FAST)) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    context->form = form;
     return VNLBHIP_OK;
 }
 

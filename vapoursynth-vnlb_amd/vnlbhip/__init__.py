@@ -49,6 +49,7 @@ class Context:
         if rc != 0:
             raise RuntimeError(f"VNLB HIP: cannot create a context on device {device} ({_ERROR_NAMES.get(rc, rc)})")
         self.device = int(device)
+        self.form = "plain"
 
     def close(self) -> None:
         if self._handle:
@@ -70,6 +71,16 @@ class Context:
         self._lib.vnlbhip_get_stats(self._handle, ctypes.byref(s))
         return {"anchors": s.anchors, "groups": s.groups,
                 "unconverged_groups": s.unconverged_groups, "batches": s.batches}
+
+    def set_form(self, form: str) -> None:
+        """'plain' (fp32 in the CPU path's order, the default) or 'fast'
+        (candidate distances from bf16x3 products on the matrix cores)."""
+        if form not in _abi.FORMS:
+            raise VNLBError(f"VNLB HIP: form must be one of {sorted(_abi.FORMS)}")
+        rc = self._lib.vnlbhip_set_form(self._handle, _abi.FORMS[form])
+        if rc != 0:
+            self._raise(rc)
+        self.form = form
 
     def set_profiling(self, enable: bool) -> None:
         self._lib.vnlbhip_set_profiling(self._handle, 1 if enable else 0)
@@ -183,12 +194,16 @@ def Basic(clip, sigma, block_size=None, block_step=None, group_size=None, bm_ran
           cap_factor=None, model_cap_factor=None, beta=None, tau=None,
           variance_threshold=None, weight_alpha=None, weight_beta=None, weight_gamma=None,
           weight_epsilon=None, membership_noise_floor=None, chroma=None, mvfw=None,
-          mvbw=None, device_id=None, num_streams=None, chunk_size=None):
-    """vnlb.Basic followed by vnlb.Aggregate(src=clip), on the GPU."""
+          mvbw=None, device_id=None, num_streams=None, chunk_size=None, form=None):
+    """vnlb.Basic followed by vnlb.Aggregate(src=clip), on the GPU.
+    form: 'plain' (default) or 'fast' (MFMA candidate distances)."""
     args = dict(locals())
     clip = args.pop("clip")
+    form = args.pop("form") or "plain"
     params = parse_stage_parameters(BASIC, args)
-    return _context(int(device_id or 0)).stage(BASIC, params, clip)
+    context = _context(int(device_id or 0))
+    context.set_form(form)
+    return context.stage(BASIC, params, clip)
 
 
 def Final(clip, ref, sigma, block_size=None, block_step=None, group_size=None, bm_range=None,
@@ -197,10 +212,14 @@ def Final(clip, ref, sigma, block_size=None, block_step=None, group_size=None, b
           variance_threshold=None, sigma_basic=None, gamma=None, flat_areas=None,
           weight_alpha=None, weight_beta=None, weight_gamma=None, weight_epsilon=None,
           membership_noise_floor=None, chroma=None, mvfw=None, mvbw=None, device_id=None,
-          num_streams=None, chunk_size=None):
-    """vnlb.Final(ref=basic) followed by vnlb.Aggregate(src=clip), on the GPU."""
+          num_streams=None, chunk_size=None, form=None):
+    """vnlb.Final(ref=basic) followed by vnlb.Aggregate(src=clip), on the GPU.
+    form: 'plain' (default) or 'fast' (MFMA candidate distances)."""
     args = dict(locals())
     clip = args.pop("clip")
     ref = args.pop("ref")
+    form = args.pop("form") or "plain"
     params = parse_stage_parameters(FINAL, args)
-    return _context(int(device_id or 0)).stage(FINAL, params, clip, ref)
+    context = _context(int(device_id or 0))
+    context.set_form(form)
+    return context.stage(FINAL, params, clip, ref)

@@ -25,9 +25,10 @@ EXPORTED_SYMBOLS = (
     "vnlbhip_get_stats",
     "vnlbhip_set_profiling",
     "vnlbhip_get_kernel_times",
+    "vnlbhip_set_form",
 )
 
-KERNEL_NAMES = ("match", "paste", "prepare", "jacobi", "filter", "normalize")
+KERNEL_NAMES = ("match", "paste", "prepare", "jacobi", "filter", "normalize", "select", "energy")
 
 
 class Params(ctypes.Structure):
@@ -70,8 +71,8 @@ class Stats(ctypes.Structure):
 
 class KernelTimes(ctypes.Structure):
     _fields_ = [
-        ("milliseconds", ctypes.c_double * 6),
-        ("launches", ctypes.c_int64 * 6),
+        ("milliseconds", ctypes.c_double * len(KERNEL_NAMES)),
+        ("launches", ctypes.c_int64 * len(KERNEL_NAMES)),
     ]
 
 
@@ -82,6 +83,9 @@ def to_struct(p: StageParameters) -> Params:
         setattr(s, name, int(value) if isinstance(value, bool) else value)
     return s
 
+
+# Arithmetic forms (VNLBHIP_FORM_*).
+FORMS = {"plain": 0, "fast": 1}
 
 _lib = None
 
@@ -128,6 +132,8 @@ def load() -> ctypes.CDLL:
     lib.vnlbhip_get_stats.restype = ctypes.c_int
     lib.vnlbhip_set_profiling.argtypes = [vp, ctypes.c_int]
     lib.vnlbhip_set_profiling.restype = ctypes.c_int
+    lib.vnlbhip_set_form.argtypes = [vp, ctypes.c_int]
+    lib.vnlbhip_set_form.restype = ctypes.c_int
     lib.vnlbhip_get_kernel_times.argtypes = [vp, P(KernelTimes)]
     lib.vnlbhip_get_kernel_times.restype = ctypes.c_int
     _lib = lib
