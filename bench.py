@@ -54,6 +54,18 @@ def algorithmic_match(params, width, height, frames, channels, anchors, stage):
     return 2.0 * nx * ny * nt * d * anchors
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (rocprofv3 FETCH_SIZE / WRITE_SIZE passes, copy-calibrated), or None."""
+    path = REPO / "profiles" / "hbm_traffic_fast_kernels_yuv444ps_1080p_16f.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def match_roofline(form, times, bp, fp, clip, sb, sf) -> dict:
     """Roofline of the distance kernel: algorithmic GEMM-form flops (2 N d
     per anchor, the candidates each anchor needs, not the tile unions or the
@@ -75,7 +87,7 @@ def match_roofline(form, times, bp, fp, clip, sb, sf) -> dict:
         "peak": peak,
         "unit": "TFLOP/s",
         "frac": round(achieved / peak, 5),
-        "traffic": None,
+        "traffic": measured_traffic("match_fast_kernel") if form == "fast" else None,
         "launch_ms": round(avg_s * 1000.0, 4),
     }
 

@@ -1,12 +1,15 @@
 """MFMA ("fast") form against the plain form, through the C ABI.
 
 The fast form takes candidate distances from bf16x3 products
-(E(r) + E(c) - 2 <r, c>), so distances agree with the exact fp32 sums to a
-small relative error, the kept candidates can differ where distances are
-within that error of a threshold, and the outputs agree in quality: the
-bound is |PSNR(fast) - PSNR(plain)| <= 0.05 dB against the clean clip.
-Shapes the fast matcher does not take (patch sizes other than 8) run the
-plain matcher and must give bit-identical results.
+(E(r) + E(c) - 2 <r, c>) and filters dual-PCA groups through their
+cross-Gram (bf16x3 on the matrix cores) instead of explicit basis vectors.
+Distances agree with the exact fp32 sums to a small relative error, the
+kept candidates can differ where distances are within that error of a
+threshold, and the outputs agree in quality: the bound is
+|PSNR(fast) - PSNR(plain)| <= 0.05 dB against the clean clip, checked both
+on the benchmark's configuration and on one that denoises by several dB.
+Shapes the fast kernels do not take (patch sizes other than 8) run the
+plain kernels and must give bit-identical results.
 """
 import numpy as np
 import pytest
@@ -109,3 +112,29 @@ def test_form_is_validated(contexts):
     with pytest.raises(vnlbhip.VNLBError):
         fast.set_form("fp8")
     assert fast.form == "fast"
+
+
+DENOISE = dict(sigma=20, block_size=8, block_step=4, group_size=16, bm_range=9, patch_time=2,
+               radius=2, rank=4, chroma=1)
+
+
+@pytest.mark.parametrize("channels", [3, 1])
+def test_fast_form_on_a_denoising_config(contexts, channels):
+    """rank 4 shrinks the dual-PCA model enough to denoise by several dB:
+    the fast matcher and the fast (dual-algebra) filter must keep the plain
+    form's quality."""
+    plain, fast = contexts
+    noisy, clean = synth.make_clip(width=128, height=96, frames=6, channels=channels, sigma8=20,
+                                   seed=9, motion="subpixel")
+    bp = parse_stage_parameters(BASIC, DENOISE)
+    fp = parse_stage_parameters(FINAL, DENOISE)
+    print(f"[fast] denoising config channels={channels}", flush=True)
+    x = _dev(noisy)
+    f_plain = plain.stage(FINAL, fp, x, plain.stage(BASIC, bp, x)).cpu().numpy()
+    f_fast = fast.stage(FINAL, fp, x, fast.stage(BASIC, bp, x)).cpu().numpy()
+    p_noisy = synth.psnr(noisy, clean)
+    p_plain = synth.psnr(f_plain, clean)
+    p_fast = synth.psnr(f_fast, clean)
+    assert p_plain > p_noisy + 3.0, (p_noisy, p_plain)
+    assert abs(p_fast - p_plain) <= PSNR_TOL, (p_fast, p_plain)
+    assert float(np.abs(f_fast - f_plain).mean()) < 2e-3

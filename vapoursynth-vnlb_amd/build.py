@@ -18,7 +18,7 @@ CSRC = PACKAGE_DIR / "csrc"
 BUILD = PACKAGE_DIR / "_build"
 LIBRARY = BUILD / "libvnlbhip.so"
 
-HIP_SOURCES = ["match.hip", "match_fast.hip", "paste.hip", "prepare.hip", "model.hip", "filter.hip", "aggregate.hip"]
+HIP_SOURCES = ["match.hip", "match_fast.hip", "paste.hip", "prepare.hip", "model.hip", "filter.hip", "filter_fast.hip", "aggregate.hip"]
 HOST_SOURCES = ["stage.cpp"]
 
 # -ffp-contract=off: only explicit fmaf calls fuse, so the summation order
@@ -38,6 +38,10 @@ DEVICE_FLAGS = [
     "-target-feature",
     "-Xclang",
     "-packed-fp32-ops",
+    # MFMA accumulators in ordinary VGPRs: loop-carried tiles are not copied
+    # between AGPRs and VGPRs around every MFMA.
+    "-mllvm",
+    "-amdgpu-mfma-vgpr-form=1",
 ]
 
 

@@ -16,13 +16,27 @@ __device__ inline float video_at(const float* __restrict__ video,
                  x];
 }
 
+// Value of the lane kShift places further along the same 16-lane row
+// (DPP row_ror: a VALU operand modifier, no LDS traffic).
+template <int kShift>
+__device__ inline float row_rotate(float v) {
+    static_assert(kShift >= 1 && kShift <= 15, "row_ror takes 1..15");
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x120 + kShift, 0xF, 0xF,
+                                           false));
+}
+
 // Halving-tree sum over a 16-lane segment (the ReduceSum order of the CPU
-// restatement); every lane of the segment ends with the total.
+// restatement); every lane of the segment ends with the total.  Level h adds
+// lane l and lane l + 16 / 2^h of the previous level: after level h the
+// values repeat with period 16 / 2^h, so the rotation by that period pairs
+// lane l with lane l XOR (16 / 2^h) of the halving tree, and float addition
+// is commutative, so every lane holds exactly the tree's sums.
 __device__ inline float segment16_sum(float v) {
-    v = v + __shfl_xor(v, 8, 16);
-    v = v + __shfl_xor(v, 4, 16);
-    v = v + __shfl_xor(v, 2, 16);
-    v = v + __shfl_xor(v, 1, 16);
+    v = v + row_rotate<8>(v);
+    v = v + row_rotate<4>(v);
+    v = v + row_rotate<2>(v);
+    v = v + row_rotate<1>(v);
     return v;
 }
 

@@ -28,9 +28,6 @@
 namespace vnlbhip {
 namespace {
 
-constexpr int kMetaValid = 1 << 17;
-constexpr int kMetaFused = 1 << 16;
-
 // LDS layout of the filter kernel in 4-byte words (host computed).
 struct FilterLayout {
     int pitch;       // words per lane slot row (>= template slots, 16-byte multiple)
@@ -93,10 +90,6 @@ FilterLayout filter_layout(const StageDesc& d) {
     L.words = at;
     return L;
 }
-
-struct Accumulators {
-    double* data;  // [frames][channels + 1][height][width]
-};
 
 // Loads a lane's slots of an LDS row (slot layout) into registers with
 // 128-bit reads; bt receives the tail slot.
@@ -186,41 +179,10 @@ __device__ inline void load_centered(float (&x)[kS], float& xt, const float* __r
     }
 }
 
-// One weighted contribution (write_sample_contributions_coupled*,
-// core.cpp:1635-1834): value and weight into the accumulator cell, the
-// weight only once per pixel (channel 0).  Samples whose frame falls outside
-// the anchor's slot range are dropped.
-__device__ inline void scatter(const StageDesc& d, double* __restrict__ cell_base, int slot_base,
-                               int meta, int acc_off, float win, float value, float w) {
-    if ((meta & kMetaValid) == 0) {
-        return;
-    }
-    const int t = meta & 0xFF;
-    const int c = (meta >> 8) & 0xFF;
-    const int slot = slot_base + t;
-    if (slot < 0 || slot >= d.slots) {
-        return;
-    }
-    const float cw = w * win;
-    double addend;
-    if ((meta & kMetaFused) != 0) {
-        addend = static_cast<double>(w) * static_cast<double>(value);
-    } else {
-        const float product = cw * value;
-        addend = static_cast<double>(product);
-    }
-    double* cell = cell_base + acc_off;
-    unsafeAtomicAdd(cell, addend);
-    if (c == 0) {
-        const size_t plane = static_cast<size_t>(d.width) * d.height;
-        unsafeAtomicAdd(cell + static_cast<size_t>(d.channels) * plane, static_cast<double>(cw));
-    }
-}
-
 // kExact: the model row is exactly 16 * kS elements (no tail, no slot
 // guards at run time).
 template <int kS, bool kExact>
-__global__ __launch_bounds__(kModelThreads) void filter_kernel(
+__global__ __launch_bounds__(kModelThreads, kS <= 24 ? 3 : 1) void filter_kernel(
     StageDesc d, FilterLayout L, const float* __restrict__ noisy,
     const float* __restrict__ basic, GroupRefs refs, int group_count, unsigned group_stride,
     int problem_stride, const int* __restrict__ group_flags, const float* __restrict__ means,

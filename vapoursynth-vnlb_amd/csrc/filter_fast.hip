@@ -1,0 +1,501 @@
+// Filter stage, MFMA form, for dual-PCA groups: model rows B <= 16 below a
+// model dimension that is a multiple of 32, 8x8 patches, one model (coupled
+// channels or a single plane), at most 64 retained samples.
+//
+// With the centered model rows Xc (B x D), the Gram G = Xc Xc^T / B, its
+// eigenpairs (lambda_j, u_j) and the dual basis
+//     b_j = Xc^T u_j / sqrt(B lambda_j)
+// (map_dual_eigenvectors_to_basis, linalg.cpp:552-595, normalises
+// Xc^T u_j / sqrt(lambda_j), whose norm is sqrt(B)), every quantity of the
+// filter follows from the cross-Gram C = Y Xc^T of the rows Y that are
+// projected:
+//     projections  p_rj = sum_s C[r][s] u_j[s] / sqrt(B lambda_j)
+//     estimate     y^_r = center + sum_s W[r][s] Xc[s],
+//                  W[r][s] = sum_j p_rj coef_j u_j[s] / sqrt(B lambda_j)
+// so no basis vector of the model dimension is formed.  C is a
+// 16 x 16 x D product on the matrix cores in bf16x3 (hi*hi + hi*lo + lo*hi,
+// fp32 accumulation), its operands read straight from the clip, one 8-sample
+// patch row per lane.  The weights, coefficients and scatter are those of
+// the exact filter (filter.hip).  One wavefront owns one group.
+#include "common.h"
+#include "device_util.h"
+#include "group.h"
+#include "kernels.h"
+
+#include <float.h>
+#include <hip/hip_bf16.h>
+
+namespace vnlbhip {
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
+
+constexpr int kWaves = kModelThreads / 64;
+constexpr int kFastRows = 64;   // retained samples per group
+constexpr int kFastModel = 16;  // model rows and rank
+
+// Per-wavefront LDS region (4-byte words).
+struct WaveLayout {
+    int center;  // [D] centre of the estimate rows
+    int xmean;   // [D] mean of the model (and member) rows
+    int gram;    // [kFastRows][16] cross-Gram, then W
+    int norm;    // [kFastRows] squared norms of the member rows
+    int u;       // [16][16] eigenvectors u_j[s]
+    int pos;     // [kFastRows]
+    int w;       // [kFastRows] aggregation weights
+    int words;
+};
+
+__host__ __device__ inline WaveLayout wave_layout(int dim) {
+    WaveLayout L{};
+    int at = 0;
+    L.center = at;
+    at += (dim + 3) / 4 * 4;
+    L.xmean = at;
+    at += (dim + 3) / 4 * 4;
+    L.gram = at;
+    at += kFastRows * 16;
+    L.norm = at;
+    at += kFastRows;
+    L.u = at;
+    at += 16 * 16;
+    L.pos = at;
+    at += kFastRows;
+    L.w = at;
+    at += kFastRows;
+    L.words = at;
+    return L;
+}
+
+__host__ __device__ inline int table_words(int dim) {
+    return 4 * ((dim + 3) / 4 * 4);
+}
+
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// C[r][s] = <rows_r - rows_mean, model_s - xmean> for r < nrows, s < B, and
+// norm[r] = |rows_r - rows_mean|^2, into LDS.  rows and model share the
+// sample positions s_pos (model rows are the first B samples).
+__device__ inline void cross_gram(const StageDesc& d, const float* __restrict__ rows_video,
+                                  const float* rows_mean, const float* __restrict__ model_video,
+                                  const float* xmean, const uint32_t* s_pos, int nrows, int B,
+                                  const int* tab_in, float* s_gram, float* s_norm, int lane) {
+    const int md = d.model_dim;
+    const int kb = lane >> 4;
+    const int col = lane & 15;
+    const size_t model_base = col < B ? patch_base(d, s_pos[col]) : 0;
+    for (int mt = 0; mt * 16 < nrows; ++mt) {
+        const int r = 16 * mt + col;
+        const bool row_ok = r < nrows;
+        const size_t row_base = row_ok ? patch_base(d, s_pos[r]) : 0;
+        floatx4 acc = {0.0F, 0.0F, 0.0F, 0.0F};
+        float norm = 0.0F;
+        for (int q = 0; q * 32 < md; ++q) {
+            const int e0 = 32 * q + 8 * kb;
+            const int off = tab_in[e0];  // 8 samples of one patch row from here
+            bf16x8 a_hi;
+            bf16x8 a_lo;
+            bf16x8 b_hi;
+            bf16x8 b_lo;
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                float a = 0.0F;
+                if (row_ok) {
+                    a = rows_video[row_base + off + v] - rows_mean[e0 + v];
+                }
+                norm = fmaf(a, a, norm);
+                float b = 0.0F;
+                if (col < B) {
+                    b = model_video[model_base + off + v] - xmean[e0 + v];
+                }
+                const __bf16 ah = static_cast<__bf16>(a);
+                const __bf16 bh = static_cast<__bf16>(b);
+                a_hi[v] = ah;
+                a_lo[v] = static_cast<__bf16>(a - static_cast<float>(ah));
+                b_hi[v] = bh;
+                b_lo[v] = static_cast<__bf16>(b - static_cast<float>(bh));
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, b_hi, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, b_lo, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_lo, b_hi, acc, 0, 0, 0);
+        }
+        // acc[v] = C[16 mt + 4 kb + v][col]
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int rr = 16 * mt + 4 * kb + v;
+            if (rr < nrows) {
+                s_gram[rr * 16 + col] = acc[v];
+            }
+        }
+        norm = norm + __shfl_xor(norm, 16, 64);
+        norm = norm + __shfl_xor(norm, 32, 64);
+        if (kb == 0 && row_ok) {
+            s_norm[r] = norm;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kModelThreads) void filter_fast_kernel(
+    StageDesc d, const float* __restrict__ noisy, const float* __restrict__ basic, GroupRefs refs,
+    int group_count, int problem_stride, const int* __restrict__ group_flags,
+    const float* __restrict__ means, const float* __restrict__ eigenvalues,
+    const float* __restrict__ vectors, Accumulators acc, float* __restrict__ weights_out) {
+    extern __shared__ float4 s_dyn4[];
+    float* s_dyn = reinterpret_cast<float*>(s_dyn4);
+    const int dim = d.dim;  // == model dimension (one model)
+    const int twords = table_words(dim);
+    const int stride4 = (dim + 3) / 4 * 4;
+    int* tab_in = reinterpret_cast<int*>(s_dyn);
+    int* tab_acc = tab_in + stride4;
+    int* tab_meta = tab_acc + stride4;
+    float* tab_win = reinterpret_cast<float*>(tab_meta + stride4);
+    const WaveLayout W = wave_layout(dim);
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63;
+    float* s_wave = s_dyn + twords + wave * W.words;
+    float* s_center = s_wave + W.center;
+    float* s_xmean = s_wave + W.xmean;
+    float* s_gram = s_wave + W.gram;
+    float* s_norm = s_wave + W.norm;
+    float* s_u = s_wave + W.u;
+    uint32_t* s_pos = reinterpret_cast<uint32_t*>(s_wave + W.pos);
+    float* s_w = s_wave + W.w;
+
+    for (int e = tid; e < dim; e += kModelThreads) {
+        const int c = e / d.patch_dim;
+        const int wi = e - c * d.patch_dim;
+        const int t = wi / d.patch_area;
+        const int yx = wi - t * d.patch_area;
+        const int y = yx / d.patch;
+        const int x = yx - y * d.patch;
+        tab_in[e] = ((t * d.channels + c) * d.height + y) * d.width + x;
+        tab_acc[e] = ((t * (d.channels + 1) + c) * d.height + y) * d.width + x;
+        tab_meta[e] = t | (c << 8) | (x < d.fused_width ? kMetaFused : 0) | kMetaValid;
+        tab_win[e] = d.window[yx];
+    }
+    __syncthreads();
+
+    const bool final_stage = d.stage == kFinal;
+    const int md = d.model_dim;
+    const int R = d.rank;
+    const float sigma = d.sigma;
+    const float beta = d.beta;
+    const float sigma2 = (beta * sigma) * sigma;
+    const float model_noise2 = final_stage ? (beta * d.sigma_basic) * d.sigma_basic : sigma2;
+    const float estimate_noise2 = sigma2;
+    const float match_sigma = final_stage ? d.sigma_basic : sigma;
+    const float match_noise2 = match_sigma * match_sigma;
+    const float floor_noise2 = (d.membership_noise_floor * sigma) * sigma;
+    const float membership_noise2 = match_noise2 > floor_noise2 ? match_noise2 : floor_noise2;
+    const float dual_floor = 128.0F * FLT_EPSILON;
+    const float* model_video = final_stage ? basic : noisy;
+
+    const int waves_total = static_cast<int>(gridDim.x) * kWaves;
+    for (int g = static_cast<int>(blockIdx.x) * kWaves + wave; g < group_count; g += waves_total) {
+        wave_sync();  // the previous group is done with this wave's LDS
+        const int a = refs.group_list[g];
+        const int af = position_frame(refs.anchors[a]);
+        const int similar = refs.counts[a];
+        const size_t list = static_cast<size_t>(a) * d.cap;
+        const int flags = group_flags[g];
+        const bool equal = (flags & kFlagEqual) != 0;
+        const bool flat = (flags & kFlagFlat) != 0;
+        const bool modelled = (flags & kFlagModelled) != 0;
+        const int B = model_rows_of(d, similar);
+        const bool fit = modelled && !equal;
+
+        if (lane < similar) {
+            s_pos[lane] = refs.positions[list + lane];
+        }
+        const float* g_mean = means + static_cast<size_t>(g) * 2 * dim;
+        for (int e = lane; e < dim; e += 64) {
+            const float mn = g_mean[e];
+            const float mb = final_stage ? g_mean[dim + e] : 0.0F;
+            s_center[e] = (final_stage && flat) ? mb : mn;
+            s_xmean[e] = final_stage ? mb : mn;
+        }
+        // Per component j (lane j): eigenpair terms.
+        float lambda = 0.0F;
+        float inv = 0.0F;
+        float coef = 0.0F;
+        float lhat = 0.0F;
+        float trace_term = 0.0F;
+        float volume_term = 0.0F;
+        if (fit && lane < R) {
+            lambda = eigenvalues[static_cast<size_t>(g) * R + lane];
+            if (!(lambda <= dual_floor)) {
+                inv = 1.0F / sqrtf(static_cast<float>(B) * lambda);
+            }
+            const float sigma_basic2 = final_stage ? (beta * d.sigma_basic) * d.sigma_basic : sigma2;
+            float variance = lambda;
+            if (sigma_basic2 > 0.0F) {
+                variance = variance - fminf(variance, sigma_basic2);
+            }
+            coef = variance > d.variance_threshold * sigma2 ? 1.0F / (1.0F + (sigma2 / variance)) : 0.0F;
+            const float lhat_raw = lambda - model_noise2;
+            lhat = lhat_raw > 0.0F ? lhat_raw : 0.0F;
+            trace_term = lhat > 0.0F ? lhat / (lhat + estimate_noise2) : 0.0F;
+            volume_term = lhat > 0.0F ? log1p_via_double(lhat / membership_noise2) : 0.0F;
+        }
+        if (fit) {
+            for (int i = lane; i < R * B; i += 64) {
+                const int j = i / B;
+                const int s = i - j * B;
+                s_u[j * 16 + s] = vectors[(static_cast<size_t>(g) * R + j) * problem_stride + s];
+            }
+        }
+        // accumulate_aggregation_model (core.cpp:326-412): group terms, in
+        // component order.
+        float log_group = 0.0F;
+        float volume_md = 0.0F;
+        if (!equal) {
+            const int rank_used = modelled ? R : 0;
+            float trace = 0.0F;
+            float volume = 0.0F;
+            for (int j = 0; j < rank_used; ++j) {
+                const float lh = __shfl(lhat, j, 64);
+                const float tt = __shfl(trace_term, j, 64);
+                const float vt = __shfl(volume_term, j, 64);
+                if (lh > 0.0F) {
+                    trace = trace + tt;
+                    volume = volume + vt;
+                }
+            }
+            const float mean_term = static_cast<float>(md) / static_cast<float>(B);
+            const float tau = (d.weight_epsilon + mean_term) + trace;
+            log_group = d.weight_alpha == 0.0F ? 0.0F : -d.weight_alpha * log_via_double(tau);
+            volume_md = volume / static_cast<float>(md);
+        }
+        wave_sync();
+
+        // Aggregation weights (finalize_aggregation_weights core.cpp:276-294).
+        const bool weights_need_scores = !equal && d.weight_beta != 0.0F;
+        const bool estimate_is_member = !final_stage;  // Basic: same rows, same centre
+        float p_est[kFastModel];
+#pragma unroll
+        for (int j = 0; j < kFastModel; ++j) {
+            p_est[j] = 0.0F;
+        }
+        bool have_estimate = false;
+        float logw = 0.0F;
+        int model_count = 0;
+        if (!equal) {
+            if (d.weight_alpha == 0.0F && d.weight_beta == 0.0F) {
+                model_count = 1;
+            } else if (!weights_need_scores) {
+                logw = log_group;
+                model_count = 1;
+            } else {
+                const int rank_used = modelled ? R : 0;
+                cross_gram(d, model_video, s_xmean, model_video, s_xmean, s_pos, similar, B, tab_in,
+                           s_gram, s_norm, lane);
+                wave_sync();
+                float score = INFINITY;
+                if (lane < similar) {
+                    float distance = 0.0F;
+                    float projected = 0.0F;
+                    for (int j = 0; j < rank_used; ++j) {
+                        const float inv_j = __builtin_bit_cast(float, __builtin_amdgcn_readlane(
+                                                                          __builtin_bit_cast(int, inv), j));
+                        const float lh = __builtin_bit_cast(float, __builtin_amdgcn_readlane(
+                                                                       __builtin_bit_cast(int, lhat), j));
+                        float p = 0.0F;
+                        if (inv_j != 0.0F) {
+                            for (int s = 0; s < B; ++s) {
+                                p = fmaf(s_gram[lane * 16 + s], s_u[j * 16 + s], p);
+                            }
+                            p = p * inv_j;
+                        }
+                        if (estimate_is_member && j < kFastModel) {
+#pragma unroll
+                            for (int jj = 0; jj < kFastModel; ++jj) {
+                                if (jj == j) {
+                                    p_est[jj] = p;
+                                }
+                            }
+                        }
+                        const float p2 = p * p;
+                        distance = distance + p2 / (lh + membership_noise2);
+                        projected = projected + p2;
+                    }
+                    const float norm2 = s_norm[lane];
+                    const float residual = norm2 - projected > 0.0F ? norm2 - projected : 0.0F;
+                    distance = distance + residual / membership_noise2;
+                    score = (distance - static_cast<float>(md)) / sqrtf(2.0F * static_cast<float>(md));
+                }
+                have_estimate = estimate_is_member && fit;
+                const float min_score = wave_min(score);
+                const float excess = score - min_score;
+                const float membership = (excess > 0.0F ? excess : 0.0F) + volume_md;
+                logw = log_group - (d.weight_beta * membership);
+                model_count = 1;
+            }
+        }
+        float w = 1.0F;
+        if (d.weights_enabled) {
+            if (model_count == 0) {
+                const float tau = d.weight_epsilon + (static_cast<float>(md) / static_cast<float>(B));
+                logw = d.weight_alpha == 0.0F ? 0.0F : -d.weight_alpha * log_via_double(tau);
+                model_count = 1;
+            }
+            float value = logw / static_cast<float>(model_count);
+            value = value < -80.0F ? -80.0F : (value > 80.0F ? 80.0F : value);
+            w = exp_via_double(value);
+        }
+        if (lane < similar) {
+            s_w[lane] = w;
+            if (weights_out != nullptr) {
+                weights_out[list + lane] = w;
+            }
+        }
+
+        // Projections of the estimate rows and W = P diag(coef / sqrt(B lambda)) U^T.
+        if (fit) {
+            if (!have_estimate) {
+                wave_sync();
+                cross_gram(d, noisy, s_center, model_video, s_xmean, s_pos, similar, B, tab_in, s_gram,
+                           s_norm, lane);
+                wave_sync();
+                if (lane < similar) {
+#pragma unroll
+                    for (int j = 0; j < kFastModel; ++j) {
+                        const float inv_j = __builtin_bit_cast(float, __builtin_amdgcn_readlane(
+                                                                          __builtin_bit_cast(int, inv), j));
+                        float p = 0.0F;
+                        if (j < R && inv_j != 0.0F) {
+                            for (int s = 0; s < B; ++s) {
+                                p = fmaf(s_gram[lane * 16 + s], s_u[j * 16 + s], p);
+                            }
+                            p = p * inv_j;
+                        }
+                        p_est[j] = p;
+                    }
+                }
+            }
+            wave_sync();
+            if (lane < similar) {
+                float wrow[kFastModel];
+#pragma unroll
+                for (int s = 0; s < kFastModel; ++s) {
+                    wrow[s] = 0.0F;
+                }
+#pragma unroll
+                for (int j = 0; j < kFastModel; ++j) {
+                    const float scale_j = __builtin_bit_cast(
+                        float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, coef * inv), j));
+                    if (j < R && scale_j != 0.0F) {
+                        const float pj = p_est[j] * scale_j;
+#pragma unroll
+                        for (int s = 0; s < kFastModel; ++s) {
+                            wrow[s] = fmaf(pj, s_u[j * 16 + s], wrow[s]);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int s = 0; s < kFastModel; ++s) {
+                    s_gram[lane * 16 + s] = s < B ? wrow[s] : 0.0F;
+                }
+            }
+        }
+        wave_sync();
+
+        // Estimate and weighted scatter, one element column per lane.
+        for (int e = lane; e < md; e += 64) {
+            const int in_off = tab_in[e];
+            const int acc_off = tab_acc[e];
+            const int meta = tab_meta[e];
+            const float win = tab_win[e];
+            const float center = s_center[e];
+            float x[kFastModel];
+            if (fit) {
+                const float mean = s_xmean[e];
+#pragma unroll
+                for (int s = 0; s < kFastModel; ++s) {
+                    x[s] = s < B ? model_video[patch_base(d, s_pos[s]) + in_off] - mean : 0.0F;
+                }
+            }
+            for (int r = 0; r < similar; ++r) {
+                const uint32_t p = s_pos[r];
+                const int pf = position_frame(p);
+                double* cell_base =
+                    acc.data + (static_cast<size_t>(pf) * (d.channels + 1) * d.height + position_y(p)) * d.width +
+                    position_x(p);
+                float value;
+                if (equal) {
+                    value = noisy[patch_base(d, p) + in_off];
+                } else if (!modelled) {
+                    value = center;
+                } else {
+                    const float4* wr = reinterpret_cast<const float4*>(s_gram + r * 16);
+                    float sum = 0.0F;
+#pragma unroll
+                    for (int q = 0; q < kFastModel / 4; ++q) {
+                        const float4 wq = wr[q];
+                        sum = fmaf(wq.x, x[4 * q + 0], sum);
+                        sum = fmaf(wq.y, x[4 * q + 1], sum);
+                        sum = fmaf(wq.z, x[4 * q + 2], sum);
+                        sum = fmaf(wq.w, x[4 * q + 3], sum);
+                    }
+                    value = center + sum;
+                }
+                scatter(d, cell_base, pf - af + d.search_bwd, meta, acc_off, win, value, s_w[r]);
+            }
+        }
+    }
+}
+
+}  // namespace
+
+bool fast_filter_supported(const StageDesc& d) {
+    return d.use_dual != 0 && d.models == 1 && d.max_rows <= kFastModel && d.rank <= kFastModel &&
+           d.patch == 8 && d.model_dim % 32 == 0 && d.cap <= kFastRows &&
+           fast_filter_shared_bytes(d) <= 96 * 1024;
+}
+
+size_t fast_filter_shared_bytes(const StageDesc& d) {
+    return (static_cast<size_t>(table_words(d.dim)) + static_cast<size_t>(kWaves) * wave_layout(d.dim).words) *
+           sizeof(float);
+}
+
+hipError_t launch_fast_filter(const StageDesc& d, const float* noisy, const float* basic,
+                              const uint32_t* anchors, const int* counts, const uint32_t* positions,
+                              const int* group_list, int group_count, int problem_stride,
+                              const int* group_flags, const float* means, const float* eigenvalues,
+                              const float* vectors, double* accumulators, float* weights_out,
+                              hipStream_t stream) {
+    if (group_count <= 0) {
+        return hipSuccess;
+    }
+    const GroupRefs refs{anchors, counts, positions, group_list};
+    const Accumulators acc{accumulators};
+    const size_t shared = fast_filter_shared_bytes(d);
+    int device = 0;
+    int units = 0;
+    hipError_t status = hipGetDevice(&device);
+    if (status == hipSuccess) {
+        status = hipDeviceGetAttribute(&units, hipDeviceAttributeMultiprocessorCount, device);
+    }
+    int per_unit = 0;
+    if (status == hipSuccess) {
+        status = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_unit, filter_fast_kernel, kModelThreads,
+                                                              shared);
+    }
+    if (status != hipSuccess) {
+        return status;
+    }
+    const long long resident = static_cast<long long>(per_unit > 0 ? per_unit : 1) * units;
+    const long long wanted = (group_count + kWaves - 1) / kWaves;
+    const unsigned blocks = static_cast<unsigned>(resident < wanted ? resident : wanted);
+    hipLaunchKernelGGL(filter_fast_kernel, dim3(blocks), dim3(kModelThreads), shared, stream, d, noisy,
+                       basic, refs, group_count, problem_stride, group_flags, means, eigenvalues, vectors,
+                       acc, weights_out);
+    return hipGetLastError();
+}
+
+}  // namespace vnlbhip

@@ -67,4 +67,43 @@ __host__ __device__ inline int slot_pitch(int model_dim) {
     return slots + (((slots / 4) % 2 == 0) ? 4 : 0);
 }
 
+// Scatter metadata of a model-row element: t | c << 8 | fused | valid.
+constexpr int kMetaValid = 1 << 17;
+constexpr int kMetaFused = 1 << 16;
+
+struct Accumulators {
+    double* data;  // [frames][channels + 1][height][width]
+};
+
+// One weighted contribution (write_sample_contributions_coupled*,
+// core.cpp:1635-1834): value and weight into the accumulator cell, the
+// weight only once per pixel (channel 0).  Samples whose frame falls outside
+// the anchor's slot range are dropped.
+__device__ inline void scatter(const StageDesc& d, double* __restrict__ cell_base, int slot_base,
+                               int meta, int acc_off, float win, float value, float w) {
+    if ((meta & kMetaValid) == 0) {
+        return;
+    }
+    const int t = meta & 0xFF;
+    const int c = (meta >> 8) & 0xFF;
+    const int slot = slot_base + t;
+    if (slot < 0 || slot >= d.slots) {
+        return;
+    }
+    const float cw = w * win;
+    double addend;
+    if ((meta & kMetaFused) != 0) {
+        addend = static_cast<double>(w) * static_cast<double>(value);
+    } else {
+        const float product = cw * value;
+        addend = static_cast<double>(product);
+    }
+    double* cell = cell_base + acc_off;
+    unsafeAtomicAdd(cell, addend);
+    if (c == 0) {
+        const size_t plane = static_cast<size_t>(d.width) * d.height;
+        unsafeAtomicAdd(cell + static_cast<size_t>(d.channels) * plane, static_cast<double>(cw));
+    }
+}
+
 }  // namespace vnlbhip

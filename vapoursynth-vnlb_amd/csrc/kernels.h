@@ -58,6 +58,17 @@ hipError_t launch_filter(const StageDesc& d, const float* noisy, const float* ba
                          const float* eigenvalues, const float* vectors, double* accumulators,
                          float* weights_out, hipStream_t stream);
 
+// MFMA form of the filter (filter_fast.hip): dual-PCA groups of one model,
+// 8x8 patches, at most 16 model rows and components, at most 64 samples.
+bool fast_filter_supported(const StageDesc& d);
+size_t fast_filter_shared_bytes(const StageDesc& d);
+hipError_t launch_fast_filter(const StageDesc& d, const float* noisy, const float* basic,
+                              const uint32_t* anchors, const int* counts, const uint32_t* positions,
+                              const int* group_list, int group_count, int problem_stride,
+                              const int* group_flags, const float* means, const float* eigenvalues,
+                              const float* vectors, double* accumulators, float* weights_out,
+                              hipStream_t stream);
+
 hipError_t launch_fill_zero(double* data, size_t count, hipStream_t stream);
 hipError_t launch_normalize(const StageDesc& d, const double* acc, const float* source, float* out,
                             hipStream_t stream);

@@ -38,6 +38,8 @@ constexpr int kFastWaves = kFastThreads / 64;
 constexpr int kFastPatch = 8;
 constexpr int kTileAnchors = 16;
 constexpr int kCopies = 4;
+constexpr int kFillPerThread = 3;          // fill items (4 output columns) per thread
+constexpr int kFillSpan = 4 + kCopies - 1;  // samples behind one item
 
 // Per-channel offset: the mean of a strided sample of the channel.
 __global__ __launch_bounds__(256) void channel_offset_kernel(const float* __restrict__ source,
@@ -142,6 +144,14 @@ struct FastShape {
     int per_frame;     // window^2
 };
 
+__device__ inline uint2 pack4(__bf16 a, __bf16 b, __bf16 c, __bf16 d) {
+    const unsigned lo = static_cast<unsigned>(__builtin_bit_cast(unsigned short, a)) |
+                        (static_cast<unsigned>(__builtin_bit_cast(unsigned short, b)) << 16);
+    const unsigned hi = static_cast<unsigned>(__builtin_bit_cast(unsigned short, c)) |
+                        (static_cast<unsigned>(__builtin_bit_cast(unsigned short, d)) << 16);
+    return uint2{lo, hi};
+}
+
 __device__ inline bf16x8 load_bf16x8(const __bf16* p) {
     // 8-byte aligned: two 64-bit LDS reads.
     const uint2* q = reinterpret_cast<const uint2*>(p);
@@ -159,8 +169,9 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
     const float* __restrict__ energy, const uint32_t* __restrict__ anchors,
     const int* __restrict__ tiles, int anchor_base, float* __restrict__ out) {
     extern __shared__ uint4 s_dyn16[];
-    __bf16* s_copy = reinterpret_cast<__bf16*>(s_dyn16);  // [hi/lo][copy][rows][pitch]
-    float* s_stage = reinterpret_cast<float*>(s_dyn16);   // epilogue staging (aliases the copies)
+    __bf16* s_copy = reinterpret_cast<__bf16*>(s_dyn16);  // [buffer][hi/lo][copy][rows][pitch]
+    // Epilogue staging after the two plane buffers.
+    float* s_stage = reinterpret_cast<float*>(s_copy + static_cast<size_t>(4) * kCopies * S.copy_elems);
     __shared__ int s_index[kTileAnchors];
     __shared__ int s_ax[kTileAnchors];
     __shared__ int s_ay[kTileAnchors];
@@ -248,135 +259,190 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
     const int my_ax = my_anchor >= 0 ? s_ax[n_lane] : 0;
     const int my_ay = my_anchor >= 0 ? s_ay[n_lane] : 0;
 
-    for (int ti = 0; ti < nt; ++ti) {
-        const int frame = scheduled_frame(af, t_low, t_high, ti);
-        floatx4 acc[kTilesPerWave];
+    // Planes (candidate frame ti, channel c, patch frame t) in order, the
+    // region of plane p + 1 loaded into registers while plane p is on the
+    // matrix cores, then stored into the other LDS buffer.
+    const int planes_per_frame = match_channels * d.patch_time;
+    const int total_planes = nt * planes_per_frame;
+    const int groups4 = S.pitch / 4;
+    const int fill_items = S.region_rows * groups4;  // <= kFillPerThread * threads (host checked)
+    float pre[kFillPerThread][kFillSpan];
+
+    const auto plane_of = [&](int p, int& frame, int& c, int& t) {
+        const int ti = p / planes_per_frame;
+        const int rem = p - ti * planes_per_frame;
+        c = rem / d.patch_time;
+        t = rem - c * d.patch_time;
+        frame = scheduled_frame(af, t_low, t_high, ti);
+    };
+    const auto prefetch = [&](int p) {
+        int frame = 0;
+        int c = 0;
+        int t = 0;
+        plane_of(p, frame, c, t);
+        const float mu = offsets[c];
+        const float* src = source + (static_cast<size_t>(frame + t) * d.channels + c) * plane;
 #pragma unroll
-        for (int i = 0; i < kTilesPerWave; ++i) {
-            acc[i] = floatx4{0.0F, 0.0F, 0.0F, 0.0F};
+        for (int k = 0; k < kFillPerThread; ++k) {
+            const int item = tid + k * kFastThreads;
+            const int ry = item / groups4;
+            const int x0 = ux0 + 4 * (item - ry * groups4);
+            const int y = uy0 + ry;
+            const bool row_ok = item < fill_items && y < d.height;
+#pragma unroll
+            for (int v = 0; v < kFillSpan; ++v) {
+                float value = 0.0F;
+                if (row_ok && x0 + v < d.width) {
+                    value = src[static_cast<size_t>(y) * d.width + x0 + v] - mu;
+                }
+                pre[k][v] = value;
+            }
         }
-        for (int c = 0; c < match_channels; ++c) {
-            const float mu = offsets[c];
-            for (int t = 0; t < d.patch_time; ++t) {
-                __syncthreads();
-                // Stage the plane (frame + t, c) of the union region.
-                const float* src = source + (static_cast<size_t>(frame + t) * d.channels + c) * plane;
-                const int fill_cols = S.pitch + kCopies - 1;
-                const int fill = S.region_rows * fill_cols;
-                for (int i = tid; i < fill; i += kFastThreads) {
-                    const int ry = i / fill_cols;
-                    const int rx = i - ry * fill_cols;
-                    const int y = uy0 + ry;
-                    const int x = ux0 + rx;
-                    float v = 0.0F;
-                    if (y < d.height && x < d.width) {
-                        v = src[static_cast<size_t>(y) * d.width + x] - mu;
-                    }
-                    const __bf16 hi = static_cast<__bf16>(v);
-                    const __bf16 lo = static_cast<__bf16>(v - static_cast<float>(hi));
+    };
+    const auto store = [&](int buffer) {
+        __bf16* base = s_copy + static_cast<size_t>(buffer) * 2 * kCopies * S.copy_elems;
 #pragma unroll
-                    for (int q = 0; q < kCopies; ++q) {
-                        const int j = rx - q;
-                        if (j >= 0 && j < S.pitch) {
-                            const int at = q * S.copy_elems + ry * S.pitch + j;
-                            s_copy[at] = hi;
-                            s_copy[kCopies * S.copy_elems + at] = lo;
-                        }
-                    }
+        for (int k = 0; k < kFillPerThread; ++k) {
+            const int item = tid + k * kFastThreads;
+            if (item < fill_items) {
+                const int ry = item / groups4;
+                const int j = 4 * (item - ry * groups4);
+                __bf16 hi[kFillSpan];
+                __bf16 lo[kFillSpan];
+#pragma unroll
+                for (int v = 0; v < kFillSpan; ++v) {
+                    hi[v] = static_cast<__bf16>(pre[k][v]);
+                    lo[v] = static_cast<__bf16>(pre[k][v] - static_cast<float>(hi[v]));
                 }
-                // Anchor operands: anchor n_lane, patch rows 4 ks + kb.
-                bf16x8 b_hi[2];
-                bf16x8 b_lo[2];
-                const float* anchor_plane = source + (static_cast<size_t>(af + t) * d.channels + c) * plane;
 #pragma unroll
-                for (int ks = 0; ks < 2; ++ks) {
-                    const int py = 4 * ks + kb;
-#pragma unroll
-                    for (int v = 0; v < 8; ++v) {
-                        float x = 0.0F;
-                        if (my_anchor >= 0) {
-                            x = anchor_plane[static_cast<size_t>(my_ay + py) * d.width + my_ax + v] - mu;
-                        }
-                        const __bf16 hi = static_cast<__bf16>(x);
-                        b_hi[ks][v] = hi;
-                        b_lo[ks][v] = static_cast<__bf16>(x - static_cast<float>(hi));
-                    }
-                }
-                __syncthreads();
-#pragma unroll
-                for (int i = 0; i < kTilesPerWave; ++i) {
-                    int mt = wave + kFastWaves * i;
-                    // Opaque to loop-invariant motion: the few scalar
-                    // address operations are redone per plane instead of
-                    // holding every tile's addresses in registers.
-                    asm volatile("" : "+s"(mt));
-                    if (mt < mtiles) {
-                        const int col = (mt * tile_magic) >> 16;
-                        const int rt = mt - col * row_tiles;
-                        const int shift = col & (kCopies - 1);
-                        const int j = col - shift;
-#pragma unroll
-                        for (int ks = 0; ks < 2; ++ks) {
-                            const int ry = 16 * rt + n_lane + 4 * ks + kb;
-                            const int at = shift * S.copy_elems + ry * S.pitch + j;
-                            const bf16x8 a_hi = load_bf16x8(s_copy + at);
-                            const bf16x8 a_lo = load_bf16x8(s_copy + kCopies * S.copy_elems + at);
-                            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, b_hi[ks], acc[i], 0, 0, 0);
-                            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, b_lo[ks], acc[i], 0, 0, 0);
-                            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_lo, b_hi[ks], acc[i], 0, 0, 0);
-                        }
-                    }
-                    // Keep one tile's operands live at a time.
-                    __builtin_amdgcn_sched_barrier(0);
+                for (int q = 0; q < kCopies; ++q) {
+                    // copy q holds plane[ry][j + q .. j + q + 3] at columns j .. j + 3
+                    const int at = q * S.copy_elems + ry * S.pitch + j;
+                    *reinterpret_cast<uint2*>(base + at) = pack4(hi[q], hi[q + 1], hi[q + 2], hi[q + 3]);
+                    *reinterpret_cast<uint2*>(base + kCopies * S.copy_elems + at) =
+                        pack4(lo[q], lo[q + 1], lo[q + 2], lo[q + 3]);
                 }
             }
         }
-        // Epilogue: distances of this candidate frame into LDS per anchor
-        // (window scan order), then out to global memory.
-        __syncthreads();
-        const float* frame_energy = energy + static_cast<size_t>(frame) * plane;
+    };
+
+    floatx4 acc[kTilesPerWave];
+    prefetch(0);
+    store(0);
+    __syncthreads();
+    for (int p = 0; p < total_planes; ++p) {
+        int frame = 0;
+        int c = 0;
+        int t = 0;
+        plane_of(p, frame, c, t);
+        if (c == 0 && t == 0) {
+#pragma unroll
+            for (int i = 0; i < kTilesPerWave; ++i) {
+                acc[i] = floatx4{0.0F, 0.0F, 0.0F, 0.0F};
+            }
+        }
+        if (p + 1 < total_planes) {
+            prefetch(p + 1);
+        }
+        // Anchor operands: anchor n_lane, patch rows 4 ks + kb of (af + t, c).
+        const float mu = offsets[c];
+        bf16x8 b_hi[2];
+        bf16x8 b_lo[2];
+        const float* anchor_plane = source + (static_cast<size_t>(af + t) * d.channels + c) * plane;
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            const int py = 4 * ks + kb;
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                float x = 0.0F;
+                if (my_anchor >= 0) {
+                    x = anchor_plane[static_cast<size_t>(my_ay + py) * d.width + my_ax + v] - mu;
+                }
+                const __bf16 hi = static_cast<__bf16>(x);
+                b_hi[ks][v] = hi;
+                b_lo[ks][v] = static_cast<__bf16>(x - static_cast<float>(hi));
+            }
+        }
+        const __bf16* copies = s_copy + static_cast<size_t>(p & 1) * 2 * kCopies * S.copy_elems;
 #pragma unroll
         for (int i = 0; i < kTilesPerWave; ++i) {
             int mt = wave + kFastWaves * i;
+            // Opaque to loop-invariant motion: the few scalar address
+            // operations are redone per plane instead of holding every
+            // tile's addresses in registers.
             asm volatile("" : "+s"(mt));
-            if (mt < mtiles && my_anchor >= 0) {
+            if (mt < mtiles) {
                 const int col = (mt * tile_magic) >> 16;
                 const int rt = mt - col * row_tiles;
-                const int cx = ux0 + col;
-                const int rel_x = cx - s_xl[n_lane];
-                if (rel_x >= 0 && rel_x < s_nx[n_lane]) {
+                const int shift = col & (kCopies - 1);
+                const int j = col - shift;
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const int cy = uy0 + 16 * rt + 4 * kb + v;
-                        const int rel_y = cy - s_yl[n_lane];
-                        if (rel_y >= 0 && rel_y < s_ny[n_lane]) {
-                            float dist;
-                            if (cx == my_ax && cy == my_ay && frame == af) {
-                                dist = 0.0F;
-                            } else {
-                                const float ec = frame_energy[static_cast<size_t>(cy) * d.width + cx];
-                                dist = (s_energy[n_lane] + ec) - 2.0F * acc[i][v];
-                                dist = dist > 0.0F ? dist : 0.0F;
+                for (int ks = 0; ks < 2; ++ks) {
+                    const int ry = 16 * rt + n_lane + 4 * ks + kb;
+                    const int at = shift * S.copy_elems + ry * S.pitch + j;
+                    const bf16x8 a_hi = load_bf16x8(copies + at);
+                    const bf16x8 a_lo = load_bf16x8(copies + kCopies * S.copy_elems + at);
+                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, b_hi[ks], acc[i], 0, 0, 0);
+                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, b_lo[ks], acc[i], 0, 0, 0);
+                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_lo, b_hi[ks], acc[i], 0, 0, 0);
+                }
+            }
+            // Keep one tile's operands live at a time.
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (p + 1 < total_planes) {
+            store((p + 1) & 1);
+        }
+        if (c == match_channels - 1 && t == d.patch_time - 1) {
+            // Epilogue: this candidate frame's distances into LDS per anchor
+            // (window scan order), then out to global memory.
+            const int ti = p / planes_per_frame;
+            const float* frame_energy = energy + static_cast<size_t>(frame) * plane;
+#pragma unroll
+            for (int i = 0; i < kTilesPerWave; ++i) {
+                int mt = wave + kFastWaves * i;
+                asm volatile("" : "+s"(mt));
+                if (mt < mtiles && my_anchor >= 0) {
+                    const int col = (mt * tile_magic) >> 16;
+                    const int rt = mt - col * row_tiles;
+                    const int cx = ux0 + col;
+                    const int rel_x = cx - s_xl[n_lane];
+                    if (rel_x >= 0 && rel_x < s_nx[n_lane]) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const int cy = uy0 + 16 * rt + 4 * kb + v;
+                            const int rel_y = cy - s_yl[n_lane];
+                            if (rel_y >= 0 && rel_y < s_ny[n_lane]) {
+                                float dist;
+                                if (cx == my_ax && cy == my_ay && frame == af) {
+                                    dist = 0.0F;
+                                } else {
+                                    const float ec = frame_energy[static_cast<size_t>(cy) * d.width + cx];
+                                    dist = (s_energy[n_lane] + ec) - 2.0F * acc[i][v];
+                                    dist = dist > 0.0F ? dist : 0.0F;
+                                }
+                                s_stage[n_lane * S.per_frame + rel_y * s_nx[n_lane] + rel_x] = dist;
                             }
-                            s_stage[n_lane * S.per_frame + rel_y * s_nx[n_lane] + rel_x] = dist;
                         }
                     }
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            __syncthreads();
+            for (int n = 0; n < kTileAnchors; ++n) {
+                const int index = s_index[n];
+                if (index < 0) {
+                    continue;
+                }
+                const int count = s_nx[n] * s_ny[n];
+                float* dst = out + static_cast<size_t>(index - anchor_base) * S.stride +
+                             static_cast<size_t>(ti) * count;
+                for (int i = tid; i < count; i += kFastThreads) {
+                    dst[i] = s_stage[n * S.per_frame + i];
                 }
             }
         }
         __syncthreads();
-        for (int n = 0; n < kTileAnchors; ++n) {
-            const int index = s_index[n];
-            if (index < 0) {
-                continue;
-            }
-            const int count = s_nx[n] * s_ny[n];
-            float* dst = out + static_cast<size_t>(index - anchor_base) * S.stride +
-                         static_cast<size_t>(ti) * count;
-            for (int i = tid; i < count; i += kFastThreads) {
-                dst[i] = s_stage[n * S.per_frame + i];
-            }
-        }
     }
 }
 
@@ -457,15 +523,17 @@ bool fast_match_supported(const StageDesc& d, int max_cols, int max_rows) {
         return false;
     }
     const int row_tiles = (max_rows + 15) / 16;
+    const FastShape S = fast_shape(d, max_cols, max_rows);
     return static_cast<long long>(max_cols) * row_tiles <= static_cast<long long>(kMaxTilesPerWave) * kFastWaves &&
-           fast_match_shared_bytes(d, max_cols, max_rows) <= 64 * 1024;
+           S.region_rows * (S.pitch / 4) <= kFillPerThread * kFastThreads &&
+           fast_match_shared_bytes(d, max_cols, max_rows) <= 80 * 1024;
 }
 
 size_t fast_match_shared_bytes(const StageDesc& d, int max_cols, int max_rows) {
     const FastShape S = fast_shape(d, max_cols, max_rows);
-    const size_t copies = static_cast<size_t>(2) * kCopies * S.copy_elems * sizeof(__bf16);
+    const size_t buffers = static_cast<size_t>(4) * kCopies * S.copy_elems * sizeof(__bf16);
     const size_t staging = static_cast<size_t>(kTileAnchors) * S.per_frame * sizeof(float);
-    return copies > staging ? copies : staging;
+    return buffers + staging;
 }
 
 size_t fast_match_stride(const StageDesc& d) {

@@ -71,36 +71,142 @@ __device__ inline uint32_t radix_select(const float* dist, int n, int rank,
     return prefix;
 }
 
-// Selection over s_dist (scan order) into the anchor's output list.
-__device__ inline void select_and_write(const StageDesc& d, const float* s_dist, int n, int af,
-                                 int t_low, int t_high, int x_low, int y_low, int nx,
-                                 int per_frame, int nt, unsigned* s_hist, int* s_int, int* s_scan,
-                                 float* s_sel_dist, int* s_sel_index, int* s_sel_rank, int a,
-                                 size_t out_base, int* __restrict__ counts,
-                                 uint32_t* __restrict__ positions,
-                                 float* __restrict__ distances) {
-    int valid_local = 0;
-    for (int c = threadIdx.x; c < n; c += blockDim.x) {
-        const float v = s_dist[c];
-        valid_local += v == v ? 1 : 0;
+// Short path of the selection.  Split the candidates into G >= k disjoint
+// subsets (the threads' strided elements, G groups of threads): the largest
+// subset minimum T0 bounds the k-th smallest distance, so the candidates
+// with key <= T0 (typically a few dozen) contain the k nearest.  When they
+// fit one per thread and T = max(tau, d_k) <= T0 (so T does not reach past
+// them), the whole selection, cap included, is decided within that list,
+// which stays in scan order.  Returns false, with nothing written, when a
+// condition fails; every decision is block-uniform.
+__device__ inline bool select_short(const StageDesc& d, const float* s_dist, int n, int k,
+                                    int buffer_cap, unsigned* s_u, int* s_int, int* s_scan,
+                                    float* s_sel_dist, int* s_sel_index, int* s_list_index,
+                                    int& selected) {
+    const int tid = threadIdx.x;
+    const int nthreads = blockDim.x;
+    const int groups = k <= 16 ? 16 : (k <= 64 ? 64 : 0);
+    if (groups == 0 || groups > nthreads) {
+        return false;
     }
+    uint32_t m = 0xFFFFFFFFu;
+    for (int c = tid; c < n; c += nthreads) {
+        const uint32_t key = distance_key(s_dist[c]);
+        m = key < m ? key : m;
+    }
+    s_u[tid] = m;
+    if (tid == 0) {
+        s_int[10] = 0;
+        s_int[12] = 0;
+    }
+    __syncthreads();
+    if (tid < 64) {
+        uint32_t gm = 0;
+        if (tid < groups) {
+            const int per = nthreads / groups;
+            gm = 0xFFFFFFFFu;
+            for (int i = 0; i < per; ++i) {
+                const uint32_t v = s_u[tid * per + i];
+                gm = v < gm ? v : gm;
+            }
+        }
+#pragma unroll
+        for (int offset = 32; offset >= 1; offset >>= 1) {
+            const uint32_t other = static_cast<uint32_t>(__shfl_xor(static_cast<int>(gm), offset, 64));
+            gm = other > gm ? other : gm;
+        }
+        if (tid == 0) {
+            s_int[8] = static_cast<int>(gm);
+        }
+    }
+    __syncthreads();
+    const uint32_t t0 = static_cast<uint32_t>(s_int[8]);
+
+    // Candidates with key <= T0, compacted in scan order.
+    const int chunk = (n + nthreads - 1) / nthreads;
+    const int begin = tid * chunk < n ? tid * chunk : n;
+    const int end = begin + chunk < n ? begin + chunk : n;
+    int local = 0;
+    for (int c = begin; c < end; ++c) {
+        local += distance_key(s_dist[c]) <= t0 ? 1 : 0;
+    }
+    int list = 0;
+    int at = block_exclusive_scan(local, list, s_scan);
+    if (list > nthreads) {
+        return false;
+    }
+    for (int c = begin; c < end; ++c) {
+        const uint32_t key = distance_key(s_dist[c]);
+        if (key <= t0) {
+            s_u[at] = key;
+            s_list_index[at] = c;
+            ++at;
+        }
+    }
+    __syncthreads();
+
+    // Rank of each entry by (key, scan position); d_k has rank k - 1.
+    uint32_t key_e = 0xFFFFFFFFu;
+    int rank = 0;
+    if (tid < list) {
+        key_e = s_u[tid];
+        for (int f = 0; f < list; ++f) {
+            const uint32_t kf = s_u[f];
+            rank += (kf < key_e || (kf == key_e && f < tid)) ? 1 : 0;
+        }
+        if (rank == k - 1) {
+            s_int[9] = static_cast<int>(key_e);
+        }
+    }
+    __syncthreads();
+    const float kth = __uint_as_float(static_cast<uint32_t>(s_int[9]));
+    const float threshold = d.tau < kth ? kth : d.tau;
+    const uint32_t threshold_key = distance_key(threshold);
+    if (threshold_key > t0) {
+        return false;
+    }
+    if (tid < list && key_e <= threshold_key) {
+        atomicAdd(&s_int[10], 1);
+    }
+    __syncthreads();
+    const int within = s_int[10];
+    uint32_t limit_key = threshold_key;
+    int need_equal = 0x7FFFFFFF;
+    if (within > buffer_cap) {
+        if (tid < list && rank == buffer_cap - 1) {
+            s_int[11] = static_cast<int>(key_e);
+        }
+        __syncthreads();
+        limit_key = static_cast<uint32_t>(s_int[11]);
+        if (tid < list && key_e < limit_key) {
+            atomicAdd(&s_int[12], 1);
+        }
+        __syncthreads();
+        need_equal = buffer_cap - s_int[12];
+    }
+    const bool equal = tid < list && key_e == limit_key;
+    int equal_total = 0;
+    const int before_equal = block_exclusive_scan(equal ? 1 : 0, equal_total, s_scan);
+    const bool take = tid < list && (key_e < limit_key || (equal && before_equal < need_equal));
+    const int out = block_exclusive_scan(take ? 1 : 0, selected, s_scan);
+    if (take) {
+        s_sel_dist[out] = __uint_as_float(key_e);
+        s_sel_index[out] = s_list_index[tid];
+    }
+    __syncthreads();
+    return true;
+}
+
+// The general path: radix selections over all candidates.  Fills
+// s_sel_dist / s_sel_index in scan order and returns the selected count.
+__device__ inline int select_general(const StageDesc& d, const float* s_dist, int n, int k,
+                                     int buffer_cap, unsigned* s_hist, int* s_int, int* s_scan,
+                                     float* s_sel_dist, int* s_sel_index) {
     if (threadIdx.x == 0) {
-        s_int[2] = 0;
         s_int[3] = 0;
         s_int[4] = 0;
     }
     __syncthreads();
-    atomicAdd(&s_int[2], valid_local);
-    __syncthreads();
-    const int valid = s_int[2];
-    if (valid == 0) {
-        if (threadIdx.x == 0) {
-            counts[a] = 0;
-        }
-        return;
-    }
-    const int k = d.similar < valid ? d.similar : valid;
-
     // T = max(tau, d_K) (select_similar_patches core.cpp:1217-1218).
     const uint32_t kth_key = radix_select(s_dist, n, k, s_hist, s_int);
     const float kth = __uint_as_float(kth_key);
@@ -113,10 +219,6 @@ __device__ inline void select_and_write(const StageDesc& d, const float* s_dist,
     atomicAdd(&s_int[3], within_local);
     __syncthreads();
     const int within = s_int[3];
-
-    // Buffer capacity of find_similar_patches (core.cpp:1279-1287).
-    const long long window_cap = static_cast<long long>(d.search_window) * d.search_window * nt;
-    const int buffer_cap = static_cast<int>(window_cap < d.soft_cap ? window_cap : d.soft_cap);
 
     // Selection predicate: d < limit, or d == limit within the first
     // `need_equal` ties in scan order.
@@ -175,8 +277,17 @@ __device__ inline void select_and_write(const StageDesc& d, const float* s_dist,
         }
     }
     __syncthreads();
+    return selected;
+}
 
-    // Rank by (distance, scan position); the K nearest lead, ascending.
+// Ranks the selected candidates by (distance, scan position), the K nearest
+// first and ascending, the rest in scan order, and writes the anchor's list.
+__device__ inline void write_selected(const float* s_sel_dist, const int* s_sel_index,
+                                      int* s_sel_rank, int selected, int k, int af, int t_low,
+                                      int t_high, int x_low, int y_low, int nx, int per_frame, int a,
+                                      size_t out_base, int* __restrict__ counts,
+                                      uint32_t* __restrict__ positions,
+                                      float* __restrict__ distances) {
     const int k_out = k < selected ? k : selected;
     for (int e = threadIdx.x; e < selected; e += blockDim.x) {
         const float de = s_sel_dist[e];
@@ -211,6 +322,50 @@ __device__ inline void select_and_write(const StageDesc& d, const float* s_dist,
     if (threadIdx.x == 0) {
         counts[a] = selected;
     }
+}
+
+// Selection over s_dist (scan order) into the anchor's output list.
+__device__ inline void select_and_write(const StageDesc& d, const float* s_dist, int n, int af,
+                                 int t_low, int t_high, int x_low, int y_low, int nx,
+                                 int per_frame, int nt, unsigned* s_hist, int* s_int, int* s_scan,
+                                 float* s_sel_dist, int* s_sel_index, int* s_sel_rank, int a,
+                                 size_t out_base, int* __restrict__ counts,
+                                 uint32_t* __restrict__ positions,
+                                 float* __restrict__ distances) {
+    int valid_local = 0;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+        const float v = s_dist[c];
+        valid_local += v == v ? 1 : 0;
+    }
+    if (threadIdx.x == 0) {
+        s_int[2] = 0;
+        s_int[3] = 0;
+        s_int[4] = 0;
+    }
+    __syncthreads();
+    atomicAdd(&s_int[2], valid_local);
+    __syncthreads();
+    const int valid = s_int[2];
+    if (valid == 0) {
+        if (threadIdx.x == 0) {
+            counts[a] = 0;
+        }
+        return;
+    }
+    const int k = d.similar < valid ? d.similar : valid;
+
+    // Buffer capacity of find_similar_patches (core.cpp:1279-1287).
+    const long long window_cap = static_cast<long long>(d.search_window) * d.search_window * nt;
+    const int buffer_cap = static_cast<int>(window_cap < d.soft_cap ? window_cap : d.soft_cap);
+
+    int selected = 0;
+    if (!select_short(d, s_dist, n, k, buffer_cap, s_hist, s_int, s_scan, s_sel_dist, s_sel_index,
+                      s_sel_rank, selected)) {
+        selected = select_general(d, s_dist, n, k, buffer_cap, s_hist, s_int, s_scan, s_sel_dist,
+                                  s_sel_index);
+    }
+    write_selected(s_sel_dist, s_sel_index, s_sel_rank, selected, k, af, t_low, t_high, x_low,
+                   y_low, nx, per_frame, a, out_base, counts, positions, distances);
 }
 
 }  // namespace vnlbhip
