@@ -16,7 +16,11 @@
 // 16 x 16 x D product on the matrix cores in bf16x3 (hi*hi + hi*lo + lo*hi,
 // fp32 accumulation), its operands read straight from the clip, one 8-sample
 // patch row per lane.  The weights, coefficients and scatter are those of
-// the exact filter (filter.hip).  One wavefront owns one group.
+// the exact filter (filter.hip).  One wavefront owns one group; its
+// contributions are combined in the wavefront's LDS tile (one channel and
+// one output frame of the group's window at a time) and reach the
+// accumulators once per touched cell: the retained patches overlap, so this
+// takes about a third of the per-sample atomics.
 #include "common.h"
 #include "device_util.h"
 #include "group.h"
@@ -37,25 +41,26 @@ constexpr int kFastModel = 16;  // model rows and rank
 
 // Per-wavefront LDS region (4-byte words).
 struct WaveLayout {
-    int center;  // [D] centre of the estimate rows
-    int xmean;   // [D] mean of the model (and member) rows
-    int gram;    // [kFastRows][16] cross-Gram, then W
-    int norm;    // [kFastRows] squared norms of the member rows
-    int u;       // [16][16] eigenvectors u_j[s]
-    int pos;     // [kFastRows]
-    int w;       // [kFastRows] aggregation weights
+    int gram;   // [16][kFastRows] C[r][s] at s * kFastRows + r, then W
+    int norm;   // [kFastRows] squared norms of the member rows
+    int u;      // [16][16] eigenvectors u_j[s]
+    int pos;    // [kFastRows]
+    int w;      // [kFastRows] aggregation weights
+    int order;  // [kFastRows] samples sorted by source slot, then the slot starts
+    int tile;   // [side][side][2] value plane and weight plane of one output frame
     int words;
 };
 
-__host__ __device__ inline WaveLayout wave_layout(int dim) {
+// Side of the aggregation tile: the search window plus a patch.
+__host__ __device__ inline int tile_side(const StageDesc& d) {
+    return d.search_window + d.patch - 1;
+}
+
+__host__ __device__ inline WaveLayout wave_layout(const StageDesc& d) {
     WaveLayout L{};
     int at = 0;
-    L.center = at;
-    at += (dim + 3) / 4 * 4;
-    L.xmean = at;
-    at += (dim + 3) / 4 * 4;
     L.gram = at;
-    at += kFastRows * 16;
+    at += 16 * kFastRows;
     L.norm = at;
     at += kFastRows;
     L.u = at;
@@ -64,10 +69,16 @@ __host__ __device__ inline WaveLayout wave_layout(int dim) {
     at += kFastRows;
     L.w = at;
     at += kFastRows;
-    L.words = at;
+    L.order = at;
+    at += (kFastRows + kMaxTemporal + 8 + 3) / 4 * 4;  // slots + patch frames <= kMaxTemporal + 4
+    L.tile = at;
+    at += 2 * tile_side(d) * tile_side(d);
+    L.words = (at + 3) / 4 * 4;
     return L;
 }
 
+// Stage-constant element tables: input offset, meta (t | c << 8), window
+// weight and in-patch position x | y << 8.
 __host__ __device__ inline int table_words(int dim) {
     return 4 * ((dim + 3) / 4 * 4);
 }
@@ -129,7 +140,7 @@ __device__ inline void cross_gram(const StageDesc& d, const float* __restrict__ 
         for (int v = 0; v < 4; ++v) {
             const int rr = 16 * mt + 4 * kb + v;
             if (rr < nrows) {
-                s_gram[rr * 16 + col] = acc[v];
+                s_gram[col * kFastRows + rr] = acc[v];
             }
         }
         norm = norm + __shfl_xor(norm, 16, 64);
@@ -140,6 +151,8 @@ __device__ inline void cross_gram(const StageDesc& d, const float* __restrict__ 
     }
 }
 
+// kPt: frames of a patch (64-element chunks of one channel's model row).
+template <int kPt>
 __global__ __launch_bounds__(kModelThreads) void filter_fast_kernel(
     StageDesc d, const float* __restrict__ noisy, const float* __restrict__ basic, GroupRefs refs,
     int group_count, int problem_stride, const int* __restrict__ group_flags,
@@ -151,21 +164,24 @@ __global__ __launch_bounds__(kModelThreads) void filter_fast_kernel(
     const int twords = table_words(dim);
     const int stride4 = (dim + 3) / 4 * 4;
     int* tab_in = reinterpret_cast<int*>(s_dyn);
-    int* tab_acc = tab_in + stride4;
-    int* tab_meta = tab_acc + stride4;
+    int* tab_meta = tab_in + stride4;
     float* tab_win = reinterpret_cast<float*>(tab_meta + stride4);
-    const WaveLayout W = wave_layout(dim);
+    int* tab_xy = reinterpret_cast<int*>(tab_win + stride4);
+    const WaveLayout W = wave_layout(d);
+    const int side = tile_side(d);
+    const int cells = side * side;
     const int tid = threadIdx.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lane = tid & 63;
     float* s_wave = s_dyn + twords + wave * W.words;
-    float* s_center = s_wave + W.center;
-    float* s_xmean = s_wave + W.xmean;
     float* s_gram = s_wave + W.gram;
     float* s_norm = s_wave + W.norm;
     float* s_u = s_wave + W.u;
     uint32_t* s_pos = reinterpret_cast<uint32_t*>(s_wave + W.pos);
     float* s_w = s_wave + W.w;
+    int* s_order = reinterpret_cast<int*>(s_wave + W.order);
+    int* s_begin = s_order + kFastRows;
+    float* s_tile = s_wave + W.tile;
 
     for (int e = tid; e < dim; e += kModelThreads) {
         const int c = e / d.patch_dim;
@@ -175,9 +191,12 @@ __global__ __launch_bounds__(kModelThreads) void filter_fast_kernel(
         const int y = yx / d.patch;
         const int x = yx - y * d.patch;
         tab_in[e] = ((t * d.channels + c) * d.height + y) * d.width + x;
-        tab_acc[e] = ((t * (d.channels + 1) + c) * d.height + y) * d.width + x;
-        tab_meta[e] = t | (c << 8) | (x < d.fused_width ? kMetaFused : 0) | kMetaValid;
+        tab_meta[e] = t | (c << 8);
         tab_win[e] = d.window[yx];
+        tab_xy[e] = x | (y << 8);
+    }
+    for (int i = lane; i < 2 * cells; i += 64) {
+        s_tile[i] = 0.0F;  // every flush leaves the tile cleared
     }
     __syncthreads();
 
@@ -213,13 +232,10 @@ __global__ __launch_bounds__(kModelThreads) void filter_fast_kernel(
         if (lane < similar) {
             s_pos[lane] = refs.positions[list + lane];
         }
+        // Centre of the estimate rows and mean of the model / member rows.
         const float* g_mean = means + static_cast<size_t>(g) * 2 * dim;
-        for (int e = lane; e < dim; e += 64) {
-            const float mn = g_mean[e];
-            const float mb = final_stage ? g_mean[dim + e] : 0.0F;
-            s_center[e] = (final_stage && flat) ? mb : mn;
-            s_xmean[e] = final_stage ? mb : mn;
-        }
+        const float* center_of = (final_stage && flat) ? g_mean + dim : g_mean;
+        const float* xmean_of = final_stage ? g_mean + dim : g_mean;
         // Per component j (lane j): eigenpair terms.
         float lambda = 0.0F;
         float inv = 0.0F;
@@ -293,7 +309,7 @@ __global__ __launch_bounds__(kModelThreads) void filter_fast_kernel(
                 model_count = 1;
             } else {
                 const int rank_used = modelled ? R : 0;
-                cross_gram(d, model_video, s_xmean, model_video, s_xmean, s_pos, similar, B, tab_in,
+                cross_gram(d, model_video, xmean_of, model_video, xmean_of, s_pos, similar, B, tab_in,
                            s_gram, s_norm, lane);
                 wave_sync();
                 float score = INFINITY;
@@ -308,7 +324,7 @@ __global__ __launch_bounds__(kModelThreads) void filter_fast_kernel(
                         float p = 0.0F;
                         if (inv_j != 0.0F) {
                             for (int s = 0; s < B; ++s) {
-                                p = fmaf(s_gram[lane * 16 + s], s_u[j * 16 + s], p);
+                                p = fmaf(s_gram[s * kFastRows + lane], s_u[j * 16 + s], p);
                             }
                             p = p * inv_j;
                         }
@@ -359,7 +375,7 @@ __global__ __launch_bounds__(kModelThreads) void filter_fast_kernel(
         if (fit) {
             if (!have_estimate) {
                 wave_sync();
-                cross_gram(d, noisy, s_center, model_video, s_xmean, s_pos, similar, B, tab_in, s_gram,
+                cross_gram(d, noisy, center_of, model_video, xmean_of, s_pos, similar, B, tab_in, s_gram,
                            s_norm, lane);
                 wave_sync();
                 if (lane < similar) {
@@ -370,7 +386,7 @@ __global__ __launch_bounds__(kModelThreads) void filter_fast_kernel(
                         float p = 0.0F;
                         if (j < R && inv_j != 0.0F) {
                             for (int s = 0; s < B; ++s) {
-                                p = fmaf(s_gram[lane * 16 + s], s_u[j * 16 + s], p);
+                                p = fmaf(s_gram[s * kFastRows + lane], s_u[j * 16 + s], p);
                             }
                             p = p * inv_j;
                         }
@@ -399,52 +415,127 @@ __global__ __launch_bounds__(kModelThreads) void filter_fast_kernel(
                 }
 #pragma unroll
                 for (int s = 0; s < kFastModel; ++s) {
-                    s_gram[lane * 16 + s] = s < B ? wrow[s] : 0.0F;
+                    s_gram[s * kFastRows + lane] = s < B ? wrow[s] : 0.0F;
                 }
             }
         }
         wave_sync();
 
-        // Estimate and weighted scatter, one element column per lane.
-        for (int e = lane; e < md; e += 64) {
-            const int in_off = tab_in[e];
-            const int acc_off = tab_acc[e];
-            const int meta = tab_meta[e];
-            const float win = tab_win[e];
-            const float center = s_center[e];
-            float x[kFastModel];
-            if (fit) {
-                const float mean = s_xmean[e];
+        // Samples sorted by source slot q = frame - af + search_bwd over
+        // [1 - kPt, slots): the sample feeds output slot q + t through patch
+        // frame t, and output slots outside [0, slots) are dropped (scatter).
+        const int sorted_slots = d.slots + kPt - 1;
+        {
+            const int my_slot =
+                lane < similar ? position_frame(s_pos[lane]) - af + d.search_bwd + kPt - 1 : -1;
+            int at = 0;
+            for (int q = 0; q < sorted_slots; ++q) {
+                const uint64_t members = __ballot(my_slot == q);
+                if (lane == 0) {
+                    s_begin[q] = at;
+                }
+                if (my_slot == q) {
+                    s_order[at + __popcll(members & ((1ULL << lane) - 1))] = lane;
+                }
+                at += __popcll(members);
+            }
+            if (lane == 0) {
+                s_begin[sorted_slots] = at;
+            }
+        }
+        wave_sync();
+
+        // Estimate and weighted aggregation: per channel and output frame of
+        // the group's window, contributions are combined in the tile (value
+        // plane; weight plane for channel 0) and flushed once per cell.
+        const uint32_t anchor_p = refs.anchors[a];
+        int x_low = 0;
+        int x_high = 0;
+        int y_low = 0;
+        int y_high = 0;
+        spatial_range(d, position_x(anchor_p), d.width, x_low, x_high);
+        spatial_range(d, position_y(anchor_p), d.height, y_low, y_high);
+        const size_t plane = static_cast<size_t>(d.width) * d.height;
+        for (int c = 0; c < d.channels; ++c) {
+            // The lane's model-row samples of this channel, read once.
+            float x[kPt][kFastModel];
+            float center[kPt];
 #pragma unroll
-                for (int s = 0; s < kFastModel; ++s) {
-                    x[s] = s < B ? model_video[patch_base(d, s_pos[s]) + in_off] - mean : 0.0F;
+            for (int tt = 0; tt < kPt; ++tt) {
+                const int e = c * d.patch_dim + 64 * tt + lane;
+                center[tt] = center_of[e];
+#pragma unroll
+                for (int s2 = 0; s2 < kFastModel; ++s2) {
+                    x[tt][s2] = 0.0F;
+                }
+                if (fit) {
+                    const int in_off = tab_in[e];
+                    const float mean = xmean_of[e];
+#pragma unroll
+                    for (int s2 = 0; s2 < kFastModel; ++s2) {
+                        if (s2 < B) {
+                            x[tt][s2] = model_video[patch_base(d, s_pos[s2]) + in_off] - mean;
+                        }
+                    }
                 }
             }
-            for (int r = 0; r < similar; ++r) {
-                const uint32_t p = s_pos[r];
-                const int pf = position_frame(p);
-                double* cell_base =
-                    acc.data + (static_cast<size_t>(pf) * (d.channels + 1) * d.height + position_y(p)) * d.width +
-                    position_x(p);
-                float value;
-                if (equal) {
-                    value = noisy[patch_base(d, p) + in_off];
-                } else if (!modelled) {
-                    value = center;
-                } else {
-                    const float4* wr = reinterpret_cast<const float4*>(s_gram + r * 16);
-                    float sum = 0.0F;
-#pragma unroll
-                    for (int q = 0; q < kFastModel / 4; ++q) {
-                        const float4 wq = wr[q];
-                        sum = fmaf(wq.x, x[4 * q + 0], sum);
-                        sum = fmaf(wq.y, x[4 * q + 1], sum);
-                        sum = fmaf(wq.z, x[4 * q + 2], sum);
-                        sum = fmaf(wq.w, x[4 * q + 3], sum);
-                    }
-                    value = center + sum;
+            for (int slot = 0; slot < d.slots; ++slot) {
+                // Samples of source slot slot - t feed this output frame
+                // through patch frame t; s_begin is indexed by q + kPt - 1.
+                if (s_begin[slot + kPt] == s_begin[slot]) {
+                    continue;
                 }
-                scatter(d, cell_base, pf - af + d.search_bwd, meta, acc_off, win, value, s_w[r]);
+#pragma unroll
+                for (int tt = 0; tt < kPt; ++tt) {
+                    const int source_slot = slot - tt + kPt - 1;
+                    const int e = c * d.patch_dim + 64 * tt + lane;
+                    const int in_off = tab_in[e];
+                    const float win = tab_win[e];
+                    const int xy = tab_xy[e];
+                    const int k_end = s_begin[source_slot + 1];
+                    for (int k = s_begin[source_slot]; k < k_end; ++k) {
+                        const int r = s_order[k];
+                        const uint32_t p = s_pos[r];
+                        float value;
+                        if (equal) {
+                            value = noisy[patch_base(d, p) + in_off];
+                        } else if (!modelled) {
+                            value = center[tt];
+                        } else {
+                            float sum = 0.0F;
+#pragma unroll
+                            for (int s2 = 0; s2 < kFastModel; ++s2) {
+                                sum = fmaf(s_gram[s2 * kFastRows + r], x[tt][s2], sum);
+                            }
+                            value = center[tt] + sum;
+                        }
+                        const float cw = s_w[r] * win;
+                        const int cell = (position_y(p) - y_low + (xy >> 8)) * side + position_x(p) - x_low + (xy & 0xFF);
+                        atomicAdd(&s_tile[2 * cell], cw * value);
+                        if (c == 0) {
+                            atomicAdd(&s_tile[2 * cell + 1], cw);
+                        }
+                    }
+                }
+                wave_sync();
+                // Flush the touched cells and clear them for the next pass.
+                const int frame = af - d.search_bwd + slot;
+                double* frame_base = acc.data + static_cast<size_t>(frame) * (d.channels + 1) * plane;
+                for (int cell = lane; cell < cells; cell += 64) {
+                    float2* tp = reinterpret_cast<float2*>(s_tile) + cell;
+                    const float2 v = *tp;
+                    if (v.x != 0.0F || v.y != 0.0F) {
+                        *tp = float2{0.0F, 0.0F};
+                        const int cy = cell / side;
+                        const int cx = cell - cy * side;
+                        double* base = frame_base + static_cast<size_t>(y_low + cy) * d.width + x_low + cx;
+                        unsafeAtomicAdd(base + static_cast<size_t>(c) * plane, static_cast<double>(v.x));
+                        if (c == 0) {
+                            unsafeAtomicAdd(base + static_cast<size_t>(d.channels) * plane, static_cast<double>(v.y));
+                        }
+                    }
+                }
+                wave_sync();
             }
         }
     }
@@ -454,12 +545,12 @@ __global__ __launch_bounds__(kModelThreads) void filter_fast_kernel(
 
 bool fast_filter_supported(const StageDesc& d) {
     return d.use_dual != 0 && d.models == 1 && d.max_rows <= kFastModel && d.rank <= kFastModel &&
-           d.patch == 8 && d.model_dim % 32 == 0 && d.cap <= kFastRows &&
-           fast_filter_shared_bytes(d) <= 96 * 1024;
+           d.patch == 8 && d.patch_time >= 1 && d.patch_time <= 4 && d.cap <= kFastRows &&
+           d.slots <= kMaxTemporal && d.fused_width == 0 && fast_filter_shared_bytes(d) <= 64 * 1024;
 }
 
 size_t fast_filter_shared_bytes(const StageDesc& d) {
-    return (static_cast<size_t>(table_words(d.dim)) + static_cast<size_t>(kWaves) * wave_layout(d.dim).words) *
+    return (static_cast<size_t>(table_words(d.dim)) + static_cast<size_t>(kWaves) * wave_layout(d).words) *
            sizeof(float);
 }
 
@@ -481,20 +572,31 @@ hipError_t launch_fast_filter(const StageDesc& d, const float* noisy, const floa
     if (status == hipSuccess) {
         status = hipDeviceGetAttribute(&units, hipDeviceAttributeMultiprocessorCount, device);
     }
-    int per_unit = 0;
-    if (status == hipSuccess) {
-        status = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_unit, filter_fast_kernel, kModelThreads,
-                                                              shared);
-    }
     if (status != hipSuccess) {
         return status;
     }
-    const long long resident = static_cast<long long>(per_unit > 0 ? per_unit : 1) * units;
     const long long wanted = (group_count + kWaves - 1) / kWaves;
-    const unsigned blocks = static_cast<unsigned>(resident < wanted ? resident : wanted);
-    hipLaunchKernelGGL(filter_fast_kernel, dim3(blocks), dim3(kModelThreads), shared, stream, d, noisy,
-                       basic, refs, group_count, problem_stride, group_flags, means, eigenvalues, vectors,
-                       acc, weights_out);
+#define VNLBHIP_FAST_FILTER(KPT)                                                                     \
+    do {                                                                                             \
+        int per_unit = 0;                                                                            \
+        status = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_unit, filter_fast_kernel<KPT>,    \
+                                                              kModelThreads, shared);                \
+        if (status != hipSuccess) {                                                                  \
+            return status;                                                                           \
+        }                                                                                            \
+        const long long resident = static_cast<long long>(per_unit > 0 ? per_unit : 1) * units;      \
+        const unsigned blocks = static_cast<unsigned>(resident < wanted ? resident : wanted);        \
+        hipLaunchKernelGGL(filter_fast_kernel<KPT>, dim3(blocks), dim3(kModelThreads), shared,       \
+                           stream, d, noisy, basic, refs, group_count, problem_stride, group_flags,  \
+                           means, eigenvalues, vectors, acc, weights_out);                           \
+    } while (0)
+    switch (d.patch_time) {
+        case 1: VNLBHIP_FAST_FILTER(1); break;
+        case 2: VNLBHIP_FAST_FILTER(2); break;
+        case 3: VNLBHIP_FAST_FILTER(3); break;
+        default: VNLBHIP_FAST_FILTER(4); break;
+    }
+#undef VNLBHIP_FAST_FILTER
     return hipGetLastError();
 }
 
