@@ -89,71 +89,128 @@ __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Eight consecutive floats from a 4-byte aligned address (dwordx4 loads).
+__device__ inline void load8(const float* p, float (&v)[8]) {
+    float4 lo;
+    float4 hi;
+    __builtin_memcpy(&lo, p, sizeof(lo));
+    __builtin_memcpy(&hi, p + 4, sizeof(hi));
+    v[0] = lo.x;
+    v[1] = lo.y;
+    v[2] = lo.z;
+    v[3] = lo.w;
+    v[4] = hi.x;
+    v[5] = hi.y;
+    v[6] = hi.z;
+    v[7] = hi.w;
+}
+
+__device__ inline void split8(const float (&x)[8], bf16x8& hi, bf16x8& lo) {
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        const __bf16 h = static_cast<__bf16>(x[v]);
+        hi[v] = h;
+        lo[v] = static_cast<__bf16>(x[v] - static_cast<float>(h));
+    }
+}
+
 // C[r][s] = <rows_r - rows_mean, model_s - xmean> for r < nrows, s < B, and
-// norm[r] = |rows_r - rows_mean|^2, into LDS.  rows and model share the
-// sample positions s_pos (model rows are the first B samples).
+// norm[r] = |rows_r - rows_mean|^2, into LDS (C column-major, s * kFastRows
+// + r).  rows and model share the sample positions s_pos (model rows are the
+// first B samples).  One k step of 32 model elements reads every operand of
+// the four 16-row tiles at once: 8 consecutive samples of one patch row per
+// lane, the model operand and the means shared by the tiles.
 __device__ inline void cross_gram(const StageDesc& d, const float* __restrict__ rows_video,
                                   const float* rows_mean, const float* __restrict__ model_video,
                                   const float* xmean, const uint32_t* s_pos, int nrows, int B,
                                   const int* tab_in, float* s_gram, float* s_norm, int lane) {
+    constexpr int kRowTiles = kFastRows / 16;
     const int md = d.model_dim;
     const int kb = lane >> 4;
     const int col = lane & 15;
+    const int row_tiles = (nrows + 15) / 16;
     const size_t model_base = col < B ? patch_base(d, s_pos[col]) : 0;
-    for (int mt = 0; mt * 16 < nrows; ++mt) {
+    size_t row_base[kRowTiles];
+#pragma unroll
+    for (int mt = 0; mt < kRowTiles; ++mt) {
         const int r = 16 * mt + col;
-        const bool row_ok = r < nrows;
-        const size_t row_base = row_ok ? patch_base(d, s_pos[r]) : 0;
-        floatx4 acc = {0.0F, 0.0F, 0.0F, 0.0F};
-        float norm = 0.0F;
-        for (int q = 0; q * 32 < md; ++q) {
-            const int e0 = 32 * q + 8 * kb;
-            const int off = tab_in[e0];  // 8 samples of one patch row from here
-            bf16x8 a_hi;
-            bf16x8 a_lo;
-            bf16x8 b_hi;
-            bf16x8 b_lo;
+        row_base[mt] = r < nrows ? patch_base(d, s_pos[r]) : 0;
+    }
+    floatx4 acc[kRowTiles];
+    float norm[kRowTiles];
 #pragma unroll
-            for (int v = 0; v < 8; ++v) {
-                float a = 0.0F;
-                if (row_ok) {
-                    a = rows_video[row_base + off + v] - rows_mean[e0 + v];
-                }
-                norm = fmaf(a, a, norm);
-                float b = 0.0F;
-                if (col < B) {
-                    b = model_video[model_base + off + v] - xmean[e0 + v];
-                }
-                const __bf16 ah = static_cast<__bf16>(a);
-                const __bf16 bh = static_cast<__bf16>(b);
-                a_hi[v] = ah;
-                a_lo[v] = static_cast<__bf16>(a - static_cast<float>(ah));
-                b_hi[v] = bh;
-                b_lo[v] = static_cast<__bf16>(b - static_cast<float>(bh));
-            }
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, b_hi, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, b_lo, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_lo, b_hi, acc, 0, 0, 0);
+    for (int mt = 0; mt < kRowTiles; ++mt) {
+        acc[mt] = floatx4{0.0F, 0.0F, 0.0F, 0.0F};
+        norm[mt] = 0.0F;
+    }
+    for (int q = 0; q * 32 < md; ++q) {
+        const int e0 = 32 * q + 8 * kb;
+        const int off = tab_in[e0];  // 8 samples of one patch row from here
+        float b[8];
+        float bm[8];
+        float am[8];
+        float a[kRowTiles][8];
+        load8(xmean + e0, bm);
+        load8(rows_mean + e0, am);
+        if (col < B) {
+            load8(model_video + model_base + off, b);
         }
-        // acc[v] = C[16 mt + 4 kb + v][col]
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int rr = 16 * mt + 4 * kb + v;
-            if (rr < nrows) {
-                s_gram[col * kFastRows + rr] = acc[v];
+        for (int mt = 0; mt < kRowTiles; ++mt) {
+            if (mt < row_tiles && 16 * mt + col < nrows) {
+                load8(rows_video + row_base[mt] + off, a[mt]);
             }
         }
-        norm = norm + __shfl_xor(norm, 16, 64);
-        norm = norm + __shfl_xor(norm, 32, 64);
-        if (kb == 0 && row_ok) {
-            s_norm[r] = norm;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+            b[v] = col < B ? b[v] - bm[v] : 0.0F;
+        }
+        bf16x8 b_hi;
+        bf16x8 b_lo;
+        split8(b, b_hi, b_lo);
+#pragma unroll
+        for (int mt = 0; mt < kRowTiles; ++mt) {
+            if (mt < row_tiles) {
+                const bool row_ok = 16 * mt + col < nrows;
+#pragma unroll
+                for (int v = 0; v < 8; ++v) {
+                    a[mt][v] = row_ok ? a[mt][v] - am[v] : 0.0F;
+                    norm[mt] = fmaf(a[mt][v], a[mt][v], norm[mt]);
+                }
+                bf16x8 a_hi;
+                bf16x8 a_lo;
+                split8(a[mt], a_hi, a_lo);
+                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, b_hi, acc[mt], 0, 0, 0);
+                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, b_lo, acc[mt], 0, 0, 0);
+                acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_lo, b_hi, acc[mt], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int mt = 0; mt < kRowTiles; ++mt) {
+        if (mt < row_tiles) {
+            // acc[v] = C[16 mt + 4 kb + v][col]
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int rr = 16 * mt + 4 * kb + v;
+                if (rr < nrows) {
+                    s_gram[col * kFastRows + rr] = acc[mt][v];
+                }
+            }
+            float n = norm[mt];
+            n = n + __shfl_xor(n, 16, 64);
+            n = n + __shfl_xor(n, 32, 64);
+            const int r = 16 * mt + col;
+            if (kb == 0 && r < nrows) {
+                s_norm[r] = n;
+            }
         }
     }
 }
 
 // kPt: frames of a patch (64-element chunks of one channel's model row).
 template <int kPt>
-__global__ __launch_bounds__(kModelThreads) void filter_fast_kernel(
+__global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
     StageDesc d, const float* __restrict__ noisy, const float* __restrict__ basic, GroupRefs refs,
     int group_count, int problem_stride, const int* __restrict__ group_flags,
     const float* __restrict__ means, const float* __restrict__ eigenvalues,
