@@ -55,17 +55,46 @@ PrepareLayout prepare_layout(const StageDesc& d) {
 }
 
 // Loads sample row r (all models) into LDS slot layout: team-wide, lane l
-// reads the elements 16 k + l of each model.
+// reads the elements 16 k + l of each model.  Every load of a lane is issued
+// before the first LDS store (kExact: kS slots, all full), so a row costs
+// one memory round trip.
+template <int kS, bool kExact>
 __device__ inline void load_row(const StageDesc& d, const PrepareLayout& L,
                                 const float* __restrict__ video, size_t base, const int* tab_in,
                                 float* dst, int lane) {
     for (int m = 0; m < d.models; ++m) {
         const int at = m * L.row + lane * L.pitch;
-        for (int k = 0; k < L.chunks; ++k) {
-            dst[at + k] = video[base + tab_in[at + k]];
-        }
-        if (lane < L.tail) {
-            dst[at + L.chunks] = video[base + tab_in[at + L.chunks]];
+        if (kExact) {
+            const int4* t4 = reinterpret_cast<const int4*>(tab_in + at);
+            float v[kS];
+#pragma unroll
+            for (int h = 0; h < kS / 4; ++h) {
+                const int4 o = t4[h];
+                v[4 * h + 0] = video[base + o.x];
+                v[4 * h + 1] = video[base + o.y];
+                v[4 * h + 2] = video[base + o.z];
+                v[4 * h + 3] = video[base + o.w];
+            }
+            float4* d4 = reinterpret_cast<float4*>(dst + at);
+#pragma unroll
+            for (int h = 0; h < kS / 4; ++h) {
+                d4[h] = float4{v[4 * h], v[4 * h + 1], v[4 * h + 2], v[4 * h + 3]};
+            }
+        } else {
+            const int n = L.chunks + (lane < L.tail ? 1 : 0);
+            for (int k0 = 0; k0 < n; k0 += 8) {
+                float v[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    v[k] = k0 + k < n ? video[base + tab_in[at + k0 + k]] : 0.0F;
+                }
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    if (k0 + k < n) {
+                        dst[at + k0 + k] = v[k];
+                    }
+                }
+            }
         }
     }
 }
@@ -147,7 +176,7 @@ __global__ __launch_bounds__(kModelThreads) void prepare_kernel(
         }
         for (int r = team; r < rows; r += kTeams) {
             float* dst = s_rows + r * width;
-            load_row(d, L, noisy, patch_base(d, s_pos[r]), tab_in, dst, lane);
+            load_row<kS, kExact>(d, L, noisy, patch_base(d, s_pos[r]), tab_in, dst, lane);
             if (want_flat) {
                 for (int m = 0; m < d.models; ++m) {
                     const int at = m * row + lane * pitch;
@@ -213,9 +242,23 @@ __global__ __launch_bounds__(kModelThreads) void prepare_kernel(
                 for (int m = 0; m < d.models; ++m) {
                     const int at = m * row + lane * pitch;
                     bool same = true;
+                    // The row's samples (kExact: every load issued first).
+                    float staged[kExact ? kS : 1];
+                    if (kExact) {
+                        const int4* t4 = reinterpret_cast<const int4*>(tab_in + at);
+#pragma unroll
+                        for (int h = 0; h < kS / 4; ++h) {
+                            const int4 o = t4[h];
+                            staged[4 * h + 0] = noisy[base + o.x];
+                            staged[4 * h + 1] = noisy[base + o.y];
+                            staged[4 * h + 2] = noisy[base + o.z];
+                            staged[4 * h + 3] = noisy[base + o.w];
+                        }
+                    }
+#pragma unroll
                     for (int k = 0; k <= chunks; ++k) {
                         if (k < chunks || lane < tail) {
-                            const float v = noisy[base + tab_in[at + k]];
+                            const float v = kExact ? staged[k < kS ? k : 0] : noisy[base + tab_in[at + k]];
                             if (want_flat) {
                                 const int c = tab_c[at + k];
                                 const double dv = static_cast<double>(v);
@@ -291,7 +334,7 @@ __global__ __launch_bounds__(kModelThreads) void prepare_kernel(
             // The basic mean is needed by the filter even when the group is
             // not modelled (flat centre); the model rows are the basic rows.
             for (int r = team; r < rows; r += kTeams) {
-                load_row(d, L, basic, patch_base(d, s_pos[r]), tab_in, s_rows + r * width, lane);
+                load_row<kS, kExact>(d, L, basic, patch_base(d, s_pos[r]), tab_in, s_rows + r * width, lane);
             }
             __syncthreads();
             for (int i = tid; i < width; i += kModelThreads) {
