@@ -159,9 +159,12 @@ int vnlbhip_get_kernel_times(const vnlbhip_context* context, vnlbhip_kernel_time
  * reference).  PLAIN, the default, sums every distance and model product in
  * fp32 in the CPU path's order.  FAST takes the candidate distances from
  * bf16x3 products on the matrix cores (8x8 patches; other shapes, and
- * windows too wide for its tiles, keep the PLAIN matcher); the selection
- * and the model are the PLAIN ones.  Returns VNLBHIP_ERROR_INVALID for an
- * unknown form. */
+ * windows too wide for its tiles, keep the PLAIN matcher), solves group
+ * models of at most 16 rows with a parallel-order fp32 Jacobi, and filters
+ * dual-PCA groups (one model, at most 16 model rows, at most 64 retained
+ * samples, 8x8 patches) through their bf16x3 cross-Gram; the selection
+ * rule, group flags, means and Grams are the PLAIN ones.  Returns
+ * VNLBHIP_ERROR_INVALID for an unknown form. */
 enum {
     VNLBHIP_FORM_PLAIN = 0,
     VNLBHIP_FORM_FAST = 1,

@@ -51,6 +51,12 @@ hipError_t launch_prepare(const StageDesc& d, const float* noisy, const float* b
 hipError_t launch_jacobi(const StageDesc& d, int problem_count, int problem_stride,
                          const int* group_flags, const int* problem_dims, const float* problems,
                          float* eigenvalues, float* vectors, int* status, hipStream_t stream);
+// FAST form: parallel-order Jacobi for problems of at most 16 rows.
+bool jacobi_parallel_supported(int problem_stride);
+hipError_t launch_jacobi_parallel(const StageDesc& d, int problem_count, int problem_stride,
+                                  const int* group_flags, const int* problem_dims,
+                                  const float* problems, float* eigenvalues, float* vectors,
+                                  int* status, hipStream_t stream);
 hipError_t launch_filter(const StageDesc& d, const float* noisy, const float* basic,
                          const uint32_t* anchors, const int* counts, const uint32_t* positions,
                          const int* group_list, int group_count, int problem_stride,

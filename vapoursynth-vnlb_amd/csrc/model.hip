@@ -11,6 +11,8 @@
 
 #include <float.h>
 
+#include <utility>
+
 namespace vnlbhip {
 namespace {
 
@@ -215,6 +217,252 @@ __global__ __launch_bounds__(64) void jacobi_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// parallel-order Jacobi (FAST form)
+// ---------------------------------------------------------------------------
+
+// v from lane (lane ^ kMask) of the same 16-lane row, with DPP moves only:
+// row_mirror is xor 15, row_half_mirror xor 7 and quad_perm any xor below 4.
+template <int kMask>
+__device__ inline float xor16(float value) {
+    static_assert(kMask >= 0 && kMask < 16, "xor within a row of 16 lanes");
+    constexpr int kAfterMirror = (kMask & 8) != 0 ? (kMask ^ 15) : kMask;
+    constexpr int kQuad = (kAfterMirror & 4) != 0 ? (kAfterMirror ^ 7) : kAfterMirror;
+    constexpr int kQuadCtrl = (0 ^ kQuad) | ((1 ^ kQuad) << 2) | ((2 ^ kQuad) << 4) | ((3 ^ kQuad) << 6);
+    int x = __builtin_bit_cast(int, value);
+    if constexpr ((kMask & 8) != 0) {
+        x = __builtin_amdgcn_update_dpp(0, x, 0x140, 0xF, 0xF, false);  // row_mirror
+    }
+    if constexpr ((kAfterMirror & 4) != 0) {
+        x = __builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    }
+    if constexpr (kQuad != 0) {
+        x = __builtin_amdgcn_update_dpp(0, x, kQuadCtrl, 0xF, 0xF, false);
+    }
+    return __builtin_bit_cast(float, x);
+}
+
+// Column pair (j, j ^ k) of the relative row: column x = r ^ j pairs with
+// x ^ k and takes the rotation of lane x.
+template <int kK, int kJ>
+__device__ inline void rotate_columns(float (&a)[16], float (&v)[16], const float (&b)[16], float c,
+                                      float sigma) {
+    constexpr int kHigh = kK >= 8 ? 8 : (kK >= 4 ? 4 : (kK >= 2 ? 2 : 1));
+    if constexpr ((kJ & kHigh) == 0) {
+        constexpr int kJJ = kJ ^ kK;
+        const float c0 = xor16<kJ>(c);
+        const float s0 = xor16<kJ>(sigma);
+        const float c1 = xor16<kJJ>(c);
+        const float s1 = xor16<kJJ>(sigma);
+        const float v0 = v[kJ];
+        const float v1 = v[kJJ];
+        a[kJ] = fmaf(s0, b[kJJ], c0 * b[kJ]);
+        a[kJJ] = fmaf(s1, b[kJ], c1 * b[kJJ]);
+        v[kJ] = fmaf(s0, v1, c0 * v0);
+        v[kJJ] = fmaf(s1, v0, c1 * v1);
+    }
+}
+
+template <int kK, int... kJ>
+__device__ inline void rotate_rows_and_columns(float (&a)[16], float (&v)[16], float c, float sigma,
+                                               std::integer_sequence<int, kJ...>) {
+    // Rows: B[r][.] = c A[r][.] + sigma A[r ^ k][.]; the partner's
+    // A[r ^ k][r ^ j] is its a[j ^ k].
+    float b[16];
+    ((b[kJ] = fmaf(sigma, xor16<kK>(a[kJ ^ kK]), c * a[kJ])), ...);
+    (rotate_columns<kK, kJ>(a, v, b, c, sigma), ...);
+}
+
+// One rotation round of the parallel ordering for problems of at most 16
+// rows: in round k every index r is paired with r ^ k (rounds 1..15 cover
+// each pair exactly once per sweep), and the eight disjoint rotations of a
+// problem are applied together as A <- J^T A J, V <- V J.  Lane r of a
+// 16-lane row holds row r of A and V in relative order, a[j] = A[r][r ^ j],
+// so the diagonal is a[0], the pivot A[r][r ^ k] is a[k] and every register
+// index and lane exchange is a compile-time constant.  The rotation is that
+// of the cyclic solver (jacobi_rotate, linalg.cpp:153-202) with hardware
+// reciprocal and square-root approximations; both lanes of a pair compute it
+// from the lower index's values, so they agree bit for bit.  Rounds in which
+// no problem of the wavefront rotates are skipped.
+template <int kK>
+__device__ inline void parallel_round(float (&a)[16], float (&v)[16], int r, bool active,
+                                      float threshold) {
+    constexpr int kHigh = kK >= 8 ? 8 : (kK >= 4 ? 4 : (kK >= 2 ? 2 : 1));
+    const bool is_p = (r & kHigh) == 0;  // r is the lower index of its pair
+    const float own_diag = a[0];
+    const float own_off = a[kK];
+    const float partner_diag = xor16<kK>(own_diag);
+    const float partner_off = xor16<kK>(own_off);
+    const float app = is_p ? own_diag : partner_diag;
+    const float aqq = is_p ? partner_diag : own_diag;
+    const float apq = is_p ? own_off : partner_off;
+    const bool rotate = active && fabsf(apq) > threshold;
+    if (!__any(rotate)) {
+        return;
+    }
+    // Branch-free: every lane evaluates the rotation (apq is replaced by 1
+    // where no rotation happens), then selects.
+    const float safe_apq = rotate ? apq : 1.0F;
+    const float tau = (aqq - app) * __builtin_amdgcn_rcpf(2.0F * safe_apq);
+    const float tau_abs = fabsf(tau);
+    const float t_abs = __builtin_amdgcn_rcpf(tau_abs + __builtin_amdgcn_sqrtf(1.0F + (tau * tau)));
+    const float t = tau < 0.0F ? -t_abs : t_abs;
+    const float c_rot = __builtin_amdgcn_rsqf(1.0F + (t * t));
+    const float s_rot = t * c_rot;
+    const float cc = c_rot * c_rot;
+    const float ss = s_rot * s_rot;
+    const float two_cs = 2.0F * (c_rot * s_rot);
+    const float new_pp = ((cc * app) - (two_cs * apq)) + (ss * aqq);
+    const float new_qq = ((ss * app) + (two_cs * apq)) + (cc * aqq);
+    const float c = rotate ? c_rot : 1.0F;
+    // -s for the lower index, +s for the upper one.
+    const float sigma = rotate ? (is_p ? -s_rot : s_rot) : 0.0F;
+    const float new_diag = is_p ? new_pp : new_qq;
+    rotate_rows_and_columns<kK>(a, v, c, sigma, std::make_integer_sequence<int, 16>{});
+    a[0] = rotate ? new_diag : a[0];
+    a[kK] = rotate ? 0.0F : a[kK];
+    // Rounds are scheduled one after the other (bounded register use).
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+template <int... kKs>
+__device__ inline void parallel_sweep(float (&a)[16], float (&v)[16], int r, bool active,
+                                      float threshold) {
+    (parallel_round<kKs>(a, v, r, active, threshold), ...);
+}
+
+// Same stopping rule, ordering and canonical vectors as jacobi_kernel, with
+// the parallel ordering above: 16 lanes per problem, four per wavefront.
+__global__ __launch_bounds__(64) void jacobi_parallel_kernel(
+    StageDesc d, int problem_count, int problem_stride, const int* __restrict__ group_flags,
+    const int* __restrict__ problem_dims, const float* __restrict__ problems,
+    float* __restrict__ eigenvalues, float* __restrict__ vectors, int* __restrict__ status) {
+    constexpr int kPerWave = 4;
+    constexpr int kStride = 17;
+    __shared__ float s_v[kPerWave][16 * kStride];
+    __shared__ float s_diag[kPerWave][16];
+    __shared__ int s_order[kPerWave][16];
+
+    const int sub = threadIdx.x / 16;
+    const int r = threadIdx.x % 16;
+    const int problem = blockIdx.x * kPerWave + sub;
+    const int group = problem / d.models;
+    bool valid = problem < problem_count;
+    int n = 0;
+    if (valid) {
+        valid = (group_flags[group] & kFlagModelled) != 0;
+        n = problem_dims[group];
+    }
+    const float* in = problems + static_cast<size_t>(problem) * problem_stride * problem_stride;
+    float a[16];
+    float v[16];
+    float row_max = 0.0F;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int col = r ^ j;
+        float value = 0.0F;
+        if (valid && r < n && col < n) {
+            value = in[r * problem_stride + col];
+            row_max = fmaxf(row_max, fabsf(value));
+        }
+        a[j] = value;
+        v[j] = j == 0 ? 1.0F : 0.0F;
+    }
+    row_max = fmaxf(row_max, xor16<8>(row_max));
+    row_max = fmaxf(row_max, xor16<4>(row_max));
+    row_max = fmaxf(row_max, xor16<2>(row_max));
+    row_max = fmaxf(row_max, xor16<1>(row_max));
+    const float matrix_scale = row_max > 1.0F ? row_max : 1.0F;
+    const float threshold = (64.0F * FLT_EPSILON) * matrix_scale;
+
+    const auto offdiag_max = [&]() {
+        float value = 0.0F;
+#pragma unroll
+        for (int j = 1; j < 16; ++j) {
+            value = fmaxf(value, fabsf(a[j]));
+        }
+        value = fmaxf(value, xor16<8>(value));
+        value = fmaxf(value, xor16<4>(value));
+        value = fmaxf(value, xor16<2>(value));
+        value = fmaxf(value, xor16<1>(value));
+        return value;
+    };
+
+    bool done = !valid || n <= 1;
+    bool converged = true;
+    int sweeps = 0;
+#pragma nounroll
+    for (int sweep = 0; sweep < d.max_sweeps; ++sweep) {
+        if (!done) {
+            sweeps = sweep + 1;
+        }
+        const float off = offdiag_max();
+        if (!done && off <= threshold) {
+            done = true;
+        }
+        if (__all(done)) {
+            break;
+        }
+        parallel_sweep<1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15>(a, v, r, !done, threshold);
+    }
+    const float final_off = offdiag_max();
+    if (!done) {
+        sweeps = d.max_sweeps;
+        converged = final_off <= 8.0F * threshold;
+    }
+
+    // V[r][r ^ j] = v[j] into LDS; the diagonal in row order.
+    float* vs = s_v[sub];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        vs[r * kStride + (r ^ j)] = v[j];
+    }
+    s_diag[sub][r] = a[0];
+    __syncthreads();
+    if (valid && r < n) {
+        const float dr = s_diag[sub][r];
+        int rank = 0;
+        for (int j = 0; j < n; ++j) {
+            const float dj = s_diag[sub][j];
+            rank += (dj > dr || (j < r && !(dr > dj))) ? 1 : 0;
+        }
+        s_order[sub][rank] = r;
+    }
+    __syncthreads();
+    if (valid && r < d.rank && r < n) {
+        const int col = s_order[sub][r];
+        float* vec_out = vectors + (static_cast<size_t>(problem) * d.rank + r) * problem_stride;
+        float norm_squared = 0.0F;
+        float max_abs = 0.0F;
+        int pivot = 0;
+        for (int i = 0; i < n; ++i) {
+            const float value = vs[i * kStride + col];
+            norm_squared = norm_squared + value * value;
+            const float value_abs = fabsf(value);
+            if (value_abs > max_abs) {
+                max_abs = value_abs;
+                pivot = i;
+            }
+        }
+        float scale = 1.0F;
+        const bool rescale = norm_squared > FLT_MIN;
+        if (rescale) {
+            scale = 1.0F / sqrtf(norm_squared);
+            if (vs[pivot * kStride + col] < 0.0F) {
+                scale = -scale;
+            }
+        }
+        for (int i = 0; i < n; ++i) {
+            const float value = vs[i * kStride + col];
+            vec_out[i] = rescale ? value * scale : value;
+        }
+        eigenvalues[static_cast<size_t>(problem) * d.rank + r] = s_diag[sub][col];
+    }
+    if (valid && r == 0) {
+        status[problem] = sweeps | (converged ? 0 : (1 << 16));
+    }
+}
+
 }  // namespace
 
 hipError_t launch_jacobi(const StageDesc& d, int problem_count, int problem_stride,
@@ -242,6 +490,30 @@ hipError_t launch_jacobi(const StageDesc& d, int problem_count, int problem_stri
         VNLBHIP_JACOBI(64);
     }
 #undef VNLBHIP_JACOBI
+    return hipGetLastError();
+}
+
+}  // namespace vnlbhip
+
+namespace vnlbhip {
+
+bool jacobi_parallel_supported(int problem_stride) {
+    return problem_stride <= 16;
+}
+
+hipError_t launch_jacobi_parallel(const StageDesc& d, int problem_count, int problem_stride,
+                                  const int* group_flags, const int* problem_dims,
+                                  const float* problems, float* eigenvalues, float* vectors,
+                                  int* status, hipStream_t stream) {
+    if (problem_count <= 0) {
+        return hipSuccess;
+    }
+    if (!jacobi_parallel_supported(problem_stride)) {
+        return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL(jacobi_parallel_kernel, dim3(static_cast<unsigned>((problem_count + 3) / 4)),
+                       dim3(64), 0, stream, d, problem_count, problem_stride, group_flags,
+                       problem_dims, problems, eigenvalues, vectors, status);
     return hipGetLastError();
 }
 

@@ -682,7 +682,14 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
                                            d_problems, stream);
         });
         if (d.rank > 0) {
+            const bool parallel_jacobi =
+                ctx.form == VNLBHIP_FORM_FAST && vnlbhip::jacobi_parallel_supported(n_stride);
             timed(ctx, VNLBHIP_KERNEL_JACOBI, stream, "jacobi kernel", [&]() {
+                if (parallel_jacobi) {
+                    return vnlbhip::launch_jacobi_parallel(d, problems, n_stride, d_flags, d_dims,
+                                                           d_problems, d_eigen, d_vectors, d_status,
+                                                           stream);
+                }
                 return vnlbhip::launch_jacobi(d, problems, n_stride, d_flags, d_dims, d_problems,
                                               d_eigen, d_vectors, d_status, stream);
             });
