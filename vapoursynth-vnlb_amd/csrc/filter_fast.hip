@@ -41,12 +41,12 @@ constexpr int kFastModel = 16;  // model rows and rank
 
 // Per-wavefront LDS region (4-byte words).
 struct WaveLayout {
-    int gram;   // [16][kFastRows] C[r][s] at s * kFastRows + r, then W
+    int gram;   // [16][kFastRows] C[r][s] at s * kFastRows + r; then W, row-major in sorted order
     int norm;   // [kFastRows] squared norms of the member rows
     int u;      // [16][16] eigenvectors u_j[s]
     int pos;    // [kFastRows]
-    int w;      // [kFastRows] aggregation weights
-    int order;  // [kFastRows] samples sorted by source slot, then the slot starts
+    int sorted; // per sample in source-slot order: [kFastRows] position, tile cell, weight
+    int begin;  // [kMaxTemporal + 8] first sorted sample of each source slot
     int tile;   // [side][side][2] value plane and weight plane of one output frame
     int words;
 };
@@ -67,20 +67,19 @@ __host__ __device__ inline WaveLayout wave_layout(const StageDesc& d) {
     at += 16 * 16;
     L.pos = at;
     at += kFastRows;
-    L.w = at;
-    at += kFastRows;
-    L.order = at;
-    at += (kFastRows + kMaxTemporal + 8 + 3) / 4 * 4;  // slots + patch frames <= kMaxTemporal + 4
+    L.sorted = at;
+    at += 3 * kFastRows;
+    L.begin = at;
+    at += (kMaxTemporal + 8 + 3) / 4 * 4;  // slots + patch frames <= kMaxTemporal + 4
     L.tile = at;
     at += 2 * tile_side(d) * tile_side(d);
     L.words = (at + 3) / 4 * 4;
     return L;
 }
 
-// Stage-constant element tables: input offset, meta (t | c << 8), window
-// weight and in-patch position x | y << 8.
+// Stage-constant element table: input offset of every model element.
 __host__ __device__ inline int table_words(int dim) {
-    return 4 * ((dim + 3) / 4 * 4);
+    return (dim + 3) / 4 * 4;
 }
 
 __device__ inline void wave_sync() {
@@ -219,11 +218,7 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
     float* s_dyn = reinterpret_cast<float*>(s_dyn4);
     const int dim = d.dim;  // == model dimension (one model)
     const int twords = table_words(dim);
-    const int stride4 = (dim + 3) / 4 * 4;
     int* tab_in = reinterpret_cast<int*>(s_dyn);
-    int* tab_meta = tab_in + stride4;
-    float* tab_win = reinterpret_cast<float*>(tab_meta + stride4);
-    int* tab_xy = reinterpret_cast<int*>(tab_win + stride4);
     const WaveLayout W = wave_layout(d);
     const int side = tile_side(d);
     const int cells = side * side;
@@ -235,10 +230,15 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
     float* s_norm = s_wave + W.norm;
     float* s_u = s_wave + W.u;
     uint32_t* s_pos = reinterpret_cast<uint32_t*>(s_wave + W.pos);
-    float* s_w = s_wave + W.w;
-    int* s_order = reinterpret_cast<int*>(s_wave + W.order);
-    int* s_begin = s_order + kFastRows;
+    uint32_t* s_kpos = reinterpret_cast<uint32_t*>(s_wave + W.sorted);
+    int* s_kcell = reinterpret_cast<int*>(s_wave + W.sorted + kFastRows);
+    float* s_kw = s_wave + W.sorted + 2 * kFastRows;
+    int* s_begin = reinterpret_cast<int*>(s_wave + W.begin);
     float* s_tile = s_wave + W.tile;
+    // Element e = c * patch_dim + 64 t + lane of an 8x8 patch sits at
+    // (lane & 7, lane >> 3) for every channel and patch frame.
+    const int lane_cell = (lane >> 3) * side + (lane & 7);
+    const float lane_window = d.window[lane];
 
     for (int e = tid; e < dim; e += kModelThreads) {
         const int c = e / d.patch_dim;
@@ -248,9 +248,6 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
         const int y = yx / d.patch;
         const int x = yx - y * d.patch;
         tab_in[e] = ((t * d.channels + c) * d.height + y) * d.width + x;
-        tab_meta[e] = t | (c << 8);
-        tab_win[e] = d.window[yx];
-        tab_xy[e] = x | (y << 8);
     }
     for (int i = lane; i < 2 * cells; i += 64) {
         s_tile[i] = 0.0F;  // every flush leaves the tile cleared
@@ -286,8 +283,45 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
         const int B = model_rows_of(d, similar);
         const bool fit = modelled && !equal;
 
+        uint32_t my_pos = 0;
         if (lane < similar) {
-            s_pos[lane] = refs.positions[list + lane];
+            my_pos = refs.positions[list + lane];
+            s_pos[lane] = my_pos;
+        }
+        // Samples sorted by source slot q = frame - af + search_bwd over
+        // [1 - kPt, slots): the sample feeds output slot q + t through patch
+        // frame t, and output slots outside [0, slots) are dropped (scatter).
+        // Lane r keeps its sorted index; the sorted tables hold the position
+        // and the tile cell of the patch origin.
+        const uint32_t anchor_p = refs.anchors[a];
+        int x_low = 0;
+        int x_high = 0;
+        int y_low = 0;
+        int y_high = 0;
+        spatial_range(d, position_x(anchor_p), d.width, x_low, x_high);
+        spatial_range(d, position_y(anchor_p), d.height, y_low, y_high);
+        const int sorted_slots = d.slots + kPt - 1;
+        int my_k = -1;
+        {
+            const int my_slot = lane < similar ? position_frame(my_pos) - af + d.search_bwd + kPt - 1 : -1;
+            int at = 0;
+            for (int q = 0; q < sorted_slots; ++q) {
+                const uint64_t members = __ballot(my_slot == q);
+                if (lane == 0) {
+                    s_begin[q] = at;
+                }
+                if (my_slot == q) {
+                    my_k = at + __popcll(members & ((1ULL << lane) - 1));
+                }
+                at += __popcll(members);
+            }
+            if (lane == 0) {
+                s_begin[sorted_slots] = at;
+            }
+            if (my_k >= 0) {
+                s_kpos[my_k] = my_pos;
+                s_kcell[my_k] = (position_y(my_pos) - y_low) * side + position_x(my_pos) - x_low;
+            }
         }
         // Centre of the estimate rows and mean of the model / member rows.
         const float* g_mean = means + static_cast<size_t>(g) * 2 * dim;
@@ -422,7 +456,9 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
             w = exp_via_double(value);
         }
         if (lane < similar) {
-            s_w[lane] = w;
+            if (my_k >= 0) {
+                s_kw[my_k] = w;
+            }
             if (weights_out != nullptr) {
                 weights_out[list + lane] = w;
             }
@@ -470,34 +506,14 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                         }
                     }
                 }
+                if (my_k >= 0) {
+                    float4* w4 = reinterpret_cast<float4*>(s_gram + my_k * kFastModel);
 #pragma unroll
-                for (int s = 0; s < kFastModel; ++s) {
-                    s_gram[s * kFastRows + lane] = s < B ? wrow[s] : 0.0F;
+                    for (int h = 0; h < kFastModel / 4; ++h) {
+                        w4[h] = float4{4 * h + 0 < B ? wrow[4 * h + 0] : 0.0F, 4 * h + 1 < B ? wrow[4 * h + 1] : 0.0F,
+                                       4 * h + 2 < B ? wrow[4 * h + 2] : 0.0F, 4 * h + 3 < B ? wrow[4 * h + 3] : 0.0F};
+                    }
                 }
-            }
-        }
-        wave_sync();
-
-        // Samples sorted by source slot q = frame - af + search_bwd over
-        // [1 - kPt, slots): the sample feeds output slot q + t through patch
-        // frame t, and output slots outside [0, slots) are dropped (scatter).
-        const int sorted_slots = d.slots + kPt - 1;
-        {
-            const int my_slot =
-                lane < similar ? position_frame(s_pos[lane]) - af + d.search_bwd + kPt - 1 : -1;
-            int at = 0;
-            for (int q = 0; q < sorted_slots; ++q) {
-                const uint64_t members = __ballot(my_slot == q);
-                if (lane == 0) {
-                    s_begin[q] = at;
-                }
-                if (my_slot == q) {
-                    s_order[at + __popcll(members & ((1ULL << lane) - 1))] = lane;
-                }
-                at += __popcll(members);
-            }
-            if (lane == 0) {
-                s_begin[sorted_slots] = at;
             }
         }
         wave_sync();
@@ -505,16 +521,10 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
         // Estimate and weighted aggregation: per channel and output frame of
         // the group's window, contributions are combined in the tile (value
         // plane; weight plane for channel 0) and flushed once per cell.
-        const uint32_t anchor_p = refs.anchors[a];
-        int x_low = 0;
-        int x_high = 0;
-        int y_low = 0;
-        int y_high = 0;
-        spatial_range(d, position_x(anchor_p), d.width, x_low, x_high);
-        spatial_range(d, position_y(anchor_p), d.height, y_low, y_high);
         const size_t plane = static_cast<size_t>(d.width) * d.height;
         for (int c = 0; c < d.channels; ++c) {
-            // The lane's model-row samples of this channel, read once.
+            // The lane's centered model-row samples of this channel, every
+            // load issued at once (rows beyond B read row 0 and are zeroed).
             float x[kPt][kFastModel];
             float center[kPt];
 #pragma unroll
@@ -525,49 +535,61 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                 for (int s2 = 0; s2 < kFastModel; ++s2) {
                     x[tt][s2] = 0.0F;
                 }
-                if (fit) {
+            }
+            if (fit) {
+#pragma unroll
+                for (int tt = 0; tt < kPt; ++tt) {
+                    const int e = c * d.patch_dim + 64 * tt + lane;
                     const int in_off = tab_in[e];
-                    const float mean = xmean_of[e];
 #pragma unroll
                     for (int s2 = 0; s2 < kFastModel; ++s2) {
-                        if (s2 < B) {
-                            x[tt][s2] = model_video[patch_base(d, s_pos[s2]) + in_off] - mean;
-                        }
+                        const uint32_t ps = s_pos[s2 < B ? s2 : 0];
+                        x[tt][s2] = model_video[patch_base(d, ps) + in_off];
+                    }
+                }
+#pragma unroll
+                for (int tt = 0; tt < kPt; ++tt) {
+                    const float mean = xmean_of[c * d.patch_dim + 64 * tt + lane];
+#pragma unroll
+                    for (int s2 = 0; s2 < kFastModel; ++s2) {
+                        x[tt][s2] = s2 < B ? x[tt][s2] - mean : 0.0F;
                     }
                 }
             }
             for (int slot = 0; slot < d.slots; ++slot) {
                 // Samples of source slot slot - t feed this output frame
                 // through patch frame t; s_begin is indexed by q + kPt - 1.
-                if (s_begin[slot + kPt] == s_begin[slot]) {
+                const int k_first = s_begin[slot];
+                const int k_last = s_begin[slot + kPt];
+                if (k_first == k_last) {
                     continue;
                 }
 #pragma unroll
                 for (int tt = 0; tt < kPt; ++tt) {
                     const int source_slot = slot - tt + kPt - 1;
-                    const int e = c * d.patch_dim + 64 * tt + lane;
-                    const int in_off = tab_in[e];
-                    const float win = tab_win[e];
-                    const int xy = tab_xy[e];
                     const int k_end = s_begin[source_slot + 1];
+                    const int in_off = equal ? tab_in[c * d.patch_dim + 64 * tt + lane] : 0;
                     for (int k = s_begin[source_slot]; k < k_end; ++k) {
-                        const int r = s_order[k];
-                        const uint32_t p = s_pos[r];
+                        const float4* w4 = reinterpret_cast<const float4*>(s_gram + k * kFastModel);
+                        const int cell = s_kcell[k] + lane_cell;
+                        const float cw = s_kw[k] * lane_window;
                         float value;
                         if (equal) {
-                            value = noisy[patch_base(d, p) + in_off];
+                            value = noisy[patch_base(d, s_kpos[k]) + in_off];
                         } else if (!modelled) {
                             value = center[tt];
                         } else {
                             float sum = 0.0F;
 #pragma unroll
-                            for (int s2 = 0; s2 < kFastModel; ++s2) {
-                                sum = fmaf(s_gram[s2 * kFastRows + r], x[tt][s2], sum);
+                            for (int h = 0; h < kFastModel / 4; ++h) {
+                                const float4 wv = w4[h];
+                                sum = fmaf(wv.x, x[tt][4 * h + 0], sum);
+                                sum = fmaf(wv.y, x[tt][4 * h + 1], sum);
+                                sum = fmaf(wv.z, x[tt][4 * h + 2], sum);
+                                sum = fmaf(wv.w, x[tt][4 * h + 3], sum);
                             }
                             value = center[tt] + sum;
                         }
-                        const float cw = s_w[r] * win;
-                        const int cell = (position_y(p) - y_low + (xy >> 8)) * side + position_x(p) - x_low + (xy & 0xFF);
                         atomicAdd(&s_tile[2 * cell], cw * value);
                         if (c == 0) {
                             atomicAdd(&s_tile[2 * cell + 1], cw);
@@ -575,10 +597,13 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                     }
                 }
                 wave_sync();
-                // Flush the touched cells and clear them for the next pass.
+                // Flush: the cells touched in this pass are those of the
+                // samples that fed it; the first visit of a cell adds it to
+                // the accumulators and clears it.
                 const int frame = af - d.search_bwd + slot;
                 double* frame_base = acc.data + static_cast<size_t>(frame) * (d.channels + 1) * plane;
-                for (int cell = lane; cell < cells; cell += 64) {
+                for (int k = k_first; k < k_last; ++k) {
+                    const int cell = s_kcell[k] + lane_cell;
                     float2* tp = reinterpret_cast<float2*>(s_tile) + cell;
                     const float2 v = *tp;
                     if (v.x != 0.0F || v.y != 0.0F) {
