@@ -567,18 +567,30 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
 #pragma unroll
                 for (int tt = 0; tt < kPt; ++tt) {
                     const int source_slot = slot - tt + kPt - 1;
+                    const int k_begin = s_begin[source_slot];
                     const int k_end = s_begin[source_slot + 1];
-                    const int in_off = equal ? tab_in[c * d.patch_dim + 64 * tt + lane] : 0;
-                    for (int k = s_begin[source_slot]; k < k_end; ++k) {
+                    if (equal) {
+                        // Bypass: the patch itself (the only global loads of
+                        // this loop, kept out of the modelled one).
+                        const int in_off = tab_in[c * d.patch_dim + 64 * tt + lane];
+                        for (int k = k_begin; k < k_end; ++k) {
+                            const int cell = s_kcell[k] + lane_cell;
+                            const float cw = s_kw[k] * lane_window;
+                            const float value = noisy[patch_base(d, s_kpos[k]) + in_off];
+                            atomicAdd(&s_tile[2 * cell], cw * value);
+                            if (c == 0) {
+                                atomicAdd(&s_tile[2 * cell + 1], cw);
+                            }
+                        }
+                        continue;
+                    }
+#pragma unroll 2
+                    for (int k = k_begin; k < k_end; ++k) {
                         const float4* w4 = reinterpret_cast<const float4*>(s_gram + k * kFastModel);
                         const int cell = s_kcell[k] + lane_cell;
                         const float cw = s_kw[k] * lane_window;
-                        float value;
-                        if (equal) {
-                            value = noisy[patch_base(d, s_kpos[k]) + in_off];
-                        } else if (!modelled) {
-                            value = center[tt];
-                        } else {
+                        float value = center[tt];
+                        if (modelled) {
                             float sum = 0.0F;
 #pragma unroll
                             for (int h = 0; h < kFastModel / 4; ++h) {
@@ -588,7 +600,7 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                                 sum = fmaf(wv.z, x[tt][4 * h + 2], sum);
                                 sum = fmaf(wv.w, x[tt][4 * h + 3], sum);
                             }
-                            value = center[tt] + sum;
+                            value = value + sum;
                         }
                         atomicAdd(&s_tile[2 * cell], cw * value);
                         if (c == 0) {
@@ -598,22 +610,37 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                 }
                 wave_sync();
                 // Flush: the cells touched in this pass are those of the
-                // samples that fed it; the first visit of a cell adds it to
-                // the accumulators and clears it.
+                // samples that fed it.  Each is taken with an LDS exchange
+                // against zero, so a cell shared by several samples reaches
+                // the accumulators once; exchanges of four samples are in
+                // flight together.
                 const int frame = af - d.search_bwd + slot;
                 double* frame_base = acc.data + static_cast<size_t>(frame) * (d.channels + 1) * plane;
-                for (int k = k_first; k < k_last; ++k) {
-                    const int cell = s_kcell[k] + lane_cell;
-                    float2* tp = reinterpret_cast<float2*>(s_tile) + cell;
-                    const float2 v = *tp;
-                    if (v.x != 0.0F || v.y != 0.0F) {
-                        *tp = float2{0.0F, 0.0F};
-                        const int cy = cell / side;
-                        const int cx = cell - cy * side;
-                        double* base = frame_base + static_cast<size_t>(y_low + cy) * d.width + x_low + cx;
-                        unsafeAtomicAdd(base + static_cast<size_t>(c) * plane, static_cast<double>(v.x));
-                        if (c == 0) {
-                            unsafeAtomicAdd(base + static_cast<size_t>(d.channels) * plane, static_cast<double>(v.y));
+                double* lane_base = frame_base + static_cast<size_t>(lane >> 3) * d.width + (lane & 7);
+                for (int k0 = k_first; k0 < k_last; k0 += 4) {
+                    unsigned long long taken[4];
+                    uint32_t origin[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        taken[u] = 0;
+                        origin[u] = 0;
+                        if (k0 + u < k_last) {
+                            const int cell = s_kcell[k0 + u] + lane_cell;
+                            origin[u] = s_kpos[k0 + u];
+                            taken[u] = atomicExch(reinterpret_cast<unsigned long long*>(s_tile) + cell, 0ULL);
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        if (taken[u] != 0ULL) {
+                            const float2 v = __builtin_bit_cast(float2, taken[u]);
+                            double* base = lane_base + static_cast<size_t>(position_y(origin[u])) * d.width +
+                                           position_x(origin[u]);
+                            unsafeAtomicAdd(base + static_cast<size_t>(c) * plane, static_cast<double>(v.x));
+                            if (c == 0) {
+                                unsafeAtomicAdd(base + static_cast<size_t>(d.channels) * plane,
+                                                static_cast<double>(v.y));
+                            }
                         }
                     }
                 }
