@@ -77,9 +77,10 @@ def match_roofline(form, times, bp, fp, clip, sb, sf) -> dict:
     avg_s = match["ms"] / launches / 1000.0
     achieved = flops / launches / avg_s / 1e12 if avg_s > 0 else 0.0
     if form == "fast":
-        kernel, peak = "match_fast_kernel (bf16x3 MFMA distances)", MI355X_PEAK["bf16_tflops"]
+        kernel, peak = "match_fast_kernel", MI355X_PEAK["bf16_tflops"]
+        traffic = measured_traffic("match_fast_kernel")
     else:
-        kernel, peak = "match_kernel (fp32 distances + k-NN select)", MI355X_PEAK["fp32_tflops"]
+        kernel, peak, traffic = "match_kernel", MI355X_PEAK["fp32_tflops"], None
     return {
         "kernel": kernel,
         "bound": "mfma",
@@ -87,8 +88,39 @@ def match_roofline(form, times, bp, fp, clip, sb, sf) -> dict:
         "peak": peak,
         "unit": "TFLOP/s",
         "frac": round(achieved / peak, 5),
-        "traffic": measured_traffic("match_fast_kernel") if form == "fast" else None,
+        "traffic": traffic,
         "launch_ms": round(avg_s * 1000.0, 4),
+        "algorithmic_per_launch": round(flops / launches, 1),
+    }
+
+
+def filter_roofline(form, times, bp, fp, clip, sb, sf) -> dict:
+    """Roofline of the filter + aggregation kernel against HBM: the
+    algorithmic bytes of the scatter design per stage, sum over groups of
+    S * D * 4 (filtered rows read) + S * Pt * P^2 * (C + 1) * 8 (accumulator
+    read-modify-write, fp32), S the retained samples of the group, over the
+    kernel's average launch time from hipEvents (one launch per stage)."""
+    channels = clip["channels"]
+    total = 0.0
+    for params, stats in ((bp, sb), (fp, sf)):
+        dim = params.patch_size ** 2 * params.patch_time * channels
+        contributions = params.patch_time * params.patch_size ** 2 * (channels + 1)
+        total += stats["retained"] * (dim * 4.0 + contributions * 8.0)
+    filt = times["filter"]
+    launches = max(1, filt["launches"])
+    avg_s = filt["ms"] / launches / 1000.0
+    achieved = total / launches / avg_s / 1e9 if avg_s > 0 else 0.0
+    kernel = "filter_fast_kernel" if form == "fast" else "filter_kernel"
+    return {
+        "kernel": kernel,
+        "bound": "hbm",
+        "achieved": round(achieved, 2),
+        "peak": MI355X_PEAK["hbm_gbs"],
+        "unit": "GB/s",
+        "frac": round(achieved / MI355X_PEAK["hbm_gbs"], 5),
+        "traffic": measured_traffic(kernel) if form == "fast" else None,
+        "launch_ms": round(avg_s * 1000.0, 4),
+        "algorithmic_per_launch": round(total / launches, 1),
     }
 
 
@@ -185,7 +217,13 @@ def main() -> int:
     ctx.set_profiling(False)
     log("[bench] kernel ms: " + ", ".join(f"{k}={v['ms']:.2f}/{v['launches']}" for k, v in times.items()))
     _, _, sb, sf = step_profile
-    roofline = match_roofline(args.form, times, bp, fp, clip, sb, sf)
+    # The dominant kernel (largest share of the step) carries the roofline.
+    rooflines = {
+        "match": match_roofline(args.form, times, bp, fp, clip, sb, sf),
+        "filter": filter_roofline(args.form, times, bp, fp, clip, sb, sf),
+    }
+    dominant = max(rooflines, key=lambda k: times[k]["ms"])
+    roofline = rooflines[dominant]
 
     # The plain form on the same clip: quality reference of the metric and
     # its own throughput (one untimed warm step, one timed step).
@@ -234,6 +272,8 @@ def main() -> int:
             "psnr_final_db": round(psnr_final, 4),
             **quality,
             "kernel_ms": {k: round(v["ms"], 3) for k, v in times.items()},
+            "retained_basic": sb["retained"], "retained_final": sf["retained"],
+            "rooflines": rooflines,
         },
         "roofline": roofline,
     }

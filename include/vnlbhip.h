@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define VNLBHIP_ABI_VERSION 1
+#define VNLBHIP_ABI_VERSION 2
 
 enum {
     VNLBHIP_STAGE_BASIC = 0,
@@ -73,6 +73,7 @@ typedef struct vnlbhip_stats {
     int64_t unconverged_groups;   /* eigen problems that hit the sweep limit */
     int32_t batches;              /* anchor-frame batches of the last call */
     int32_t reserved;
+    int64_t retained;             /* retained samples summed over the groups */
 } vnlbhip_stats;
 
 typedef struct vnlbhip_context vnlbhip_context;

@@ -47,8 +47,13 @@ __global__ __launch_bounds__(256) void normalize_kernel(StageDesc d,
 // cover the rank (the reference throws for those, core.cpp:645-648),
 // summary[1] eigen problems that reached the sweep limit without meeting the
 // convergence test.
+// summary[0]: groups whose rows cannot cover the rank; summary[1]:
+// unconverged eigen problems; summary[2..3]: retained samples over the
+// groups (64-bit).
 __global__ __launch_bounds__(256) void summarize_kernel(const int* __restrict__ flags,
                                                         const int* __restrict__ status,
+                                                        const int* __restrict__ counts,
+                                                        const int* __restrict__ group_list,
                                                         int groups, int models,
                                                         int* __restrict__ summary) {
     const int problem = blockIdx.x * blockDim.x + threadIdx.x;
@@ -59,6 +64,10 @@ __global__ __launch_bounds__(256) void summarize_kernel(const int* __restrict__ 
     const int flag = flags[group];
     if ((problem % models) == 0 && (flag & 8) != 0) {
         atomicAdd(&summary[0], 1);
+    }
+    if ((problem % models) == 0) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(summary + 2),
+                  static_cast<unsigned long long>(counts[group_list[group]]));
     }
     if (status != nullptr && (flag & kFlagModelled) != 0 && (status[problem] >> 16) != 0) {
         atomicAdd(&summary[1], 1);
@@ -75,14 +84,16 @@ unsigned grid_for(size_t count) {
 
 }  // namespace
 
-hipError_t launch_summarize(const int* flags, const int* status, int groups, int models,
-                            int* summary, hipStream_t stream) {
+hipError_t launch_summarize(const int* flags, const int* status, const int* counts,
+                            const int* group_list, int groups, int models, int* summary,
+                            hipStream_t stream) {
     const int problems = groups * models;
     if (problems <= 0) {
         return hipSuccess;
     }
     hipLaunchKernelGGL(summarize_kernel, dim3(static_cast<unsigned>((problems + 255) / 256)),
-                       dim3(256), 0, stream, flags, status, groups, models, summary);
+                       dim3(256), 0, stream, flags, status, counts, group_list, groups, models,
+                       summary);
     return hipGetLastError();
 }
 

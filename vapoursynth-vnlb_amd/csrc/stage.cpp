@@ -439,8 +439,9 @@ struct vnlbhip_context {
 };
 
 namespace vnlbhip {
-hipError_t launch_summarize(const int* flags, const int* status, int groups, int models,
-                            int* summary, hipStream_t stream);
+hipError_t launch_summarize(const int* flags, const int* status, const int* counts,
+                            const int* group_list, int groups, int models, int* summary,
+                            hipStream_t stream);
 }
 
 namespace {
@@ -705,8 +706,8 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
                                           d_list, groups, n_stride, d_flags, d_dims, d_means,
                                           d_eigen, d_vectors, acc, d_weights, stream);
         });
-        check(vnlbhip::launch_summarize(d_flags, d.rank > 0 ? d_status : nullptr, groups,
-                                        d.models, summary, stream), "summarize kernel");
+        check(vnlbhip::launch_summarize(d_flags, d.rank > 0 ? d_status : nullptr, d_counts, d_list,
+                                        groups, d.models, summary, stream), "summarize kernel");
 
         if (debug != nullptr) {
             require(anchor_count <= debug->capacity_anchors, "debug capacity too small");
@@ -799,6 +800,9 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
     check(hipStreamSynchronize(stream), "hipStreamSynchronize");
     collect_times(ctx);
     ctx.stats.unconverged_groups = host_summary[1];
+    int64_t retained = 0;
+    std::memcpy(&retained, host_summary + 2, sizeof(retained));
+    ctx.stats.retained += retained;
     if (host_summary[0] != 0) {
         throw ModelFailure("group_size must cover the effective VNLB rank");
     }

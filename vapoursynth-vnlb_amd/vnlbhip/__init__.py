@@ -70,7 +70,8 @@ class Context:
         s = _abi.Stats()
         self._lib.vnlbhip_get_stats(self._handle, ctypes.byref(s))
         return {"anchors": s.anchors, "groups": s.groups,
-                "unconverged_groups": s.unconverged_groups, "batches": s.batches}
+                "unconverged_groups": s.unconverged_groups, "batches": s.batches,
+                "retained": s.retained}
 
     def set_form(self, form: str) -> None:
         """'plain' (fp32 in the CPU path's order, the default) or 'fast'

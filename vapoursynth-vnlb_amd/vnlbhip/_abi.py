@@ -66,6 +66,7 @@ class Stats(ctypes.Structure):
         ("unconverged_groups", ctypes.c_int64),
         ("batches", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
+        ("retained", ctypes.c_int64),
     ]
 
 
