@@ -40,6 +40,7 @@ constexpr int kTileAnchors = 16;
 constexpr int kCopies = 4;
 constexpr int kFillPerThread = 3;          // fill items (4 output columns) per thread
 constexpr int kFillSpan = 4 + kCopies - 1;  // samples behind one item
+constexpr int kEnergyPerThread = 5;         // energy box items per thread
 
 // Per-channel offset: the mean of a strided sample of the channel.
 __global__ __launch_bounds__(256) void channel_offset_kernel(const float* __restrict__ source,
@@ -172,6 +173,8 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
     __bf16* s_copy = reinterpret_cast<__bf16*>(s_dyn16);  // [buffer][hi/lo][copy][rows][pitch]
     // Epilogue staging after the two plane buffers.
     float* s_stage = reinterpret_cast<float*>(s_copy + static_cast<size_t>(4) * kCopies * S.copy_elems);
+    // Patch energies of the candidate frame over the tile's window union
+    // go into the plane buffer that the epilogue's plane has finished with.
     __shared__ int s_index[kTileAnchors];
     __shared__ int s_ax[kTileAnchors];
     __shared__ int s_ay[kTileAnchors];
@@ -326,15 +329,82 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
         }
     };
 
+    // Anchor operands (anchor n_lane, patch rows 4 ks + kb of plane
+    // (af + t, c), offset by the channel mean) for the next plane.
+    float pre_anchor[2][8];
+    const auto prefetch_anchor = [&](int p) {
+        int frame = 0;
+        int c = 0;
+        int t = 0;
+        plane_of(p, frame, c, t);
+        const float mu = offsets[c];
+        const float* anchor_plane = source + (static_cast<size_t>(af + t) * d.channels + c) * plane;
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            const int py = 4 * ks + kb;
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                pre_anchor[ks][v] = my_anchor >= 0
+                                        ? anchor_plane[static_cast<size_t>(my_ay + py) * d.width + my_ax + v] - mu
+                                        : 0.0F;
+            }
+        }
+    };
+    // Energies of a candidate frame over the union box, kEnergyPerThread
+    // items per thread (host checked), for the epilogue.
+    const int box_items = urows * ucols;
+    float pre_energy[kEnergyPerThread];
+    const auto prefetch_energy = [&](int frame) {
+        const float* frame_energy = energy + static_cast<size_t>(frame) * plane;
+#pragma unroll
+        for (int k = 0; k < kEnergyPerThread; ++k) {
+            const int item = tid + k * kFastThreads;
+            const int ry = item / ucols;
+            const int cx = item - ry * ucols;
+            pre_energy[k] = item < box_items
+                                ? frame_energy[static_cast<size_t>(uy0 + ry) * d.width + ux0 + cx]
+                                : 0.0F;
+        }
+    };
+
+    // Operands of tile i, patch rows 4 ks .. 4 ks + 3: mt clamped to a valid
+    // tile so every LDS read stays in the staged region (accumulators of
+    // tiles past mtiles are never used).
+    const auto load_tile = [&](const __bf16* copies, int i, int ks, bf16x8& a_hi, bf16x8& a_lo) {
+        int mt = wave + kFastWaves * i;
+        // Opaque to loop-invariant motion: the few scalar address operations
+        // are redone per plane instead of holding every tile's addresses.
+        asm volatile("" : "+s"(mt));
+        mt = mt < mtiles ? mt : mtiles - 1;
+        const int col = (mt * tile_magic) >> 16;
+        const int rt = mt - col * row_tiles;
+        const int shift = col & (kCopies - 1);
+        const int j = col - shift;
+        const int ry = 16 * rt + n_lane + 4 * ks + kb;
+        const int at = shift * S.copy_elems + ry * S.pitch + j;
+        a_hi = load_bf16x8(copies + at);
+        a_lo = load_bf16x8(copies + kCopies * S.copy_elems + at);
+    };
+
     floatx4 acc[kTilesPerWave];
     prefetch(0);
+    prefetch_anchor(0);
     store(0);
+    float cur_anchor[2][8];
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+            cur_anchor[ks][v] = pre_anchor[ks][v];
+        }
+    }
     __syncthreads();
     for (int p = 0; p < total_planes; ++p) {
         int frame = 0;
         int c = 0;
         int t = 0;
         plane_of(p, frame, c, t);
+        const bool frame_done = c == match_channels - 1 && t == d.patch_time - 1;
         if (c == 0 && t == 0) {
 #pragma unroll
             for (int i = 0; i < kTilesPerWave; ++i) {
@@ -343,61 +413,81 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
         }
         if (p + 1 < total_planes) {
             prefetch(p + 1);
+            prefetch_anchor(p + 1);
         }
-        // Anchor operands: anchor n_lane, patch rows 4 ks + kb of (af + t, c).
-        const float mu = offsets[c];
+        if (frame_done) {
+            prefetch_energy(frame);
+        }
         bf16x8 b_hi[2];
         bf16x8 b_lo[2];
-        const float* anchor_plane = source + (static_cast<size_t>(af + t) * d.channels + c) * plane;
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
-            const int py = 4 * ks + kb;
 #pragma unroll
             for (int v = 0; v < 8; ++v) {
-                float x = 0.0F;
-                if (my_anchor >= 0) {
-                    x = anchor_plane[static_cast<size_t>(my_ay + py) * d.width + my_ax + v] - mu;
-                }
+                const float x = cur_anchor[ks][v];
                 const __bf16 hi = static_cast<__bf16>(x);
                 b_hi[ks][v] = hi;
                 b_lo[ks][v] = static_cast<__bf16>(x - static_cast<float>(hi));
             }
         }
         const __bf16* copies = s_copy + static_cast<size_t>(p & 1) * 2 * kCopies * S.copy_elems;
+        // Pairs of tiles, their products alternating so consecutive MFMAs
+        // are independent; the LDS reads of the next pair's half (ks) go out
+        // as soon as the current pair's products of that half are issued.
+        bf16x8 cur_hi[2][2];
+        bf16x8 cur_lo[2][2];
 #pragma unroll
-        for (int i = 0; i < kTilesPerWave; ++i) {
+        for (int ks = 0; ks < 2; ++ks) {
+            load_tile(copies, 0, ks, cur_hi[0][ks], cur_lo[0][ks]);
+            load_tile(copies, 1, ks, cur_hi[1][ks], cur_lo[1][ks]);
+        }
+#pragma unroll
+        for (int i = 0; i < kTilesPerWave; i += 2) {
             int mt = wave + kFastWaves * i;
-            // Opaque to loop-invariant motion: the few scalar address
-            // operations are redone per plane instead of holding every
-            // tile's addresses in registers.
             asm volatile("" : "+s"(mt));
-            if (mt < mtiles) {
-                const int col = (mt * tile_magic) >> 16;
-                const int rt = mt - col * row_tiles;
-                const int shift = col & (kCopies - 1);
-                const int j = col - shift;
+            if (mt >= mtiles) {
+                break;  // uniform: later pairs are past the tile count too
+            }
 #pragma unroll
-                for (int ks = 0; ks < 2; ++ks) {
-                    const int ry = 16 * rt + n_lane + 4 * ks + kb;
-                    const int at = shift * S.copy_elems + ry * S.pitch + j;
-                    const bf16x8 a_hi = load_bf16x8(copies + at);
-                    const bf16x8 a_lo = load_bf16x8(copies + kCopies * S.copy_elems + at);
-                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, b_hi[ks], acc[i], 0, 0, 0);
-                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, b_lo[ks], acc[i], 0, 0, 0);
-                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_lo, b_hi[ks], acc[i], 0, 0, 0);
+            for (int ks = 0; ks < 2; ++ks) {
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur_hi[0][ks], b_hi[ks], acc[i], 0, 0, 0);
+                acc[i + 1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur_hi[1][ks], b_hi[ks], acc[i + 1], 0, 0, 0);
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur_hi[0][ks], b_lo[ks], acc[i], 0, 0, 0);
+                acc[i + 1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur_hi[1][ks], b_lo[ks], acc[i + 1], 0, 0, 0);
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur_lo[0][ks], b_hi[ks], acc[i], 0, 0, 0);
+                acc[i + 1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur_lo[1][ks], b_hi[ks], acc[i + 1], 0, 0, 0);
+                if (i + 2 < kTilesPerWave) {
+                    load_tile(copies, i + 2, ks, cur_hi[0][ks], cur_lo[0][ks]);
+                    load_tile(copies, i + 3, ks, cur_hi[1][ks], cur_lo[1][ks]);
                 }
             }
-            // Keep one tile's operands live at a time.
             __builtin_amdgcn_sched_barrier(0);
         }
         if (p + 1 < total_planes) {
             store((p + 1) & 1);
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks) {
+#pragma unroll
+                for (int v = 0; v < 8; ++v) {
+                    cur_anchor[ks][v] = pre_anchor[ks][v];
+                }
+            }
         }
-        if (c == match_channels - 1 && t == d.patch_time - 1) {
+        if (frame_done) {
             // Epilogue: this candidate frame's distances into LDS per anchor
-            // (window scan order), then out to global memory.
+            // (window scan order), then out to global memory.  The energy
+            // box takes the buffer of plane p once every wave is past it.
+            float* s_ebox = reinterpret_cast<float*>(s_copy + static_cast<size_t>(p & 1) * 2 * kCopies * S.copy_elems);
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < kEnergyPerThread; ++k) {
+                const int item = tid + k * kFastThreads;
+                if (item < box_items) {
+                    s_ebox[item] = pre_energy[k];
+                }
+            }
+            __syncthreads();
             const int ti = p / planes_per_frame;
-            const float* frame_energy = energy + static_cast<size_t>(frame) * plane;
 #pragma unroll
             for (int i = 0; i < kTilesPerWave; ++i) {
                 int mt = wave + kFastWaves * i;
@@ -410,14 +500,15 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
                     if (rel_x >= 0 && rel_x < s_nx[n_lane]) {
 #pragma unroll
                         for (int v = 0; v < 4; ++v) {
-                            const int cy = uy0 + 16 * rt + 4 * kb + v;
+                            const int ry = 16 * rt + 4 * kb + v;
+                            const int cy = uy0 + ry;
                             const int rel_y = cy - s_yl[n_lane];
                             if (rel_y >= 0 && rel_y < s_ny[n_lane]) {
                                 float dist;
                                 if (cx == my_ax && cy == my_ay && frame == af) {
                                     dist = 0.0F;
                                 } else {
-                                    const float ec = frame_energy[static_cast<size_t>(cy) * d.width + cx];
+                                    const float ec = s_ebox[ry * ucols + col];
                                     dist = (s_energy[n_lane] + ec) - 2.0F * acc[i][v];
                                     dist = dist > 0.0F ? dist : 0.0F;
                                 }
@@ -526,6 +617,8 @@ bool fast_match_supported(const StageDesc& d, int max_cols, int max_rows) {
     const FastShape S = fast_shape(d, max_cols, max_rows);
     return static_cast<long long>(max_cols) * row_tiles <= static_cast<long long>(kMaxTilesPerWave) * kFastWaves &&
            S.region_rows * (S.pitch / 4) <= kFillPerThread * kFastThreads &&
+           static_cast<long long>(max_cols) * max_rows <= static_cast<long long>(kEnergyPerThread) * kFastThreads &&
+           static_cast<long long>(max_cols) * max_rows * 4 <= static_cast<long long>(2) * kCopies * S.copy_elems * 2 &&
            fast_match_shared_bytes(d, max_cols, max_rows) <= 80 * 1024;
 }
 
@@ -576,6 +669,8 @@ hipError_t launch_fast_match(const StageDesc& d, int max_cols, int max_rows, con
         VNLBHIP_FAST(12);
     } else if (per_wave <= 16) {
         VNLBHIP_FAST(16);
+    } else if (per_wave <= 18) {
+        VNLBHIP_FAST(18);
     } else if (per_wave <= 20) {
         VNLBHIP_FAST(20);
     } else {
