@@ -169,3 +169,22 @@ def test_parallel_jacobi_matches_cyclic(contexts, stage):
         assert np.max(np.abs(g - w)) <= 1e-4 * scale, (a, g, w)
         compared += 1
     assert compared >= 20, compared
+
+
+def test_fast_form_nan_region(contexts):
+    """NaN regions: the fast form forms the same groups as the plain form
+    (anchors whose candidates are all NaN have none) and yields NaN exactly
+    where the plain form does."""
+    plain, fast = contexts
+    rng = np.random.default_rng(17)
+    noisy = rng.random((4, 1, 48, 64), dtype=np.float32)
+    noisy[:, :, :, :24] = np.nan
+    args = dict(sigma=20, block_size=8, block_step=4, group_size=16, bm_range=5, patch_time=2,
+                radius=1, rank=4)
+    bp = parse_stage_parameters(BASIC, args)
+    x = _dev(noisy)
+    print("[fast] nan region", flush=True)
+    want = plain.stage(BASIC, bp, x).cpu().numpy()
+    got = fast.stage(BASIC, bp, x).cpu().numpy()
+    np.testing.assert_array_equal(np.isnan(got), np.isnan(want))
+    assert plain.stats()["groups"] > 0

@@ -119,6 +119,32 @@ def test_integer_valued_clip_is_exact(gpu, oracle_lib):
     assert np.abs(got - want).max() <= 1e-6
 
 
+def _nan_clip():
+    rng = np.random.default_rng(17)
+    noisy = rng.random((4, 1, 48, 64), dtype=np.float32)
+    noisy[:, :, :, :24] = np.nan
+    return noisy
+
+
+def test_nan_region_anchors_are_skipped(gpu, oracle_lib):
+    """Anchors whose every candidate distance is NaN have no matches: they
+    form no group and clear nothing (the oracle skips them), and their
+    pixels fall back to the (NaN) source."""
+    noisy = _nan_clip()
+    p = StageParameters(sigma=20 / 255, patch_size=4, patch_time=1, search_window=7, search_bwd=1,
+                        search_fwd=1, similar=8, rank=4, proc_step=2)
+    print("[nan] debug frame", flush=True)
+    want_dbg = _compare_frame(gpu, oracle_lib, BASIC, p, noisy, None, 1)
+    assert (want_dbg["counts"] == 0).any() and (want_dbg["counts"] > 0).any()
+    print("[nan] stage", flush=True)
+    want, groups = oracle_lib.stage(BASIC, p, noisy, flags=oracle_lib.ACCUM_DOUBLE)
+    got = gpu.stage(BASIC, p, _dev(noisy)).cpu().numpy()
+    assert gpu.stats()["groups"] == groups
+    np.testing.assert_array_equal(np.isnan(got), np.isnan(want))
+    finite = ~np.isnan(want)
+    assert np.abs(got[finite] - want[finite]).max() <= MAX_ABS
+
+
 def test_constant_clip(gpu):
     noisy = np.full((3, 1, 24, 20), 0.25, np.float32)
     p = StageParameters(sigma=0.1, patch_size=2, patch_time=1, search_window=3, search_bwd=1,

@@ -101,4 +101,42 @@ __device__ inline double wave_sum_double(double v) {
     return v;
 }
 
+// v from lane (lane ^ kMask) of the same 16-lane row, with DPP moves only:
+// row_mirror is xor 15, row_half_mirror xor 7 and quad_perm any xor below 4.
+template <int kMask>
+__device__ inline int xor16_bits(int x) {
+    static_assert(kMask >= 0 && kMask < 16, "xor within a row of 16 lanes");
+    constexpr int kAfterMirror = (kMask & 8) != 0 ? (kMask ^ 15) : kMask;
+    constexpr int kQuad = (kAfterMirror & 4) != 0 ? (kAfterMirror ^ 7) : kAfterMirror;
+    constexpr int kQuadCtrl = (0 ^ kQuad) | ((1 ^ kQuad) << 2) | ((2 ^ kQuad) << 4) | ((3 ^ kQuad) << 6);
+    if constexpr ((kMask & 8) != 0) {
+        x = __builtin_amdgcn_update_dpp(0, x, 0x140, 0xF, 0xF, false);  // row_mirror
+    }
+    if constexpr ((kAfterMirror & 4) != 0) {
+        x = __builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    }
+    if constexpr (kQuad != 0) {
+        x = __builtin_amdgcn_update_dpp(0, x, kQuadCtrl, 0xF, 0xF, false);
+    }
+    return x;
+}
+
+template <int kMask>
+__device__ inline float xor16(float value) {
+    return __builtin_bit_cast(float, xor16_bits<kMask>(__builtin_bit_cast(int, value)));
+}
+
+// 64-bit value from lane (lane ^ kMask) of the wavefront: DPP within rows of
+// 16 lanes, ds_bpermute across rows.
+template <int kMask>
+__device__ inline uint64_t xor64(uint64_t value) {
+    if constexpr (kMask < 16) {
+        const uint32_t lo = static_cast<uint32_t>(xor16_bits<kMask>(static_cast<int>(value & 0xFFFFFFFFu)));
+        const uint32_t hi = static_cast<uint32_t>(xor16_bits<kMask>(static_cast<int>(value >> 32)));
+        return (static_cast<uint64_t>(hi) << 32) | lo;
+    } else {
+        return __shfl_xor(value, kMask, 64);
+    }
+}
+
 }  // namespace vnlbhip

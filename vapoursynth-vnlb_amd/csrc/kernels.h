@@ -22,9 +22,14 @@ size_t fast_match_stride(const StageDesc& d);
 hipError_t launch_fast_energy(const StageDesc& d, const float* source, float* offsets, float* rows,
                               float* energy, hipStream_t stream);
 // Distances of anchor index i land at staged[(i - anchor_base) * stride].
+// fused: k-nearest selection inside the matcher (fast_match_fused), results
+// straight into counts / positions / distances; otherwise distances are
+// staged for launch_fast_select.
+bool fast_match_fused(const StageDesc& d, int max_cols, int max_rows);
 hipError_t launch_fast_match(const StageDesc& d, int max_cols, int max_rows, const float* source,
                              const float* offsets, const float* energy, const uint32_t* anchors,
                              const int* tiles, int tile_count, int anchor_base, float* staged,
+                             bool fused, int* counts, uint32_t* positions, float* distances,
                              hipStream_t stream);
 hipError_t launch_fast_select(const StageDesc& d, const float* staged, const uint32_t* anchors,
                               int anchor_count, int* counts, uint32_t* positions, float* distances,

@@ -24,6 +24,7 @@
 #include "kernels.h"
 #include "select.h"
 
+#include <float.h>
 #include <hip/hip_bf16.h>
 
 namespace vnlbhip {
@@ -42,7 +43,9 @@ constexpr int kFillPerThread = 3;          // fill items (4 output columns) per 
 constexpr int kFillSpan = 4 + kCopies - 1;  // samples behind one item
 constexpr int kEnergyPerThread = 5;         // energy box items per thread
 
-// Per-channel offset: the mean of a strided sample of the channel.
+// Per-channel offset: the mean of the finite values of a strided sample of
+// the channel (any finite offset leaves the distances unchanged; non-finite
+// samples only poison the patches that hold them).
 __global__ __launch_bounds__(256) void channel_offset_kernel(const float* __restrict__ source,
                                                              int channels, size_t plane,
                                                              int frames, float* __restrict__ offsets) {
@@ -55,8 +58,11 @@ __global__ __launch_bounds__(256) void channel_offset_kernel(const float* __rest
     for (size_t i = threadIdx.x * stride; i < per_channel; i += blockDim.x * stride) {
         const size_t f = i / plane;
         const size_t rem = i - f * plane;
-        sum += static_cast<double>(source[(f * channels + c) * plane + rem]);
-        ++count;
+        const float v = source[(f * channels + c) * plane + rem];
+        if (v == v && fabsf(v) <= FLT_MAX) {
+            sum += static_cast<double>(v);
+            ++count;
+        }
     }
     sum = wave_sum_double(sum);
     double n = wave_sum_double(static_cast<double>(count));
@@ -143,6 +149,8 @@ struct FastShape {
     int copy_elems;    // region_rows * pitch
     int stride;        // distance slots per anchor (window^2 * temporal)
     int per_frame;     // window^2
+    int ebox_words;    // energy box (max cols x max rows), rounded to 4
+    int scratch_keys;  // compaction list per anchor (window^2 16-bit entries, rounded to 2)
 };
 
 __device__ inline uint2 pack4(__bf16 a, __bf16 b, __bf16 c, __bf16 d) {
@@ -162,13 +170,146 @@ __device__ inline bf16x8 load_bf16x8(const __bf16* p) {
     return __builtin_bit_cast(bf16x8, v);
 }
 
+// One compare-exchange step of a bitonic network over the wavefront:
+// partner lane ^ kJ, blocks of kK lanes alternately ascending (kK = 64: the
+// whole wavefront ascending).
+template <int kK, int kJ>
+__device__ inline uint64_t bitonic_step(uint64_t key, int lane) {
+    const uint64_t other = xor64<kJ>(key);
+    const bool ascending = (lane & kK) == 0;
+    const bool lower = (lane & kJ) == 0;
+    const uint64_t low = other < key ? other : key;
+    const uint64_t high = other < key ? key : other;
+    return lower == ascending ? low : high;
+}
+
+// Wave-wide bitonic sort of 64 keys, one per lane, ascending by lane.
+__device__ inline uint64_t wave_sort64(uint64_t key, int lane) {
+    key = bitonic_step<2, 1>(key, lane);
+    key = bitonic_step<4, 2>(key, lane);
+    key = bitonic_step<4, 1>(key, lane);
+    key = bitonic_step<8, 4>(key, lane);
+    key = bitonic_step<8, 2>(key, lane);
+    key = bitonic_step<8, 1>(key, lane);
+    key = bitonic_step<16, 8>(key, lane);
+    key = bitonic_step<16, 4>(key, lane);
+    key = bitonic_step<16, 2>(key, lane);
+    key = bitonic_step<16, 1>(key, lane);
+    key = bitonic_step<32, 16>(key, lane);
+    key = bitonic_step<32, 8>(key, lane);
+    key = bitonic_step<32, 4>(key, lane);
+    key = bitonic_step<32, 2>(key, lane);
+    key = bitonic_step<32, 1>(key, lane);
+    key = bitonic_step<64, 32>(key, lane);
+    key = bitonic_step<64, 16>(key, lane);
+    key = bitonic_step<64, 8>(key, lane);
+    key = bitonic_step<64, 4>(key, lane);
+    key = bitonic_step<64, 2>(key, lane);
+    key = bitonic_step<64, 1>(key, lane);
+    return key;
+}
+
+// The 64 smallest of two ascending wave-wide lists, ascending: the first
+// half-cleaner of the bitonic merge of r and reversed c, then the merge.
+__device__ inline uint64_t wave_merge_lower64(uint64_t r, uint64_t c, int lane) {
+    const uint64_t reversed = __shfl(c, 63 - lane, 64);
+    uint64_t key = reversed < r ? reversed : r;
+    key = bitonic_step<64, 32>(key, lane);
+    key = bitonic_step<64, 16>(key, lane);
+    key = bitonic_step<64, 8>(key, lane);
+    key = bitonic_step<64, 4>(key, lane);
+    key = bitonic_step<64, 2>(key, lane);
+    key = bitonic_step<64, 1>(key, lane);
+    return key;
+}
+
+// Four independent lists at once (one per anchor of a wavefront): the
+// exchanges of a step are issued together so their latencies overlap.
+template <int kK, int kJ>
+__device__ inline void bitonic_step4(uint64_t (&key)[4], int lane) {
+    uint64_t other[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        other[q] = xor64<kJ>(key[q]);
+    }
+    const bool ascending = (lane & kK) == 0;
+    const bool lower = (lane & kJ) == 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint64_t low = other[q] < key[q] ? other[q] : key[q];
+        const uint64_t high = other[q] < key[q] ? key[q] : other[q];
+        key[q] = lower == ascending ? low : high;
+    }
+}
+
+__device__ inline void wave_sort64x4(uint64_t (&key)[4], int lane) {
+    bitonic_step4<2, 1>(key, lane);
+    bitonic_step4<4, 2>(key, lane);
+    bitonic_step4<4, 1>(key, lane);
+    bitonic_step4<8, 4>(key, lane);
+    bitonic_step4<8, 2>(key, lane);
+    bitonic_step4<8, 1>(key, lane);
+    bitonic_step4<16, 8>(key, lane);
+    bitonic_step4<16, 4>(key, lane);
+    bitonic_step4<16, 2>(key, lane);
+    bitonic_step4<16, 1>(key, lane);
+    bitonic_step4<32, 16>(key, lane);
+    bitonic_step4<32, 8>(key, lane);
+    bitonic_step4<32, 4>(key, lane);
+    bitonic_step4<32, 2>(key, lane);
+    bitonic_step4<32, 1>(key, lane);
+    bitonic_step4<64, 32>(key, lane);
+    bitonic_step4<64, 16>(key, lane);
+    bitonic_step4<64, 8>(key, lane);
+    bitonic_step4<64, 4>(key, lane);
+    bitonic_step4<64, 2>(key, lane);
+    bitonic_step4<64, 1>(key, lane);
+}
+
+// best[q] <- the 64 smallest of best[q] and sorted[q], ascending.
+__device__ inline void wave_merge_lower64x4(uint64_t (&best)[4], const uint64_t (&sorted)[4], int lane) {
+    uint64_t key[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint64_t reversed = __shfl(sorted[q], 63 - lane, 64);
+        key[q] = reversed < best[q] ? reversed : best[q];
+    }
+    bitonic_step4<64, 32>(key, lane);
+    bitonic_step4<64, 16>(key, lane);
+    bitonic_step4<64, 8>(key, lane);
+    bitonic_step4<64, 4>(key, lane);
+    bitonic_step4<64, 2>(key, lane);
+    bitonic_step4<64, 1>(key, lane);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        best[q] = key[q];
+    }
+}
+
+__device__ inline uint64_t readlane64(uint64_t v, int lane) {
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v & 0xFFFFFFFFu), lane));
+    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v >> 32), lane));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+constexpr uint64_t kNoKey = ~static_cast<uint64_t>(0);
+
 // Cross terms of one anchor tile against its candidates, all candidate
-// frames; distances land in `out` ([anchor][stride], scan order).
-template <int kTilesPerWave>
+// frames.  kFused = false: distances land in `out` ([anchor][stride], scan
+// order) for select_fast_kernel.  kFused = true (retained cap <= 64): the
+// k-nearest selection runs here.  Each wavefront keeps, for four anchors of
+// the tile, the 64 smallest keys (distance key << 32 | scan position) seen
+// so far in lane order; after every candidate frame the candidates below the
+// 64th key are compacted, sorted and merged in.  The selected set of
+// select.h is a prefix of the (distance, scan position) order no longer
+// than the retained cap, so it lies within those 64 keys, and the lists
+// written are those of select_and_write.
+template <int kTilesPerWave, bool kFused>
 __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
     StageDesc d, FastShape S, const float* __restrict__ source, const float* __restrict__ offsets,
     const float* __restrict__ energy, const uint32_t* __restrict__ anchors,
-    const int* __restrict__ tiles, int anchor_base, float* __restrict__ out) {
+    const int* __restrict__ tiles, int anchor_base, float* __restrict__ out, int* __restrict__ counts,
+    uint32_t* __restrict__ positions, float* __restrict__ distances) {
     extern __shared__ uint4 s_dyn16[];
     __bf16* s_copy = reinterpret_cast<__bf16*>(s_dyn16);  // [buffer][hi/lo][copy][rows][pitch]
     // Epilogue staging after the two plane buffers.
@@ -387,6 +528,8 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
     };
 
     floatx4 acc[kTilesPerWave];
+    uint64_t best[4] = {kNoKey, kNoKey, kNoKey, kNoKey};  // kFused: anchors 4 wave + q
+    int valid_count[4] = {0, 0, 0, 0};
     prefetch(0);
     prefetch_anchor(0);
     store(0);
@@ -506,11 +649,14 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
                             if (rel_y >= 0 && rel_y < s_ny[n_lane]) {
                                 float dist;
                                 if (cx == my_ax && cy == my_ay && frame == af) {
-                                    dist = 0.0F;
+                                    // The anchor against itself: exactly 0 (NaN for a
+                                    // patch holding NaN, as the exact sum gives).
+                                    const float e_anchor = s_energy[n_lane];
+                                    dist = e_anchor == e_anchor ? 0.0F : e_anchor;
                                 } else {
                                     const float ec = s_ebox[ry * ucols + col];
                                     dist = (s_energy[n_lane] + ec) - 2.0F * acc[i][v];
-                                    dist = dist > 0.0F ? dist : 0.0F;
+                                    dist = dist < 0.0F ? 0.0F : dist;  // rounding below 0; NaN stays NaN
                                 }
                                 s_stage[n_lane * S.per_frame + rel_y * s_nx[n_lane] + rel_x] = dist;
                             }
@@ -520,20 +666,143 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
                 __builtin_amdgcn_sched_barrier(0);
             }
             __syncthreads();
-            for (int n = 0; n < kTileAnchors; ++n) {
-                const int index = s_index[n];
-                if (index < 0) {
-                    continue;
+            if constexpr (kFused) {
+                // Compacted candidates (window positions of the keys below the
+                // anchor's 64th), one list per anchor, in the idle plane
+                // buffer after the energy box.
+                uint16_t* scratch = reinterpret_cast<uint16_t*>(s_ebox + S.ebox_words) + wave * 4 * S.scratch_keys;
+                int m[4] = {0, 0, 0, 0};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int n = 4 * wave + q;
+                    if (s_index[n] < 0) {
+                        continue;
+                    }
+                    const int count = s_nx[n] * s_ny[n];
+                    const uint32_t base_index = static_cast<uint32_t>(ti * count);
+                    const uint64_t threshold = readlane64(best[q], 63);
+                    uint16_t* list = scratch + q * S.scratch_keys;
+                    for (int c0 = 0; c0 < count; c0 += 64) {
+                        const int i = c0 + lane;
+                        bool pass = false;
+                        bool valid = false;
+                        uint64_t key = kNoKey;
+                        if (i < count) {
+                            const float dist = s_stage[n * S.per_frame + i];
+                            valid = dist == dist;
+                            key = (static_cast<uint64_t>(distance_key(dist)) << 32) | (base_index + i);
+                            pass = key < threshold;
+                        }
+                        valid_count[q] += __popcll(__ballot(valid));
+                        const uint64_t mask = __ballot(pass);
+                        if (pass) {
+                            list[m[q] + __popcll(mask & ((1ULL << lane) - 1))] = static_cast<uint16_t>(i);
+                        }
+                        m[q] += __popcll(mask);
+                    }
                 }
-                const int count = s_nx[n] * s_ny[n];
-                float* dst = out + static_cast<size_t>(index - anchor_base) * S.stride +
-                             static_cast<size_t>(ti) * count;
-                for (int i = tid; i < count; i += kFastThreads) {
-                    dst[i] = s_stage[n * S.per_frame + i];
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                // Rounds of 64 compacted keys, the four anchors side by side;
+                // keys no longer below an anchor's 64th are dropped.
+                const int rounds = (max(max(m[0], m[1]), max(m[2], m[3])) + 63) / 64;
+                for (int r = 0; r < rounds; ++r) {
+                    uint64_t key[4];
+                    bool any = false;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int at = 64 * r + lane;
+                        const uint64_t current = readlane64(best[q], 63);
+                        uint64_t v = kNoKey;
+                        if (at < m[q]) {
+                            const int n = 4 * wave + q;
+                            const int i = scratch[q * S.scratch_keys + at];
+                            const uint32_t base_index = static_cast<uint32_t>(ti * s_nx[n] * s_ny[n]);
+                            v = (static_cast<uint64_t>(distance_key(s_stage[n * S.per_frame + i])) << 32) |
+                                (base_index + i);
+                        }
+                        v = v < current ? v : kNoKey;
+                        key[q] = v;
+                        any = any || __ballot(v != kNoKey) != 0;
+                    }
+                    if (!any) {
+                        continue;
+                    }
+                    wave_sort64x4(key, lane);
+                    wave_merge_lower64x4(best, key, lane);
+                }
+            } else {
+                for (int n = 0; n < kTileAnchors; ++n) {
+                    const int index = s_index[n];
+                    if (index < 0) {
+                        continue;
+                    }
+                    const int count = s_nx[n] * s_ny[n];
+                    float* dst = out + static_cast<size_t>(index - anchor_base) * S.stride +
+                                 static_cast<size_t>(ti) * count;
+                    for (int i = tid; i < count; i += kFastThreads) {
+                        dst[i] = s_stage[n * S.per_frame + i];
+                    }
                 }
             }
         }
         __syncthreads();
+    }
+    if constexpr (kFused) {
+        // select_and_write over the retained keys: k = min(K, valid), the
+        // threshold max(tau, d_k), the prefix within it (at most the buffer
+        // capacity of find_similar_patches, core.cpp:1279-1287), the K
+        // nearest first, the rest in scan order.
+        const long long window_cap = static_cast<long long>(d.search_window) * d.search_window * nt;
+        const int buffer_cap = static_cast<int>(window_cap < d.soft_cap ? window_cap : d.soft_cap);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int n = 4 * wave + q;
+            const int a = s_index[n];
+            if (a < 0) {
+                continue;
+            }
+            if (valid_count[q] == 0) {
+                if (lane == 0) {
+                    counts[a] = 0;
+                }
+                continue;
+            }
+            const int k = d.similar < valid_count[q] ? d.similar : valid_count[q];
+            const uint64_t key = best[q];
+            const uint32_t dist_key = static_cast<uint32_t>(key >> 32);
+            const uint32_t index = static_cast<uint32_t>(key & 0xFFFFFFFFu);
+            const float kth = __uint_as_float(static_cast<uint32_t>(readlane64(key, k - 1) >> 32));
+            const float threshold = d.tau < kth ? kth : d.tau;
+            const uint32_t threshold_key = distance_key(threshold);
+            const bool selected = lane < buffer_cap && key != kNoKey && dist_key <= threshold_key;
+            const int total = __popcll(__ballot(selected));
+            int slot = lane;
+            if (lane >= k) {
+                int before = 0;
+                for (int f = k; f < total; ++f) {
+                    const uint32_t other = static_cast<uint32_t>(__shfl(static_cast<int>(index), f, 64));
+                    before += other < index ? 1 : 0;
+                }
+                slot = k + before;
+            }
+            if (selected) {
+                const int per_frame = s_nx[n] * s_ny[n];
+                const int ti = static_cast<int>(index) / per_frame;
+                const int rem = static_cast<int>(index) - ti * per_frame;
+                const int yy = rem / s_nx[n];
+                const int cx = s_xl[n] + (rem - yy * s_nx[n]);
+                const int cy = s_yl[n] + yy;
+                const int frame = scheduled_frame(af, t_low, t_high, ti);
+                const size_t out_base = static_cast<size_t>(a) * d.cap;
+                positions[out_base + slot] = pack_position(cx, cy, frame);
+                distances[out_base + slot] = __uint_as_float(dist_key);
+            }
+            if (lane == 0) {
+                counts[a] = total;
+            }
+        }
     }
 }
 
@@ -602,6 +871,8 @@ FastShape fast_shape(const StageDesc& d, int max_cols, int max_rows) {
     S.copy_elems = S.region_rows * S.pitch;
     S.per_frame = d.search_window * d.search_window;
     S.stride = S.per_frame * (d.search_bwd + d.search_fwd + 1);
+    S.ebox_words = (max_cols * max_rows + 3) / 4 * 4;
+    S.scratch_keys = (S.per_frame + 1) / 2 * 2;
     return S;
 }
 
@@ -620,6 +891,14 @@ bool fast_match_supported(const StageDesc& d, int max_cols, int max_rows) {
            static_cast<long long>(max_cols) * max_rows <= static_cast<long long>(kEnergyPerThread) * kFastThreads &&
            static_cast<long long>(max_cols) * max_rows * 4 <= static_cast<long long>(2) * kCopies * S.copy_elems * 2 &&
            fast_match_shared_bytes(d, max_cols, max_rows) <= 80 * 1024;
+}
+
+bool fast_match_fused(const StageDesc& d, int max_cols, int max_rows) {
+    // The running lists hold 64 keys per anchor; the compaction scratch of
+    // four wavefronts shares the idle plane buffer with the energy box.
+    const FastShape S = fast_shape(d, max_cols, max_rows);
+    const long long scratch = static_cast<long long>(S.ebox_words) * 4 + 2LL * kTileAnchors * S.scratch_keys;
+    return d.cap <= 64 && d.similar <= 64 && scratch <= static_cast<long long>(2) * kCopies * S.copy_elems * 2;
 }
 
 size_t fast_match_shared_bytes(const StageDesc& d, int max_cols, int max_rows) {
@@ -651,6 +930,7 @@ hipError_t launch_fast_energy(const StageDesc& d, const float* source, float* of
 hipError_t launch_fast_match(const StageDesc& d, int max_cols, int max_rows, const float* source,
                              const float* offsets, const float* energy, const uint32_t* anchors,
                              const int* tiles, int tile_count, int anchor_base, float* staged,
+                             bool fused, int* counts, uint32_t* positions, float* distances,
                              hipStream_t stream) {
     if (tile_count <= 0) {
         return hipSuccess;
@@ -659,10 +939,18 @@ hipError_t launch_fast_match(const StageDesc& d, int max_cols, int max_rows, con
     const size_t shared = fast_match_shared_bytes(d, max_cols, max_rows);
     const int mtiles = max_cols * S.row_tiles;
     const int per_wave = (mtiles + kFastWaves - 1) / kFastWaves;
-#define VNLBHIP_FAST(T)                                                                          \
-    hipLaunchKernelGGL(match_fast_kernel<T>, dim3(static_cast<unsigned>(tile_count)),            \
-                       dim3(kFastThreads), shared, stream, d, S, source, offsets, energy, anchors, \
-                       tiles, anchor_base, staged)
+#define VNLBHIP_FAST(T)                                                                            \
+    do {                                                                                           \
+        if (fused) {                                                                               \
+            hipLaunchKernelGGL((match_fast_kernel<T, true>), dim3(static_cast<unsigned>(tile_count)), \
+                               dim3(kFastThreads), shared, stream, d, S, source, offsets, energy,  \
+                               anchors, tiles, anchor_base, staged, counts, positions, distances); \
+        } else {                                                                                   \
+            hipLaunchKernelGGL((match_fast_kernel<T, false>), dim3(static_cast<unsigned>(tile_count)), \
+                               dim3(kFastThreads), shared, stream, d, S, source, offsets, energy,  \
+                               anchors, tiles, anchor_base, staged, counts, positions, distances); \
+        }                                                                                          \
+    } while (0)
     if (per_wave <= 8) {
         VNLBHIP_FAST(8);
     } else if (per_wave <= 12) {

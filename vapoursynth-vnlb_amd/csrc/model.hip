@@ -221,27 +221,6 @@ __global__ __launch_bounds__(64) void jacobi_kernel(
 // parallel-order Jacobi (FAST form)
 // ---------------------------------------------------------------------------
 
-// v from lane (lane ^ kMask) of the same 16-lane row, with DPP moves only:
-// row_mirror is xor 15, row_half_mirror xor 7 and quad_perm any xor below 4.
-template <int kMask>
-__device__ inline float xor16(float value) {
-    static_assert(kMask >= 0 && kMask < 16, "xor within a row of 16 lanes");
-    constexpr int kAfterMirror = (kMask & 8) != 0 ? (kMask ^ 15) : kMask;
-    constexpr int kQuad = (kAfterMirror & 4) != 0 ? (kAfterMirror ^ 7) : kAfterMirror;
-    constexpr int kQuadCtrl = (0 ^ kQuad) | ((1 ^ kQuad) << 2) | ((2 ^ kQuad) << 4) | ((3 ^ kQuad) << 6);
-    int x = __builtin_bit_cast(int, value);
-    if constexpr ((kMask & 8) != 0) {
-        x = __builtin_amdgcn_update_dpp(0, x, 0x140, 0xF, 0xF, false);  // row_mirror
-    }
-    if constexpr ((kAfterMirror & 4) != 0) {
-        x = __builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, false);  // row_half_mirror
-    }
-    if constexpr (kQuad != 0) {
-        x = __builtin_amdgcn_update_dpp(0, x, kQuadCtrl, 0xF, 0xF, false);
-    }
-    return __builtin_bit_cast(float, x);
-}
-
 // Column pair (j, j ^ k) of the relative row: column x = r ^ j pairs with
 // x ^ k and takes the rotation of lane x.
 template <int kK, int kJ>

@@ -128,7 +128,10 @@ __global__ __launch_bounds__(256) void paste_targets_kernel(
             }
         });
     }
-    target_counts[a] = n;
+    // An anchor without matches (every candidate distance NaN) is not a
+    // group and clears nothing (process_anchor_impl skips it,
+    // core.cpp:1851-1903): marked with a negative target count.
+    target_counts[a] = similar > 0 ? n : -1;
 }
 
 // The serial raster walk, 64 anchors at a time: a scalar scan over the
@@ -182,7 +185,7 @@ __global__ __launch_bounds__(kPasteThreads) void paste_kernel(
             prefetch(base + 64);
         }
 
-        const bool start = lane < n && ((s_mask[a >> 5] >> (a & 31)) & 1u) != 0;
+        const bool start = lane < n && my_n >= 0 && ((s_mask[a >> 5] >> (a & 31)) & 1u) != 0;
         const unsigned long long start_bits = __ballot(start);
         const unsigned dep_lo = static_cast<unsigned>(my_dep);
         const unsigned dep_hi = static_cast<unsigned>(my_dep >> 32);

@@ -585,50 +585,60 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
                 int* d_tiles = ctx.fast_tiles.reserve(tiles.slots.size());
                 check(hipMemcpyAsync(d_tiles, tiles.slots.data(), tiles.slots.size() * sizeof(int),
                                      hipMemcpyHostToDevice, stream), "upload tiles");
-                // Distances are staged for a chunk of anchor frames at a time
-                // (tiles and anchors are both in frame order) and selected
-                // before the next chunk reuses the buffer.
-                const size_t stride = vnlbhip::fast_match_stride(d);
-                const int chunk_anchors = static_cast<int>(std::max<size_t>(
-                    static_cast<size_t>(max_anchors), ctx.fast_stage_budget / stride));
-                float* staged = ctx.fast_staged.reserve(static_cast<size_t>(
-                    std::min(chunk_anchors, anchor_count)) * stride);
-                int tile_begin = 0;
-                int frame = 0;
-                while (frame < frame_count) {
-                    const int anchor_begin = grid.offsets[static_cast<size_t>(frame)];
-                    int frame_end = frame + 1;
-                    while (frame_end < frame_count &&
-                           grid.offsets[static_cast<size_t>(frame_end + 1)] - anchor_begin <= chunk_anchors) {
-                        ++frame_end;
-                    }
-                    const int anchor_end = grid.offsets[static_cast<size_t>(frame_end)];
-                    int tile_end = tile_begin;
-                    while (tile_end < tile_count) {
-                        int first = -1;
-                        for (int slot = 0; slot < 16 && first < 0; ++slot) {
-                            first = tiles.slots[static_cast<size_t>(tile_end) * 16 + slot];
-                        }
-                        if (first >= anchor_end) {
-                            break;
-                        }
-                        ++tile_end;
-                    }
+                if (vnlbhip::fast_match_fused(d, tiles.max_cols, tiles.max_rows)) {
+                    // Selection inside the matcher: one launch over every tile.
                     timed(ctx, VNLBHIP_KERNEL_MATCH, stream, "fast match kernel", [&]() {
                         return vnlbhip::launch_fast_match(d, tiles.max_cols, tiles.max_rows, source,
-                                                          fast_offsets, fast_energy, d_anchors,
-                                                          d_tiles + static_cast<size_t>(tile_begin) * 16,
-                                                          tile_end - tile_begin, anchor_begin, staged, stream);
+                                                          fast_offsets, fast_energy, d_anchors, d_tiles,
+                                                          tile_count, 0, nullptr, true, d_counts,
+                                                          d_positions, d_distances, stream);
                     });
-                    timed(ctx, VNLBHIP_KERNEL_SELECT, stream, "select kernel", [&]() {
-                        return vnlbhip::launch_fast_select(d, staged, d_anchors + anchor_begin,
-                                                           anchor_end - anchor_begin, d_counts + anchor_begin,
-                                                           d_positions + static_cast<size_t>(anchor_begin) * cap,
-                                                           d_distances + static_cast<size_t>(anchor_begin) * cap,
-                                                           stream);
-                    });
-                    tile_begin = tile_end;
-                    frame = frame_end;
+                } else {
+                    // Distances are staged for a chunk of anchor frames at a
+                    // time (tiles and anchors are both in frame order) and
+                    // selected before the next chunk reuses the buffer.
+                    const size_t stride = vnlbhip::fast_match_stride(d);
+                    const int chunk_anchors = static_cast<int>(std::max<size_t>(
+                        static_cast<size_t>(max_anchors), ctx.fast_stage_budget / stride));
+                    float* staged = ctx.fast_staged.reserve(static_cast<size_t>(
+                        std::min(chunk_anchors, anchor_count)) * stride);
+                    int tile_begin = 0;
+                    int frame = 0;
+                    while (frame < frame_count) {
+                        const int anchor_begin = grid.offsets[static_cast<size_t>(frame)];
+                        int frame_end = frame + 1;
+                        while (frame_end < frame_count &&
+                               grid.offsets[static_cast<size_t>(frame_end + 1)] - anchor_begin <= chunk_anchors) {
+                            ++frame_end;
+                        }
+                        const int anchor_end = grid.offsets[static_cast<size_t>(frame_end)];
+                        int tile_end = tile_begin;
+                        while (tile_end < tile_count) {
+                            int first = -1;
+                            for (int slot = 0; slot < 16 && first < 0; ++slot) {
+                                first = tiles.slots[static_cast<size_t>(tile_end) * 16 + slot];
+                            }
+                            if (first >= anchor_end) {
+                                break;
+                            }
+                            ++tile_end;
+                        }
+                        timed(ctx, VNLBHIP_KERNEL_MATCH, stream, "fast match kernel", [&]() {
+                            return vnlbhip::launch_fast_match(d, tiles.max_cols, tiles.max_rows, source,
+                                                              fast_offsets, fast_energy, d_anchors,
+                                                              d_tiles + static_cast<size_t>(tile_begin) * 16,
+                                                              tile_end - tile_begin, anchor_begin, staged,
+                                                              false, nullptr, nullptr, nullptr, stream);
+                        });
+                        timed(ctx, VNLBHIP_KERNEL_SELECT, stream, "select kernel", [&]() {
+                            return vnlbhip::launch_fast_select(
+                                d, staged, d_anchors + anchor_begin, anchor_end - anchor_begin,
+                                d_counts + anchor_begin, d_positions + static_cast<size_t>(anchor_begin) * cap,
+                                d_distances + static_cast<size_t>(anchor_begin) * cap, stream);
+                        });
+                        tile_begin = tile_end;
+                        frame = frame_end;
+                    }
                 }
                 // The tile list must outlive its upload.
                 check(hipStreamSynchronize(stream), "hipStreamSynchronize");
