@@ -342,6 +342,15 @@ Tiles build_tiles(const StageDesc& d, const Grid& grid, int height) {
     return tiles;
 }
 
+// One batch of anchor frames: its grid and, once the MFMA matcher has
+// needed them, its anchor tiles.
+struct BatchPlan {
+    Grid grid;  // offsets end with the batch's anchor count
+    int max_anchors = 0;
+    bool tiles_built = false;
+    Tiles tiles;
+};
+
 template <class T> struct DeviceBuffer {
     T* ptr = nullptr;
     size_t capacity = 0;
@@ -423,6 +432,8 @@ struct vnlbhip_context {
     DeviceBuffer<float> fast_energy;
     DeviceBuffer<int> fast_tiles;
     DeviceBuffer<float> fast_staged;
+    std::vector<long long> plan_key;  // see run_stage
+    std::vector<BatchPlan> plans;
     bool profiling = false;
     vnlbhip_kernel_times times{};
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
@@ -530,29 +541,48 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
     }
     const int cap = d.cap;
     const int n_stride = d.jacobi_dim;
-    size_t next = 0;
-    while (next < frames.size()) {
-        // Gather frames into a batch within the entry budget.
-        Grid grid;
-        int max_anchors = 0;
+    // Anchor grids of the batches (and, lazily, their matcher tiles) depend
+    // only on the geometry, the grid step, the search window, the cap and the
+    // frame list: Basic and Final, and repeated calls, reuse them.
+    const std::vector<long long> plan_key = [&]() {
+        std::vector<long long> key = {g.width, g.height, g.frames, d.step, d.patch, d.patch_time,
+                                      d.search_window, cap, static_cast<long long>(ctx.batch_entry_budget)};
+        key.insert(key.end(), frames.begin(), frames.end());
+        return key;
+    }();
+    if (ctx.plan_key != plan_key) {
+        ctx.plan_key.clear();
+        ctx.plans.clear();
+        size_t next = 0;
         while (next < frames.size()) {
-            const size_t before = grid.anchors.size();
-            append_grid_frame(d, frames[next], grid);
-            const int added = static_cast<int>(grid.anchors.size() - before);
-            const bool over = grid.anchors.size() * static_cast<size_t>(cap) > ctx.batch_entry_budget;
-            if (over && grid.offsets.size() > 1) {
-                // Roll back this frame; it starts the next batch.
-                grid.anchors.resize(before);
-                grid.offsets.pop_back();
-                grid.ids.pop_back();
-                grid.rows.resize(grid.rows.size() - static_cast<size_t>(3) * g.height);
-                break;
+            // Gather frames into a batch within the entry budget.
+            BatchPlan plan;
+            Grid& grid = plan.grid;
+            while (next < frames.size()) {
+                const size_t before = grid.anchors.size();
+                append_grid_frame(d, frames[next], grid);
+                const int added = static_cast<int>(grid.anchors.size() - before);
+                const bool over = grid.anchors.size() * static_cast<size_t>(cap) > ctx.batch_entry_budget;
+                if (over && grid.offsets.size() > 1) {
+                    // Roll back this frame; it starts the next batch.
+                    grid.anchors.resize(before);
+                    grid.offsets.pop_back();
+                    grid.ids.pop_back();
+                    grid.rows.resize(grid.rows.size() - static_cast<size_t>(3) * g.height);
+                    break;
+                }
+                plan.max_anchors = std::max(plan.max_anchors, added);
+                ++next;
             }
-            max_anchors = std::max(max_anchors, added);
-            ++next;
+            grid.offsets.push_back(static_cast<int>(grid.anchors.size()));
+            ctx.plans.push_back(std::move(plan));
         }
-        const int frame_count = static_cast<int>(grid.offsets.size());
-        grid.offsets.push_back(static_cast<int>(grid.anchors.size()));
+        ctx.plan_key = plan_key;
+    }
+    for (BatchPlan& plan : ctx.plans) {
+        const Grid& grid = plan.grid;
+        const int max_anchors = plan.max_anchors;
+        const int frame_count = static_cast<int>(grid.offsets.size()) - 1;
         const int anchor_count = static_cast<int>(grid.anchors.size());
         ctx.stats.anchors += anchor_count;
         ++ctx.stats.batches;
@@ -579,7 +609,11 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
 
         bool matched = false;
         if (fast_energy != nullptr) {
-            const Tiles tiles = build_tiles(d, grid, g.height);
+            if (!plan.tiles_built) {
+                plan.tiles = build_tiles(d, grid, g.height);
+                plan.tiles_built = true;
+            }
+            const Tiles& tiles = plan.tiles;
             if (vnlbhip::fast_match_supported(d, tiles.max_cols, tiles.max_rows)) {
                 const int tile_count = static_cast<int>(tiles.slots.size() / 16);
                 int* d_tiles = ctx.fast_tiles.reserve(tiles.slots.size());
