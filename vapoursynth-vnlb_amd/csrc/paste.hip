@@ -136,8 +136,8 @@ __global__ __launch_bounds__(256) void paste_targets_kernel(
 
 // The serial raster walk, 64 anchors at a time: a scalar scan over the
 // chunk's predecessor masks makes the sequential decisions exactly, then the
-// survivors clear their targets beyond the chunk.  The next chunk's inputs
-// are loaded while the current one is decided.
+// survivors clear their targets beyond the chunk.  The inputs of the next
+// two chunks are in flight while the current one is decided.
 constexpr int kPrefetchTargets = 24;
 
 __global__ __launch_bounds__(kPasteThreads) void paste_kernel(
@@ -156,33 +156,46 @@ __global__ __launch_bounds__(kPasteThreads) void paste_kernel(
     }
     __syncthreads();
 
-    unsigned long long next_dep = 0ull;
-    int next_n = 0;
-    int next_t[kPrefetchTargets];
-    const auto prefetch = [&](int base) {
+    struct Chunk {
+        unsigned long long dep;
+        int n;
+        int t[kPrefetchTargets];
+    };
+    const auto load = [&](int base, Chunk& c) {
         const int a = base + lane;
         const bool valid = a < count;
-        next_dep = valid ? dep[first + a] : 0ull;
-        next_n = valid ? target_counts[first + a] : 0;
-        const int* list = targets + static_cast<size_t>(first + (valid ? a : 0)) * target_stride;
+        c.dep = valid ? dep[first + a] : 0ull;
+        c.n = valid ? target_counts[first + a] : 0;
+        const int4* list = reinterpret_cast<const int4*>(targets + static_cast<size_t>(first + (valid ? a : 0)) * target_stride);
 #pragma unroll
-        for (int i = 0; i < kPrefetchTargets; ++i) {
-            next_t[i] = (valid && i < target_stride) ? list[i] : -1;
+        for (int h = 0; h < kPrefetchTargets / 4; ++h) {
+            int4 v = int4{-1, -1, -1, -1};
+            if (valid && 4 * h < target_stride) {
+                v = list[h];
+            }
+            c.t[4 * h + 0] = v.x;
+            c.t[4 * h + 1] = v.y;
+            c.t[4 * h + 2] = v.z;
+            c.t[4 * h + 3] = v.w;
         }
     };
-    prefetch(0);
+    Chunk next1;
+    Chunk next2;
+    load(0, next1);
+    load(64, next2);
     for (int base = 0; base < count; base += 64) {
         const int n = count - base < 64 ? count - base : 64;
         const int a = base + lane;
-        const unsigned long long my_dep = next_dep;
-        const int my_n = next_n;
+        const unsigned long long my_dep = next1.dep;
+        const int my_n = next1.n;
         int my_t[kPrefetchTargets];
 #pragma unroll
         for (int i = 0; i < kPrefetchTargets; ++i) {
-            my_t[i] = next_t[i];
+            my_t[i] = next1.t[i];
         }
-        if (base + 64 < count) {
-            prefetch(base + 64);
+        next1 = next2;
+        if (base + 128 < count) {
+            load(base + 128, next2);
         }
 
         const bool start = lane < n && my_n >= 0 && ((s_mask[a >> 5] >> (a & 31)) & 1u) != 0;
@@ -251,7 +264,7 @@ size_t paste_shared_bytes(const StageDesc& d, int max_anchors) {
 }
 
 int paste_target_stride(const StageDesc& d) {
-    return 5 * d.cap;
+    return (5 * d.cap + 3) / 4 * 4;  // rows of 16-byte multiples
 }
 
 hipError_t launch_paste(const StageDesc& d, int frame_count, const int* frame_offsets,
