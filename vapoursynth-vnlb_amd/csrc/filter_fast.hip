@@ -213,7 +213,8 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
     StageDesc d, const float* __restrict__ noisy, const float* __restrict__ basic, GroupRefs refs,
     int group_count, int problem_stride, const int* __restrict__ group_flags,
     const float* __restrict__ means, const float* __restrict__ eigenvalues,
-    const float* __restrict__ vectors, Accumulators acc, float* __restrict__ weights_out) {
+    const float* __restrict__ vectors, const float* __restrict__ grams, Accumulators acc,
+    float* __restrict__ weights_out) {
     extern __shared__ float4 s_dyn4[];
     float* s_dyn = reinterpret_cast<float*>(s_dyn4);
     const int dim = d.dim;  // == model dimension (one model)
@@ -400,8 +401,23 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                 model_count = 1;
             } else {
                 const int rank_used = modelled ? R : 0;
-                cross_gram(d, model_video, xmean_of, model_video, xmean_of, s_pos, similar, B, tab_in,
-                           s_gram, s_norm, lane);
+                if (modelled && similar == B) {
+                    // Every member is a model row: C = B G with the Gram the
+                    // model kernel built from the same centered rows.
+                    const float* gram = grams + static_cast<size_t>(g) * problem_stride * problem_stride;
+                    const float scale_b = static_cast<float>(B);
+                    for (int i = lane; i < B * B; i += 64) {
+                        const int r = i / B;
+                        const int s = i - r * B;
+                        s_gram[s * kFastRows + r] = scale_b * gram[r * problem_stride + s];
+                    }
+                    if (lane < B) {
+                        s_norm[lane] = scale_b * gram[lane * problem_stride + lane];
+                    }
+                } else {
+                    cross_gram(d, model_video, xmean_of, model_video, xmean_of, s_pos, similar, B, tab_in,
+                               s_gram, s_norm, lane);
+                }
                 wave_sync();
                 float score = INFINITY;
                 if (lane < similar) {
@@ -667,8 +683,8 @@ hipError_t launch_fast_filter(const StageDesc& d, const float* noisy, const floa
                               const uint32_t* anchors, const int* counts, const uint32_t* positions,
                               const int* group_list, int group_count, int problem_stride,
                               const int* group_flags, const float* means, const float* eigenvalues,
-                              const float* vectors, double* accumulators, float* weights_out,
-                              hipStream_t stream) {
+                              const float* vectors, const float* grams, double* accumulators,
+                              float* weights_out, hipStream_t stream) {
     if (group_count <= 0) {
         return hipSuccess;
     }
@@ -697,7 +713,7 @@ hipError_t launch_fast_filter(const StageDesc& d, const float* noisy, const floa
         const unsigned blocks = static_cast<unsigned>(resident < wanted ? resident : wanted);        \
         hipLaunchKernelGGL(filter_fast_kernel<KPT>, dim3(blocks), dim3(kModelThreads), shared,       \
                            stream, d, noisy, basic, refs, group_count, problem_stride, group_flags,  \
-                           means, eigenvalues, vectors, acc, weights_out);                           \
+                           means, eigenvalues, vectors, grams, acc, weights_out);                    \
     } while (0)
     switch (d.patch_time) {
         case 1: VNLBHIP_FAST_FILTER(1); break;

@@ -744,7 +744,7 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
             if (fast_filter) {
                 return vnlbhip::launch_fast_filter(d, noisy, basic, d_anchors, d_counts, d_positions,
                                                    d_list, groups, n_stride, d_flags, d_means, d_eigen,
-                                                   d_vectors, acc, d_weights, stream);
+                                                   d_vectors, d_problems, acc, d_weights, stream);
             }
             return vnlbhip::launch_filter(d, noisy, basic, d_anchors, d_counts, d_positions,
                                           d_list, groups, n_stride, d_flags, d_dims, d_means,
