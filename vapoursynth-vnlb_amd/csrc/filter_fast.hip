@@ -45,7 +45,7 @@ struct WaveLayout {
     int norm;   // [kFastRows] squared norms of the member rows
     int u;      // [16][16] eigenvectors u_j[s]
     int pos;    // [kFastRows]
-    int sorted; // per sample in source-slot order: [kFastRows] position, tile cell, weight
+    int sorted; // per sample in source-slot order: [kFastRows] position, tile cell, weight, plane offset
     int begin;  // [kMaxTemporal + 8] first sorted sample of each source slot
     int tile;   // [side][side][2] value plane and weight plane of one output frame
     int words;
@@ -68,7 +68,7 @@ __host__ __device__ inline WaveLayout wave_layout(const StageDesc& d) {
     L.pos = at;
     at += kFastRows;
     L.sorted = at;
-    at += 3 * kFastRows;
+    at += 4 * kFastRows;
     L.begin = at;
     at += (kMaxTemporal + 8 + 3) / 4 * 4;  // slots + patch frames <= kMaxTemporal + 4
     L.tile = at;
@@ -234,11 +234,13 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
     uint32_t* s_kpos = reinterpret_cast<uint32_t*>(s_wave + W.sorted);
     int* s_kcell = reinterpret_cast<int*>(s_wave + W.sorted + kFastRows);
     float* s_kw = s_wave + W.sorted + 2 * kFastRows;
+    int* s_koff = reinterpret_cast<int*>(s_wave + W.sorted + 3 * kFastRows);  // y * width + x of the origin
     int* s_begin = reinterpret_cast<int*>(s_wave + W.begin);
     float* s_tile = s_wave + W.tile;
     // Element e = c * patch_dim + 64 t + lane of an 8x8 patch sits at
     // (lane & 7, lane >> 3) for every channel and patch frame.
     const int lane_cell = (lane >> 3) * side + (lane & 7);
+    const int lane_off = (lane >> 3) * d.width + (lane & 7);
     const float lane_window = d.window[lane];
 
     for (int e = tid; e < dim; e += kModelThreads) {
@@ -322,6 +324,7 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
             if (my_k >= 0) {
                 s_kpos[my_k] = my_pos;
                 s_kcell[my_k] = (position_y(my_pos) - y_low) * side + position_x(my_pos) - x_low;
+                s_koff[my_k] = position_y(my_pos) * d.width + position_x(my_pos);
             }
         }
         // Centre of the estimate rows and mean of the model / member rows.
@@ -631,18 +634,18 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                 // the accumulators once; exchanges of four samples are in
                 // flight together.
                 const int frame = af - d.search_bwd + slot;
-                double* frame_base = acc.data + static_cast<size_t>(frame) * (d.channels + 1) * plane;
-                double* lane_base = frame_base + static_cast<size_t>(lane >> 3) * d.width + (lane & 7);
+                double* value_base = acc.data + (static_cast<size_t>(frame) * (d.channels + 1) + c) * plane + lane_off;
+                double* weight_base = value_base + static_cast<size_t>(d.channels - c) * plane;
                 for (int k0 = k_first; k0 < k_last; k0 += 4) {
                     unsigned long long taken[4];
-                    uint32_t origin[4];
+                    int offset[4];
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         taken[u] = 0;
-                        origin[u] = 0;
+                        offset[u] = 0;
                         if (k0 + u < k_last) {
                             const int cell = s_kcell[k0 + u] + lane_cell;
-                            origin[u] = s_kpos[k0 + u];
+                            offset[u] = s_koff[k0 + u];
                             taken[u] = atomicExch(reinterpret_cast<unsigned long long*>(s_tile) + cell, 0ULL);
                         }
                     }
@@ -650,12 +653,9 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                     for (int u = 0; u < 4; ++u) {
                         if (taken[u] != 0ULL) {
                             const float2 v = __builtin_bit_cast(float2, taken[u]);
-                            double* base = lane_base + static_cast<size_t>(position_y(origin[u])) * d.width +
-                                           position_x(origin[u]);
-                            unsafeAtomicAdd(base + static_cast<size_t>(c) * plane, static_cast<double>(v.x));
+                            unsafeAtomicAdd(value_base + offset[u], static_cast<double>(v.x));
                             if (c == 0) {
-                                unsafeAtomicAdd(base + static_cast<size_t>(d.channels) * plane,
-                                                static_cast<double>(v.y));
+                                unsafeAtomicAdd(weight_base + offset[u], static_cast<double>(v.y));
                             }
                         }
                     }
