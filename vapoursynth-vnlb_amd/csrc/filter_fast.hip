@@ -207,6 +207,21 @@ __device__ inline void cross_gram(const StageDesc& d, const float* __restrict__ 
     }
 }
 
+// sum_s c[s] u[s] in ascending s, u a 16-float LDS row read as four float4.
+__device__ inline float dot16(const float (&c)[16], const float* u) {
+    const float4* u4 = reinterpret_cast<const float4*>(u);
+    float p = 0.0F;
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        const float4 uv = u4[h];
+        p = fmaf(c[4 * h + 0], uv.x, p);
+        p = fmaf(c[4 * h + 1], uv.y, p);
+        p = fmaf(c[4 * h + 2], uv.z, p);
+        p = fmaf(c[4 * h + 3], uv.w, p);
+    }
+    return p;
+}
+
 // kPt: frames of a patch (64-element chunks of one channel's model row).
 template <int kPt>
 __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
@@ -355,10 +370,11 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
             volume_term = lhat > 0.0F ? log1p_via_double(lhat / membership_noise2) : 0.0F;
         }
         if (fit) {
-            for (int i = lane; i < R * B; i += 64) {
-                const int j = i / B;
-                const int s = i - j * B;
-                s_u[j * 16 + s] = vectors[(static_cast<size_t>(g) * R + j) * problem_stride + s];
+            // U zero-padded to 16 x 16 so rows are read as four float4.
+            for (int i = lane; i < 16 * 16; i += 64) {
+                const int j = i >> 4;
+                const int s = i & 15;
+                s_u[i] = (j < R && s < B) ? vectors[(static_cast<size_t>(g) * R + j) * problem_stride + s] : 0.0F;
             }
         }
         // accumulate_aggregation_model (core.cpp:326-412): group terms, in
@@ -426,6 +442,11 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                 if (lane < similar) {
                     float distance = 0.0F;
                     float projected = 0.0F;
+                    float crow[kFastModel];
+#pragma unroll
+                    for (int s = 0; s < kFastModel; ++s) {
+                        crow[s] = s < B ? s_gram[s * kFastRows + lane] : 0.0F;
+                    }
                     for (int j = 0; j < rank_used; ++j) {
                         const float inv_j = __builtin_bit_cast(float, __builtin_amdgcn_readlane(
                                                                           __builtin_bit_cast(int, inv), j));
@@ -433,9 +454,7 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                                                                        __builtin_bit_cast(int, lhat), j));
                         float p = 0.0F;
                         if (inv_j != 0.0F) {
-                            for (int s = 0; s < B; ++s) {
-                                p = fmaf(s_gram[s * kFastRows + lane], s_u[j * 16 + s], p);
-                            }
+                            p = dot16(crow, s_u + j * 16);
                             p = p * inv_j;
                         }
                         if (estimate_is_member && j < kFastModel) {
@@ -491,15 +510,18 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                            s_norm, lane);
                 wave_sync();
                 if (lane < similar) {
+                    float crow[kFastModel];
+#pragma unroll
+                    for (int s = 0; s < kFastModel; ++s) {
+                        crow[s] = s < B ? s_gram[s * kFastRows + lane] : 0.0F;
+                    }
 #pragma unroll
                     for (int j = 0; j < kFastModel; ++j) {
                         const float inv_j = __builtin_bit_cast(float, __builtin_amdgcn_readlane(
                                                                           __builtin_bit_cast(int, inv), j));
                         float p = 0.0F;
                         if (j < R && inv_j != 0.0F) {
-                            for (int s = 0; s < B; ++s) {
-                                p = fmaf(s_gram[s * kFastRows + lane], s_u[j * 16 + s], p);
-                            }
+                            p = dot16(crow, s_u + j * 16);
                             p = p * inv_j;
                         }
                         p_est[j] = p;
@@ -519,9 +541,14 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                         float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, coef * inv), j));
                     if (j < R && scale_j != 0.0F) {
                         const float pj = p_est[j] * scale_j;
+                        const float4* u4 = reinterpret_cast<const float4*>(s_u + j * 16);
 #pragma unroll
-                        for (int s = 0; s < kFastModel; ++s) {
-                            wrow[s] = fmaf(pj, s_u[j * 16 + s], wrow[s]);
+                        for (int h = 0; h < kFastModel / 4; ++h) {
+                            const float4 uv = u4[h];
+                            wrow[4 * h + 0] = fmaf(pj, uv.x, wrow[4 * h + 0]);
+                            wrow[4 * h + 1] = fmaf(pj, uv.y, wrow[4 * h + 1]);
+                            wrow[4 * h + 2] = fmaf(pj, uv.z, wrow[4 * h + 2]);
+                            wrow[4 * h + 3] = fmaf(pj, uv.w, wrow[4 * h + 3]);
                         }
                     }
                 }
