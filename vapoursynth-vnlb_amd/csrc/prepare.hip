@@ -126,6 +126,9 @@ __global__ __launch_bounds__(kModelThreads) void prepare_kernel(
     const int md = d.model_dim;
     const bool final_stage = d.stage == kFinal;
     const bool want_flat = final_stage && d.flat_areas != 0;
+    // Slots per channel when one model covers whole 16-element slots of each
+    // channel (coupled, patch_dim a multiple of 16), else 0.
+    const int channel_runs = (d.models == 1 && d.patch_dim % kLanes == 0) ? d.patch_dim / kLanes : 0;
 
     for (int i = tid; i < width; i += kModelThreads) {
         const int m = i / row;
@@ -177,7 +180,26 @@ __global__ __launch_bounds__(kModelThreads) void prepare_kernel(
         for (int r = team; r < rows; r += kTeams) {
             float* dst = s_rows + r * width;
             load_row<kS, kExact>(d, L, noisy, patch_base(d, s_pos[r]), tab_in, dst, lane);
-            if (want_flat) {
+            if (want_flat && channel_runs > 0) {
+                // One model over whole 16-element slots per channel: the
+                // lane's slots k of channel c are c * runs .. (c + 1) * runs - 1
+                // (same per-channel order as below).
+                const int at = lane * pitch;
+#pragma unroll
+                for (int cc = 0; cc < kMaxChannels; ++cc) {
+                    if (cc < d.channels) {
+                        double s0 = sums[cc][0];
+                        double s1 = sums[cc][1];
+                        for (int k = cc * channel_runs; k < (cc + 1) * channel_runs; ++k) {
+                            const double v = static_cast<double>(dst[at + k]);
+                            s0 += v;
+                            s1 += v * v;
+                        }
+                        sums[cc][0] = s0;
+                        sums[cc][1] = s1;
+                    }
+                }
+            } else if (want_flat) {
                 for (int m = 0; m < d.models; ++m) {
                     const int at = m * row + lane * pitch;
                     for (int k = 0; k <= chunks; ++k) {
@@ -369,9 +391,14 @@ __global__ __launch_bounds__(kModelThreads) void prepare_kernel(
                 // the 16-lane order, one team per entry.
                 const int pairs = rows * (rows + 1) / 2;
                 for (int q = team; q < pairs; q += kTeams) {
-                    int r = 0;
+                    // Row of the triangular index q: the float estimate,
+                    // corrected to the exact integer.
+                    int r = static_cast<int>((sqrtf(8.0F * static_cast<float>(q) + 1.0F) - 1.0F) * 0.5F);
                     while ((r + 1) * (r + 2) / 2 <= q) {
                         ++r;
+                    }
+                    while (r > 0 && r * (r + 1) / 2 > q) {
+                        --r;
                     }
                     const int c = q - r * (r + 1) / 2;
                     const float* left = s_rows + r * width + m * row + lane * pitch;
