@@ -176,11 +176,10 @@ __device__ inline bf16x8 load_bf16x8(const __bf16* p) {
 template <int kK, int kJ>
 __device__ inline uint64_t bitonic_step(uint64_t key, int lane) {
     const uint64_t other = xor64<kJ>(key);
-    const bool ascending = (lane & kK) == 0;
-    const bool lower = (lane & kJ) == 0;
-    const uint64_t low = other < key ? other : key;
-    const uint64_t high = other < key ? key : other;
-    return lower == ascending ? low : high;
+    // Keep the smaller key where the lane is the lower end of an ascending
+    // pair or the upper end of a descending one (keys are distinct).
+    const bool want_low = ((lane & kJ) == 0) == ((lane & kK) == 0);
+    return (other < key) == want_low ? other : key;
 }
 
 // Wave-wide bitonic sort of 64 keys, one per lane, ascending by lane.
@@ -232,13 +231,10 @@ __device__ inline void bitonic_step4(uint64_t (&key)[4], int lane) {
     for (int q = 0; q < 4; ++q) {
         other[q] = xor64<kJ>(key[q]);
     }
-    const bool ascending = (lane & kK) == 0;
-    const bool lower = (lane & kJ) == 0;
+    const bool want_low = ((lane & kJ) == 0) == ((lane & kK) == 0);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const uint64_t low = other[q] < key[q] ? other[q] : key[q];
-        const uint64_t high = other[q] < key[q] ? key[q] : other[q];
-        key[q] = lower == ascending ? low : high;
+        key[q] = (other[q] < key[q]) == want_low ? other[q] : key[q];
     }
 }
 
