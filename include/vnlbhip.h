@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define VNLBHIP_ABI_VERSION 2
+#define VNLBHIP_ABI_VERSION 3
 
 enum {
     VNLBHIP_STAGE_BASIC = 0,
@@ -74,6 +74,7 @@ typedef struct vnlbhip_stats {
     int32_t batches;              /* anchor-frame batches of the last call */
     int32_t reserved;
     int64_t retained;             /* retained samples summed over the groups */
+    int64_t refined;              /* FAST form: eigen problems re-solved in fp64 */
 } vnlbhip_stats;
 
 typedef struct vnlbhip_context vnlbhip_context;
@@ -172,6 +173,14 @@ enum {
 };
 
 int vnlbhip_set_form(vnlbhip_context* context, int form);
+
+/* FAST form: group models whose eigenvalue gap at the rank cutoff (the
+ * rank-th minus the next eigenvalue, both of the fp32 solve) is below
+ * relative_gap times the largest eigenvalue are re-solved in fp64 (no
+ * counterpart in the reference, whose solver runs at one precision).
+ * Default 1e-3; 0 disables; negative or non-finite values are rejected with
+ * VNLBHIP_ERROR_INVALID.  Only models with rank < rows have a cutoff. */
+int vnlbhip_set_eigen_refine(vnlbhip_context* context, float relative_gap);
 
 #ifdef __cplusplus
 }

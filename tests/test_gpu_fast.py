@@ -188,3 +188,44 @@ def test_fast_form_nan_region(contexts):
     got = fast.stage(BASIC, bp, x).cpu().numpy()
     np.testing.assert_array_equal(np.isnan(got), np.isnan(want))
     assert plain.stats()["groups"] > 0
+
+
+def test_eigen_refinement_in_fp64(gpu, oracle_lib):
+    """Rank 4 of 16 model rows puts a cutoff inside every spectrum.  With the
+    gap threshold at 1e30 every modelled group is re-solved by the fp64
+    Jacobi: the refined count equals the modelled groups, eigenvalues match
+    the oracle's float64 model to 1e-5 of the largest, and the output stays
+    within 0.05 dB of the fp32 solve's; threshold 0 refines none."""
+    noisy, clean = synth.make_clip(width=96, height=80, frames=5, channels=3, sigma8=20, seed=41,
+                                   motion="subpixel")
+    bp = parse_stage_parameters(BASIC, DENOISE)
+    ctx = vnlbhip.Context(0)
+    ctx.set_form("fast")
+    try:
+        ctx.set_eigen_refine(0.0)
+        base = ctx.stage(BASIC, bp, _dev(noisy)).cpu().numpy()
+        assert ctx.stats()["refined"] == 0
+        ctx.set_eigen_refine(1e30)
+        print("[fast] eigen refinement", flush=True)
+        refined = ctx.stage(BASIC, bp, _dev(noisy)).cpu().numpy()
+        stats = ctx.stats()
+        got = ctx.debug_frame(BASIC, bp, _dev(noisy), None, 2)
+        with pytest.raises(vnlbhip.VNLBError):
+            ctx.set_eigen_refine(-1.0)
+    finally:
+        ctx.close()
+    assert stats["refined"] == stats["groups"], stats
+    want = oracle_lib.frame_debug(BASIC, bp, noisy, None, 2,
+                                  flags=oracle_lib.ACCUM_DOUBLE | oracle_lib.MODEL_DOUBLE)
+    compared = 0
+    for a, n in enumerate(want["counts"]):
+        if not (want["active"][a] and got["active"][a]) or (want["group_flags"][a] & 4) == 0:
+            continue
+        if got["counts"][a] != n or not np.array_equal(got["positions"][a, :n], want["positions"][a, :n]):
+            continue
+        w = want["eigenvalues"][a].astype(np.float64)
+        g = got["eigenvalues"][a].astype(np.float64)
+        assert np.max(np.abs(g - w)) <= 1e-5 * np.max(np.abs(w)), (a, g, w)
+        compared += 1
+    assert compared >= 20, compared
+    assert abs(synth.psnr(refined, clean) - synth.psnr(base, clean)) <= PSNR_TOL

@@ -21,6 +21,14 @@ __global__ __launch_bounds__(256) void fill_zero_kernel(double* __restrict__ dat
     }
 }
 
+__global__ __launch_bounds__(256) void fill_zero_int_kernel(int* __restrict__ data, size_t count) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+         i += stride) {
+        data[i] = 0;
+    }
+}
+
 __global__ __launch_bounds__(256) void normalize_kernel(StageDesc d,
                                                         const double* __restrict__ acc,
                                                         const float* __restrict__ source,
@@ -102,6 +110,14 @@ hipError_t launch_fill_zero(double* data, size_t count, hipStream_t stream) {
         return hipSuccess;
     }
     hipLaunchKernelGGL(fill_zero_kernel, dim3(grid_for(count)), dim3(256), 0, stream, data, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_zero_int(int* data, size_t count, hipStream_t stream) {
+    if (count == 0) {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(fill_zero_int_kernel, dim3(grid_for(count)), dim3(256), 0, stream, data, count);
     return hipGetLastError();
 }
 

@@ -93,6 +93,14 @@ __device__ inline float wave_min(float v) {
     return v;
 }
 
+__device__ inline double wave_max_double(double v) {
+#pragma unroll
+    for (int offset = 32; offset >= 1; offset >>= 1) {
+        v = fmax(v, __shfl_xor(v, offset, 64));
+    }
+    return v;
+}
+
 __device__ inline double wave_sum_double(double v) {
 #pragma unroll
     for (int offset = 32; offset >= 1; offset >>= 1) {

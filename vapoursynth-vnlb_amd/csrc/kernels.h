@@ -58,10 +58,14 @@ hipError_t launch_jacobi(const StageDesc& d, int problem_count, int problem_stri
                          float* eigenvalues, float* vectors, int* status, hipStream_t stream);
 // FAST form: parallel-order Jacobi for problems of at most 16 rows.
 bool jacobi_parallel_supported(int problem_stride);
+// refine_list / refine_count (zeroed by the caller; may be null): problems
+// whose relative eigenvalue gap at the rank cutoff is below refine_gap are
+// listed and re-solved in fp64 by a second kernel on the same stream.
 hipError_t launch_jacobi_parallel(const StageDesc& d, int problem_count, int problem_stride,
                                   const int* group_flags, const int* problem_dims,
                                   const float* problems, float* eigenvalues, float* vectors,
-                                  int* status, hipStream_t stream);
+                                  int* status, float refine_gap, int* refine_list, int* refine_count,
+                                  int* refined_total, hipStream_t stream);
 hipError_t launch_filter(const StageDesc& d, const float* noisy, const float* basic,
                          const uint32_t* anchors, const int* counts, const uint32_t* positions,
                          const int* group_list, int group_count, int problem_stride,
@@ -81,6 +85,7 @@ hipError_t launch_fast_filter(const StageDesc& d, const float* noisy, const floa
                               float* weights_out, hipStream_t stream);
 
 hipError_t launch_fill_zero(double* data, size_t count, hipStream_t stream);
+hipError_t launch_fill_zero_int(int* data, size_t count, hipStream_t stream);
 hipError_t launch_normalize(const StageDesc& d, const double* acc, const float* source, float* out,
                             hipStream_t stream);
 

@@ -315,7 +315,8 @@ __device__ inline void parallel_sweep(float (&a)[16], float (&v)[16], int r, boo
 __global__ __launch_bounds__(64) void jacobi_parallel_kernel(
     StageDesc d, int problem_count, int problem_stride, const int* __restrict__ group_flags,
     const int* __restrict__ problem_dims, const float* __restrict__ problems,
-    float* __restrict__ eigenvalues, float* __restrict__ vectors, int* __restrict__ status) {
+    float* __restrict__ eigenvalues, float* __restrict__ vectors, int* __restrict__ status,
+    float refine_gap, int* __restrict__ refine_list, int* __restrict__ refine_count) {
     constexpr int kPerWave = 4;
     constexpr int kStride = 17;
     __shared__ float s_v[kPerWave][16 * kStride];
@@ -439,6 +440,171 @@ __global__ __launch_bounds__(64) void jacobi_parallel_kernel(
     }
     if (valid && r == 0) {
         status[problem] = sweeps | (converged ? 0 : (1 << 16));
+        // A rank cutoff inside the spectrum whose gap is small against the
+        // largest eigenvalue: the kept subspace is re-solved in fp64.
+        const int rank = d.rank;
+        if (rank < n && rank > 0 && refine_list != nullptr) {
+            const float top = fabsf(s_diag[sub][s_order[sub][0]]);
+            const float gap = s_diag[sub][s_order[sub][rank - 1]] - s_diag[sub][s_order[sub][rank]];
+            if (!(gap >= refine_gap * fmaxf(top, FLT_MIN))) {
+                refine_list[atomicAdd(refine_count, 1)] = problem;
+            }
+        }
+    }
+}
+
+// fp64 re-solve of the flagged problems (cyclic Jacobi in double, the
+// reference's solver at its double instantiation, linalg.cpp:447-550: same
+// rotation, threshold 64 eps max(1, max|A|) with the double eps, stopping
+// rule, ordering and canonical vectors), results rounded to float.  One
+// problem per wavefront at a time, lane r owning row r; the list length is
+// read on the device, so no host round trip is needed.
+__global__ __launch_bounds__(64) void jacobi_refine_kernel(
+    StageDesc d, int problem_stride, const int* __restrict__ problem_dims,
+    const float* __restrict__ problems, const int* __restrict__ refine_list,
+    const int* __restrict__ refine_count, float* __restrict__ eigenvalues,
+    float* __restrict__ vectors, int* __restrict__ refined_total) {
+    constexpr int kN = 16;
+    constexpr int kStride = kN + 1;
+    __shared__ double s_m[kN * kStride];
+    __shared__ double s_v[kN * kStride];
+    __shared__ int s_order[kN];
+    const int r = threadIdx.x;
+    const int total = *refine_count;
+    if (blockIdx.x == 0 && r == 0 && refined_total != nullptr) {
+        atomicAdd(refined_total, total);
+    }
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        const int problem = refine_list[item];
+        const int group = problem / d.models;
+        const int n = problem_dims[group];
+        const float* in = problems + static_cast<size_t>(problem) * problem_stride * problem_stride;
+        __syncthreads();
+        double row_max = 0.0;
+        if (r < kN) {
+            for (int c = 0; c < kN; ++c) {
+                double value = 0.0;
+                if (r < n && c < n) {
+                    value = static_cast<double>(in[r * problem_stride + c]);
+                    row_max = fmax(row_max, fabs(value));
+                }
+                s_m[r * kStride + c] = value;
+                s_v[r * kStride + c] = (r == c) ? 1.0 : 0.0;
+            }
+        }
+        row_max = wave_max_double(row_max);
+        const double threshold = (64.0 * DBL_EPSILON) * (row_max > 1.0 ? row_max : 1.0);
+        __syncthreads();
+        for (int sweep = 0; sweep < d.max_sweeps; ++sweep) {
+            double off = 0.0;
+            if (r < n) {
+                for (int c = r + 1; c < n; ++c) {
+                    off = fmax(off, fabs(s_m[r * kStride + c]));
+                }
+            }
+            off = wave_max_double(off);
+            if (off <= threshold) {
+                break;
+            }
+            for (int p = 0; p + 1 < n; ++p) {
+                for (int q = p + 1; q < n; ++q) {
+                    const double apq = s_m[p * kStride + q];
+                    const bool rotate = fabs(apq) > threshold;  // uniform
+                    double c = 1.0;
+                    double sn = 0.0;
+                    double new_pp = 0.0;
+                    double new_qq = 0.0;
+                    double new_rp = 0.0;
+                    double new_rq = 0.0;
+                    double new_vp = 0.0;
+                    double new_vq = 0.0;
+                    if (rotate) {
+                        const double app = s_m[p * kStride + p];
+                        const double aqq = s_m[q * kStride + q];
+                        const double tau = (aqq - app) / (2.0 * apq);
+                        const double t_abs = 1.0 / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        const double t = tau < 0.0 ? -t_abs : t_abs;
+                        c = 1.0 / sqrt(1.0 + t * t);
+                        sn = t * c;
+                        const double cc = c * c;
+                        const double ss = sn * sn;
+                        const double two_cs = 2.0 * (c * sn);
+                        new_pp = ((cc * app) - (two_cs * apq)) + (ss * aqq);
+                        new_qq = ((ss * app) + (two_cs * apq)) + (cc * aqq);
+                        if (r < n && r != p && r != q) {
+                            const double arp = s_m[r * kStride + p];
+                            const double arq = s_m[r * kStride + q];
+                            new_rp = (c * arp) - (sn * arq);
+                            new_rq = (sn * arp) + (c * arq);
+                        }
+                        if (r < n) {
+                            const double vrp = s_v[r * kStride + p];
+                            const double vrq = s_v[r * kStride + q];
+                            new_vp = (c * vrp) - (sn * vrq);
+                            new_vq = (sn * vrp) + (c * vrq);
+                        }
+                    }
+                    __syncthreads();
+                    if (rotate) {
+                        if (r < n && r != p && r != q) {
+                            s_m[r * kStride + p] = new_rp;
+                            s_m[p * kStride + r] = new_rp;
+                            s_m[r * kStride + q] = new_rq;
+                            s_m[q * kStride + r] = new_rq;
+                        }
+                        if (r == 0) {
+                            s_m[p * kStride + p] = new_pp;
+                            s_m[q * kStride + q] = new_qq;
+                            s_m[p * kStride + q] = 0.0;
+                            s_m[q * kStride + p] = 0.0;
+                        }
+                        if (r < n) {
+                            s_v[r * kStride + p] = new_vp;
+                            s_v[r * kStride + q] = new_vq;
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+        }
+        if (r < n) {
+            const double dr = s_m[r * kStride + r];
+            int rank = 0;
+            for (int j = 0; j < n; ++j) {
+                const double dj = s_m[j * kStride + j];
+                rank += (dj > dr || (j < r && !(dr > dj))) ? 1 : 0;
+            }
+            s_order[rank] = r;
+        }
+        __syncthreads();
+        if (r < d.rank && r < n) {
+            const int col = s_order[r];
+            float* vec_out = vectors + (static_cast<size_t>(problem) * d.rank + r) * problem_stride;
+            double norm_squared = 0.0;
+            double max_abs = 0.0;
+            int pivot = 0;
+            for (int i = 0; i < n; ++i) {
+                const double value = s_v[i * kStride + col];
+                norm_squared += value * value;
+                if (fabs(value) > max_abs) {
+                    max_abs = fabs(value);
+                    pivot = i;
+                }
+            }
+            double scale = 1.0;
+            const bool rescale = norm_squared > DBL_MIN;
+            if (rescale) {
+                scale = 1.0 / sqrt(norm_squared);
+                if (s_v[pivot * kStride + col] < 0.0) {
+                    scale = -scale;
+                }
+            }
+            for (int i = 0; i < n; ++i) {
+                const double value = s_v[i * kStride + col];
+                vec_out[i] = static_cast<float>(rescale ? value * scale : value);
+            }
+            eigenvalues[static_cast<size_t>(problem) * d.rank + r] = static_cast<float>(s_m[col * kStride + col]);
+        }
     }
 }
 
@@ -483,7 +649,8 @@ bool jacobi_parallel_supported(int problem_stride) {
 hipError_t launch_jacobi_parallel(const StageDesc& d, int problem_count, int problem_stride,
                                   const int* group_flags, const int* problem_dims,
                                   const float* problems, float* eigenvalues, float* vectors,
-                                  int* status, hipStream_t stream) {
+                                  int* status, float refine_gap, int* refine_list, int* refine_count,
+                                  int* refined_total, hipStream_t stream) {
     if (problem_count <= 0) {
         return hipSuccess;
     }
@@ -492,7 +659,14 @@ hipError_t launch_jacobi_parallel(const StageDesc& d, int problem_count, int pro
     }
     hipLaunchKernelGGL(jacobi_parallel_kernel, dim3(static_cast<unsigned>((problem_count + 3) / 4)),
                        dim3(64), 0, stream, d, problem_count, problem_stride, group_flags,
-                       problem_dims, problems, eigenvalues, vectors, status);
+                       problem_dims, problems, eigenvalues, vectors, status, refine_gap, refine_list,
+                       refine_count);
+    if (refine_list != nullptr && d.rank > 0) {
+        const int blocks = problem_count < 2048 ? problem_count : 2048;
+        hipLaunchKernelGGL(jacobi_refine_kernel, dim3(static_cast<unsigned>(blocks)), dim3(64), 0, stream, d,
+                           problem_stride, problem_dims, problems, refine_list, refine_count, eigenvalues,
+                           vectors, refined_total);
+    }
     return hipGetLastError();
 }
 

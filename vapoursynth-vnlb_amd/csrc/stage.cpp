@@ -418,6 +418,8 @@ struct vnlbhip_context {
     DeviceBuffer<float> eigenvalues;
     DeviceBuffer<float> vectors;
     DeviceBuffer<int> status;
+    DeviceBuffer<int> refine_list;
+    float refine_gap = 1e-3F;  // FAST form: relative eigen gap below which groups are re-solved in fp64
     DeviceBuffer<float> weights;
     DeviceBuffer<double> accumulators;
     DeviceBuffer<int> summary;
@@ -523,8 +525,8 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
     const size_t acc_count = plane * (g.channels + 1) * g.frames;
     double* acc = ctx.accumulators.reserve(acc_count);
     check(vnlbhip::launch_fill_zero(acc, acc_count, stream), "fill accumulators");
-    int* summary = ctx.summary.reserve(4);
-    check(hipMemsetAsync(summary, 0, 4 * sizeof(int), stream), "hipMemsetAsync");
+    int* summary = ctx.summary.reserve(8);
+    check(hipMemsetAsync(summary, 0, 8 * sizeof(int), stream), "hipMemsetAsync");
 
     const float* source = stage == VNLBHIP_STAGE_FINAL ? ref : noisy;
     // MFMA matcher: patch energies of the matched source, once per stage.
@@ -729,11 +731,18 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
         if (d.rank > 0) {
             const bool parallel_jacobi =
                 ctx.form == VNLBHIP_FORM_FAST && vnlbhip::jacobi_parallel_supported(n_stride);
+            int* d_refine = nullptr;
+            if (parallel_jacobi && ctx.refine_gap > 0.0F) {
+                // refine_list[0] counts, the list follows.
+                d_refine = ctx.refine_list.reserve(static_cast<size_t>(problems) + 1);
+                check(vnlbhip::launch_fill_zero_int(d_refine, 1, stream), "clear refine count");
+            }
             timed(ctx, VNLBHIP_KERNEL_JACOBI, stream, "jacobi kernel", [&]() {
                 if (parallel_jacobi) {
-                    return vnlbhip::launch_jacobi_parallel(d, problems, n_stride, d_flags, d_dims,
-                                                           d_problems, d_eigen, d_vectors, d_status,
-                                                           stream);
+                    return vnlbhip::launch_jacobi_parallel(
+                        d, problems, n_stride, d_flags, d_dims, d_problems, d_eigen, d_vectors, d_status,
+                        ctx.refine_gap, d_refine != nullptr ? d_refine + 1 : nullptr, d_refine,
+                        summary + 4, stream);
                 }
                 return vnlbhip::launch_jacobi(d, problems, n_stride, d_flags, d_dims, d_problems,
                                               d_eigen, d_vectors, d_status, stream);
@@ -838,7 +847,7 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
         timed(ctx, VNLBHIP_KERNEL_NORMALIZE, stream, "normalize kernel",
               [&]() { return vnlbhip::launch_normalize(d, acc, noisy, out, stream); });
     }
-    int host_summary[4] = {0, 0, 0, 0};
+    int host_summary[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     check(hipMemcpyAsync(host_summary, summary, sizeof(host_summary), hipMemcpyDeviceToHost, stream),
           "read summary");
     check(hipStreamSynchronize(stream), "hipStreamSynchronize");
@@ -847,6 +856,7 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
     int64_t retained = 0;
     std::memcpy(&retained, host_summary + 2, sizeof(retained));
     ctx.stats.retained += retained;
+    ctx.stats.refined = host_summary[4];
     if (host_summary[0] != 0) {
         throw ModelFailure("group_size must cover the effective VNLB rank");
     }
@@ -1004,6 +1014,14 @@ int vnlbhip_set_form(vnlbhip_context* context, int form) {
         return VNLBHIP_ERROR_INVALID;
     }
     context->form = form;
+    return VNLBHIP_OK;
+}
+
+int vnlbhip_set_eigen_refine(vnlbhip_context* context, float relative_gap) {
+    if (context == nullptr || !(relative_gap >= 0.0F) || !std::isfinite(relative_gap)) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    context->refine_gap = relative_gap;
     return VNLBHIP_OK;
 }
 

@@ -71,7 +71,7 @@ class Context:
         self._lib.vnlbhip_get_stats(self._handle, ctypes.byref(s))
         return {"anchors": s.anchors, "groups": s.groups,
                 "unconverged_groups": s.unconverged_groups, "batches": s.batches,
-                "retained": s.retained}
+                "retained": s.retained, "refined": s.refined}
 
     def set_form(self, form: str) -> None:
         """'plain' (fp32 in the CPU path's order, the default) or 'fast'
@@ -82,6 +82,14 @@ class Context:
         if rc != 0:
             self._raise(rc)
         self.form = form
+
+    def set_eigen_refine(self, relative_gap: float) -> None:
+        """Fast form: re-solve in fp64 the group models whose eigenvalue gap at
+        the rank cutoff is below relative_gap times the largest eigenvalue
+        (default 1e-3, 0 disables)."""
+        rc = self._lib.vnlbhip_set_eigen_refine(self._handle, float(relative_gap))
+        if rc != 0:
+            raise VNLBError("VNLB HIP: relative_gap must be finite and non-negative")
 
     def set_profiling(self, enable: bool) -> None:
         self._lib.vnlbhip_set_profiling(self._handle, 1 if enable else 0)

@@ -26,6 +26,7 @@ EXPORTED_SYMBOLS = (
     "vnlbhip_set_profiling",
     "vnlbhip_get_kernel_times",
     "vnlbhip_set_form",
+    "vnlbhip_set_eigen_refine",
 )
 
 KERNEL_NAMES = ("match", "paste", "prepare", "jacobi", "filter", "normalize", "select", "energy")
@@ -67,6 +68,7 @@ class Stats(ctypes.Structure):
         ("batches", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
         ("retained", ctypes.c_int64),
+        ("refined", ctypes.c_int64),
     ]
 
 
@@ -135,6 +137,8 @@ def load() -> ctypes.CDLL:
     lib.vnlbhip_set_profiling.restype = ctypes.c_int
     lib.vnlbhip_set_form.argtypes = [vp, ctypes.c_int]
     lib.vnlbhip_set_form.restype = ctypes.c_int
+    lib.vnlbhip_set_eigen_refine.argtypes = [vp, ctypes.c_float]
+    lib.vnlbhip_set_eigen_refine.restype = ctypes.c_int
     lib.vnlbhip_get_kernel_times.argtypes = [vp, P(KernelTimes)]
     lib.vnlbhip_get_kernel_times.restype = ctypes.c_int
     _lib = lib
