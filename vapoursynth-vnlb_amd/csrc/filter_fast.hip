@@ -45,7 +45,7 @@ struct WaveLayout {
     int norm;   // [kFastRows] squared norms of the member rows
     int u;      // [16][16] eigenvectors u_j[s]
     int pos;    // [kFastRows]
-    int sorted; // per sample in source-slot order: [kFastRows] position, tile cell, weight, plane offset
+    int sorted; // per sample in source-slot order: [kFastRows] position, tile cell, weight, plane offset, lane
     int begin;  // [kMaxTemporal + 8] first sorted sample of each source slot
     int tile;   // [side][side][2] value plane and weight plane of one output frame
     int words;
@@ -68,7 +68,7 @@ __host__ __device__ inline WaveLayout wave_layout(const StageDesc& d) {
     L.pos = at;
     at += kFastRows;
     L.sorted = at;
-    at += 4 * kFastRows;
+    at += 5 * kFastRows;
     L.begin = at;
     at += (kMaxTemporal + 8 + 3) / 4 * 4;  // slots + patch frames <= kMaxTemporal + 4
     L.tile = at;
@@ -250,6 +250,7 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
     int* s_kcell = reinterpret_cast<int*>(s_wave + W.sorted + kFastRows);
     float* s_kw = s_wave + W.sorted + 2 * kFastRows;
     int* s_koff = reinterpret_cast<int*>(s_wave + W.sorted + 3 * kFastRows);  // y * width + x of the origin
+    int* s_klane = reinterpret_cast<int*>(s_wave + W.sorted + 4 * kFastRows);  // the sample's lane
     int* s_begin = reinterpret_cast<int*>(s_wave + W.begin);
     float* s_tile = s_wave + W.tile;
     // Element e = c * patch_dim + 64 t + lane of an 8x8 patch sits at
@@ -340,6 +341,7 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                 s_kpos[my_k] = my_pos;
                 s_kcell[my_k] = (position_y(my_pos) - y_low) * side + position_x(my_pos) - x_low;
                 s_koff[my_k] = position_y(my_pos) * d.width + position_x(my_pos);
+                s_klane[my_k] = lane;
             }
         }
         // Centre of the estimate rows and mean of the model / member rows.
@@ -502,7 +504,13 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
             }
         }
 
-        // Projections of the estimate rows and W = P diag(coef / sqrt(B lambda)) U^T.
+        // Projections of the estimate rows and W = P diag(coef / sqrt(B lambda)) U^T:
+        // lane r keeps row r of W.
+        float wrow[kFastModel];
+#pragma unroll
+        for (int s = 0; s < kFastModel; ++s) {
+            wrow[s] = 0.0F;
+        }
         if (fit) {
             if (!have_estimate) {
                 wave_sync();
@@ -530,11 +538,6 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
             }
             wave_sync();
             if (lane < similar) {
-                float wrow[kFastModel];
-#pragma unroll
-                for (int s = 0; s < kFastModel; ++s) {
-                    wrow[s] = 0.0F;
-                }
 #pragma unroll
                 for (int j = 0; j < kFastModel; ++j) {
                     const float scale_j = __builtin_bit_cast(
@@ -552,17 +555,26 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                         }
                     }
                 }
-                if (my_k >= 0) {
-                    float4* w4 = reinterpret_cast<float4*>(s_gram + my_k * kFastModel);
 #pragma unroll
-                    for (int h = 0; h < kFastModel / 4; ++h) {
-                        w4[h] = float4{4 * h + 0 < B ? wrow[4 * h + 0] : 0.0F, 4 * h + 1 < B ? wrow[4 * h + 1] : 0.0F,
-                                       4 * h + 2 < B ? wrow[4 * h + 2] : 0.0F, 4 * h + 3 < B ? wrow[4 * h + 3] : 0.0F};
-                    }
+                for (int s = 0; s < kFastModel; ++s) {
+                    wrow[s] = s < B ? wrow[s] : 0.0F;
                 }
             }
         }
         wave_sync();
+        // The sorted tables in registers, sample k in lane k, so the loops
+        // below take them with v_readlane instead of LDS broadcasts.
+        const int sorted_count = __builtin_amdgcn_readfirstlane(s_begin[sorted_slots]);
+        int kcell_l = 0;
+        int koff_l = 0;
+        int klane_l = 0;
+        float kw_l = 0.0F;
+        if (lane < sorted_count) {
+            kcell_l = s_kcell[lane];
+            koff_l = s_koff[lane];
+            klane_l = s_klane[lane];
+            kw_l = s_kw[lane];
+        }
 
         // Estimate and weighted aggregation: per channel and output frame of
         // the group's window, contributions are combined in the tile (value
@@ -605,8 +617,8 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
             for (int slot = 0; slot < d.slots; ++slot) {
                 // Samples of source slot slot - t feed this output frame
                 // through patch frame t; s_begin is indexed by q + kPt - 1.
-                const int k_first = s_begin[slot];
-                const int k_last = s_begin[slot + kPt];
+                const int k_first = __builtin_amdgcn_readfirstlane(s_begin[slot]);
+                const int k_last = __builtin_amdgcn_readfirstlane(s_begin[slot + kPt]);
                 if (k_first == k_last) {
                     continue;
                 }
@@ -630,21 +642,21 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                         }
                         continue;
                     }
-#pragma unroll 2
-                    for (int k = k_begin; k < k_end; ++k) {
-                        const float4* w4 = reinterpret_cast<const float4*>(s_gram + k * kFastModel);
-                        const int cell = s_kcell[k] + lane_cell;
-                        const float cw = s_kw[k] * lane_window;
+                    for (int k = __builtin_amdgcn_readfirstlane(k_begin); k < __builtin_amdgcn_readfirstlane(k_end);
+                         ++k) {
+                        const int cell = __builtin_amdgcn_readlane(kcell_l, k) + lane_cell;
+                        const float cw =
+                            __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, kw_l), k)) *
+                            lane_window;
                         float value = center[tt];
                         if (modelled) {
+                            const int r = __builtin_amdgcn_readlane(klane_l, k);
                             float sum = 0.0F;
 #pragma unroll
-                            for (int h = 0; h < kFastModel / 4; ++h) {
-                                const float4 wv = w4[h];
-                                sum = fmaf(wv.x, x[tt][4 * h + 0], sum);
-                                sum = fmaf(wv.y, x[tt][4 * h + 1], sum);
-                                sum = fmaf(wv.z, x[tt][4 * h + 2], sum);
-                                sum = fmaf(wv.w, x[tt][4 * h + 3], sum);
+                            for (int s2 = 0; s2 < kFastModel; ++s2) {
+                                const float w_s = __builtin_bit_cast(
+                                    float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, wrow[s2]), r));
+                                sum = fmaf(w_s, x[tt][s2], sum);
                             }
                             value = value + sum;
                         }
@@ -671,8 +683,8 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                         taken[u] = 0;
                         offset[u] = 0;
                         if (k0 + u < k_last) {
-                            const int cell = s_kcell[k0 + u] + lane_cell;
-                            offset[u] = s_koff[k0 + u];
+                            const int cell = __builtin_amdgcn_readlane(kcell_l, k0 + u) + lane_cell;
+                            offset[u] = __builtin_amdgcn_readlane(koff_l, k0 + u);
                             taken[u] = atomicExch(reinterpret_cast<unsigned long long*>(s_tile) + cell, 0ULL);
                         }
                     }
