@@ -207,6 +207,23 @@ __device__ inline void cross_gram(const StageDesc& d, const float* __restrict__ 
     }
 }
 
+// Adds one contribution to the wavefront's tile cell (value plane; weight
+// plane too when with_weight).  The 64 lanes of a call address 64 distinct
+// cells (the pixels of one patch) and the wavefront's LDS operations
+// complete in issue order, so a plain read-modify-write is exact, in the
+// order of the calls, without LDS atomics.
+__device__ inline void tile_add(float* s_tile, int cell, float value, float weight, bool with_weight) {
+    if (with_weight) {
+        float2* t2 = reinterpret_cast<float2*>(s_tile) + cell;
+        float2 t = *t2;
+        t.x = t.x + value;
+        t.y = t.y + weight;
+        *t2 = t;
+    } else {
+        s_tile[2 * cell] = s_tile[2 * cell] + value;
+    }
+}
+
 // sum_s c[s] u[s] in ascending s, u a 16-float LDS row read as four float4.
 __device__ inline float dot16(const float (&c)[16], const float* u) {
     const float4* u4 = reinterpret_cast<const float4*>(u);
@@ -635,10 +652,7 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                             const int cell = s_kcell[k] + lane_cell;
                             const float cw = s_kw[k] * lane_window;
                             const float value = noisy[patch_base(d, s_kpos[k]) + in_off];
-                            atomicAdd(&s_tile[2 * cell], cw * value);
-                            if (c == 0) {
-                                atomicAdd(&s_tile[2 * cell + 1], cw);
-                            }
+                            tile_add(s_tile, cell, cw * value, cw, c == 0);
                         }
                         continue;
                     }
@@ -660,18 +674,16 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                             }
                             value = value + sum;
                         }
-                        atomicAdd(&s_tile[2 * cell], cw * value);
-                        if (c == 0) {
-                            atomicAdd(&s_tile[2 * cell + 1], cw);
-                        }
+                        tile_add(s_tile, cell, cw * value, cw, c == 0);
                     }
                 }
                 wave_sync();
                 // Flush: the cells touched in this pass are those of the
-                // samples that fed it.  Each is taken with an LDS exchange
-                // against zero, so a cell shared by several samples reaches
-                // the accumulators once; exchanges of four samples are in
-                // flight together.
+                // samples that fed it.  Each is read and cleared in sample
+                // order (LDS operations of a wavefront complete in issue
+                // order), so a cell shared by several samples reaches the
+                // accumulators once; four samples' reads are in flight
+                // together.
                 const int frame = af - d.search_bwd + slot;
                 double* value_base = acc.data + (static_cast<size_t>(frame) * (d.channels + 1) + c) * plane + lane_off;
                 double* weight_base = value_base + static_cast<size_t>(d.channels - c) * plane;
@@ -685,7 +697,9 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                         if (k0 + u < k_last) {
                             const int cell = __builtin_amdgcn_readlane(kcell_l, k0 + u) + lane_cell;
                             offset[u] = __builtin_amdgcn_readlane(koff_l, k0 + u);
-                            taken[u] = atomicExch(reinterpret_cast<unsigned long long*>(s_tile) + cell, 0ULL);
+                            unsigned long long* slot64 = reinterpret_cast<unsigned long long*>(s_tile) + cell;
+                            taken[u] = *slot64;
+                            *slot64 = 0ULL;
                         }
                     }
 #pragma unroll
