@@ -576,6 +576,14 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                 for (int s = 0; s < kFastModel; ++s) {
                     wrow[s] = s < B ? wrow[s] : 0.0F;
                 }
+                if (my_k >= 0) {
+                    // Row-major in sorted order for the broadcast reads below.
+                    float4* w4 = reinterpret_cast<float4*>(s_gram + my_k * kFastModel);
+#pragma unroll
+                    for (int h = 0; h < kFastModel / 4; ++h) {
+                        w4[h] = float4{wrow[4 * h + 0], wrow[4 * h + 1], wrow[4 * h + 2], wrow[4 * h + 3]};
+                    }
+                }
             }
         }
         wave_sync();
@@ -664,13 +672,15 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                             lane_window;
                         float value = center[tt];
                         if (modelled) {
-                            const int r = __builtin_amdgcn_readlane(klane_l, k);
+                            const float4* w4 = reinterpret_cast<const float4*>(s_gram + k * kFastModel);
                             float sum = 0.0F;
 #pragma unroll
-                            for (int s2 = 0; s2 < kFastModel; ++s2) {
-                                const float w_s = __builtin_bit_cast(
-                                    float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, wrow[s2]), r));
-                                sum = fmaf(w_s, x[tt][s2], sum);
+                            for (int h = 0; h < kFastModel / 4; ++h) {
+                                const float4 wv = w4[h];
+                                sum = fmaf(wv.x, x[tt][4 * h + 0], sum);
+                                sum = fmaf(wv.y, x[tt][4 * h + 1], sum);
+                                sum = fmaf(wv.z, x[tt][4 * h + 2], sum);
+                                sum = fmaf(wv.w, x[tt][4 * h + 3], sum);
                             }
                             value = value + sum;
                         }
