@@ -52,7 +52,7 @@ hipError_t launch_prepare(const StageDesc& d, const float* noisy, const float* b
                           const uint32_t* anchors, const int* counts, const uint32_t* positions,
                           const int* group_list, int group_count, int problem_stride,
                           int* group_flags, int* problem_dims, float* means, float* problems,
-                          hipStream_t stream);
+                          bool mfma_gram, hipStream_t stream);
 hipError_t launch_jacobi(const StageDesc& d, int problem_count, int problem_stride,
                          const int* group_flags, const int* problem_dims, const float* problems,
                          float* eigenvalues, float* vectors, int* status, hipStream_t stream);

@@ -17,6 +17,8 @@
 #include "group.h"
 #include "kernels.h"
 
+#include <hip/hip_bf16.h>
+
 namespace vnlbhip {
 namespace {
 
@@ -99,7 +101,12 @@ __device__ inline void load_row(const StageDesc& d, const PrepareLayout& L,
     }
 }
 
-template <int kS, bool kExact>
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
+
+// kMfma (FAST form): the dual Gram on the matrix cores in bf16x3 instead of
+// the 16-lane fp32 team dots.
+template <int kS, bool kExact, bool kMfma>
 __global__ __launch_bounds__(kModelThreads) void prepare_kernel(
     StageDesc d, PrepareLayout L, const float* __restrict__ noisy,
     const float* __restrict__ basic, GroupRefs refs, int group_count, int problem_stride,
@@ -386,7 +393,53 @@ __global__ __launch_bounds__(kModelThreads) void prepare_kernel(
         const float scale = 1.0F / static_cast<float>(rows);
         for (int m = 0; m < d.models; ++m) {
             float* out = problems + (static_cast<size_t>(g) * d.models + m) * problem_stride * problem_stride;
-            if (dual) {
+            if (kMfma && dual) {
+                // G = X X^T / rows with X the centered model rows (at most 16),
+                // one wavefront: v_mfma_f32_16x16x32_bf16 over the elements in
+                // blocks of 8 slots of one lane column (any element order gives
+                // the same sum; A and B share the lane's registers since
+                // B = A^T), three products hi.hi + hi.lo + lo.hi.
+                if (tid < 64) {
+                    const int col = tid & 15;
+                    const int kb = tid >> 4;
+                    const int blocks_per_lane = kS / 8;
+                    floatx4 acc = {0.0F, 0.0F, 0.0F, 0.0F};
+                    for (int q = 0; q < kS / 2; ++q) {
+                        const int b = 4 * q + kb;
+                        const int src_lane = b / blocks_per_lane;
+                        const int k0 = 8 * (b - src_lane * blocks_per_lane);
+                        float4 lo4 = {0.0F, 0.0F, 0.0F, 0.0F};
+                        float4 hi4 = {0.0F, 0.0F, 0.0F, 0.0F};
+                        if (col < rows) {  // rows past the model rows are not staged
+                            const float4* src = reinterpret_cast<const float4*>(
+                                s_rows + col * width + m * row + src_lane * pitch + k0);
+                            lo4 = src[0];
+                            hi4 = src[1];
+                        }
+                        const float v[8] = {lo4.x, lo4.y, lo4.z, lo4.w, hi4.x, hi4.y, hi4.z, hi4.w};
+                        bf16x8 a_hi;
+                        bf16x8 a_lo;
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) {
+                            const float x = v[e];
+                            const __bf16 h = static_cast<__bf16>(x);
+                            a_hi[e] = h;
+                            a_lo[e] = static_cast<__bf16>(x - static_cast<float>(h));
+                        }
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, a_hi, acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, a_lo, acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_lo, a_hi, acc, 0, 0, 0);
+                    }
+                    // acc[v] = G[4 kb + v][col]
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int r = 4 * kb + v;
+                        if (r < rows && col < rows) {
+                            out[r * problem_stride + col] = scale * acc[v];
+                        }
+                    }
+                }
+            } else if (dual) {
                 // compute_gram_centered (linalg.cpp:326-354): scale * dot in
                 // the 16-lane order, one team per entry.
                 const int pairs = rows * (rows + 1) / 2;
@@ -477,7 +530,7 @@ hipError_t launch_prepare(const StageDesc& d, const float* noisy, const float* b
                           const uint32_t* anchors, const int* counts, const uint32_t* positions,
                           const int* group_list, int group_count, int problem_stride,
                           int* group_flags, int* problem_dims, float* means, float* problems,
-                          hipStream_t stream) {
+                          bool mfma_gram, hipStream_t stream) {
     if (group_count <= 0) {
         return hipSuccess;
     }
@@ -495,10 +548,13 @@ hipError_t launch_prepare(const StageDesc& d, const float* noisy, const float* b
     }
     const int slots = template_slots(d.model_dim);
     const bool exact = d.model_dim == kLanes * slots;
-#define VNLBHIP_PREPARE(KS, EXACT)                                                                 \
+    // The matrix-core Gram needs whole 8-slot blocks and at most 16 rows.
+    const bool mfma = mfma_gram && exact && slots % 8 == 0 && d.max_rows <= 16;
+#define VNLBHIP_PREPARE3(KS, EXACT, MFMA)                                                          \
     do {                                                                                           \
         int per_unit = 0;                                                                          \
-        status = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_unit, prepare_kernel<KS, EXACT>, \
+        status = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_unit,                           \
+                                                              prepare_kernel<KS, EXACT, MFMA>,     \
                                                               kModelThreads, shared);              \
         if (status != hipSuccess) {                                                                \
             return status;                                                                         \
@@ -506,10 +562,16 @@ hipError_t launch_prepare(const StageDesc& d, const float* noisy, const float* b
         const long long resident = static_cast<long long>(per_unit > 0 ? per_unit : 1) * units;    \
         const unsigned blocks =                                                                    \
             static_cast<unsigned>(resident < group_count ? resident : group_count);                \
-        hipLaunchKernelGGL((prepare_kernel<KS, EXACT>), dim3(blocks), dim3(kModelThreads), shared, \
+        hipLaunchKernelGGL((prepare_kernel<KS, EXACT, MFMA>), dim3(blocks), dim3(kModelThreads), shared, \
                            stream, d, L, noisy, basic, refs, group_count, problem_stride,          \
                            group_flags, problem_dims, means, problems);                            \
     } while (0)
+#define VNLBHIP_PREPARE(KS, EXACT)                                                                 \
+    if (EXACT && mfma) {                                                                           \
+        VNLBHIP_PREPARE3(KS, EXACT, true);                                                         \
+    } else {                                                                                       \
+        VNLBHIP_PREPARE3(KS, EXACT, false);                                                        \
+    }
 #define VNLBHIP_PREPARE_SLOTS(KS)      \
     if (exact) {                       \
         VNLBHIP_PREPARE(KS, true);     \
@@ -526,6 +588,7 @@ hipError_t launch_prepare(const StageDesc& d, const float* noisy, const float* b
     }
 #undef VNLBHIP_PREPARE_SLOTS
 #undef VNLBHIP_PREPARE
+#undef VNLBHIP_PREPARE3
     return hipGetLastError();
 }
 

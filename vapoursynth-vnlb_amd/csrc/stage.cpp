@@ -726,7 +726,8 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
         timed(ctx, VNLBHIP_KERNEL_PREPARE, stream, "prepare kernel", [&]() {
             return vnlbhip::launch_prepare(d, noisy, basic, d_anchors, d_counts, d_positions,
                                            d_list, groups, n_stride, d_flags, d_dims, d_means,
-                                           d_problems, stream);
+                                           d_problems,
+                                           ctx.form == VNLBHIP_FORM_FAST, stream);
         });
         if (d.rank > 0) {
             const bool parallel_jacobi =
