@@ -101,6 +101,53 @@ __device__ inline void load_row(const StageDesc& d, const PrepareLayout& L,
     }
 }
 
+// Means over the model rows of four consecutive slots per thread (kExact:
+// every slot k < kS holds an element), in row order as the scalar loop does,
+// written to mean_out; optionally the rows are centered and the equality of
+// every row with row 0 is tracked.
+template <int kS>
+__device__ inline void slot_means4(const StageDesc& d, const PrepareLayout& L, float* s_rows, int rows,
+                                   float inv_rows, float* mean_out, bool center, bool& equal_local,
+                                   int tid) {
+    constexpr int kGroups = kS / 4;  // float4 groups per lane slot row
+    const int width = L.width;
+    const int total = d.models * kLanes * kGroups;
+    for (int t = tid; t < total; t += kModelThreads) {
+        const int m = t / (kLanes * kGroups);
+        const int rem = t - m * kLanes * kGroups;
+        const int l = rem / kGroups;
+        const int k = 4 * (rem - l * kGroups);
+        const int i = m * L.row + l * L.pitch + k;
+        const float4 v0 = *reinterpret_cast<const float4*>(s_rows + i);
+        float4 acc = {0.0F, 0.0F, 0.0F, 0.0F};
+        for (int r = 0; r < rows; ++r) {
+            const float4 v = *reinterpret_cast<const float4*>(s_rows + r * width + i);
+            equal_local = equal_local && v.x == v0.x && v.y == v0.y && v.z == v0.z && v.w == v0.w;
+            acc.x = acc.x + v.x;
+            acc.y = acc.y + v.y;
+            acc.z = acc.z + v.z;
+            acc.w = acc.w + v.w;
+        }
+        const float4 mean = {acc.x * inv_rows, acc.y * inv_rows, acc.z * inv_rows, acc.w * inv_rows};
+        float* out = mean_out + (d.coupled ? 0 : m * d.patch_dim) + kLanes * k + l;
+        out[0] = mean.x;
+        out[kLanes] = mean.y;
+        out[2 * kLanes] = mean.z;
+        out[3 * kLanes] = mean.w;
+        if (center) {
+            for (int r = 0; r < rows; ++r) {
+                float4* v = reinterpret_cast<float4*>(s_rows + r * width + i);
+                float4 x = *v;
+                x.x = x.x - mean.x;
+                x.y = x.y - mean.y;
+                x.z = x.z - mean.z;
+                x.w = x.w - mean.w;
+                *v = x;
+            }
+        }
+    }
+}
+
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
 
@@ -233,7 +280,10 @@ __global__ __launch_bounds__(kModelThreads) void prepare_kernel(
         float* mean_noisy = means + static_cast<size_t>(g) * 2 * d.dim;
         float* mean_basic = mean_noisy + d.dim;
         bool equal_local = true;
-        for (int i = tid; i < width; i += kModelThreads) {
+        if (kExact) {
+            slot_means4<kS>(d, L, s_rows, rows, inv_rows, mean_noisy, !final_stage, equal_local, tid);
+        }
+        for (int i = tid; i < (kExact ? 0 : width); i += kModelThreads) {
             if (tab_c[i] < 0) {
                 continue;
             }
@@ -366,7 +416,11 @@ __global__ __launch_bounds__(kModelThreads) void prepare_kernel(
                 load_row<kS, kExact>(d, L, basic, patch_base(d, s_pos[r]), tab_in, s_rows + r * width, lane);
             }
             __syncthreads();
-            for (int i = tid; i < width; i += kModelThreads) {
+            if (kExact) {
+                bool unused = true;
+                slot_means4<kS>(d, L, s_rows, rows, inv_rows, mean_basic, true, unused, tid);
+            }
+            for (int i = tid; i < (kExact ? 0 : width); i += kModelThreads) {
                 if (tab_c[i] < 0) {
                     continue;
                 }
