@@ -282,13 +282,28 @@ __device__ inline void wave_merge_lower64x4(uint64_t (&best)[4], const uint64_t 
     }
 }
 
+constexpr uint64_t kNoKey = ~static_cast<uint64_t>(0);
+
 __device__ inline uint64_t readlane64(uint64_t v, int lane) {
     const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v & 0xFFFFFFFFu), lane));
     const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v >> 32), lane));
     return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
-constexpr uint64_t kNoKey = ~static_cast<uint64_t>(0);
+// Exclusive bound on the keys that can still be selected, for a running
+// list `best` (ascending over the lanes): a key at or above the 64th can not
+// enter it, and a key above both the K-th so far and tau can not be
+// selected (the final threshold max(tau, d_K) only falls as candidates
+// arrive, and k = min(K, valid) <= K).
+__device__ inline uint64_t admit_bound(uint64_t best, int similar, uint64_t tau_key) {
+    const uint64_t last = readlane64(best, 63);
+    const uint64_t kth = readlane64(best, similar - 1);
+    const uint64_t reach = kth > tau_key ? kth : tau_key;
+    // keys <= reach may be selected: as an exclusive bound, reach + 1
+    // (reach < kNoKey unless nothing is held yet).
+    const uint64_t open = reach == kNoKey ? kNoKey : reach + 1;
+    return open < last ? open : last;
+}
 
 // Cross terms of one anchor tile against its candidates, all candidate
 // frames.  kFused = false: distances land in `out` ([anchor][stride], scan
@@ -525,6 +540,8 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
 
     floatx4 acc[kTilesPerWave];
     uint64_t best[4] = {kNoKey, kNoKey, kNoKey, kNoKey};  // kFused: anchors 4 wave + q
+    // Keys at or below tau (any scan position) may be selected whatever d_K is.
+    const uint64_t tau_key = (static_cast<uint64_t>(distance_key(d.tau)) << 32) | 0xFFFFFFFFull;
     int valid_count[4] = {0, 0, 0, 0};
     prefetch(0);
     prefetch_anchor(0);
@@ -676,7 +693,7 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
                     }
                     const int count = s_nx[n] * s_ny[n];
                     const uint32_t base_index = static_cast<uint32_t>(ti * count);
-                    const uint64_t threshold = readlane64(best[q], 63);
+                    const uint64_t threshold = admit_bound(best[q], d.similar, tau_key);
                     uint16_t* list = scratch + q * S.scratch_keys;
                     for (int c0 = 0; c0 < count; c0 += 64) {
                         const int i = c0 + lane;
@@ -709,7 +726,7 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const int at = 64 * r + lane;
-                        const uint64_t current = readlane64(best[q], 63);
+                        const uint64_t current = admit_bound(best[q], d.similar, tau_key);
                         uint64_t v = kNoKey;
                         if (at < m[q]) {
                             const int n = 4 * wave + q;
