@@ -444,13 +444,27 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
             const int x0 = ux0 + 4 * (item - ry * groups4);
             const int y = uy0 + ry;
             const bool row_ok = item < fill_items && y < d.height;
+            const float* row_src = src + static_cast<size_t>(y) * d.width + x0;
+            if (row_ok && x0 + 8 <= d.width) {
+                // Interior: two 16-byte loads (4-byte aligned) cover the span.
+                float4 a;
+                float4 b;
+                __builtin_memcpy(&a, row_src, sizeof(a));
+                __builtin_memcpy(&b, row_src + 4, sizeof(b));
+                const float span[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
-            for (int v = 0; v < kFillSpan; ++v) {
-                float value = 0.0F;
-                if (row_ok && x0 + v < d.width) {
-                    value = src[static_cast<size_t>(y) * d.width + x0 + v] - mu;
+                for (int v = 0; v < kFillSpan; ++v) {
+                    pre[k][v] = span[v] - mu;
                 }
-                pre[k][v] = value;
+            } else {
+#pragma unroll
+                for (int v = 0; v < kFillSpan; ++v) {
+                    float value = 0.0F;
+                    if (row_ok && x0 + v < d.width) {
+                        value = row_src[v] - mu;
+                    }
+                    pre[k][v] = value;
+                }
             }
         }
     };
