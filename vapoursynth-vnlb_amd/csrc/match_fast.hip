@@ -508,11 +508,18 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
             const int py = 4 * ks + kb;
+            // The anchor patch lies inside the frame: 8 samples, two 16-byte loads.
+            float4 a = {0.0F, 0.0F, 0.0F, 0.0F};
+            float4 b = {0.0F, 0.0F, 0.0F, 0.0F};
+            if (my_anchor >= 0) {
+                const float* row = anchor_plane + static_cast<size_t>(my_ay + py) * d.width + my_ax;
+                __builtin_memcpy(&a, row, sizeof(a));
+                __builtin_memcpy(&b, row + 4, sizeof(b));
+            }
+            const float span[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
             for (int v = 0; v < 8; ++v) {
-                pre_anchor[ks][v] = my_anchor >= 0
-                                        ? anchor_plane[static_cast<size_t>(my_ay + py) * d.width + my_ax + v] - mu
-                                        : 0.0F;
+                pre_anchor[ks][v] = my_anchor >= 0 ? span[v] - mu : 0.0F;
             }
         }
     };
