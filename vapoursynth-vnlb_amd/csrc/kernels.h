@@ -37,14 +37,18 @@ hipError_t launch_fast_select(const StageDesc& d, const float* staged, const uin
 
 size_t paste_shared_bytes(const StageDesc& d, int max_anchors);
 int paste_target_stride(const StageDesc& d);
-// dep ([anchor_count] 64-bit) must be zero on entry.
+// frame_offsets holds frame_count + 1 clip-wide anchor indices (the range
+// may start past anchor 0); anchor_count is their span.  dep (64-bit, indexed
+// by clip-wide anchor) must be zero on entry over that span.
 hipError_t launch_paste(const StageDesc& d, int frame_count, const int* frame_offsets,
                         const int* frame_ids, const int* row_tables, const int* counts,
                         const uint32_t* positions, unsigned char* active, int max_anchors,
                         int anchor_count, unsigned long long* dep, int* target_counts,
                         int* targets, hipStream_t stream);
-hipError_t launch_compact(const unsigned char* active, int anchor_count, int* group_list,
-                          int* group_count, hipStream_t stream);
+// Appends the active anchors of [anchor_begin, anchor_begin + anchor_count)
+// to group_list at atomically reserved places of *group_count.
+hipError_t launch_compact(const unsigned char* active, int anchor_begin, int anchor_count,
+                          int* group_list, int* group_count, hipStream_t stream);
 
 size_t prepare_shared_bytes(const StageDesc& d);
 size_t filter_shared_bytes(const StageDesc& d);

@@ -94,7 +94,8 @@ __global__ __launch_bounds__(256) void paste_targets_kernel(
     const int* __restrict__ counts, const uint32_t* __restrict__ positions,
     unsigned long long* __restrict__ dep, int* __restrict__ target_counts,
     int* __restrict__ targets, int target_stride) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    // frame_offsets are clip-wide anchor indices; the range may start past 0.
+    const int a = frame_offsets[0] + blockIdx.x * blockDim.x + threadIdx.x;
     const int total = frame_offsets[frame_count];
     if (a >= total) {
         return;
@@ -235,14 +236,15 @@ __global__ __launch_bounds__(kPasteThreads) void paste_kernel(
     }
 }
 
-// Appends the active anchors of a batch to a group list (order is not
-// significant: every later stage is per group and accumulation is
-// order-independent).
+// Appends the active anchors of [anchor_begin, anchor_begin + anchor_count)
+// to a group list (order is not significant: every later stage is per group
+// and accumulation is order-independent).
 __global__ __launch_bounds__(256) void compact_kernel(
-    const unsigned char* __restrict__ active, int anchor_count,
+    const unsigned char* __restrict__ active, int anchor_begin, int anchor_count,
     int* __restrict__ group_list, int* __restrict__ group_count) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool take = a < anchor_count && active[a] != 0;
+    const int local = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = anchor_begin + local;
+    const bool take = local < anchor_count && active[a] != 0;
     const unsigned long long ballot = __ballot(take);
     const int lane = threadIdx.x & 63;
     int base = 0;
@@ -289,14 +291,14 @@ hipError_t launch_paste(const StageDesc& d, int frame_count, const int* frame_of
     return hipGetLastError();
 }
 
-hipError_t launch_compact(const unsigned char* active, int anchor_count,
+hipError_t launch_compact(const unsigned char* active, int anchor_begin, int anchor_count,
                           int* group_list, int* group_count, hipStream_t stream) {
     if (anchor_count <= 0) {
         return hipSuccess;
     }
     const dim3 grid(static_cast<unsigned>((anchor_count + 255) / 256));
-    hipLaunchKernelGGL(compact_kernel, grid, dim3(256), 0, stream, active, anchor_count,
-                       group_list, group_count);
+    hipLaunchKernelGGL(compact_kernel, grid, dim3(256), 0, stream, active, anchor_begin,
+                       anchor_count, group_list, group_count);
     return hipGetLastError();
 }
 

@@ -370,6 +370,8 @@ template <class T> struct DeviceBuffer {
             count = 1;
         }
         if (count > capacity) {
+            // Work queued on any stream may still use the old allocation.
+            check(hipDeviceSynchronize(), "hipDeviceSynchronize");
             release();
             check(hipMalloc(reinterpret_cast<void**>(&ptr), count * sizeof(T)), "hipMalloc");
             capacity = count;
@@ -393,6 +395,9 @@ struct DebugOutput {
 };
 
 }  // namespace
+
+// Anchor-frame parts a batch is pipelined in (matching, paste scan, model).
+constexpr int kParts = 2;
 
 struct vnlbhip_context {
     int device = 0;
@@ -440,7 +445,33 @@ struct vnlbhip_context {
     vnlbhip_kernel_times times{};
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
     std::vector<hipEvent_t> event_pool;
+    // Side stream of the paste scans (default priority), its events and the
+    // pinned group counts read back per part; created on first use.
+    hipStream_t side = nullptr;
+    hipEvent_t ready_event = nullptr;
+    hipEvent_t match_event[kParts] = {};
+    hipEvent_t paste_event[kParts] = {};
+    int* host_counts = nullptr;
     ~vnlbhip_context() {
+        if (host_counts != nullptr) {
+            (void)hipHostFree(host_counts);
+        }
+        for (hipEvent_t e : match_event) {
+            if (e != nullptr) {
+                (void)hipEventDestroy(e);
+            }
+        }
+        for (hipEvent_t e : paste_event) {
+            if (e != nullptr) {
+                (void)hipEventDestroy(e);
+            }
+        }
+        if (ready_event != nullptr) {
+            (void)hipEventDestroy(ready_event);
+        }
+        if (side != nullptr) {
+            (void)hipStreamDestroy(side);
+        }
         for (auto& p : pending) {
             event_pool.push_back(p.second.first);
             event_pool.push_back(p.second.second);
@@ -581,6 +612,16 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
         }
         ctx.plan_key = plan_key;
     }
+    if (ctx.side == nullptr) {
+        check(hipStreamCreateWithFlags(&ctx.side, hipStreamNonBlocking), "hipStreamCreate");
+        check(hipEventCreateWithFlags(&ctx.ready_event, hipEventDisableTiming), "hipEventCreate");
+        for (int h = 0; h < kParts; ++h) {
+            check(hipEventCreateWithFlags(&ctx.match_event[h], hipEventDisableTiming), "hipEventCreate");
+            check(hipEventCreateWithFlags(&ctx.paste_event[h], hipEventDisableTiming), "hipEventCreate");
+        }
+        check(hipHostMalloc(reinterpret_cast<void**>(&ctx.host_counts), kParts * sizeof(int)),
+              "hipHostMalloc");
+    }
     for (BatchPlan& plan : ctx.plans) {
         const Grid& grid = plan.grid;
         const int max_anchors = plan.max_anchors;
@@ -607,162 +648,213 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
         float* d_distances = ctx.distances.reserve(static_cast<size_t>(anchor_count) * cap);
         unsigned char* d_active = ctx.active.reserve(anchor_count);
         int* d_list = ctx.group_list.reserve(anchor_count);
-        int* d_group_count = ctx.group_count.reserve(1);
-
-        bool matched = false;
-        if (fast_energy != nullptr) {
-            if (!plan.tiles_built) {
-                plan.tiles = build_tiles(d, grid, g.height);
-                plan.tiles_built = true;
-            }
-            const Tiles& tiles = plan.tiles;
-            if (vnlbhip::fast_match_supported(d, tiles.max_cols, tiles.max_rows)) {
-                const int tile_count = static_cast<int>(tiles.slots.size() / 16);
-                int* d_tiles = ctx.fast_tiles.reserve(tiles.slots.size());
-                check(hipMemcpyAsync(d_tiles, tiles.slots.data(), tiles.slots.size() * sizeof(int),
-                                     hipMemcpyHostToDevice, stream), "upload tiles");
-                if (vnlbhip::fast_match_fused(d, tiles.max_cols, tiles.max_rows)) {
-                    // Selection inside the matcher: one launch over every tile.
-                    timed(ctx, VNLBHIP_KERNEL_MATCH, stream, "fast match kernel", [&]() {
-                        return vnlbhip::launch_fast_match(d, tiles.max_cols, tiles.max_rows, source,
-                                                          fast_offsets, fast_energy, d_anchors, d_tiles,
-                                                          tile_count, 0, nullptr, true, d_counts,
-                                                          d_positions, d_distances, stream);
-                    });
-                } else {
-                    // Distances are staged for a chunk of anchor frames at a
-                    // time (tiles and anchors are both in frame order) and
-                    // selected before the next chunk reuses the buffer.
-                    const size_t stride = vnlbhip::fast_match_stride(d);
-                    const int chunk_anchors = static_cast<int>(std::max<size_t>(
-                        static_cast<size_t>(max_anchors), ctx.fast_stage_budget / stride));
-                    float* staged = ctx.fast_staged.reserve(static_cast<size_t>(
-                        std::min(chunk_anchors, anchor_count)) * stride);
-                    int tile_begin = 0;
-                    int frame = 0;
-                    while (frame < frame_count) {
-                        const int anchor_begin = grid.offsets[static_cast<size_t>(frame)];
-                        int frame_end = frame + 1;
-                        while (frame_end < frame_count &&
-                               grid.offsets[static_cast<size_t>(frame_end + 1)] - anchor_begin <= chunk_anchors) {
-                            ++frame_end;
-                        }
-                        const int anchor_end = grid.offsets[static_cast<size_t>(frame_end)];
-                        int tile_end = tile_begin;
-                        while (tile_end < tile_count) {
-                            int first = -1;
-                            for (int slot = 0; slot < 16 && first < 0; ++slot) {
-                                first = tiles.slots[static_cast<size_t>(tile_end) * 16 + slot];
-                            }
-                            if (first >= anchor_end) {
-                                break;
-                            }
-                            ++tile_end;
-                        }
-                        timed(ctx, VNLBHIP_KERNEL_MATCH, stream, "fast match kernel", [&]() {
-                            return vnlbhip::launch_fast_match(d, tiles.max_cols, tiles.max_rows, source,
-                                                              fast_offsets, fast_energy, d_anchors,
-                                                              d_tiles + static_cast<size_t>(tile_begin) * 16,
-                                                              tile_end - tile_begin, anchor_begin, staged,
-                                                              false, nullptr, nullptr, nullptr, stream);
-                        });
-                        timed(ctx, VNLBHIP_KERNEL_SELECT, stream, "select kernel", [&]() {
-                            return vnlbhip::launch_fast_select(
-                                d, staged, d_anchors + anchor_begin, anchor_end - anchor_begin,
-                                d_counts + anchor_begin, d_positions + static_cast<size_t>(anchor_begin) * cap,
-                                d_distances + static_cast<size_t>(anchor_begin) * cap, stream);
-                        });
-                        tile_begin = tile_end;
-                        frame = frame_end;
-                    }
-                }
-                // The tile list must outlive its upload.
-                check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-                matched = true;
-            }
-        }
-        if (!matched) {
-            timed(ctx, VNLBHIP_KERNEL_MATCH, stream, "match kernel", [&]() {
-                return vnlbhip::launch_match(d, source, d_anchors, anchor_count, d_counts, d_positions,
-                                             d_distances, stream);
-            });
-        }
+        int* d_group_count = ctx.group_count.reserve(kParts);
         unsigned long long* d_dep = ctx.paste_dep.reserve(anchor_count);
         int* d_tcount = ctx.paste_target_counts.reserve(anchor_count);
         int* d_targets = ctx.paste_targets.reserve(static_cast<size_t>(anchor_count) *
                                                    vnlbhip::paste_target_stride(d));
         check(vnlbhip::launch_fill_zero(reinterpret_cast<double*>(d_dep), anchor_count, stream),
               "clear paste masks");
-        timed(ctx, VNLBHIP_KERNEL_PASTE, stream, "paste kernel", [&]() {
-            return vnlbhip::launch_paste(d, frame_count, d_offsets, d_ids, d_rows, d_counts,
-                                         d_positions, d_active, max_anchors, anchor_count, d_dep,
-                                         d_tcount, d_targets, stream);
-        });
-        check(hipMemsetAsync(d_group_count, 0, sizeof(int), stream), "hipMemsetAsync");
-        check(vnlbhip::launch_compact(d_active, anchor_count, d_list, d_group_count, stream),
-              "compact kernel");
-        int groups = 0;
-        check(hipMemcpyAsync(&groups, d_group_count, sizeof(int), hipMemcpyDeviceToHost, stream),
-              "read group count");
-        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-        collect_times(ctx);
-        ctx.stats.groups += groups;
+        check(vnlbhip::launch_fill_zero_int(d_group_count, kParts, stream), "clear group counts");
 
-        const int problems = groups * d.models;
-        int* d_flags = ctx.group_flags.reserve(groups);
-        int* d_dims = ctx.problem_dims.reserve(groups);
-        float* d_means = ctx.means.reserve(static_cast<size_t>(groups) * 2 * d.dim);
-        float* d_problems =
-            ctx.problems.reserve(static_cast<size_t>(problems) * n_stride * n_stride);
-        float* d_eigen = ctx.eigenvalues.reserve(static_cast<size_t>(problems) * std::max(d.rank, 1));
-        float* d_vectors =
-            ctx.vectors.reserve(static_cast<size_t>(problems) * std::max(d.rank, 1) * n_stride);
-        int* d_status = ctx.status.reserve(problems);
+        // The batch runs as up to two parts of consecutive anchor frames: the
+        // serial paste scan of a part (a few wavefronts) runs on the side
+        // stream beside the next part's matching and the previous part's
+        // model, and each part's model starts once its groups are listed.
+        // Debug runs keep one part.
+        const int parts = (debug == nullptr && frame_count >= 2) ? kParts : 1;
+        int part_frame[kParts + 1];
+        for (int h = 0; h <= parts; ++h) {
+            part_frame[h] = frame_count * h / parts;
+        }
+        check(hipEventRecord(ctx.ready_event, stream), "hipEventRecord");
+        check(hipStreamWaitEvent(ctx.side, ctx.ready_event, 0), "hipStreamWaitEvent");
+
+        const Tiles* tiles = nullptr;
+        int* d_tiles = nullptr;
+        bool fast_matcher = false;
+        if (fast_energy != nullptr) {
+            if (!plan.tiles_built) {
+                plan.tiles = build_tiles(d, grid, g.height);
+                plan.tiles_built = true;
+            }
+            tiles = &plan.tiles;
+            fast_matcher = vnlbhip::fast_match_supported(d, tiles->max_cols, tiles->max_rows);
+            if (fast_matcher) {
+                // The tile list lives in the cached plan, so the upload may
+                // complete asynchronously.
+                d_tiles = ctx.fast_tiles.reserve(tiles->slots.size());
+                check(hipMemcpyAsync(d_tiles, tiles->slots.data(), tiles->slots.size() * sizeof(int),
+                                     hipMemcpyHostToDevice, stream), "upload tiles");
+            }
+        }
+        const int tile_count = fast_matcher ? static_cast<int>(tiles->slots.size() / 16) : 0;
+        // First tile of the frame that starts at anchor_begin: tiles hold
+        // anchors of one frame and come in frame order, so "first anchor >=
+        // anchor_begin" is false and then true along the tile list.
+        const auto tile_at = [&](int anchor_begin) {
+            int lo = 0;
+            int hi = tile_count;
+            while (lo < hi) {
+                const int t = lo + (hi - lo) / 2;
+                int first = -1;
+                for (int slot = 0; slot < 16 && first < 0; ++slot) {
+                    first = tiles->slots[static_cast<size_t>(t) * 16 + slot];
+                }
+                if (first >= anchor_begin) {
+                    hi = t;
+                } else {
+                    lo = t + 1;
+                }
+            }
+            return lo;
+        };
+
+        for (int h = 0; h < parts; ++h) {
+            const int fb = part_frame[h];
+            const int fe = part_frame[h + 1];
+            const int ab = grid.offsets[static_cast<size_t>(fb)];
+            const int ae = grid.offsets[static_cast<size_t>(fe)];
+            // Matching of the part's anchors (stage stream).
+            if (fast_matcher && vnlbhip::fast_match_fused(d, tiles->max_cols, tiles->max_rows)) {
+                const int tb = tile_at(ab);
+                const int te = tile_at(ae);
+                timed(ctx, VNLBHIP_KERNEL_MATCH, stream, "fast match kernel", [&]() {
+                    return vnlbhip::launch_fast_match(d, tiles->max_cols, tiles->max_rows, source,
+                                                      fast_offsets, fast_energy, d_anchors,
+                                                      d_tiles + static_cast<size_t>(tb) * 16, te - tb, 0,
+                                                      nullptr, true, d_counts, d_positions, d_distances,
+                                                      stream);
+                });
+            } else if (fast_matcher) {
+                // Distances are staged for a chunk of anchor frames at a
+                // time (tiles and anchors are both in frame order) and
+                // selected before the next chunk reuses the buffer.
+                const size_t stride = vnlbhip::fast_match_stride(d);
+                const int chunk_anchors = static_cast<int>(std::max<size_t>(
+                    static_cast<size_t>(max_anchors), ctx.fast_stage_budget / stride));
+                float* staged = ctx.fast_staged.reserve(static_cast<size_t>(
+                    std::min(chunk_anchors, anchor_count)) * stride);
+                int frame = fb;
+                while (frame < fe) {
+                    const int anchor_begin = grid.offsets[static_cast<size_t>(frame)];
+                    int frame_end = frame + 1;
+                    while (frame_end < fe &&
+                           grid.offsets[static_cast<size_t>(frame_end + 1)] - anchor_begin <= chunk_anchors) {
+                        ++frame_end;
+                    }
+                    const int anchor_end = grid.offsets[static_cast<size_t>(frame_end)];
+                    const int tb = tile_at(anchor_begin);
+                    const int te = tile_at(anchor_end);
+                    timed(ctx, VNLBHIP_KERNEL_MATCH, stream, "fast match kernel", [&]() {
+                        return vnlbhip::launch_fast_match(d, tiles->max_cols, tiles->max_rows, source,
+                                                          fast_offsets, fast_energy, d_anchors,
+                                                          d_tiles + static_cast<size_t>(tb) * 16, te - tb,
+                                                          anchor_begin, staged, false, nullptr, nullptr,
+                                                          nullptr, stream);
+                    });
+                    timed(ctx, VNLBHIP_KERNEL_SELECT, stream, "select kernel", [&]() {
+                        return vnlbhip::launch_fast_select(
+                            d, staged, d_anchors + anchor_begin, anchor_end - anchor_begin,
+                            d_counts + anchor_begin, d_positions + static_cast<size_t>(anchor_begin) * cap,
+                            d_distances + static_cast<size_t>(anchor_begin) * cap, stream);
+                    });
+                    frame = frame_end;
+                }
+            } else {
+                timed(ctx, VNLBHIP_KERNEL_MATCH, stream, "match kernel", [&]() {
+                    return vnlbhip::launch_match(d, source, d_anchors + ab, ae - ab, d_counts + ab,
+                                                 d_positions + static_cast<size_t>(ab) * cap,
+                                                 d_distances + static_cast<size_t>(ab) * cap, stream);
+                });
+            }
+            check(hipEventRecord(ctx.match_event[h], stream), "hipEventRecord");
+            // Paste mask and group list of the part (side stream).
+            check(hipStreamWaitEvent(ctx.side, ctx.match_event[h], 0), "hipStreamWaitEvent");
+            timed(ctx, VNLBHIP_KERNEL_PASTE, ctx.side, "paste kernel", [&]() {
+                return vnlbhip::launch_paste(d, fe - fb, d_offsets + fb, d_ids + fb,
+                                             d_rows + static_cast<size_t>(fb) * 3 * g.height, d_counts,
+                                             d_positions, d_active, max_anchors, ae - ab, d_dep, d_tcount,
+                                             d_targets, ctx.side);
+            });
+            check(vnlbhip::launch_compact(d_active, ab, ae - ab, d_list + ab, d_group_count + h, ctx.side),
+                  "compact kernel");
+            check(hipMemcpyAsync(ctx.host_counts + h, d_group_count + h, sizeof(int),
+                                 hipMemcpyDeviceToHost, ctx.side), "read group count");
+            check(hipEventRecord(ctx.paste_event[h], ctx.side), "hipEventRecord");
+        }
+
+        // Model, filter and aggregation of one list of groups (stage stream).
+        // The model buffers are shared by the parts in stream order; the
+        // variables below keep the last part's for the debug read-back.
+        const float* basic = stage == VNLBHIP_STAGE_FINAL ? ref : noisy;
+        int groups = 0;
+        int problems = 0;
+        int* d_flags = nullptr;
+        float* d_eigen = nullptr;
         float* d_weights = nullptr;
         if (debug != nullptr) {
             d_weights = ctx.weights.reserve(static_cast<size_t>(anchor_count) * cap);
         }
-        const float* basic = stage == VNLBHIP_STAGE_FINAL ? ref : noisy;
-        timed(ctx, VNLBHIP_KERNEL_PREPARE, stream, "prepare kernel", [&]() {
-            return vnlbhip::launch_prepare(d, noisy, basic, d_anchors, d_counts, d_positions,
-                                           d_list, groups, n_stride, d_flags, d_dims, d_means,
-                                           d_problems,
-                                           ctx.form == VNLBHIP_FORM_FAST, stream);
-        });
-        if (d.rank > 0) {
-            const bool parallel_jacobi =
-                ctx.form == VNLBHIP_FORM_FAST && vnlbhip::jacobi_parallel_supported(n_stride);
-            int* d_refine = nullptr;
-            if (parallel_jacobi && ctx.refine_gap > 0.0F) {
-                // refine_list[0] counts, the list follows.
-                d_refine = ctx.refine_list.reserve(static_cast<size_t>(problems) + 1);
-                check(vnlbhip::launch_fill_zero_int(d_refine, 1, stream), "clear refine count");
-            }
-            timed(ctx, VNLBHIP_KERNEL_JACOBI, stream, "jacobi kernel", [&]() {
-                if (parallel_jacobi) {
-                    return vnlbhip::launch_jacobi_parallel(
-                        d, problems, n_stride, d_flags, d_dims, d_problems, d_eigen, d_vectors, d_status,
-                        ctx.refine_gap, d_refine != nullptr ? d_refine + 1 : nullptr, d_refine,
-                        summary + 4, stream);
-                }
-                return vnlbhip::launch_jacobi(d, problems, n_stride, d_flags, d_dims, d_problems,
-                                              d_eigen, d_vectors, d_status, stream);
+        const auto run_model = [&](int* d_part_list) {
+            problems = groups * d.models;
+            d_flags = ctx.group_flags.reserve(groups);
+            int* d_dims = ctx.problem_dims.reserve(groups);
+            float* d_means = ctx.means.reserve(static_cast<size_t>(groups) * 2 * d.dim);
+            float* d_problems =
+                ctx.problems.reserve(static_cast<size_t>(problems) * n_stride * n_stride);
+            d_eigen = ctx.eigenvalues.reserve(static_cast<size_t>(problems) * std::max(d.rank, 1));
+            float* d_vectors =
+                ctx.vectors.reserve(static_cast<size_t>(problems) * std::max(d.rank, 1) * n_stride);
+            int* d_status = ctx.status.reserve(problems);
+            timed(ctx, VNLBHIP_KERNEL_PREPARE, stream, "prepare kernel", [&]() {
+                return vnlbhip::launch_prepare(d, noisy, basic, d_anchors, d_counts, d_positions,
+                                               d_part_list, groups, n_stride, d_flags, d_dims, d_means,
+                                               d_problems, ctx.form == VNLBHIP_FORM_FAST, stream);
             });
-        }
-        const bool fast_filter = ctx.form == VNLBHIP_FORM_FAST && vnlbhip::fast_filter_supported(d);
-        timed(ctx, VNLBHIP_KERNEL_FILTER, stream, "filter kernel", [&]() {
-            if (fast_filter) {
-                return vnlbhip::launch_fast_filter(d, noisy, basic, d_anchors, d_counts, d_positions,
-                                                   d_list, groups, n_stride, d_flags, d_means, d_eigen,
-                                                   d_vectors, d_problems, acc, d_weights, stream);
+            if (d.rank > 0) {
+                const bool parallel_jacobi =
+                    ctx.form == VNLBHIP_FORM_FAST && vnlbhip::jacobi_parallel_supported(n_stride);
+                int* d_refine = nullptr;
+                if (parallel_jacobi && ctx.refine_gap > 0.0F) {
+                    // refine_list[0] counts, the list follows.
+                    d_refine = ctx.refine_list.reserve(static_cast<size_t>(problems) + 1);
+                    check(vnlbhip::launch_fill_zero_int(d_refine, 1, stream), "clear refine count");
+                }
+                timed(ctx, VNLBHIP_KERNEL_JACOBI, stream, "jacobi kernel", [&]() {
+                    if (parallel_jacobi) {
+                        return vnlbhip::launch_jacobi_parallel(
+                            d, problems, n_stride, d_flags, d_dims, d_problems, d_eigen, d_vectors, d_status,
+                            ctx.refine_gap, d_refine != nullptr ? d_refine + 1 : nullptr, d_refine,
+                            summary + 4, stream);
+                    }
+                    return vnlbhip::launch_jacobi(d, problems, n_stride, d_flags, d_dims, d_problems,
+                                                  d_eigen, d_vectors, d_status, stream);
+                });
             }
-            return vnlbhip::launch_filter(d, noisy, basic, d_anchors, d_counts, d_positions,
-                                          d_list, groups, n_stride, d_flags, d_dims, d_means,
-                                          d_eigen, d_vectors, acc, d_weights, stream);
-        });
-        check(vnlbhip::launch_summarize(d_flags, d.rank > 0 ? d_status : nullptr, d_counts, d_list,
-                                        groups, d.models, summary, stream), "summarize kernel");
-
+            const bool fast_filter = ctx.form == VNLBHIP_FORM_FAST && vnlbhip::fast_filter_supported(d);
+            timed(ctx, VNLBHIP_KERNEL_FILTER, stream, "filter kernel", [&]() {
+                if (fast_filter) {
+                    return vnlbhip::launch_fast_filter(d, noisy, basic, d_anchors, d_counts, d_positions,
+                                                       d_part_list, groups, n_stride, d_flags, d_means,
+                                                       d_eigen, d_vectors, d_problems, acc, d_weights,
+                                                       stream);
+                }
+                return vnlbhip::launch_filter(d, noisy, basic, d_anchors, d_counts, d_positions,
+                                              d_part_list, groups, n_stride, d_flags, d_dims, d_means,
+                                              d_eigen, d_vectors, acc, d_weights, stream);
+            });
+            check(vnlbhip::launch_summarize(d_flags, d.rank > 0 ? d_status : nullptr, d_counts,
+                                            d_part_list, groups, d.models, summary, stream),
+                  "summarize kernel");
+        };
+        for (int h = 0; h < parts; ++h) {
+            check(hipEventSynchronize(ctx.paste_event[h]), "hipEventSynchronize");
+            groups = ctx.host_counts[h];
+            ctx.stats.groups += groups;
+            check(hipStreamWaitEvent(stream, ctx.paste_event[h], 0), "hipStreamWaitEvent");
+            run_model(d_list + grid.offsets[static_cast<size_t>(part_frame[h])]);
+        }
         if (debug != nullptr) {
             require(anchor_count <= debug->capacity_anchors, "debug capacity too small");
             std::vector<int> counts(anchor_count);
