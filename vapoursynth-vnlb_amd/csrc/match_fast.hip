@@ -149,7 +149,8 @@ struct FastShape {
     int copy_elems;    // region_rows * pitch
     int stride;        // distance slots per anchor (window^2 * temporal)
     int per_frame;     // window^2
-    int ebox_words;    // energy box (max cols x max rows), rounded to 4
+    int ebox_pitch;    // energy box column stride: 16 * row_tiles + 4 (16-byte rows, banks spread)
+    int ebox_words;    // energy box, column-major: max cols x ebox_pitch
     int scratch_keys;  // compaction list per anchor (window^2 16-bit entries, rounded to 2)
 };
 
@@ -531,12 +532,14 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
         const float* frame_energy = energy + static_cast<size_t>(frame) * plane;
 #pragma unroll
         for (int k = 0; k < kEnergyPerThread; ++k) {
-            const int item = tid + k * kFastThreads;
+            int item = tid + k * kFastThreads;
+            // Opaque: the box offsets are recomputed per frame instead of
+            // being held in registers across the plane loop.
+            asm volatile("" : "+v"(item));
+            item = item < box_items ? item : box_items - 1;  // a cell of the box
             const int ry = item / ucols;
             const int cx = item - ry * ucols;
-            pre_energy[k] = item < box_items
-                                ? frame_energy[static_cast<size_t>(uy0 + ry) * d.width + ux0 + cx]
-                                : 0.0F;
+            pre_energy[k] = frame_energy[(uy0 + ry) * d.width + ux0 + cx];
         }
     };
 
@@ -658,43 +661,53 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < kEnergyPerThread; ++k) {
-                const int item = tid + k * kFastThreads;
+                int item = tid + k * kFastThreads;
+                asm volatile("" : "+v"(item));
                 if (item < box_items) {
-                    s_ebox[item] = pre_energy[k];
+                    const int ry = item / ucols;
+                    s_ebox[(item - ry * ucols) * S.ebox_pitch + ry] = pre_energy[k];
                 }
             }
             __syncthreads();
             const int ti = p / planes_per_frame;
+            const bool anchor_frame = frame == af;
+            // The lane's anchor window in registers for the loop (an empty
+            // window for an empty slot).
+            const float my_energy = my_anchor >= 0 ? s_energy[n_lane] : 0.0F;
+            const int my_xl = my_anchor >= 0 ? s_xl[n_lane] : 0;
+            const int my_yl = my_anchor >= 0 ? s_yl[n_lane] : 0;
+            const int my_nx = my_anchor >= 0 ? s_nx[n_lane] : 0;
+            const int my_ny = my_anchor >= 0 ? s_ny[n_lane] : 0;
+            // The anchor against itself: exactly 0 (NaN for a patch holding
+            // NaN, as the exact sum gives).
+            const float my_self = my_energy == my_energy ? 0.0F : my_energy;
+            // Lane (n_lane, kb) holds candidates 16 rt + 4 kb + v of column
+            // col against anchor n_lane; their energies are consecutive in
+            // the column-major box (one broadcast 16-byte read).
 #pragma unroll
             for (int i = 0; i < kTilesPerWave; ++i) {
                 int mt = wave + kFastWaves * i;
                 asm volatile("" : "+s"(mt));
-                if (mt < mtiles && my_anchor >= 0) {
-                    const int col = (mt * tile_magic) >> 16;
-                    const int rt = mt - col * row_tiles;
-                    const int cx = ux0 + col;
-                    const int rel_x = cx - s_xl[n_lane];
-                    if (rel_x >= 0 && rel_x < s_nx[n_lane]) {
+                if (mt >= mtiles) {
+                    break;  // uniform
+                }
+                const int col = (mt * tile_magic) >> 16;
+                const int rt = mt - col * row_tiles;
+                const int ry0 = 16 * rt + 4 * kb;
+                const float4 ec = *reinterpret_cast<const float4*>(s_ebox + col * S.ebox_pitch + ry0);
+                const float ecv[4] = {ec.x, ec.y, ec.z, ec.w};
+                const int rel_x = ux0 + col - my_xl;
+                const int rel_y0 = uy0 + ry0 - my_yl;
+                const bool x_ok = static_cast<unsigned>(rel_x) < static_cast<unsigned>(my_nx);
+                const bool self_x = anchor_frame && ux0 + col == my_ax;
 #pragma unroll
-                        for (int v = 0; v < 4; ++v) {
-                            const int ry = 16 * rt + 4 * kb + v;
-                            const int cy = uy0 + ry;
-                            const int rel_y = cy - s_yl[n_lane];
-                            if (rel_y >= 0 && rel_y < s_ny[n_lane]) {
-                                float dist;
-                                if (cx == my_ax && cy == my_ay && frame == af) {
-                                    // The anchor against itself: exactly 0 (NaN for a
-                                    // patch holding NaN, as the exact sum gives).
-                                    const float e_anchor = s_energy[n_lane];
-                                    dist = e_anchor == e_anchor ? 0.0F : e_anchor;
-                                } else {
-                                    const float ec = s_ebox[ry * ucols + col];
-                                    dist = (s_energy[n_lane] + ec) - 2.0F * acc[i][v];
-                                    dist = dist < 0.0F ? 0.0F : dist;  // rounding below 0; NaN stays NaN
-                                }
-                                s_stage[n_lane * S.per_frame + rel_y * s_nx[n_lane] + rel_x] = dist;
-                            }
-                        }
+                for (int v = 0; v < 4; ++v) {
+                    const int rel_y = rel_y0 + v;
+                    float dist = (my_energy + ecv[v]) - 2.0F * acc[i][v];
+                    dist = dist < 0.0F ? 0.0F : dist;  // rounding below 0; NaN stays NaN
+                    dist = self_x && uy0 + ry0 + v == my_ay ? my_self : dist;
+                    if (x_ok && static_cast<unsigned>(rel_y) < static_cast<unsigned>(my_ny)) {
+                        s_stage[n_lane * S.per_frame + __mul24(rel_y, my_nx) + rel_x] = dist;
                     }
                 }
                 __builtin_amdgcn_sched_barrier(0);
@@ -905,7 +918,8 @@ FastShape fast_shape(const StageDesc& d, int max_cols, int max_rows) {
     S.copy_elems = S.region_rows * S.pitch;
     S.per_frame = d.search_window * d.search_window;
     S.stride = S.per_frame * (d.search_bwd + d.search_fwd + 1);
-    S.ebox_words = (max_cols * max_rows + 3) / 4 * 4;
+    S.ebox_pitch = 16 * S.row_tiles + 4;
+    S.ebox_words = max_cols * S.ebox_pitch;
     S.scratch_keys = (S.per_frame + 1) / 2 * 2;
     return S;
 }
@@ -923,7 +937,7 @@ bool fast_match_supported(const StageDesc& d, int max_cols, int max_rows) {
     return static_cast<long long>(max_cols) * row_tiles <= static_cast<long long>(kMaxTilesPerWave) * kFastWaves &&
            S.region_rows * (S.pitch / 4) <= kFillPerThread * kFastThreads &&
            static_cast<long long>(max_cols) * max_rows <= static_cast<long long>(kEnergyPerThread) * kFastThreads &&
-           static_cast<long long>(max_cols) * max_rows * 4 <= static_cast<long long>(2) * kCopies * S.copy_elems * 2 &&
+           static_cast<long long>(S.ebox_words) * 4 <= static_cast<long long>(2) * kCopies * S.copy_elems * 2 &&
            fast_match_shared_bytes(d, max_cols, max_rows) <= 80 * 1024;
 }
 
