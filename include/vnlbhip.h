@@ -182,6 +182,14 @@ int vnlbhip_set_form(vnlbhip_context* context, int form);
  * VNLBHIP_ERROR_INVALID.  Only models with rank < rows have a cutoff. */
 int vnlbhip_set_eigen_refine(vnlbhip_context* context, float relative_gap);
 
+/* FAST form: 1 (default) runs the k-nearest selection inside the MFMA
+ * matcher where the retained cap allows (at most 64); 0 stages every
+ * candidate distance in device memory and selects in a second kernel with
+ * the PLAIN form's selection (no counterpart in the reference).  Both give
+ * the same match lists; 0 exists to check that.  Values other than 0 and 1
+ * are rejected with VNLBHIP_ERROR_INVALID. */
+int vnlbhip_set_fused_selection(vnlbhip_context* context, int enable);
+
 #ifdef __cplusplus
 }
 #endif

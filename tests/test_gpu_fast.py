@@ -74,6 +74,44 @@ def test_fast_distances_track_exact(contexts, stage):
     assert same_sets >= 0.9 * len(want["counts"]), (same_sets, len(want["counts"]))
 
 
+@pytest.mark.parametrize("stage", [BASIC, FINAL])
+def test_fused_selection_matches_staged(stage):
+    """The k-nearest selection inside the MFMA matcher keeps exactly the
+    lists that the plain form's selection (select.h) makes from the same
+    staged distances: counts, positions and distances are equal on every
+    anchor, and the stage outputs agree up to the order of the fp64
+    accumulator adds."""
+    fused = vnlbhip.Context(0)
+    staged = vnlbhip.Context(0)
+    try:
+        fused.set_form("fast")
+        staged.set_form("fast")
+        staged.set_fused_selection(False)
+        noisy, _ = synth.make_clip(width=128, height=96, frames=6, channels=3, sigma8=15, seed=5,
+                                   motion="subpixel")
+        params = parse_stage_parameters(stage, ARGS)
+        ref = None
+        if stage == FINAL:
+            ref = fused.stage(BASIC, parse_stage_parameters(BASIC, ARGS), _dev(noisy))
+        for frame in (0, 2, 4):
+            print(f"[fast] fused vs staged selection stage={stage} frame={frame}", flush=True)
+            a = fused.debug_frame(stage, params, _dev(noisy), ref, frame)
+            b = staged.debug_frame(stage, params, _dev(noisy), ref, frame)
+            np.testing.assert_array_equal(a["counts"], b["counts"])
+            assert np.all(a["counts"] >= 1)
+            for i, n in enumerate(a["counts"]):
+                np.testing.assert_array_equal(a["positions"][i, :n], b["positions"][i, :n], err_msg=str(i))
+                np.testing.assert_array_equal(a["distances"][i, :n], b["distances"][i, :n], err_msg=str(i))
+        out_a = fused.stage(stage, params, _dev(noisy), ref).cpu().numpy()
+        out_b = staged.stage(stage, params, _dev(noisy), ref).cpu().numpy()
+        np.testing.assert_allclose(out_a, out_b, rtol=0, atol=1e-5)
+        with pytest.raises(vnlbhip.VNLBError):
+            staged.set_fused_selection(2)
+    finally:
+        fused.close()
+        staged.close()
+
+
 @pytest.mark.parametrize("channels,width,height", [(3, 128, 96), (1, 112, 80), (3, 40, 36)])
 def test_fast_stage_quality_matches_plain(contexts, channels, width, height):
     plain, fast = contexts

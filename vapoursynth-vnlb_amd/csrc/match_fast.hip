@@ -718,6 +718,34 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
                 // anchor's 64th), one list per anchor, in the idle plane
                 // buffer after the energy box.
                 uint16_t* scratch = reinterpret_cast<uint16_t*>(s_ebox + S.ebox_words) + wave * 4 * S.scratch_keys;
+                // First candidate frame: the lists are still empty, so every
+                // key would be admitted.  Each lane's smallest key (over the
+                // candidates i = lane mod 64) goes in first, which bounds the
+                // K-th key by the K-th of those minima; the compaction below
+                // then skips the lane's own minimum (keys are unique).
+                uint64_t own_min[4] = {kNoKey, kNoKey, kNoKey, kNoKey};
+                if (ti == 0) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int n = 4 * wave + q;
+                        if (s_index[n] < 0) {
+                            continue;
+                        }
+                        const int count = s_nx[n] * s_ny[n];
+                        for (int i = lane; i < count; i += 64) {
+                            const uint64_t key =
+                                (static_cast<uint64_t>(distance_key(s_stage[n * S.per_frame + i])) << 32) |
+                                static_cast<uint32_t>(i);
+                            own_min[q] = key < own_min[q] ? key : own_min[q];
+                        }
+                    }
+                    uint64_t first[4] = {own_min[0], own_min[1], own_min[2], own_min[3]};
+                    wave_sort64x4(first, lane);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        best[q] = first[q];  // the lists were empty
+                    }
+                }
                 int m[4] = {0, 0, 0, 0};
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
@@ -738,7 +766,7 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
                             const float dist = s_stage[n * S.per_frame + i];
                             valid = dist == dist;
                             key = (static_cast<uint64_t>(distance_key(dist)) << 32) | (base_index + i);
-                            pass = key < threshold;
+                            pass = key < threshold && key != own_min[q];
                         }
                         valid_count[q] += __popcll(__ballot(valid));
                         const uint64_t mask = __ballot(pass);

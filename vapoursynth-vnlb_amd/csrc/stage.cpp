@@ -425,6 +425,7 @@ struct vnlbhip_context {
     DeviceBuffer<int> status;
     DeviceBuffer<int> refine_list;
     float refine_gap = 1e-3F;  // FAST form: relative eigen gap below which groups are re-solved in fp64
+    bool fused_selection = true;  // FAST form: k-nearest selection inside the MFMA matcher
     DeviceBuffer<float> weights;
     DeviceBuffer<double> accumulators;
     DeviceBuffer<int> summary;
@@ -716,7 +717,8 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
             const int ab = grid.offsets[static_cast<size_t>(fb)];
             const int ae = grid.offsets[static_cast<size_t>(fe)];
             // Matching of the part's anchors (stage stream).
-            if (fast_matcher && vnlbhip::fast_match_fused(d, tiles->max_cols, tiles->max_rows)) {
+            if (fast_matcher && ctx.fused_selection &&
+                vnlbhip::fast_match_fused(d, tiles->max_cols, tiles->max_rows)) {
                 const int tb = tile_at(ab);
                 const int te = tile_at(ae);
                 timed(ctx, VNLBHIP_KERNEL_MATCH, stream, "fast match kernel", [&]() {
@@ -1115,6 +1117,14 @@ int vnlbhip_set_eigen_refine(vnlbhip_context* context, float relative_gap) {
         return VNLBHIP_ERROR_INVALID;
     }
     context->refine_gap = relative_gap;
+    return VNLBHIP_OK;
+}
+
+int vnlbhip_set_fused_selection(vnlbhip_context* context, int enable) {
+    if (context == nullptr || (enable != 0 && enable != 1)) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    context->fused_selection = enable != 0;
     return VNLBHIP_OK;
 }
 

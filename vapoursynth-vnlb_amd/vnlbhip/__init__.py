@@ -91,6 +91,14 @@ class Context:
         if rc != 0:
             raise VNLBError("VNLB HIP: relative_gap must be finite and non-negative")
 
+    def set_fused_selection(self, enable: bool) -> None:
+        """Fast form: select the k nearest inside the MFMA matcher (True,
+        the default) or from staged distances in a second kernel (False);
+        both give the same match lists."""
+        rc = self._lib.vnlbhip_set_fused_selection(self._handle, int(enable))
+        if rc != 0:
+            self._raise(rc)
+
     def set_profiling(self, enable: bool) -> None:
         self._lib.vnlbhip_set_profiling(self._handle, 1 if enable else 0)
 

@@ -27,6 +27,7 @@ EXPORTED_SYMBOLS = (
     "vnlbhip_get_kernel_times",
     "vnlbhip_set_form",
     "vnlbhip_set_eigen_refine",
+    "vnlbhip_set_fused_selection",
 )
 
 KERNEL_NAMES = ("match", "paste", "prepare", "jacobi", "filter", "normalize", "select", "energy")
@@ -139,6 +140,8 @@ def load() -> ctypes.CDLL:
     lib.vnlbhip_set_form.restype = ctypes.c_int
     lib.vnlbhip_set_eigen_refine.argtypes = [vp, ctypes.c_float]
     lib.vnlbhip_set_eigen_refine.restype = ctypes.c_int
+    lib.vnlbhip_set_fused_selection.argtypes = [vp, ctypes.c_int]
+    lib.vnlbhip_set_fused_selection.restype = ctypes.c_int
     lib.vnlbhip_get_kernel_times.argtypes = [vp, P(KernelTimes)]
     lib.vnlbhip_get_kernel_times.restype = ctypes.c_int
     _lib = lib
