@@ -93,6 +93,22 @@ __device__ inline float wave_min(float v) {
     return v;
 }
 
+__device__ inline int wave_min_int(int v) {
+#pragma unroll
+    for (int offset = 32; offset >= 1; offset >>= 1) {
+        v = min(v, __shfl_xor(v, offset, 64));
+    }
+    return v;
+}
+
+__device__ inline int wave_max_int(int v) {
+#pragma unroll
+    for (int offset = 32; offset >= 1; offset >>= 1) {
+        v = max(v, __shfl_xor(v, offset, 64));
+    }
+    return v;
+}
+
 __device__ inline double wave_max_double(double v) {
 #pragma unroll
     for (int offset = 32; offset >= 1; offset >>= 1) {

@@ -45,7 +45,7 @@ struct WaveLayout {
     int norm;   // [kFastRows] squared norms of the member rows
     int u;      // [16][16] eigenvectors u_j[s]
     int pos;    // [kFastRows]
-    int sorted; // per sample in source-slot order: [kFastRows] position, tile cell, weight, plane offset, lane
+    int sorted; // per sample in source-slot order: [kFastRows] position, tile cell, weight
     int begin;  // [kMaxTemporal + 8] first sorted sample of each source slot
     int tile;   // [side][side][2] value plane and weight plane of one output frame
     int words;
@@ -68,7 +68,7 @@ __host__ __device__ inline WaveLayout wave_layout(const StageDesc& d) {
     L.pos = at;
     at += kFastRows;
     L.sorted = at;
-    at += 5 * kFastRows;
+    at += 3 * kFastRows;
     L.begin = at;
     at += (kMaxTemporal + 8 + 3) / 4 * 4;  // slots + patch frames <= kMaxTemporal + 4
     L.tile = at;
@@ -266,14 +266,11 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
     uint32_t* s_kpos = reinterpret_cast<uint32_t*>(s_wave + W.sorted);
     int* s_kcell = reinterpret_cast<int*>(s_wave + W.sorted + kFastRows);
     float* s_kw = s_wave + W.sorted + 2 * kFastRows;
-    int* s_koff = reinterpret_cast<int*>(s_wave + W.sorted + 3 * kFastRows);  // y * width + x of the origin
-    int* s_klane = reinterpret_cast<int*>(s_wave + W.sorted + 4 * kFastRows);  // the sample's lane
     int* s_begin = reinterpret_cast<int*>(s_wave + W.begin);
     float* s_tile = s_wave + W.tile;
     // Element e = c * patch_dim + 64 t + lane of an 8x8 patch sits at
     // (lane & 7, lane >> 3) for every channel and patch frame.
     const int lane_cell = (lane >> 3) * side + (lane & 7);
-    const int lane_off = (lane >> 3) * d.width + (lane & 7);
     const float lane_window = d.window[lane];
 
     for (int e = tid; e < dim; e += kModelThreads) {
@@ -357,8 +354,6 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
             if (my_k >= 0) {
                 s_kpos[my_k] = my_pos;
                 s_kcell[my_k] = (position_y(my_pos) - y_low) * side + position_x(my_pos) - x_low;
-                s_koff[my_k] = position_y(my_pos) * d.width + position_x(my_pos);
-                s_klane[my_k] = lane;
             }
         }
         // Centre of the estimate rows and mean of the model / member rows.
@@ -591,14 +586,14 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
         // below take them with v_readlane instead of LDS broadcasts.
         const int sorted_count = __builtin_amdgcn_readfirstlane(s_begin[sorted_slots]);
         int kcell_l = 0;
-        int koff_l = 0;
-        int klane_l = 0;
         float kw_l = 0.0F;
+        int ky_l = 0;  // tile row and column of the sample's patch origin
+        int kx_l = 0;
         if (lane < sorted_count) {
             kcell_l = s_kcell[lane];
-            koff_l = s_koff[lane];
-            klane_l = s_klane[lane];
             kw_l = s_kw[lane];
+            ky_l = kcell_l / side;
+            kx_l = kcell_l - ky_l * side;
         }
 
         // Estimate and weighted aggregation: per channel and output frame of
@@ -688,37 +683,39 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                     }
                 }
                 wave_sync();
-                // Flush: the cells touched in this pass are those of the
-                // samples that fed it.  Each is read and cleared in sample
-                // order (LDS operations of a wavefront complete in issue
-                // order), so a cell shared by several samples reaches the
-                // accumulators once; four samples' reads are in flight
-                // together.
+                // Flush.  A global atomic costs one memory-side operation per
+                // 64-byte line an instruction touches, so the tile is
+                // flushed over the bounding box of the patches that fed this
+                // pass in blocks of 8 rows x 8 line-aligned pixels (lane =
+                // row * 8 + pixel): every line is touched once per block, and
+                // every touched cell is read, cleared and added once.
                 const int frame = af - d.search_bwd + slot;
-                double* value_base = acc.data + (static_cast<size_t>(frame) * (d.channels + 1) + c) * plane + lane_off;
-                double* weight_base = value_base + static_cast<size_t>(d.channels - c) * plane;
-                for (int k0 = k_first; k0 < k_last; k0 += 4) {
-                    unsigned long long taken[4];
-                    int offset[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        taken[u] = 0;
-                        offset[u] = 0;
-                        if (k0 + u < k_last) {
-                            const int cell = __builtin_amdgcn_readlane(kcell_l, k0 + u) + lane_cell;
-                            offset[u] = __builtin_amdgcn_readlane(koff_l, k0 + u);
-                            unsigned long long* slot64 = reinterpret_cast<unsigned long long*>(s_tile) + cell;
-                            taken[u] = *slot64;
+                double* value_plane = acc.data + (static_cast<size_t>(frame) * (d.channels + 1) + c) * plane;
+                double* weight_plane = value_plane + static_cast<size_t>(d.channels - c) * plane;
+                const bool feeds = lane >= k_first && lane < k_last;
+                const int ty0 = __builtin_amdgcn_readfirstlane(wave_min_int(feeds ? ky_l : side));
+                const int ty1 = __builtin_amdgcn_readfirstlane(wave_max_int(feeds ? ky_l : -1)) + 7;  // the 8x8 footprint
+                const int tx0 = __builtin_amdgcn_readfirstlane(wave_min_int(feeds ? kx_l : side));
+                const int tx1 = __builtin_amdgcn_readfirstlane(wave_max_int(feeds ? kx_l : -1)) + 7;  // the 8x8 footprint
+                const int gx_first = (x_low + tx0) & ~7;  // line-aligned image column
+                const int gx_last = x_low + tx1;
+                unsigned long long* tile64 = reinterpret_cast<unsigned long long*>(s_tile);
+                for (int ty_b = ty0; ty_b <= ty1; ty_b += 8) {
+                    const int ty = ty_b + (lane >> 3);
+                    for (int gx_b = gx_first; gx_b <= gx_last; gx_b += 8) {
+                        const int tx = gx_b + (lane & 7) - x_low;
+                        unsigned long long taken = 0ULL;
+                        if (ty <= ty1 && tx >= tx0 && tx <= tx1) {
+                            unsigned long long* slot64 = tile64 + ty * side + tx;
+                            taken = *slot64;
                             *slot64 = 0ULL;
                         }
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        if (taken[u] != 0ULL) {
-                            const float2 v = __builtin_bit_cast(float2, taken[u]);
-                            unsafeAtomicAdd(value_base + offset[u], static_cast<double>(v.x));
+                        if (taken != 0ULL) {
+                            const float2 v = __builtin_bit_cast(float2, taken);
+                            const size_t offset = static_cast<size_t>(y_low + ty) * d.width + x_low + tx;
+                            unsafeAtomicAdd(value_plane + offset, static_cast<double>(v.x));
                             if (c == 0) {
-                                unsafeAtomicAdd(weight_base + offset[u], static_cast<double>(v.y));
+                                unsafeAtomicAdd(weight_plane + offset, static_cast<double>(v.y));
                             }
                         }
                     }
