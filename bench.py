@@ -99,7 +99,9 @@ def filter_roofline(form, times, bp, fp, clip, sb, sf) -> dict:
     algorithmic bytes of the scatter design per stage, sum over groups of
     S * D * 4 (filtered rows read) + S * Pt * P^2 * (C + 1) * 8 (accumulator
     read-modify-write, fp32), S the retained samples of the group, over the
-    kernel's average launch time from hipEvents (one launch per stage)."""
+    kernel's average launch time from hipEvents (a stage launches it once per
+    part of its anchor frames, so per launch is the stage total over the
+    launches)."""
     channels = clip["channels"]
     total = 0.0
     for params, stats in ((bp, sb), (fp, sf)):

@@ -79,8 +79,8 @@ def test_fused_selection_matches_staged(stage):
     """The k-nearest selection inside the MFMA matcher keeps exactly the
     lists that the plain form's selection (select.h) makes from the same
     staged distances: counts, positions and distances are equal on every
-    anchor, and the stage outputs agree up to the order of the fp64
-    accumulator adds."""
+    anchor, and the stage outputs agree up to the order of the fp32
+    accumulator adds (a few float ulps of each sum)."""
     fused = vnlbhip.Context(0)
     staged = vnlbhip.Context(0)
     try:
@@ -104,7 +104,7 @@ def test_fused_selection_matches_staged(stage):
                 np.testing.assert_array_equal(a["distances"][i, :n], b["distances"][i, :n], err_msg=str(i))
         out_a = fused.stage(stage, params, _dev(noisy), ref).cpu().numpy()
         out_b = staged.stage(stage, params, _dev(noisy), ref).cpu().numpy()
-        np.testing.assert_allclose(out_a, out_b, rtol=0, atol=1e-5)
+        np.testing.assert_allclose(out_a, out_b, rtol=0, atol=1e-4)
         with pytest.raises(vnlbhip.VNLBError):
             staged.set_fused_selection(2)
     finally:

@@ -29,8 +29,9 @@ __global__ __launch_bounds__(256) void fill_zero_int_kernel(int* __restrict__ da
     }
 }
 
+template <class T>
 __global__ __launch_bounds__(256) void normalize_kernel(StageDesc d,
-                                                        const double* __restrict__ acc,
+                                                        const T* __restrict__ acc,
                                                         const float* __restrict__ source,
                                                         float* __restrict__ out) {
     const size_t plane = static_cast<size_t>(d.width) * d.height;
@@ -40,7 +41,7 @@ __global__ __launch_bounds__(256) void normalize_kernel(StageDesc d,
          i += stride) {
         const size_t frame = i / plane;
         const size_t pixel = i - frame * plane;
-        const double* base = acc + frame * (d.channels + 1) * plane;
+        const T* base = acc + frame * (d.channels + 1) * plane;
         const float weight = static_cast<float>(base[static_cast<size_t>(d.channels) * plane + pixel]);
         for (int c = 0; c < d.channels; ++c) {
             const size_t index = (frame * d.channels + c) * plane + pixel;
@@ -124,7 +125,15 @@ hipError_t launch_fill_zero_int(int* data, size_t count, hipStream_t stream) {
 hipError_t launch_normalize(const StageDesc& d, const double* acc, const float* source,
                             float* out, hipStream_t stream) {
     const size_t total = static_cast<size_t>(d.width) * d.height * d.frames;
-    hipLaunchKernelGGL(normalize_kernel, dim3(grid_for(total)), dim3(256), 0, stream, d, acc,
+    hipLaunchKernelGGL(normalize_kernel<double>, dim3(grid_for(total)), dim3(256), 0, stream, d, acc,
+                       source, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_normalize(const StageDesc& d, const float* acc, const float* source,
+                            float* out, hipStream_t stream) {
+    const size_t total = static_cast<size_t>(d.width) * d.height * d.frames;
+    hipLaunchKernelGGL(normalize_kernel<float>, dim3(grid_for(total)), dim3(256), 0, stream, d, acc,
                        source, out);
     return hipGetLastError();
 }

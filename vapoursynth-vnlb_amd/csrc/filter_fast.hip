@@ -245,7 +245,7 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
     StageDesc d, const float* __restrict__ noisy, const float* __restrict__ basic, GroupRefs refs,
     int group_count, int problem_stride, const int* __restrict__ group_flags,
     const float* __restrict__ means, const float* __restrict__ eigenvalues,
-    const float* __restrict__ vectors, const float* __restrict__ grams, Accumulators acc,
+    const float* __restrict__ vectors, const float* __restrict__ grams, float* __restrict__ acc,
     float* __restrict__ weights_out) {
     extern __shared__ float4 s_dyn4[];
     float* s_dyn = reinterpret_cast<float*>(s_dyn4);
@@ -684,26 +684,28 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                 }
                 wave_sync();
                 // Flush.  A global atomic costs one memory-side operation per
-                // 64-byte line an instruction touches, so the tile is
-                // flushed over the bounding box of the patches that fed this
-                // pass in blocks of 8 rows x 8 line-aligned pixels (lane =
-                // row * 8 + pixel): every line is touched once per block, and
-                // every touched cell is read, cleared and added once.
+                // 64-byte line an instruction touches, whatever the element
+                // size, so the accumulators are fp32 (16 per line) and the
+                // tile is flushed over the bounding box of the patches that
+                // fed this pass in blocks of 4 rows x 16 line-aligned pixels
+                // (lane = row * 16 + pixel; aligned when the width is a
+                // multiple of 16): every touched cell is read, cleared and
+                // added once.
                 const int frame = af - d.search_bwd + slot;
-                double* value_plane = acc.data + (static_cast<size_t>(frame) * (d.channels + 1) + c) * plane;
-                double* weight_plane = value_plane + static_cast<size_t>(d.channels - c) * plane;
+                float* value_plane = acc + (static_cast<size_t>(frame) * (d.channels + 1) + c) * plane;
+                float* weight_plane = value_plane + static_cast<size_t>(d.channels - c) * plane;
                 const bool feeds = lane >= k_first && lane < k_last;
                 const int ty0 = __builtin_amdgcn_readfirstlane(wave_min_int(feeds ? ky_l : side));
                 const int ty1 = __builtin_amdgcn_readfirstlane(wave_max_int(feeds ? ky_l : -1)) + 7;  // the 8x8 footprint
                 const int tx0 = __builtin_amdgcn_readfirstlane(wave_min_int(feeds ? kx_l : side));
-                const int tx1 = __builtin_amdgcn_readfirstlane(wave_max_int(feeds ? kx_l : -1)) + 7;  // the 8x8 footprint
-                const int gx_first = (x_low + tx0) & ~7;  // line-aligned image column
+                const int tx1 = __builtin_amdgcn_readfirstlane(wave_max_int(feeds ? kx_l : -1)) + 7;
+                const int gx_first = (x_low + tx0) & ~15;  // line-aligned image column
                 const int gx_last = x_low + tx1;
                 unsigned long long* tile64 = reinterpret_cast<unsigned long long*>(s_tile);
-                for (int ty_b = ty0; ty_b <= ty1; ty_b += 8) {
-                    const int ty = ty_b + (lane >> 3);
-                    for (int gx_b = gx_first; gx_b <= gx_last; gx_b += 8) {
-                        const int tx = gx_b + (lane & 7) - x_low;
+                for (int ty_b = ty0; ty_b <= ty1; ty_b += 4) {
+                    const int ty = ty_b + (lane >> 4);
+                    for (int gx_b = gx_first; gx_b <= gx_last; gx_b += 16) {
+                        const int tx = gx_b + (lane & 15) - x_low;
                         unsigned long long taken = 0ULL;
                         if (ty <= ty1 && tx >= tx0 && tx <= tx1) {
                             unsigned long long* slot64 = tile64 + ty * side + tx;
@@ -713,9 +715,9 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                         if (taken != 0ULL) {
                             const float2 v = __builtin_bit_cast(float2, taken);
                             const size_t offset = static_cast<size_t>(y_low + ty) * d.width + x_low + tx;
-                            unsafeAtomicAdd(value_plane + offset, static_cast<double>(v.x));
+                            unsafeAtomicAdd(value_plane + offset, v.x);
                             if (c == 0) {
-                                unsafeAtomicAdd(weight_plane + offset, static_cast<double>(v.y));
+                                unsafeAtomicAdd(weight_plane + offset, v.y);
                             }
                         }
                     }
@@ -743,13 +745,12 @@ hipError_t launch_fast_filter(const StageDesc& d, const float* noisy, const floa
                               const uint32_t* anchors, const int* counts, const uint32_t* positions,
                               const int* group_list, int group_count, int problem_stride,
                               const int* group_flags, const float* means, const float* eigenvalues,
-                              const float* vectors, const float* grams, double* accumulators,
+                              const float* vectors, const float* grams, float* accumulators,
                               float* weights_out, hipStream_t stream) {
     if (group_count <= 0) {
         return hipSuccess;
     }
     const GroupRefs refs{anchors, counts, positions, group_list};
-    const Accumulators acc{accumulators};
     const size_t shared = fast_filter_shared_bytes(d);
     int device = 0;
     int units = 0;
@@ -773,7 +774,7 @@ hipError_t launch_fast_filter(const StageDesc& d, const float* noisy, const floa
         const unsigned blocks = static_cast<unsigned>(resident < wanted ? resident : wanted);        \
         hipLaunchKernelGGL(filter_fast_kernel<KPT>, dim3(blocks), dim3(kModelThreads), shared,       \
                            stream, d, noisy, basic, refs, group_count, problem_stride, group_flags,  \
-                           means, eigenvalues, vectors, grams, acc, weights_out);                    \
+                           means, eigenvalues, vectors, grams, accumulators, weights_out);                    \
     } while (0)
     switch (d.patch_time) {
         case 1: VNLBHIP_FAST_FILTER(1); break;

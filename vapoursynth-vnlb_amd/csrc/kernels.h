@@ -85,12 +85,15 @@ hipError_t launch_fast_filter(const StageDesc& d, const float* noisy, const floa
                               const uint32_t* anchors, const int* counts, const uint32_t* positions,
                               const int* group_list, int group_count, int problem_stride,
                               const int* group_flags, const float* means, const float* eigenvalues,
-                              const float* vectors, const float* grams, double* accumulators,
+                              const float* vectors, const float* grams, float* accumulators,
                               float* weights_out, hipStream_t stream);
 
 hipError_t launch_fill_zero(double* data, size_t count, hipStream_t stream);
 hipError_t launch_fill_zero_int(int* data, size_t count, hipStream_t stream);
 hipError_t launch_normalize(const StageDesc& d, const double* acc, const float* source, float* out,
+                            hipStream_t stream);
+// FAST form accumulators (fp32, same layout).
+hipError_t launch_normalize(const StageDesc& d, const float* acc, const float* source, float* out,
                             hipStream_t stream);
 
 }  // namespace vnlbhip

@@ -555,8 +555,13 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
 
     const size_t plane = static_cast<size_t>(g.width) * g.height;
     const size_t acc_count = plane * (g.channels + 1) * g.frames;
+    // The FAST filter aggregates into fp32 accumulators (same layout, in
+    // the same buffer); every other path into fp64 ones.
+    const bool acc_f32 = ctx.form == VNLBHIP_FORM_FAST && vnlbhip::fast_filter_supported(d);
     double* acc = ctx.accumulators.reserve(acc_count);
-    check(vnlbhip::launch_fill_zero(acc, acc_count, stream), "fill accumulators");
+    float* acc32 = reinterpret_cast<float*>(acc);
+    check(vnlbhip::launch_fill_zero(acc, acc_f32 ? (acc_count + 1) / 2 : acc_count, stream),
+          "fill accumulators");
     int* summary = ctx.summary.reserve(8);
     check(hipMemsetAsync(summary, 0, 8 * sizeof(int), stream), "hipMemsetAsync");
 
@@ -834,12 +839,12 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
                                                   d_eigen, d_vectors, d_status, stream);
                 });
             }
-            const bool fast_filter = ctx.form == VNLBHIP_FORM_FAST && vnlbhip::fast_filter_supported(d);
+            const bool fast_filter = acc_f32;
             timed(ctx, VNLBHIP_KERNEL_FILTER, stream, "filter kernel", [&]() {
                 if (fast_filter) {
                     return vnlbhip::launch_fast_filter(d, noisy, basic, d_anchors, d_counts, d_positions,
                                                        d_part_list, groups, n_stride, d_flags, d_means,
-                                                       d_eigen, d_vectors, d_problems, acc, d_weights,
+                                                       d_eigen, d_vectors, d_problems, acc32, d_weights,
                                                        stream);
                 }
                 return vnlbhip::launch_filter(d, noisy, basic, d_anchors, d_counts, d_positions,
@@ -940,7 +945,10 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
 
     if (out != nullptr) {
         timed(ctx, VNLBHIP_KERNEL_NORMALIZE, stream, "normalize kernel",
-              [&]() { return vnlbhip::launch_normalize(d, acc, noisy, out, stream); });
+              [&]() {
+                  return acc_f32 ? vnlbhip::launch_normalize(d, acc32, noisy, out, stream)
+                                 : vnlbhip::launch_normalize(d, acc, noisy, out, stream);
+              });
     }
     int host_summary[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     check(hipMemcpyAsync(host_summary, summary, sizeof(host_summary), hipMemcpyDeviceToHost, stream),
