@@ -187,7 +187,7 @@ __global__ __launch_bounds__(kModelThreads, kS <= 24 ? 3 : 1) void filter_kernel
     const float* __restrict__ basic, GroupRefs refs, int group_count, unsigned group_stride,
     int problem_stride, const int* __restrict__ group_flags, const float* __restrict__ means,
     const float* __restrict__ eigenvalues, const float* __restrict__ vectors, Accumulators acc,
-    float* __restrict__ weights_out) {
+    float* __restrict__ weights_out, int* __restrict__ work_counter) {
     extern __shared__ float4 s_dyn4[];
     float* s_dyn = reinterpret_cast<float*>(s_dyn4);
     __shared__ float s_lambda[kMaxChannels * kMaxRank];
@@ -268,11 +268,29 @@ __global__ __launch_bounds__(kModelThreads, kS <= 24 ? 3 : 1) void filter_kernel
     const float* member_video = final_stage ? basic : noisy;
     const float* member_mean_all = final_stage ? s_mean_b : s_mean_n;
 
-    for (int gi = static_cast<int>(blockIdx.x); gi < group_count;
-         gi += static_cast<int>(gridDim.x)) {
+    // Group indices are taken in order from the work counter (the next one
+    // requested while the current group is processed) and spread over the
+    // list by group_stride.
+    __shared__ int s_next;
+    if (threadIdx.x == 0) {
+        s_next = atomicAdd(work_counter, 1);
+    }
+    int following = 0;
+    const auto advance = [&]() {
+        __syncthreads();  // the group is done with the LDS and s_next
+        if (threadIdx.x == 0) {
+            s_next = following;
+        }
+        __syncthreads();
+        return s_next;
+    };
+    __syncthreads();
+    for (int gi = s_next; gi < group_count; gi = advance()) {
+        if (threadIdx.x == 0) {
+            following = atomicAdd(work_counter, 1);
+        }
         const int g = static_cast<int>((static_cast<unsigned long long>(gi) * group_stride) %
                                        static_cast<unsigned>(group_count));
-        __syncthreads();  // the previous group is done with the LDS
         const int a = refs.group_list[g];
         const int af = position_frame(refs.anchors[a]);
         const int similar = refs.counts[a];
@@ -679,7 +697,8 @@ hipError_t launch_filter(const StageDesc& d, const float* noisy, const float* ba
                          const uint32_t* positions, const int* group_list, int group_count,
                          int problem_stride, const int* group_flags, const int* problem_dims,
                          const float* means, const float* eigenvalues, const float* vectors,
-                         double* accumulators, float* weights_out, hipStream_t stream) {
+                         double* accumulators, float* weights_out, int* work_counter,
+                         hipStream_t stream) {
     (void)problem_dims;
     if (group_count <= 0) {
         return hipSuccess;
@@ -729,7 +748,7 @@ hipError_t launch_filter(const StageDesc& d, const float* noisy, const float* ba
         hipLaunchKernelGGL((filter_kernel<KS, EXACT>), dim3(blocks), dim3(kModelThreads), shared, \
                            stream, d, L, noisy, basic, refs, group_count, stride,                 \
                            problem_stride, group_flags, means, eigenvalues, vectors, acc,         \
-                           weights_out);                                                          \
+                           weights_out, work_counter);                                            \
     } while (0)
 #define VNLBHIP_FILTER_SLOTS(KS)          \
     if (exact) {                          \

@@ -246,7 +246,7 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
     int group_count, int problem_stride, const int* __restrict__ group_flags,
     const float* __restrict__ means, const float* __restrict__ eigenvalues,
     const float* __restrict__ vectors, const float* __restrict__ grams, float* __restrict__ acc,
-    float* __restrict__ weights_out) {
+    float* __restrict__ weights_out, int* __restrict__ work_counter) {
     extern __shared__ float4 s_dyn4[];
     float* s_dyn = reinterpret_cast<float*>(s_dyn4);
     const int dim = d.dim;  // == model dimension (one model)
@@ -302,8 +302,18 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
     const float dual_floor = 128.0F * FLT_EPSILON;
     const float* model_video = final_stage ? basic : noisy;
 
-    const int waves_total = static_cast<int>(gridDim.x) * kWaves;
-    for (int g = static_cast<int>(blockIdx.x) * kWaves + wave; g < group_count; g += waves_total) {
+    // Groups are taken in order from the work counter, one request per
+    // wavefront; the next index is requested while the current group runs.
+    int request = 0;
+    if (lane == 0) {
+        request = atomicAdd(work_counter, 1);
+    }
+    int following = 0;
+    for (int g = __builtin_amdgcn_readlane(request, 0); g < group_count;
+         g = __builtin_amdgcn_readlane(following, 0)) {
+        if (lane == 0) {
+            following = atomicAdd(work_counter, 1);
+        }
         wave_sync();  // the previous group is done with this wave's LDS
         const int a = refs.group_list[g];
         const int af = position_frame(refs.anchors[a]);
@@ -746,7 +756,7 @@ hipError_t launch_fast_filter(const StageDesc& d, const float* noisy, const floa
                               const int* group_list, int group_count, int problem_stride,
                               const int* group_flags, const float* means, const float* eigenvalues,
                               const float* vectors, const float* grams, float* accumulators,
-                              float* weights_out, hipStream_t stream) {
+                              float* weights_out, int* work_counter, hipStream_t stream) {
     if (group_count <= 0) {
         return hipSuccess;
     }
@@ -774,7 +784,7 @@ hipError_t launch_fast_filter(const StageDesc& d, const float* noisy, const floa
         const unsigned blocks = static_cast<unsigned>(resident < wanted ? resident : wanted);        \
         hipLaunchKernelGGL(filter_fast_kernel<KPT>, dim3(blocks), dim3(kModelThreads), shared,       \
                            stream, d, noisy, basic, refs, group_count, problem_stride, group_flags,  \
-                           means, eigenvalues, vectors, grams, accumulators, weights_out);                    \
+                           means, eigenvalues, vectors, grams, accumulators, weights_out, work_counter);                    \
     } while (0)
     switch (d.patch_time) {
         case 1: VNLBHIP_FAST_FILTER(1); break;

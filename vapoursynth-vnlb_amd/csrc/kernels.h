@@ -52,11 +52,15 @@ hipError_t launch_compact(const unsigned char* active, int anchor_begin, int anc
 
 size_t prepare_shared_bytes(const StageDesc& d);
 size_t filter_shared_bytes(const StageDesc& d);
+// prepare, filter and fast filter: persistent grids that take groups in
+// order from *work_counter (zero on entry), so workgroups that become
+// resident late (beside other kernels) take fewer groups instead of
+// stretching the launch.
 hipError_t launch_prepare(const StageDesc& d, const float* noisy, const float* basic,
                           const uint32_t* anchors, const int* counts, const uint32_t* positions,
                           const int* group_list, int group_count, int problem_stride,
                           int* group_flags, int* problem_dims, float* means, float* problems,
-                          bool mfma_gram, hipStream_t stream);
+                          bool mfma_gram, int* work_counter, hipStream_t stream);
 hipError_t launch_jacobi(const StageDesc& d, int problem_count, int problem_stride,
                          const int* group_flags, const int* problem_dims, const float* problems,
                          float* eigenvalues, float* vectors, int* status, hipStream_t stream);
@@ -75,7 +79,7 @@ hipError_t launch_filter(const StageDesc& d, const float* noisy, const float* ba
                          const int* group_list, int group_count, int problem_stride,
                          const int* group_flags, const int* problem_dims, const float* means,
                          const float* eigenvalues, const float* vectors, double* accumulators,
-                         float* weights_out, hipStream_t stream);
+                         float* weights_out, int* work_counter, hipStream_t stream);
 
 // MFMA form of the filter (filter_fast.hip): dual-PCA groups of one model,
 // 8x8 patches, at most 16 model rows and components, at most 64 samples.
@@ -86,7 +90,7 @@ hipError_t launch_fast_filter(const StageDesc& d, const float* noisy, const floa
                               const int* group_list, int group_count, int problem_stride,
                               const int* group_flags, const float* means, const float* eigenvalues,
                               const float* vectors, const float* grams, float* accumulators,
-                              float* weights_out, hipStream_t stream);
+                              float* weights_out, int* work_counter, hipStream_t stream);
 
 hipError_t launch_fill_zero(double* data, size_t count, hipStream_t stream);
 hipError_t launch_fill_zero_int(int* data, size_t count, hipStream_t stream);

@@ -152,16 +152,19 @@ typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
 
 // kMfma (FAST form): the dual Gram on the matrix cores in bf16x3 instead of
-// the 16-lane fp32 team dots.
+// the 16-lane fp32 team dots.  At least four workgroups per compute unit
+// (at most 128 VGPRs): a group is a chain of dependent memory round trips,
+// so groups in flight set the rate.
 template <int kS, bool kExact, bool kMfma>
-__global__ __launch_bounds__(kModelThreads) void prepare_kernel(
+__global__ __launch_bounds__(kModelThreads, 4) void prepare_kernel(
     StageDesc d, PrepareLayout L, const float* __restrict__ noisy,
     const float* __restrict__ basic, GroupRefs refs, int group_count, int problem_stride,
     int* __restrict__ group_flags, int* __restrict__ problem_dims, float* __restrict__ means,
-    float* __restrict__ problems) {
+    float* __restrict__ problems, int* __restrict__ work_counter) {
     extern __shared__ float4 s_dyn4[];
     float* s_dyn = reinterpret_cast<float*>(s_dyn4);
     __shared__ int s_equal;
+    __shared__ int s_next;
     __shared__ double s_sum[kModelThreads / 64][kMaxChannels][2];
 
     int* tab_in = reinterpret_cast<int*>(s_dyn + L.tab_in);
@@ -207,10 +210,25 @@ __global__ __launch_bounds__(kModelThreads) void prepare_kernel(
         tab_c[i] = channel;
     }
 
-    for (int gi = static_cast<int>(blockIdx.x); gi < group_count;
-         gi += static_cast<int>(gridDim.x)) {
-        const int g = gi;
-        __syncthreads();  // the previous group is done with the LDS
+    // Groups are taken in order from the work counter; the next index is
+    // requested while the current group is processed.
+    if (tid == 0) {
+        s_next = atomicAdd(work_counter, 1);
+    }
+    int following = 0;
+    const auto advance = [&]() {
+        __syncthreads();  // the group is done with the LDS and s_next
+        if (tid == 0) {
+            s_next = following;
+        }
+        __syncthreads();
+        return s_next;
+    };
+    __syncthreads();
+    for (int g = s_next; g < group_count; g = advance()) {
+        if (tid == 0) {
+            following = atomicAdd(work_counter, 1);
+        }
         const int a = refs.group_list[g];
         const int similar = refs.counts[a];
         const size_t list = static_cast<size_t>(a) * d.cap;
@@ -584,7 +602,7 @@ hipError_t launch_prepare(const StageDesc& d, const float* noisy, const float* b
                           const uint32_t* anchors, const int* counts, const uint32_t* positions,
                           const int* group_list, int group_count, int problem_stride,
                           int* group_flags, int* problem_dims, float* means, float* problems,
-                          bool mfma_gram, hipStream_t stream) {
+                          bool mfma_gram, int* work_counter, hipStream_t stream) {
     if (group_count <= 0) {
         return hipSuccess;
     }
@@ -618,7 +636,7 @@ hipError_t launch_prepare(const StageDesc& d, const float* noisy, const float* b
             static_cast<unsigned>(resident < group_count ? resident : group_count);                \
         hipLaunchKernelGGL((prepare_kernel<KS, EXACT, MFMA>), dim3(blocks), dim3(kModelThreads), shared, \
                            stream, d, L, noisy, basic, refs, group_count, problem_stride,          \
-                           group_flags, problem_dims, means, problems);                            \
+                           group_flags, problem_dims, means, problems, work_counter);              \
     } while (0)
 #define VNLBHIP_PREPARE(KS, EXACT)                                                                 \
     if (EXACT && mfma) {                                                                           \

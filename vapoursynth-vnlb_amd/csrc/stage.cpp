@@ -424,6 +424,7 @@ struct vnlbhip_context {
     DeviceBuffer<float> vectors;
     DeviceBuffer<int> status;
     DeviceBuffer<int> refine_list;
+    DeviceBuffer<int> work_counters;  // prepare, filter: next group to take
     float refine_gap = 1e-3F;  // FAST form: relative eigen gap below which groups are re-solved in fp64
     bool fused_selection = true;  // FAST form: k-nearest selection inside the MFMA matcher
     DeviceBuffer<float> weights;
@@ -814,10 +815,12 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
             float* d_vectors =
                 ctx.vectors.reserve(static_cast<size_t>(problems) * std::max(d.rank, 1) * n_stride);
             int* d_status = ctx.status.reserve(problems);
+            int* d_work = ctx.work_counters.reserve(2);
+            check(vnlbhip::launch_fill_zero_int(d_work, 2, stream), "clear work counters");
             timed(ctx, VNLBHIP_KERNEL_PREPARE, stream, "prepare kernel", [&]() {
                 return vnlbhip::launch_prepare(d, noisy, basic, d_anchors, d_counts, d_positions,
                                                d_part_list, groups, n_stride, d_flags, d_dims, d_means,
-                                               d_problems, ctx.form == VNLBHIP_FORM_FAST, stream);
+                                               d_problems, ctx.form == VNLBHIP_FORM_FAST, d_work, stream);
             });
             if (d.rank > 0) {
                 const bool parallel_jacobi =
@@ -845,11 +848,11 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
                     return vnlbhip::launch_fast_filter(d, noisy, basic, d_anchors, d_counts, d_positions,
                                                        d_part_list, groups, n_stride, d_flags, d_means,
                                                        d_eigen, d_vectors, d_problems, acc32, d_weights,
-                                                       stream);
+                                                       d_work + 1, stream);
                 }
                 return vnlbhip::launch_filter(d, noisy, basic, d_anchors, d_counts, d_positions,
                                               d_part_list, groups, n_stride, d_flags, d_dims, d_means,
-                                              d_eigen, d_vectors, acc, d_weights, stream);
+                                              d_eigen, d_vectors, acc, d_weights, d_work + 1, stream);
             });
             check(vnlbhip::launch_summarize(d_flags, d.rank > 0 ? d_status : nullptr, d_counts,
                                             d_part_list, groups, d.models, summary, stream),
