@@ -106,6 +106,22 @@ int vnlbhip_stage(vnlbhip_context* context, int stage,
                   const float* ref, float* out, int width, int height,
                   int frames, int channels, void* stream);
 
+/* One anchor frame's contribution stack on device memory, as a vnlb.Basic /
+ * vnlb.Final output frame holds it (src/vs/plugin.cpp:264-271,625-641; the
+ * CUDA plugin's pack_kernel, src/cuda/aggregation.cu:508-528, for
+ * vnlb.Aggregate, src/vs/plugin.cpp:966-1045): slots = search_bwd +
+ * search_fwd + patch_time; for channel c and slot s (output frame
+ * anchor_frame - search_bwd + s) the numerator plane, then the weight plane,
+ * at stack + ((c * slots + s) * 2 + {0, 1}) * height * width.  Slots outside
+ * the clip, and every slot of an anchor frame past frames - patch_time, are
+ * zero.  Summing the stacks of the anchor frames that reach an output frame
+ * and dividing gives vnlbhip_stage's output.  Blocks until done. */
+int vnlbhip_anchor_stack(vnlbhip_context* context, int stage,
+                         const vnlbhip_params* params, const float* noisy,
+                         const float* ref, int width, int height, int frames,
+                         int channels, int anchor_frame, float* stack,
+                         void* stream);
+
 /* Same with host buffers: copies in, runs, copies out. */
 int vnlbhip_stage_host(vnlbhip_context* context, int stage,
                        const vnlbhip_params* params, const float* noisy,

@@ -68,6 +68,8 @@ def load() -> ctypes.CDLL:
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int, ip, ip, ip, ip, ip, fp, P(ctypes.c_ubyte),
                                        fp, fp, ip, ctypes.c_int]
+        lib.vo_anchor_stack.argtypes = [ctypes.c_int, P(VoParams), fp, fp, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, fp, ctypes.c_int]
         lib.vo_retained_cap.argtypes = [P(VoParams)]
         lib.vo_grid_count.argtypes = [P(VoParams), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int]
@@ -107,6 +109,23 @@ def stage(stage_id: int, params, noisy, ref=None, flags: int = ACCUM_DOUBLE, thr
     if rc != 0:
         raise ValueError(lib.vo_last_error().decode())
     return out, int(groups.value)
+
+
+def anchor_stack(stage_id: int, params, noisy, ref, anchor_frame: int, flags: int = ACCUM_DOUBLE):
+    """Contribution stack of one anchor frame, [channels, slots * 2 * height,
+    width]: per slot a numerator plane then the weight plane (the layout of a
+    vnlb.Basic / vnlb.Final output frame)."""
+    lib = load()
+    src, (w, h, f, c) = _video(noisy)
+    guide = _video(ref)[0] if ref is not None else None
+    p = to_struct(params)
+    slots = int(p.search_bwd) + int(p.search_fwd) + int(p.patch_time)
+    packed = np.empty((c, slots * 2 * h, w), np.float32)
+    rc = lib.vo_anchor_stack(stage_id, ctypes.byref(p), _fp(src), _fp(guide), w, h, f, c,
+                             anchor_frame, _fp(packed), flags)
+    if rc != 0:
+        raise ValueError(lib.vo_last_error().decode())
+    return packed
 
 
 def frame_debug(stage_id: int, params, noisy, ref, anchor_frame: int, flags: int = ACCUM_DOUBLE) -> dict:

@@ -1671,6 +1671,57 @@ int vo_grid_count(const vo_params* params, int width, int height, int frames,
     return count;
 }
 
+int vo_anchor_stack(int stage, const vo_params* params, const float* noisy,
+                    const float* ref, int width, int height, int frames,
+                    int channels, int anchor_frame, float* packed, int flags) {
+    try {
+        require(params != nullptr && packed != nullptr, "null argument");
+        require(stage == VO_STAGE_BASIC || stage == VO_STAGE_FINAL, "bad stage");
+        require(anchor_frame >= 0 && anchor_frame < frames, "anchor frame out of range");
+        const Geometry g{width, height, frames, channels};
+        const StageSetup s = make_setup(stage, *params, noisy, ref, g, flags);
+        const auto pack = [&](auto& stack) {
+            const std::size_t pixels = static_cast<std::size_t>(width) * height;
+            for (int c = 0; c < channels; ++c) {
+                for (int slot = 0; slot < s.slots; ++slot) {
+                    float* num = packed + (static_cast<std::size_t>(c) * s.slots + slot) * 2 * pixels;
+                    float* wgt = num + pixels;
+                    for (int y = 0; y < height; ++y) {
+                        for (int x = 0; x < width; ++x) {
+                            num[static_cast<std::size_t>(y) * width + x] =
+                                static_cast<float>(stack.at(slot, c, x, y));
+                            wgt[static_cast<std::size_t>(y) * width + x] =
+                                static_cast<float>(stack.at(slot, channels, x, y));
+                        }
+                    }
+                }
+            }
+        };
+        const bool model_double = (flags & VO_MODEL_DOUBLE) != 0;
+        if ((flags & VO_ACCUM_DOUBLE) != 0) {
+            Stack<double> stack;
+            if (model_double) {
+                process_anchor_frame<double, double>(s, anchor_frame, stack, nullptr);
+            } else {
+                process_anchor_frame<float, double>(s, anchor_frame, stack, nullptr);
+            }
+            pack(stack);
+        } else {
+            Stack<float> stack;
+            if (model_double) {
+                process_anchor_frame<double, float>(s, anchor_frame, stack, nullptr);
+            } else {
+                process_anchor_frame<float, float>(s, anchor_frame, stack, nullptr);
+            }
+            pack(stack);
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
 int vo_stage(int stage, const vo_params* params, const float* noisy,
              const float* ref, int width, int height, int frames, int channels,
              float* out, int flags, int threads, int64_t* groups_out) {

@@ -92,6 +92,17 @@ int vo_frame_debug(int stage, const vo_params* params, const float* noisy,
                    unsigned char* active, float* weights, float* eigenvalues,
                    int* group_flags, int flags);
 
+/* Contribution stack of one anchor frame in the vnlb.Basic / vnlb.Final
+ * output-frame layout (src/vs/plugin.cpp:264-271,625-641; the CUDA
+ * pack_kernel, src/cuda/aggregation.cu:508-528): per channel c and slot s =
+ * output frame - anchor_frame + search_bwd, a numerator plane then the
+ * weight plane, at ((c * slots + s) * 2 + {0, 1}) * height * width, slots =
+ * search_bwd + search_fwd + patch_time.  An anchor frame past
+ * frames - patch_time gives zeros.  Returns 0 or -1. */
+int vo_anchor_stack(int stage, const vo_params* params, const float* noisy,
+                    const float* ref, int width, int height, int frames,
+                    int channels, int anchor_frame, float* packed, int flags);
+
 /* Retained group cap (max(K, ceil(K * cap_factor)) limited by the window). */
 int vo_retained_cap(const vo_params* params);
 

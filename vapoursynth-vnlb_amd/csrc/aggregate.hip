@@ -52,6 +52,37 @@ __global__ __launch_bounds__(256) void normalize_kernel(StageDesc d,
 }
 
 
+// One anchor frame's contribution stack in the vnlb.Basic / vnlb.Final
+// output-frame layout (src/vs/plugin.cpp:264-271,625-641; pack_kernel,
+// src/cuda/aggregation.cu:508-528): plane c * slots + s holds the
+// numerators of channel c for output frame first_frame + s, then that
+// frame's weights.  The accumulators hold only this anchor frame's groups;
+// slots outside the clip are zero.
+template <class T>
+__global__ __launch_bounds__(256) void pack_stack_kernel(StageDesc d, const T* __restrict__ acc,
+                                                         int first_frame, int slots,
+                                                         float* __restrict__ stack) {
+    const size_t pixels = static_cast<size_t>(d.width) * d.height;
+    const size_t total = pixels * d.channels * slots;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const size_t plane = i / pixels;
+        const size_t pixel = i - plane * pixels;
+        const int c = static_cast<int>(plane / slots);
+        const int slot = static_cast<int>(plane - static_cast<size_t>(c) * slots);
+        const int frame = first_frame + slot;
+        float numerator = 0.0F;
+        float weight = 0.0F;
+        if (frame >= 0 && frame < d.frames) {
+            const T* base = acc + static_cast<size_t>(frame) * (d.channels + 1) * pixels;
+            numerator = static_cast<float>(base[static_cast<size_t>(c) * pixels + pixel]);
+            weight = static_cast<float>(base[static_cast<size_t>(d.channels) * pixels + pixel]);
+        }
+        stack[plane * 2 * pixels + pixel] = numerator;
+        stack[(plane * 2 + 1) * pixels + pixel] = weight;
+    }
+}
+
 // Stage health counters: summary[0] counts groups whose model rows cannot
 // cover the rank (the reference throws for those, core.cpp:645-648),
 // summary[1] eigen problems that reached the sweep limit without meeting the
@@ -120,6 +151,25 @@ hipError_t launch_fill_zero_int(int* data, size_t count, hipStream_t stream) {
     }
     hipLaunchKernelGGL(fill_zero_int_kernel, dim3(grid_for(count)), dim3(256), 0, stream, data, count);
     return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_pack(const StageDesc& d, const T* acc, int first_frame, int slots, float* stack,
+                       hipStream_t stream) {
+    const size_t total = static_cast<size_t>(d.width) * d.height * d.channels * slots;
+    hipLaunchKernelGGL(pack_stack_kernel<T>, dim3(grid_for(total)), dim3(256), 0, stream, d, acc,
+                       first_frame, slots, stack);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_stack(const StageDesc& d, const double* acc, int first_frame, int slots,
+                             float* stack, hipStream_t stream) {
+    return launch_pack(d, acc, first_frame, slots, stack, stream);
+}
+
+hipError_t launch_pack_stack(const StageDesc& d, const float* acc, int first_frame, int slots,
+                             float* stack, hipStream_t stream) {
+    return launch_pack(d, acc, first_frame, slots, stack, stream);
 }
 
 hipError_t launch_normalize(const StageDesc& d, const double* acc, const float* source,

@@ -92,6 +92,12 @@ hipError_t launch_fast_filter(const StageDesc& d, const float* noisy, const floa
                               const float* vectors, const float* grams, float* accumulators,
                               float* weights_out, int* work_counter, hipStream_t stream);
 
+// One anchor frame's contribution stack (accumulator slots first_frame ..
+// first_frame + slots - 1) in the packed vnlb output-frame layout.
+hipError_t launch_pack_stack(const StageDesc& d, const double* acc, int first_frame, int slots,
+                             float* stack, hipStream_t stream);
+hipError_t launch_pack_stack(const StageDesc& d, const float* acc, int first_frame, int slots,
+                             float* stack, hipStream_t stream);
 hipError_t launch_fill_zero(double* data, size_t count, hipStream_t stream);
 hipError_t launch_fill_zero_int(int* data, size_t count, hipStream_t stream);
 hipError_t launch_normalize(const StageDesc& d, const double* acc, const float* source, float* out,

@@ -531,9 +531,10 @@ void collect_times(vnlbhip_context& ctx) {
     ctx.pending.clear();
 }
 
+// stack (with only_frame): that anchor frame's contribution stack, packed.
 void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const float* noisy,
                const float* ref, float* out, const Geometry& g, hipStream_t stream,
-               int only_frame, DebugOutput* debug) {
+               int only_frame, DebugOutput* debug, float* stack = nullptr) {
     const StageDesc d = make_desc(stage, p, g);
     require(noisy != nullptr, "video data pointer must not be null");
     if (stage == VNLBHIP_STAGE_FINAL) {
@@ -946,6 +947,14 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
         }
     }
 
+    if (stack != nullptr) {
+        // Slot s of the stack is output frame only_frame - search_bwd + s.
+        const int slots = p.search_bwd + p.search_fwd + p.patch_time;
+        const int first = only_frame - p.search_bwd;
+        check(acc_f32 ? vnlbhip::launch_pack_stack(d, acc32, first, slots, stack, stream)
+                      : vnlbhip::launch_pack_stack(d, acc, first, slots, stack, stream),
+              "pack kernel");
+    }
     if (out != nullptr) {
         timed(ctx, VNLBHIP_KERNEL_NORMALIZE, stream, "normalize kernel",
               [&]() {
@@ -1158,6 +1167,21 @@ int vnlbhip_stage(vnlbhip_context* context, int stage, const vnlbhip_params* par
         run_stage(*context, stage, *params, noisy, ref, out,
                   Geometry{width, height, frames, channels},
                   reinterpret_cast<hipStream_t>(stream), -1, nullptr);
+    });
+}
+
+int vnlbhip_anchor_stack(vnlbhip_context* context, int stage, const vnlbhip_params* params,
+                         const float* noisy, const float* ref, int width, int height, int frames,
+                         int channels, int anchor_frame, float* stack, void* stream) {
+    if (context == nullptr) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    return guarded(context, [&]() {
+        require(params != nullptr && stack != nullptr, "params and stack must not be null");
+        require(anchor_frame >= 0 && anchor_frame < frames, "anchor_frame must be a frame of the clip");
+        run_stage(*context, stage, *params, noisy, ref, nullptr,
+                  Geometry{width, height, frames, channels},
+                  reinterpret_cast<hipStream_t>(stream), anchor_frame, nullptr, stack);
     });
 }
 

@@ -108,6 +108,32 @@ class Context:
         return {name: {"ms": t.milliseconds[i], "launches": t.launches[i]}
                 for i, name in enumerate(_abi.KERNEL_NAMES)}
 
+    def anchor_stack(self, stage: int, params: StageParameters, noisy, ref, anchor_frame: int):
+        """Contribution stack of one anchor frame as a vnlb.Basic / vnlb.Final
+        output frame holds it: a float32 GPU tensor [channels, slots * 2 *
+        height, width], per slot a numerator plane then the weight plane."""
+        import torch
+
+        frames, channels, height, width = _canonical_shape(tuple(noisy.shape))
+        if not _is_torch(noisy) or noisy.dtype != torch.float32 or not noisy.is_cuda:
+            raise VNLBError("VNLB HIP: device clips must be float32 tensors on the GPU")
+        if stage == FINAL and (ref is None or tuple(ref.shape) != tuple(noisy.shape)):
+            raise VNLBError("VNLB HIP: ref clip properties must match input clip")
+        p = _abi.to_struct(params)
+        slots = int(params.search_bwd) + int(params.search_fwd) + int(params.patch_time)
+        src = noisy.contiguous()
+        guide = ref.contiguous() if ref is not None else None
+        stack = torch.empty((channels, slots * 2 * height, width), dtype=torch.float32, device=src.device)
+        stream = torch.cuda.current_stream(src.device).cuda_stream
+        rc = self._lib.vnlbhip_anchor_stack(
+            self._handle, stage, ctypes.byref(p), ctypes.c_void_p(src.data_ptr()),
+            ctypes.c_void_p(guide.data_ptr()) if guide is not None else None, width, height,
+            frames, channels, int(anchor_frame), ctypes.c_void_p(stack.data_ptr()),
+            ctypes.c_void_p(stream))
+        if rc != 0:
+            self._raise(rc)
+        return stack
+
     def stage(self, stage: int, params: StageParameters, noisy, ref=None):
         """Run one stage over a whole clip and return the aggregated frames."""
         frames, channels, height, width = _canonical_shape(tuple(noisy.shape))

@@ -20,6 +20,7 @@ EXPORTED_SYMBOLS = (
     "vnlbhip_last_error",
     "vnlbhip_stage",
     "vnlbhip_stage_host",
+    "vnlbhip_anchor_stack",
     "vnlbhip_debug_frame",
     "vnlbhip_retained_cap",
     "vnlbhip_get_stats",
@@ -122,6 +123,10 @@ def load() -> ctypes.CDLL:
     lib.vnlbhip_stage.argtypes = [vp, ctypes.c_int, P(Params), vp, vp, vp, ctypes.c_int,
                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
     lib.vnlbhip_stage.restype = ctypes.c_int
+    lib.vnlbhip_anchor_stack.argtypes = [vp, ctypes.c_int, P(Params), vp, vp, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         vp, vp]
+    lib.vnlbhip_anchor_stack.restype = ctypes.c_int
     lib.vnlbhip_stage_host.argtypes = [vp, ctypes.c_int, P(Params), fp, fp, fp, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.vnlbhip_stage_host.restype = ctypes.c_int
