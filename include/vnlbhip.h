@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define VNLBHIP_ABI_VERSION 3
+#define VNLBHIP_ABI_VERSION 4
 
 enum {
     VNLBHIP_STAGE_BASIC = 0,
@@ -132,10 +132,13 @@ int vnlbhip_stage_host(vnlbhip_context* context, int stage,
  * the grid anchors in raster order and, per anchor, the match list
  * (positions as linear clip indices ((frame*height)+y)*width+x, distances),
  * the paste-mask decision, aggregation weights (active anchors), model
- * eigenvalues (first model, active modelled anchors) and group flags
- * (bit0 equal, bit1 flat, bit2 modelled).  Per-anchor arrays have
- * capacity_anchors entries, match arrays capacity_anchors * cap entries
- * with cap from vnlbhip_retained_cap; optional outputs may be NULL. */
+ * eigenvalues (first model, active modelled anchors), group flags
+ * (bit0 equal, bit1 flat, bit2 modelled) and the dual Gram XX^T / B the
+ * eigen solver received (first model of active modelled dual groups with
+ * B <= gram_dim, at grams + (a * gram_dim + r) * gram_dim + c).  Per-anchor
+ * arrays have capacity_anchors entries, match arrays capacity_anchors * cap
+ * entries with cap from vnlbhip_retained_cap; optional outputs may be
+ * NULL. */
 int vnlbhip_debug_frame(vnlbhip_context* context, int stage,
                         const vnlbhip_params* params, const float* noisy,
                         const float* ref, int width, int height, int frames,
@@ -143,7 +146,8 @@ int vnlbhip_debug_frame(vnlbhip_context* context, int stage,
                         int* num_anchors, int* anchor_x, int* anchor_y,
                         int* counts, int* positions, float* distances,
                         unsigned char* active, float* weights,
-                        float* eigenvalues, int* group_flags);
+                        float* eigenvalues, int* group_flags, int gram_dim,
+                        float* grams);
 
 int vnlbhip_retained_cap(const vnlbhip_params* params);
 int vnlbhip_get_stats(const vnlbhip_context* context, vnlbhip_stats* out);

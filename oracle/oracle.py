@@ -67,7 +67,7 @@ def load() -> ctypes.CDLL:
         lib.vo_frame_debug.argtypes = [ctypes.c_int, P(VoParams), fp, fp, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int, ip, ip, ip, ip, ip, fp, P(ctypes.c_ubyte),
-                                       fp, fp, ip, ctypes.c_int]
+                                       fp, fp, ip, ctypes.c_int, ctypes.c_int, dp]
         lib.vo_anchor_stack.argtypes = [ctypes.c_int, P(VoParams), fp, fp, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, fp, ctypes.c_int]
         lib.vo_retained_cap.argtypes = [P(VoParams)]
@@ -128,7 +128,8 @@ def anchor_stack(stage_id: int, params, noisy, ref, anchor_frame: int, flags: in
     return packed
 
 
-def frame_debug(stage_id: int, params, noisy, ref, anchor_frame: int, flags: int = ACCUM_DOUBLE) -> dict:
+def frame_debug(stage_id: int, params, noisy, ref, anchor_frame: int, flags: int = ACCUM_DOUBLE,
+                gram_dim: int = 16) -> dict:
     lib = load()
     src, (w, h, f, c) = _video(noisy)
     guide = _video(ref)[0] if ref is not None else None
@@ -146,13 +147,14 @@ def frame_debug(stage_id: int, params, noisy, ref, anchor_frame: int, flags: int
     weights = np.full(capacity * cap, np.nan, np.float32)
     eig = np.full(capacity * rank, np.nan, np.float32)
     gflags = np.zeros(capacity, np.int32)
+    grams = np.full(capacity * gram_dim * gram_dim, np.nan, np.float64)
     ip = ctypes.POINTER(ctypes.c_int)
     rc = lib.vo_frame_debug(stage_id, ctypes.byref(p), _fp(src), _fp(guide), w, h, f, c,
                             anchor_frame, capacity, ctypes.byref(num), ax.ctypes.data_as(ip),
                             ay.ctypes.data_as(ip), counts.ctypes.data_as(ip),
                             positions.ctypes.data_as(ip), _fp(distances),
                             active.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), _fp(weights),
-                            _fp(eig), gflags.ctypes.data_as(ip), flags)
+                            _fp(eig), gflags.ctypes.data_as(ip), flags, gram_dim, _dp(grams))
     if rc != 0:
         raise ValueError(lib.vo_last_error().decode())
     n = num.value
@@ -162,6 +164,7 @@ def frame_debug(stage_id: int, params, noisy, ref, anchor_frame: int, flags: int
         "distances": distances[: n * cap].reshape(n, cap), "active": active[:n],
         "weights": weights[: n * cap].reshape(n, cap),
         "eigenvalues": eig[: n * rank].reshape(n, rank), "group_flags": gflags[:n],
+        "grams": grams[: n * gram_dim * gram_dim].reshape(n, gram_dim, gram_dim),
     }
 
 

@@ -970,6 +970,8 @@ template <class T> T positive_or_zero(T value) {
 struct ModelInfo {
     bool modelled = false;
     std::vector<float> eigenvalues;
+    int gram_rows = 0;         // dual models: the B x B Gram XX^T / B, else 0
+    std::vector<double> gram;  // in the model's precision, widened
 };
 
 // filter_vnlb_samples (core.cpp:638-792) including compute_filter_coefficients
@@ -1040,6 +1042,10 @@ void filter_samples(const StageSetup& s, ModelScratch<T>& m, int similar,
             m.gram.resize(static_cast<std::size_t>(basis_similar) * basis_similar);
             compute_gram(model_rows.data(), basis_similar, d, scale, m.gram.data());
             m.dual_vectors.resize(static_cast<std::size_t>(rank) * basis_similar);
+            if (info != nullptr) {
+                info->gram_rows = basis_similar;
+                info->gram.assign(m.gram.begin(), m.gram.end());
+            }
             topk_eigen(m.gram.data(), basis_similar, rank, m.eigenvalues.data(),
                        m.dual_vectors.data(), m.jm, m.jv, m.order);
             dual_basis(model_rows.data(), basis_similar, d, m.eigenvalues.data(),
@@ -1259,6 +1265,8 @@ struct FrameDebug {
     float* weights = nullptr;
     float* eigenvalues = nullptr;
     int* group_flags = nullptr;
+    int gram_dim = 0;          // grams: [capacity][gram_dim][gram_dim]
+    double* grams = nullptr;
 };
 
 template <class T, class Acc>
@@ -1383,6 +1391,7 @@ long long process_anchor_frame(const StageSetup& s, int anchor_frame, Stack<Acc>
             gw.weights.assign(static_cast<std::size_t>(similar), 1.0F);
             gw.models = 0;
             info.modelled = false;
+            info.gram_rows = 0;
             int group_flag = 0;
 
             filtered = rows_noisy;
@@ -1529,6 +1538,17 @@ long long process_anchor_frame(const StageSetup& s, int anchor_frame, Stack<Acc>
                 }
                 if (debug->group_flags != nullptr) {
                     debug->group_flags[a] = group_flag;
+                }
+                if (debug->grams != nullptr && info.modelled && info.gram_rows > 0 &&
+                    info.gram_rows <= debug->gram_dim) {
+                    const int n = info.gram_rows;
+                    for (int r = 0; r < n; ++r) {
+                        for (int c2 = 0; c2 < n; ++c2) {
+                            debug->grams[(static_cast<std::size_t>(a) * debug->gram_dim + r) *
+                                             debug->gram_dim + c2] =
+                                info.gram[static_cast<std::size_t>(r) * n + c2];
+                        }
+                    }
                 }
             }
 
@@ -1756,7 +1776,7 @@ int vo_frame_debug(int stage, const vo_params* params, const float* noisy,
                    int* num_anchors, int* anchor_x, int* anchor_y, int* counts,
                    int* positions, float* distances, unsigned char* active,
                    float* weights, float* eigenvalues, int* group_flags,
-                   int flags) {
+                   int flags, int gram_dim, double* grams) {
     try {
         require(params != nullptr, "null argument");
         const Geometry g{width, height, frames, channels};
@@ -1772,6 +1792,8 @@ int vo_frame_debug(int stage, const vo_params* params, const float* noisy,
         debug.weights = weights;
         debug.eigenvalues = eigenvalues;
         debug.group_flags = group_flags;
+        debug.gram_dim = gram_dim;
+        debug.grams = grams;
         if ((flags & VO_MODEL_DOUBLE) != 0) {
             Stack<double> stack;
             process_anchor_frame<double, double>(s, anchor_frame, stack, &debug);

@@ -83,14 +83,17 @@ int vo_stage(int stage, const vo_params* params, const float* noisy,
  *   weights[a*cap + i]: final aggregation weight of each retained patch
  *     (active anchors only);
  *   eigenvalues[a*rank + j]: model eigenvalues (active, modelled groups);
- *   group_flags[a]: bit0 equal group, bit1 flat group, bit2 modelled. */
+ *   group_flags[a]: bit0 equal group, bit1 flat group, bit2 modelled;
+ *   grams[(a*gram_dim + r)*gram_dim + c]: the dual Gram XX^T / B of the
+ *     first model (active, modelled, dual groups with B <= gram_dim), in
+ *     the model's precision (double with VO_MODEL_DOUBLE). */
 int vo_frame_debug(int stage, const vo_params* params, const float* noisy,
                    const float* ref, int width, int height, int frames,
                    int channels, int anchor_frame, int capacity_anchors,
                    int* num_anchors, int* anchor_x, int* anchor_y,
                    int* counts, int* positions, float* distances,
                    unsigned char* active, float* weights, float* eigenvalues,
-                   int* group_flags, int flags);
+                   int* group_flags, int flags, int gram_dim, double* grams);
 
 /* Contribution stack of one anchor frame in the vnlb.Basic / vnlb.Final
  * output-frame layout (src/vs/plugin.cpp:264-271,625-641; the CUDA

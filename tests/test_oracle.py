@@ -233,3 +233,39 @@ def test_fp64_model_close_to_fp32(oracle_lib):
     b32, _ = oracle_lib.stage(0, bp, noisy, flags=1, threads=4)
     b64, _ = oracle_lib.stage(0, bp, noisy, flags=1 | oracle_lib.MODEL_DOUBLE, threads=4)
     assert abs(synth.psnr(b32, clean) - synth.psnr(b64, clean)) < 0.02
+
+
+def test_frame_debug_grams_are_centered_xxt(oracle_lib):
+    """The Gram the oracle's model solves (frame_debug "grams", float64
+    model) equals Xc Xc^T / B recomputed here from the group's first B
+    matches: the coupled patches (every channel and patch frame) of the noisy
+    clip, centered by their mean (compute_gram_centered, linalg.cpp:326-354;
+    the Gram does not depend on the element order)."""
+    noisy, _, bp, _ = workloads.make("yuv_pt2_tau200")
+    clip = np.ascontiguousarray(noisy, np.float64)
+    frames, channels, height, width = clip.shape
+    dbg = oracle_lib.frame_debug(0, bp, noisy, None, 2,
+                                 flags=oracle_lib.ACCUM_DOUBLE | oracle_lib.MODEL_DOUBLE)
+    P, Pt = bp.patch_size, bp.patch_time
+    rows = min(bp.similar, int(np.ceil(bp.similar * bp.model_cap_factor)))
+    checked = 0
+    for a, n in enumerate(dbg["counts"]):
+        if not dbg["active"][a] or (dbg["group_flags"][a] & 4) == 0 or n < rows:
+            continue
+        B = rows
+        x = []
+        for pos in dbg["positions"][a, :B]:
+            f, rem = divmod(int(pos), height * width)
+            y0, x0 = divmod(rem, width)
+            x.append(clip[f:f + Pt, :, y0:y0 + P, x0:x0 + P].ravel())
+        x = np.array(x)
+        xc = x - x.mean(axis=0)
+        want = xc @ xc.T / B
+        got = dbg["grams"][a, :B, :B]
+        assert np.all(np.isfinite(got)), a
+        np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-12 * np.abs(want).max())
+        assert np.all(np.isnan(dbg["grams"][a, B:, :]))
+        checked += 1
+        if checked >= 40:
+            break
+    assert checked >= 20, checked

@@ -392,6 +392,8 @@ struct DebugOutput {
     float* weights = nullptr;
     float* eigenvalues = nullptr;
     int* group_flags = nullptr;
+    int gram_dim = 0;  // grams: [capacity_anchors][gram_dim][gram_dim]
+    float* grams = nullptr;
 };
 
 }  // namespace
@@ -802,6 +804,8 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
         int* d_flags = nullptr;
         float* d_eigen = nullptr;
         float* d_weights = nullptr;
+        int* d_dims_last = nullptr;
+        float* d_problems_last = nullptr;
         if (debug != nullptr) {
             d_weights = ctx.weights.reserve(static_cast<size_t>(anchor_count) * cap);
         }
@@ -816,6 +820,8 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
             float* d_vectors =
                 ctx.vectors.reserve(static_cast<size_t>(problems) * std::max(d.rank, 1) * n_stride);
             int* d_status = ctx.status.reserve(problems);
+            d_dims_last = d_dims;
+            d_problems_last = d_problems;
             int* d_work = ctx.work_counters.reserve(2);
             check(vnlbhip::launch_fill_zero_int(d_work, 2, stream), "clear work counters");
             timed(ctx, VNLBHIP_KERNEL_PREPARE, stream, "prepare kernel", [&]() {
@@ -876,6 +882,8 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
             std::vector<int> list(groups);
             std::vector<int> flags(groups);
             std::vector<float> eig(static_cast<size_t>(problems) * std::max(d.rank, 1));
+            std::vector<int> dims(groups);
+            std::vector<float> gram_rows;
             check(hipMemcpy(counts.data(), d_counts, counts.size() * sizeof(int),
                             hipMemcpyDeviceToHost), "read counts");
             check(hipMemcpy(packed.data(), d_positions, packed.size() * sizeof(uint32_t),
@@ -893,6 +901,16 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
                 if (d.rank > 0) {
                     check(hipMemcpy(eig.data(), d_eigen, eig.size() * sizeof(float),
                                     hipMemcpyDeviceToHost), "read eigenvalues");
+                }
+                if (debug->grams != nullptr && d.rank > 0) {
+                    // The model kernel's Gram / covariance per problem,
+                    // [problem][n_stride][n_stride]; the eigen solvers only
+                    // read it.
+                    gram_rows.resize(static_cast<size_t>(problems) * n_stride * n_stride);
+                    check(hipMemcpy(dims.data(), d_dims_last, dims.size() * sizeof(int),
+                                    hipMemcpyDeviceToHost), "read problem dims");
+                    check(hipMemcpy(gram_rows.data(), d_problems_last, gram_rows.size() * sizeof(float),
+                                    hipMemcpyDeviceToHost), "read Grams");
                 }
             }
             if (debug->num_anchors != nullptr) {
@@ -936,6 +954,17 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
                 const int a = list[static_cast<size_t>(gi)];
                 if (debug->group_flags != nullptr) {
                     debug->group_flags[a] = flags[static_cast<size_t>(gi)] & 7;
+                }
+                const int n = gram_rows.empty() ? 0 : dims[static_cast<size_t>(gi)];
+                if (debug->grams != nullptr && (flags[static_cast<size_t>(gi)] & 4) != 0 && n > 0 &&
+                    n <= debug->gram_dim && n < d.model_dim) {
+                    const float* src = gram_rows.data() + static_cast<size_t>(gi) * d.models * n_stride * n_stride;
+                    for (int r = 0; r < n; ++r) {
+                        for (int c = 0; c < n; ++c) {
+                            debug->grams[(static_cast<size_t>(a) * debug->gram_dim + r) * debug->gram_dim + c] =
+                                src[r * n_stride + c];
+                        }
+                    }
                 }
                 if (debug->eigenvalues != nullptr && (flags[static_cast<size_t>(gi)] & 4) != 0) {
                     for (int j = 0; j < d.rank; ++j) {
@@ -1218,7 +1247,7 @@ int vnlbhip_debug_frame(vnlbhip_context* context, int stage, const vnlbhip_param
                         int channels, int anchor_frame, int capacity_anchors, int* num_anchors,
                         int* anchor_x, int* anchor_y, int* counts, int* positions,
                         float* distances, unsigned char* active, float* weights,
-                        float* eigenvalues, int* group_flags) {
+                        float* eigenvalues, int* group_flags, int gram_dim, float* grams) {
     if (context == nullptr) {
         return VNLBHIP_ERROR_INVALID;
     }
@@ -1237,6 +1266,9 @@ int vnlbhip_debug_frame(vnlbhip_context* context, int stage, const vnlbhip_param
         debug.weights = weights;
         debug.eigenvalues = eigenvalues;
         debug.group_flags = group_flags;
+        require(grams == nullptr || gram_dim > 0, "gram_dim must be positive when grams are requested");
+        debug.gram_dim = gram_dim;
+        debug.grams = grams;
         if (num_anchors != nullptr) {
             *num_anchors = 0;
         }

@@ -174,8 +174,11 @@ class Context:
             self._raise(rc)
         return out
 
-    def debug_frame(self, stage: int, params: StageParameters, noisy, ref, anchor_frame: int) -> dict:
-        """Per-anchor diagnostics of one anchor frame (device tensors in)."""
+    def debug_frame(self, stage: int, params: StageParameters, noisy, ref, anchor_frame: int,
+                    gram_dim: int = 16) -> dict:
+        """Per-anchor diagnostics of one anchor frame (device tensors in); grams
+        holds the dual Gram of each modelled dual group with at most gram_dim
+        model rows (NaN elsewhere)."""
         import torch
 
         frames, channels, height, width = _canonical_shape(tuple(noisy.shape))
@@ -193,6 +196,7 @@ class Context:
         weights = np.full(capacity * cap, np.nan, dtype=np.float32)
         eig = np.full(capacity * max(params.rank, 1), np.nan, dtype=np.float32)
         flags = ints(capacity)
+        grams = np.full(capacity * gram_dim * gram_dim, np.nan, dtype=np.float32)
         src = noisy.contiguous()
         guide = ref.contiguous() if ref is not None else None
         ip = ctypes.POINTER(ctypes.c_int)
@@ -204,7 +208,7 @@ class Context:
             ax.ctypes.data_as(ip), ay.ctypes.data_as(ip), counts.ctypes.data_as(ip),
             positions.ctypes.data_as(ip), distances.ctypes.data_as(fp),
             active.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), weights.ctypes.data_as(fp),
-            eig.ctypes.data_as(fp), flags.ctypes.data_as(ip))
+            eig.ctypes.data_as(fp), flags.ctypes.data_as(ip), gram_dim, grams.ctypes.data_as(fp))
         if rc != 0:
             self._raise(rc)
         torch.cuda.synchronize()
@@ -217,6 +221,7 @@ class Context:
             "active": active[:n], "weights": weights[: n * cap].reshape(n, cap),
             "eigenvalues": eig[: n * max(params.rank, 1)].reshape(n, max(params.rank, 1)),
             "group_flags": flags[:n],
+            "grams": grams[: n * gram_dim * gram_dim].reshape(n, gram_dim, gram_dim),
         }
 
 

@@ -133,7 +133,7 @@ def load() -> ctypes.CDLL:
     lib.vnlbhip_debug_frame.argtypes = [vp, ctypes.c_int, P(Params), vp, vp, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int, ip, ip, ip, ip, ip, fp,
-                                        P(ctypes.c_ubyte), fp, fp, ip]
+                                        P(ctypes.c_ubyte), fp, fp, ip, ctypes.c_int, fp]
     lib.vnlbhip_debug_frame.restype = ctypes.c_int
     lib.vnlbhip_retained_cap.argtypes = [P(Params)]
     lib.vnlbhip_retained_cap.restype = ctypes.c_int
