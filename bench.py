@@ -164,6 +164,8 @@ def main() -> int:
     parser.add_argument("--warmup", type=int, default=1)
     parser.add_argument("--workload", default="yuv444ps_1080p_16f", choices=sorted(WORKLOADS))
     parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--no-autotune", action="store_true",
+                        help="keep the default pipeline part count instead of timing the candidates")
     parser.add_argument("--form", default="fast", choices=("fast", "plain"),
                         help="fast: MFMA candidate distances (the metric's form); plain: exact fp32 order")
     args = parser.parse_args()
@@ -189,6 +191,9 @@ def main() -> int:
     fp = parse_stage_parameters(FINAL, workload["args"])
     ctx = vnlbhip.Context(0)
     ctx.set_form(args.form)
+    # The first warmup step picks each stage's pipeline part count by timing
+    # (without warmup steps the default count is used).
+    ctx.set_autotune(args.warmup > 0 and not args.no_autotune)
 
     def step():
         basic = ctx.stage(BASIC, bp, noisy)
@@ -275,6 +280,7 @@ def main() -> int:
             **quality,
             "kernel_ms": {k: round(v["ms"], 3) for k, v in times.items()},
             "retained_basic": sb["retained"], "retained_final": sf["retained"],
+            "pipeline_parts": {"basic": sb["parts"], "final": sf["parts"]},
             "rooflines": rooflines,
         },
         "roofline": roofline,

@@ -72,7 +72,7 @@ typedef struct vnlbhip_stats {
     int64_t groups;               /* anchors that survived the paste mask */
     int64_t unconverged_groups;   /* eigen problems that hit the sweep limit */
     int32_t batches;              /* anchor-frame batches of the last call */
-    int32_t reserved;
+    int32_t parts;                /* pipeline parts per batch of the last call */
     int64_t retained;             /* retained samples summed over the groups */
     int64_t refined;              /* FAST form: eigen problems re-solved in fp64 */
 } vnlbhip_stats;
@@ -201,6 +201,15 @@ int vnlbhip_set_form(vnlbhip_context* context, int form);
  * Default 1e-3; 0 disables; negative or non-finite values are rejected with
  * VNLBHIP_ERROR_INVALID.  Only models with rank < rows have a cutoff. */
 int vnlbhip_set_eigen_refine(vnlbhip_context* context, float relative_gap);
+
+/* 1: vnlbhip_stage times the stage once per pipeline part count (1..4
+ * parts of consecutive anchor frames per batch; a part's serial paste scan
+ * runs beside the next part's matching) on the first call for a stage,
+ * form, geometry and parameter set, and keeps the fastest for later calls;
+ * the first call then runs the stage six times.  0 (default): two parts.
+ * vnlbhip_stats.parts reports the count used.  No counterpart in the
+ * reference.  Values other than 0 and 1 are rejected. */
+int vnlbhip_set_autotune(vnlbhip_context* context, int enable);
 
 /* FAST form: 1 (default) runs the k-nearest selection inside the MFMA
  * matcher where the retained cap allows (at most 64); 0 stages every

@@ -267,3 +267,33 @@ def test_eigen_refinement_in_fp64(gpu, oracle_lib):
         compared += 1
     assert compared >= 20, compared
     assert abs(synth.psnr(refined, clean) - synth.psnr(base, clean)) <= PSNR_TOL
+
+
+def test_autotune_picks_a_part_count_and_keeps_results():
+    """The autotuner times the stage at 1..4 pipeline parts on the first call
+    and keeps the fastest; the part count changes only the launch sequence,
+    so outputs match the default two-part run (plain form: identical up to
+    the order of the fp64 accumulator adds)."""
+    noisy, _ = synth.make_clip(width=96, height=80, frames=8, channels=3, sigma8=15, seed=21,
+                               motion="subpixel")
+    params = parse_stage_parameters(BASIC, ARGS)
+    x = _dev(noisy)
+    tuned = vnlbhip.Context(0)
+    default = vnlbhip.Context(0)
+    try:
+        tuned.set_autotune(True)
+        print("[fast] autotune plain", flush=True)
+        a = tuned.stage(BASIC, params, x).cpu().numpy()
+        parts = tuned.stats()["parts"]
+        assert 1 <= parts <= 4
+        b = tuned.stage(BASIC, params, x).cpu().numpy()
+        assert tuned.stats()["parts"] == parts
+        c = default.stage(BASIC, params, x).cpu().numpy()
+        assert default.stats()["parts"] == 2
+        np.testing.assert_allclose(a, c, rtol=0, atol=1e-6)
+        np.testing.assert_allclose(b, c, rtol=0, atol=1e-6)
+        with pytest.raises(vnlbhip.VNLBError):
+            tuned.set_autotune(2)
+    finally:
+        tuned.close()
+        default.close()

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -23,6 +24,7 @@
 #include <limits>
 #include <stdexcept>
 #include <string>
+#include <map>
 #include <vector>
 
 namespace {
@@ -398,8 +400,10 @@ struct DebugOutput {
 
 }  // namespace
 
-// Anchor-frame parts a batch is pipelined in (matching, paste scan, model).
-constexpr int kParts = 2;
+// Anchor-frame parts a batch is pipelined in (matching, paste scan, model):
+// at most kMaxParts, kDefaultParts unless the autotuner chose another count.
+constexpr int kMaxParts = 4;
+constexpr int kDefaultParts = 2;
 
 struct vnlbhip_context {
     int device = 0;
@@ -453,8 +457,12 @@ struct vnlbhip_context {
     // pinned group counts read back per part; created on first use.
     hipStream_t side = nullptr;
     hipEvent_t ready_event = nullptr;
-    hipEvent_t match_event[kParts] = {};
-    hipEvent_t paste_event[kParts] = {};
+    hipEvent_t match_event[kMaxParts] = {};
+    hipEvent_t paste_event[kMaxParts] = {};
+    int parts = kDefaultParts;  // pipeline parts of the next stage
+    bool autotune = false;
+    // (stage, form, geometry, parameters) -> the part count timed fastest
+    std::map<std::vector<long long>, int> tuned_parts;
     int* host_counts = nullptr;
     ~vnlbhip_context() {
         if (host_counts != nullptr) {
@@ -625,11 +633,11 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
     if (ctx.side == nullptr) {
         check(hipStreamCreateWithFlags(&ctx.side, hipStreamNonBlocking), "hipStreamCreate");
         check(hipEventCreateWithFlags(&ctx.ready_event, hipEventDisableTiming), "hipEventCreate");
-        for (int h = 0; h < kParts; ++h) {
+        for (int h = 0; h < kMaxParts; ++h) {
             check(hipEventCreateWithFlags(&ctx.match_event[h], hipEventDisableTiming), "hipEventCreate");
             check(hipEventCreateWithFlags(&ctx.paste_event[h], hipEventDisableTiming), "hipEventCreate");
         }
-        check(hipHostMalloc(reinterpret_cast<void**>(&ctx.host_counts), kParts * sizeof(int)),
+        check(hipHostMalloc(reinterpret_cast<void**>(&ctx.host_counts), kMaxParts * sizeof(int)),
               "hipHostMalloc");
     }
     for (BatchPlan& plan : ctx.plans) {
@@ -658,22 +666,23 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
         float* d_distances = ctx.distances.reserve(static_cast<size_t>(anchor_count) * cap);
         unsigned char* d_active = ctx.active.reserve(anchor_count);
         int* d_list = ctx.group_list.reserve(anchor_count);
-        int* d_group_count = ctx.group_count.reserve(kParts);
+        int* d_group_count = ctx.group_count.reserve(kMaxParts);
         unsigned long long* d_dep = ctx.paste_dep.reserve(anchor_count);
         int* d_tcount = ctx.paste_target_counts.reserve(anchor_count);
         int* d_targets = ctx.paste_targets.reserve(static_cast<size_t>(anchor_count) *
                                                    vnlbhip::paste_target_stride(d));
         check(vnlbhip::launch_fill_zero(reinterpret_cast<double*>(d_dep), anchor_count, stream),
               "clear paste masks");
-        check(vnlbhip::launch_fill_zero_int(d_group_count, kParts, stream), "clear group counts");
+        check(vnlbhip::launch_fill_zero_int(d_group_count, kMaxParts, stream), "clear group counts");
 
-        // The batch runs as up to two parts of consecutive anchor frames: the
-        // serial paste scan of a part (a few wavefronts) runs on the side
-        // stream beside the next part's matching and the previous part's
-        // model, and each part's model starts once its groups are listed.
-        // Debug runs keep one part.
-        const int parts = (debug == nullptr && frame_count >= 2) ? kParts : 1;
-        int part_frame[kParts + 1];
+        // The batch runs as parts of consecutive anchor frames: the serial
+        // paste scan of a part (a few wavefronts) runs on the side stream
+        // beside the next part's matching and the previous part's model, and
+        // each part's model starts once its groups are listed.  Debug runs
+        // keep one part.
+        const int parts = debug == nullptr ? std::max(1, std::min(ctx.parts, frame_count)) : 1;
+        ctx.stats.parts = std::max(ctx.stats.parts, parts);
+        int part_frame[kMaxParts + 1];
         for (int h = 0; h <= parts; ++h) {
             part_frame[h] = frame_count * h / parts;
         }
@@ -1169,6 +1178,14 @@ int vnlbhip_set_eigen_refine(vnlbhip_context* context, float relative_gap) {
     return VNLBHIP_OK;
 }
 
+int vnlbhip_set_autotune(vnlbhip_context* context, int enable) {
+    if (context == nullptr || (enable != 0 && enable != 1)) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    context->autotune = enable != 0;
+    return VNLBHIP_OK;
+}
+
 int vnlbhip_set_fused_selection(vnlbhip_context* context, int enable) {
     if (context == nullptr || (enable != 0 && enable != 1)) {
         return VNLBHIP_ERROR_INVALID;
@@ -1193,9 +1210,41 @@ int vnlbhip_stage(vnlbhip_context* context, int stage, const vnlbhip_params* par
     }
     return guarded(context, [&]() {
         require(params != nullptr && out != nullptr, "params and out must not be null");
-        run_stage(*context, stage, *params, noisy, ref, out,
-                  Geometry{width, height, frames, channels},
-                  reinterpret_cast<hipStream_t>(stream), -1, nullptr);
+        const Geometry g{width, height, frames, channels};
+        const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+        context->parts = kDefaultParts;
+        if (context->autotune) {
+            // Key: everything the stage's kernels and grids depend on.
+            std::vector<long long> key = {stage, context->form, width, height, frames, channels};
+            const unsigned char* raw = reinterpret_cast<const unsigned char*>(params);
+            key.insert(key.end(), raw, raw + sizeof(vnlbhip_params));
+            auto it = context->tuned_parts.find(key);
+            if (it == context->tuned_parts.end()) {
+                // One untimed run (plans, buffers), then every part count
+                // once; the stage blocks until done, so host time is device
+                // time plus the fixed launch work.
+                const bool profiling = context->profiling;
+                context->profiling = false;
+                run_stage(*context, stage, *params, noisy, ref, out, g, s, -1, nullptr);
+                int best = kDefaultParts;
+                double best_ms = 0.0;
+                for (int parts = 1; parts <= kMaxParts; ++parts) {
+                    context->parts = parts;
+                    const auto t0 = std::chrono::steady_clock::now();
+                    run_stage(*context, stage, *params, noisy, ref, out, g, s, -1, nullptr);
+                    const double ms =
+                        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                    if (parts == 1 || ms < best_ms) {
+                        best = parts;
+                        best_ms = ms;
+                    }
+                }
+                context->profiling = profiling;
+                it = context->tuned_parts.emplace(key, best).first;
+            }
+            context->parts = it->second;
+        }
+        run_stage(*context, stage, *params, noisy, ref, out, g, s, -1, nullptr);
     });
 }
 

@@ -71,7 +71,7 @@ class Context:
         self._lib.vnlbhip_get_stats(self._handle, ctypes.byref(s))
         return {"anchors": s.anchors, "groups": s.groups,
                 "unconverged_groups": s.unconverged_groups, "batches": s.batches,
-                "retained": s.retained, "refined": s.refined}
+                "retained": s.retained, "refined": s.refined, "parts": s.parts}
 
     def set_form(self, form: str) -> None:
         """'plain' (fp32 in the CPU path's order, the default) or 'fast'
@@ -90,6 +90,14 @@ class Context:
         rc = self._lib.vnlbhip_set_eigen_refine(self._handle, float(relative_gap))
         if rc != 0:
             raise VNLBError("VNLB HIP: relative_gap must be finite and non-negative")
+
+    def set_autotune(self, enable: bool) -> None:
+        """Time the stage once per pipeline part count (1..4) on the first
+        call for a stage / form / geometry / parameter set and keep the
+        fastest (stats()["parts"] reports the count used)."""
+        rc = self._lib.vnlbhip_set_autotune(self._handle, int(enable))
+        if rc != 0:
+            self._raise(rc)
 
     def set_fused_selection(self, enable: bool) -> None:
         """Fast form: select the k nearest inside the MFMA matcher (True,

@@ -29,6 +29,7 @@ EXPORTED_SYMBOLS = (
     "vnlbhip_set_form",
     "vnlbhip_set_eigen_refine",
     "vnlbhip_set_fused_selection",
+    "vnlbhip_set_autotune",
 )
 
 KERNEL_NAMES = ("match", "paste", "prepare", "jacobi", "filter", "normalize", "select", "energy")
@@ -68,7 +69,7 @@ class Stats(ctypes.Structure):
         ("groups", ctypes.c_int64),
         ("unconverged_groups", ctypes.c_int64),
         ("batches", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("parts", ctypes.c_int32),
         ("retained", ctypes.c_int64),
         ("refined", ctypes.c_int64),
     ]
@@ -147,6 +148,8 @@ def load() -> ctypes.CDLL:
     lib.vnlbhip_set_eigen_refine.restype = ctypes.c_int
     lib.vnlbhip_set_fused_selection.argtypes = [vp, ctypes.c_int]
     lib.vnlbhip_set_fused_selection.restype = ctypes.c_int
+    lib.vnlbhip_set_autotune.argtypes = [vp, ctypes.c_int]
+    lib.vnlbhip_set_autotune.restype = ctypes.c_int
     lib.vnlbhip_get_kernel_times.argtypes = [vp, P(KernelTimes)]
     lib.vnlbhip_get_kernel_times.restype = ctypes.c_int
     _lib = lib
