@@ -86,17 +86,20 @@ __global__ __launch_bounds__(256) void pack_stack_kernel(StageDesc d, const T* _
 // Stage health counters: summary[0] counts groups whose model rows cannot
 // cover the rank (the reference throws for those, core.cpp:645-648),
 // summary[1] eigen problems that reached the sweep limit without meeting the
-// convergence test.
-// summary[0]: groups whose rows cannot cover the rank; summary[1]:
-// unconverged eigen problems; summary[2..3]: retained samples over the
-// groups (64-bit).
+// convergence test, summary[2..3] retained samples over the groups (64-bit),
+// summary[4] problems re-solved in fp64, summary[5] groups.  The group count
+// of the list is read on the device (grids are sized by the capacity).
 __global__ __launch_bounds__(256) void summarize_kernel(const int* __restrict__ flags,
                                                         const int* __restrict__ status,
                                                         const int* __restrict__ counts,
                                                         const int* __restrict__ group_list,
-                                                        int groups, int models,
+                                                        const int* __restrict__ groups_dev, int models,
                                                         int* __restrict__ summary) {
+    const int groups = *groups_dev;
     const int problem = blockIdx.x * blockDim.x + threadIdx.x;
+    if (problem == 0) {
+        atomicAdd(&summary[5], groups);
+    }
     if (problem >= groups * models) {
         return;
     }
@@ -125,14 +128,14 @@ unsigned grid_for(size_t count) {
 }  // namespace
 
 hipError_t launch_summarize(const int* flags, const int* status, const int* counts,
-                            const int* group_list, int groups, int models, int* summary,
-                            hipStream_t stream) {
-    const int problems = groups * models;
+                            const int* group_list, const int* groups_dev, int group_capacity,
+                            int models, int* summary, hipStream_t stream) {
+    const int problems = group_capacity * models;
     if (problems <= 0) {
         return hipSuccess;
     }
     hipLaunchKernelGGL(summarize_kernel, dim3(static_cast<unsigned>((problems + 255) / 256)),
-                       dim3(256), 0, stream, flags, status, counts, group_list, groups, models,
+                       dim3(256), 0, stream, flags, status, counts, group_list, groups_dev, models,
                        summary);
     return hipGetLastError();
 }

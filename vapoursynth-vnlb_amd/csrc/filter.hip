@@ -184,7 +184,7 @@ __device__ inline void load_centered(float (&x)[kS], float& xt, const float* __r
 template <int kS, bool kExact>
 __global__ __launch_bounds__(kModelThreads, kS <= 24 ? 3 : 1) void filter_kernel(
     StageDesc d, FilterLayout L, const float* __restrict__ noisy,
-    const float* __restrict__ basic, GroupRefs refs, int group_count, unsigned group_stride,
+    const float* __restrict__ basic, GroupRefs refs, const int* __restrict__ group_count_dev,
     int problem_stride, const int* __restrict__ group_flags, const float* __restrict__ means,
     const float* __restrict__ eigenvalues, const float* __restrict__ vectors, Accumulators acc,
     float* __restrict__ weights_out, int* __restrict__ work_counter) {
@@ -270,7 +270,26 @@ __global__ __launch_bounds__(kModelThreads, kS <= 24 ? 3 : 1) void filter_kernel
 
     // Group indices are taken in order from the work counter (the next one
     // requested while the current group is processed) and spread over the
-    // list by group_stride.
+    // list by the largest odd stride near 0.618 * count coprime with the
+    // count (uniform: every thread computes it from the device count).
+    const int group_count = *group_count_dev;
+    unsigned group_stride = static_cast<unsigned>(static_cast<double>(group_count) * 0.6180339887) | 1u;
+    while (group_stride > 1) {
+        unsigned x = group_stride;
+        unsigned y = static_cast<unsigned>(group_count);
+        while (y != 0) {
+            const unsigned r = x % y;
+            x = y;
+            y = r;
+        }
+        if (x == 1) {
+            break;
+        }
+        group_stride -= 2;
+    }
+    if (group_stride == 0) {
+        group_stride = 1;
+    }
     __shared__ int s_next;
     if (threadIdx.x == 0) {
         s_next = atomicAdd(work_counter, 1);
@@ -694,35 +713,20 @@ size_t filter_shared_bytes(const StageDesc& d) {
 
 hipError_t launch_filter(const StageDesc& d, const float* noisy, const float* basic,
                          const uint32_t* anchors, const int* counts,
-                         const uint32_t* positions, const int* group_list, int group_count,
+                         const uint32_t* positions, const int* group_list,
+                         const int* group_count_dev, int group_capacity,
                          int problem_stride, const int* group_flags, const int* problem_dims,
                          const float* means, const float* eigenvalues, const float* vectors,
                          double* accumulators, float* weights_out, int* work_counter,
                          hipStream_t stream) {
     (void)problem_dims;
-    if (group_count <= 0) {
+    if (group_capacity <= 0) {
         return hipSuccess;
     }
     const GroupRefs refs{anchors, counts, positions, group_list};
     const Accumulators acc{accumulators};
     const FilterLayout L = filter_layout(d);
     const size_t shared = static_cast<size_t>(L.words) * sizeof(float);
-    // Largest odd stride near 0.618 * count that is coprime with the count.
-    unsigned stride = static_cast<unsigned>(static_cast<double>(group_count) * 0.6180339887) | 1u;
-    const auto gcd = [](unsigned x, unsigned y) {
-        while (y != 0) {
-            const unsigned r = x % y;
-            x = y;
-            y = r;
-        }
-        return x;
-    };
-    while (stride > 1 && gcd(stride, static_cast<unsigned>(group_count)) != 1) {
-        stride -= 2;
-    }
-    if (stride == 0) {
-        stride = 1;
-    }
     int device = 0;
     int units = 0;
     hipError_t status = hipGetDevice(&device);
@@ -744,9 +748,9 @@ hipError_t launch_filter(const StageDesc& d, const float* noisy, const float* ba
         }                                                                                         \
         const long long resident = static_cast<long long>(per_unit > 0 ? per_unit : 1) * units;   \
         const unsigned blocks =                                                                   \
-            static_cast<unsigned>(resident < group_count ? resident : group_count);               \
+            static_cast<unsigned>(resident < group_capacity ? resident : group_capacity);         \
         hipLaunchKernelGGL((filter_kernel<KS, EXACT>), dim3(blocks), dim3(kModelThreads), shared, \
-                           stream, d, L, noisy, basic, refs, group_count, stride,                 \
+                           stream, d, L, noisy, basic, refs, group_count_dev,                     \
                            problem_stride, group_flags, means, eigenvalues, vectors, acc,         \
                            weights_out, work_counter);                                            \
     } while (0)

@@ -243,7 +243,7 @@ __device__ inline float dot16(const float (&c)[16], const float* u) {
 template <int kPt>
 __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
     StageDesc d, const float* __restrict__ noisy, const float* __restrict__ basic, GroupRefs refs,
-    int group_count, int problem_stride, const int* __restrict__ group_flags,
+    const int* __restrict__ group_count_dev, int problem_stride, const int* __restrict__ group_flags,
     const float* __restrict__ means, const float* __restrict__ eigenvalues,
     const float* __restrict__ vectors, const float* __restrict__ grams, float* __restrict__ acc,
     float* __restrict__ weights_out, int* __restrict__ work_counter) {
@@ -302,6 +302,7 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
     const float dual_floor = 128.0F * FLT_EPSILON;
     const float* model_video = final_stage ? basic : noisy;
 
+    const int group_count = *group_count_dev;
     // Groups are taken in order from the work counter, one request per
     // wavefront; the next index is requested while the current group runs.
     int request = 0;
@@ -753,11 +754,12 @@ size_t fast_filter_shared_bytes(const StageDesc& d) {
 
 hipError_t launch_fast_filter(const StageDesc& d, const float* noisy, const float* basic,
                               const uint32_t* anchors, const int* counts, const uint32_t* positions,
-                              const int* group_list, int group_count, int problem_stride,
-                              const int* group_flags, const float* means, const float* eigenvalues,
-                              const float* vectors, const float* grams, float* accumulators,
-                              float* weights_out, int* work_counter, hipStream_t stream) {
-    if (group_count <= 0) {
+                              const int* group_list, const int* group_count_dev, int group_capacity,
+                              int problem_stride, const int* group_flags, const float* means,
+                              const float* eigenvalues, const float* vectors, const float* grams,
+                              float* accumulators, float* weights_out, int* work_counter,
+                              hipStream_t stream) {
+    if (group_capacity <= 0) {
         return hipSuccess;
     }
     const GroupRefs refs{anchors, counts, positions, group_list};
@@ -771,7 +773,7 @@ hipError_t launch_fast_filter(const StageDesc& d, const float* noisy, const floa
     if (status != hipSuccess) {
         return status;
     }
-    const long long wanted = (group_count + kWaves - 1) / kWaves;
+    const long long wanted = (group_capacity + kWaves - 1) / kWaves;
 #define VNLBHIP_FAST_FILTER(KPT)                                                                     \
     do {                                                                                             \
         int per_unit = 0;                                                                            \
@@ -783,7 +785,7 @@ hipError_t launch_fast_filter(const StageDesc& d, const float* noisy, const floa
         const long long resident = static_cast<long long>(per_unit > 0 ? per_unit : 1) * units;      \
         const unsigned blocks = static_cast<unsigned>(resident < wanted ? resident : wanted);        \
         hipLaunchKernelGGL(filter_fast_kernel<KPT>, dim3(blocks), dim3(kModelThreads), shared,       \
-                           stream, d, noisy, basic, refs, group_count, problem_stride, group_flags,  \
+                           stream, d, noisy, basic, refs, group_count_dev, problem_stride, group_flags, \
                            means, eigenvalues, vectors, grams, accumulators, weights_out, work_counter);                    \
     } while (0)
     switch (d.patch_time) {

@@ -55,31 +55,35 @@ size_t filter_shared_bytes(const StageDesc& d);
 // prepare, filter and fast filter: persistent grids that take groups in
 // order from *work_counter (zero on entry), so workgroups that become
 // resident late (beside other kernels) take fewer groups instead of
-// stretching the launch.
+// stretching the launch.  The model kernels read the group count from
+// *group_count_dev (written by the compaction on the device); grids and
+// buffers are sized by the capacity, an upper bound (the part's anchors).
 hipError_t launch_prepare(const StageDesc& d, const float* noisy, const float* basic,
                           const uint32_t* anchors, const int* counts, const uint32_t* positions,
-                          const int* group_list, int group_count, int problem_stride,
-                          int* group_flags, int* problem_dims, float* means, float* problems,
-                          bool mfma_gram, int* work_counter, hipStream_t stream);
-hipError_t launch_jacobi(const StageDesc& d, int problem_count, int problem_stride,
-                         const int* group_flags, const int* problem_dims, const float* problems,
-                         float* eigenvalues, float* vectors, int* status, hipStream_t stream);
+                          const int* group_list, const int* group_count_dev, int group_capacity,
+                          int problem_stride, int* group_flags, int* problem_dims, float* means,
+                          float* problems, bool mfma_gram, int* work_counter, hipStream_t stream);
+hipError_t launch_jacobi(const StageDesc& d, const int* group_count_dev, int problem_capacity,
+                         int problem_stride, const int* group_flags, const int* problem_dims,
+                         const float* problems, float* eigenvalues, float* vectors, int* status,
+                         hipStream_t stream);
 // FAST form: parallel-order Jacobi for problems of at most 16 rows.
 bool jacobi_parallel_supported(int problem_stride);
 // refine_list / refine_count (zeroed by the caller; may be null): problems
 // whose relative eigenvalue gap at the rank cutoff is below refine_gap are
 // listed and re-solved in fp64 by a second kernel on the same stream.
-hipError_t launch_jacobi_parallel(const StageDesc& d, int problem_count, int problem_stride,
-                                  const int* group_flags, const int* problem_dims,
+hipError_t launch_jacobi_parallel(const StageDesc& d, const int* group_count_dev, int problem_capacity,
+                                  int problem_stride, const int* group_flags, const int* problem_dims,
                                   const float* problems, float* eigenvalues, float* vectors,
                                   int* status, float refine_gap, int* refine_list, int* refine_count,
                                   int* refined_total, hipStream_t stream);
 hipError_t launch_filter(const StageDesc& d, const float* noisy, const float* basic,
                          const uint32_t* anchors, const int* counts, const uint32_t* positions,
-                         const int* group_list, int group_count, int problem_stride,
-                         const int* group_flags, const int* problem_dims, const float* means,
-                         const float* eigenvalues, const float* vectors, double* accumulators,
-                         float* weights_out, int* work_counter, hipStream_t stream);
+                         const int* group_list, const int* group_count_dev, int group_capacity,
+                         int problem_stride, const int* group_flags, const int* problem_dims,
+                         const float* means, const float* eigenvalues, const float* vectors,
+                         double* accumulators, float* weights_out, int* work_counter,
+                         hipStream_t stream);
 
 // MFMA form of the filter (filter_fast.hip): dual-PCA groups of one model,
 // 8x8 patches, at most 16 model rows and components, at most 64 samples.
@@ -87,10 +91,11 @@ bool fast_filter_supported(const StageDesc& d);
 size_t fast_filter_shared_bytes(const StageDesc& d);
 hipError_t launch_fast_filter(const StageDesc& d, const float* noisy, const float* basic,
                               const uint32_t* anchors, const int* counts, const uint32_t* positions,
-                              const int* group_list, int group_count, int problem_stride,
-                              const int* group_flags, const float* means, const float* eigenvalues,
-                              const float* vectors, const float* grams, float* accumulators,
-                              float* weights_out, int* work_counter, hipStream_t stream);
+                              const int* group_list, const int* group_count_dev, int group_capacity,
+                              int problem_stride, const int* group_flags, const float* means,
+                              const float* eigenvalues, const float* vectors, const float* grams,
+                              float* accumulators, float* weights_out, int* work_counter,
+                              hipStream_t stream);
 
 // One anchor frame's contribution stack (accumulator slots first_frame ..
 // first_frame + slots - 1) in the packed vnlb output-frame layout.

@@ -28,10 +28,11 @@ namespace {
 // (canonicalize_vector_row :204-233).  Each lane owns one matrix row.
 template <int kLP>  // lanes per problem (power of two >= n)
 __global__ __launch_bounds__(64) void jacobi_kernel(
-    StageDesc d, int problem_count, int problem_stride,
+    StageDesc d, const int* __restrict__ group_count_dev, int problem_stride,
     const int* __restrict__ group_flags, const int* __restrict__ problem_dims,
     const float* __restrict__ problems, float* __restrict__ eigenvalues,
     float* __restrict__ vectors, int* __restrict__ status) {
+    const int problem_count = *group_count_dev * d.models;
     constexpr int kPerWave = 64 / kLP;
     constexpr int kStride = kLP + 1;
     __shared__ float s_m[kPerWave][kLP * kStride];
@@ -313,10 +314,12 @@ __device__ inline void parallel_sweep(float (&a)[16], float (&v)[16], int r, boo
 // Same stopping rule, ordering and canonical vectors as jacobi_kernel, with
 // the parallel ordering above: 16 lanes per problem, four per wavefront.
 __global__ __launch_bounds__(64) void jacobi_parallel_kernel(
-    StageDesc d, int problem_count, int problem_stride, const int* __restrict__ group_flags,
-    const int* __restrict__ problem_dims, const float* __restrict__ problems,
-    float* __restrict__ eigenvalues, float* __restrict__ vectors, int* __restrict__ status,
-    float refine_gap, int* __restrict__ refine_list, int* __restrict__ refine_count) {
+    StageDesc d, const int* __restrict__ group_count_dev, int problem_stride,
+    const int* __restrict__ group_flags, const int* __restrict__ problem_dims,
+    const float* __restrict__ problems, float* __restrict__ eigenvalues, float* __restrict__ vectors,
+    int* __restrict__ status, float refine_gap, int* __restrict__ refine_list,
+    int* __restrict__ refine_count) {
+    const int problem_count = *group_count_dev * d.models;
     constexpr int kPerWave = 4;
     constexpr int kStride = 17;
     __shared__ float s_v[kPerWave][16 * kStride];
@@ -610,18 +613,18 @@ __global__ __launch_bounds__(64) void jacobi_refine_kernel(
 
 }  // namespace
 
-hipError_t launch_jacobi(const StageDesc& d, int problem_count, int problem_stride,
-                         const int* group_flags, const int* problem_dims,
+hipError_t launch_jacobi(const StageDesc& d, const int* group_count_dev, int problem_capacity,
+                         int problem_stride, const int* group_flags, const int* problem_dims,
                          const float* problems, float* eigenvalues, float* vectors,
                          int* status, hipStream_t stream) {
-    if (problem_count <= 0) {
+    if (problem_capacity <= 0) {
         return hipSuccess;
     }
     const int n = problem_stride;
 #define VNLBHIP_JACOBI(LP)                                                                      \
     hipLaunchKernelGGL(jacobi_kernel<LP>,                                                       \
-                       dim3(static_cast<unsigned>((problem_count + (64 / LP) - 1) / (64 / LP))), \
-                       dim3(64), 0, stream, d, problem_count, problem_stride, group_flags,       \
+                       dim3(static_cast<unsigned>((problem_capacity + (64 / LP) - 1) / (64 / LP))), \
+                       dim3(64), 0, stream, d, group_count_dev, problem_stride, group_flags,     \
                        problem_dims, problems, eigenvalues, vectors, status)
     if (n <= 4) {
         VNLBHIP_JACOBI(4);
@@ -646,23 +649,23 @@ bool jacobi_parallel_supported(int problem_stride) {
     return problem_stride <= 16;
 }
 
-hipError_t launch_jacobi_parallel(const StageDesc& d, int problem_count, int problem_stride,
-                                  const int* group_flags, const int* problem_dims,
+hipError_t launch_jacobi_parallel(const StageDesc& d, const int* group_count_dev, int problem_capacity,
+                                  int problem_stride, const int* group_flags, const int* problem_dims,
                                   const float* problems, float* eigenvalues, float* vectors,
                                   int* status, float refine_gap, int* refine_list, int* refine_count,
                                   int* refined_total, hipStream_t stream) {
-    if (problem_count <= 0) {
+    if (problem_capacity <= 0) {
         return hipSuccess;
     }
     if (!jacobi_parallel_supported(problem_stride)) {
         return hipErrorInvalidValue;
     }
-    hipLaunchKernelGGL(jacobi_parallel_kernel, dim3(static_cast<unsigned>((problem_count + 3) / 4)),
-                       dim3(64), 0, stream, d, problem_count, problem_stride, group_flags,
+    hipLaunchKernelGGL(jacobi_parallel_kernel, dim3(static_cast<unsigned>((problem_capacity + 3) / 4)),
+                       dim3(64), 0, stream, d, group_count_dev, problem_stride, group_flags,
                        problem_dims, problems, eigenvalues, vectors, status, refine_gap, refine_list,
                        refine_count);
     if (refine_list != nullptr && d.rank > 0) {
-        const int blocks = problem_count < 2048 ? problem_count : 2048;
+        const int blocks = problem_capacity < 2048 ? problem_capacity : 2048;
         hipLaunchKernelGGL(jacobi_refine_kernel, dim3(static_cast<unsigned>(blocks)), dim3(64), 0, stream, d,
                            problem_stride, problem_dims, problems, refine_list, refine_count, eigenvalues,
                            vectors, refined_total);

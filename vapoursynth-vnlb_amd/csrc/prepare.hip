@@ -158,7 +158,8 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
 template <int kS, bool kExact, bool kMfma>
 __global__ __launch_bounds__(kModelThreads, 4) void prepare_kernel(
     StageDesc d, PrepareLayout L, const float* __restrict__ noisy,
-    const float* __restrict__ basic, GroupRefs refs, int group_count, int problem_stride,
+    const float* __restrict__ basic, GroupRefs refs, const int* __restrict__ group_count_dev,
+    int problem_stride,
     int* __restrict__ group_flags, int* __restrict__ problem_dims, float* __restrict__ means,
     float* __restrict__ problems, int* __restrict__ work_counter) {
     extern __shared__ float4 s_dyn4[];
@@ -210,6 +211,7 @@ __global__ __launch_bounds__(kModelThreads, 4) void prepare_kernel(
         tab_c[i] = channel;
     }
 
+    const int group_count = *group_count_dev;
     // Groups are taken in order from the work counter; the next index is
     // requested while the current group is processed.
     if (tid == 0) {
@@ -600,10 +602,10 @@ size_t prepare_shared_bytes(const StageDesc& d) {
 
 hipError_t launch_prepare(const StageDesc& d, const float* noisy, const float* basic,
                           const uint32_t* anchors, const int* counts, const uint32_t* positions,
-                          const int* group_list, int group_count, int problem_stride,
-                          int* group_flags, int* problem_dims, float* means, float* problems,
-                          bool mfma_gram, int* work_counter, hipStream_t stream) {
-    if (group_count <= 0) {
+                          const int* group_list, const int* group_count_dev, int group_capacity,
+                          int problem_stride, int* group_flags, int* problem_dims, float* means,
+                          float* problems, bool mfma_gram, int* work_counter, hipStream_t stream) {
+    if (group_capacity <= 0) {
         return hipSuccess;
     }
     const GroupRefs refs{anchors, counts, positions, group_list};
@@ -633,9 +635,9 @@ hipError_t launch_prepare(const StageDesc& d, const float* noisy, const float* b
         }                                                                                          \
         const long long resident = static_cast<long long>(per_unit > 0 ? per_unit : 1) * units;    \
         const unsigned blocks =                                                                    \
-            static_cast<unsigned>(resident < group_count ? resident : group_count);                \
+            static_cast<unsigned>(resident < group_capacity ? resident : group_capacity);          \
         hipLaunchKernelGGL((prepare_kernel<KS, EXACT, MFMA>), dim3(blocks), dim3(kModelThreads), shared, \
-                           stream, d, L, noisy, basic, refs, group_count, problem_stride,          \
+                           stream, d, L, noisy, basic, refs, group_count_dev, problem_stride,      \
                            group_flags, problem_dims, means, problems, work_counter);              \
     } while (0)
 #define VNLBHIP_PREPARE(KS, EXACT)                                                                 \

@@ -496,8 +496,8 @@ struct vnlbhip_context {
 
 namespace vnlbhip {
 hipError_t launch_summarize(const int* flags, const int* status, const int* counts,
-                            const int* group_list, int groups, int models, int* summary,
-                            hipStream_t stream);
+                            const int* group_list, const int* groups_dev, int group_capacity,
+                            int models, int* summary, hipStream_t stream);
 }
 
 namespace {
@@ -575,7 +575,7 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
     check(vnlbhip::launch_fill_zero(acc, acc_f32 ? (acc_count + 1) / 2 : acc_count, stream),
           "fill accumulators");
     int* summary = ctx.summary.reserve(8);
-    check(hipMemsetAsync(summary, 0, 8 * sizeof(int), stream), "hipMemsetAsync");
+    check(vnlbhip::launch_fill_zero_int(summary, 8, stream), "clear summary");
 
     const float* source = stage == VNLBHIP_STAGE_FINAL ? ref : noisy;
     // MFMA matcher: patch energies of the matched source, once per stage.
@@ -799,8 +799,6 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
             });
             check(vnlbhip::launch_compact(d_active, ab, ae - ab, d_list + ab, d_group_count + h, ctx.side),
                   "compact kernel");
-            check(hipMemcpyAsync(ctx.host_counts + h, d_group_count + h, sizeof(int),
-                                 hipMemcpyDeviceToHost, ctx.side), "read group count");
             check(hipEventRecord(ctx.paste_event[h], ctx.side), "hipEventRecord");
         }
 
@@ -818,25 +816,35 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
         if (debug != nullptr) {
             d_weights = ctx.weights.reserve(static_cast<size_t>(anchor_count) * cap);
         }
-        const auto run_model = [&](int* d_part_list) {
-            problems = groups * d.models;
-            d_flags = ctx.group_flags.reserve(groups);
-            int* d_dims = ctx.problem_dims.reserve(groups);
-            float* d_means = ctx.means.reserve(static_cast<size_t>(groups) * 2 * d.dim);
-            float* d_problems =
-                ctx.problems.reserve(static_cast<size_t>(problems) * n_stride * n_stride);
-            d_eigen = ctx.eigenvalues.reserve(static_cast<size_t>(problems) * std::max(d.rank, 1));
-            float* d_vectors =
-                ctx.vectors.reserve(static_cast<size_t>(problems) * std::max(d.rank, 1) * n_stride);
-            int* d_status = ctx.status.reserve(problems);
+        // The group count of a part is read on the device (the compaction
+        // wrote it); grids and buffers are sized by the largest part's anchor
+        // count, so the host never waits for the paste scans.
+        int part_capacity = 0;
+        for (int h = 0; h < parts; ++h) {
+            part_capacity = std::max(part_capacity, grid.offsets[static_cast<size_t>(part_frame[h + 1])] -
+                                                        grid.offsets[static_cast<size_t>(part_frame[h])]);
+        }
+        const auto run_model = [&](int* d_part_list, const int* d_count, int capacity) {
+            const int problem_capacity = capacity * d.models;
+            d_flags = ctx.group_flags.reserve(part_capacity);
+            int* d_dims = ctx.problem_dims.reserve(part_capacity);
+            float* d_means = ctx.means.reserve(static_cast<size_t>(part_capacity) * 2 * d.dim);
+            float* d_problems = ctx.problems.reserve(static_cast<size_t>(part_capacity) * d.models *
+                                                     n_stride * n_stride);
+            d_eigen = ctx.eigenvalues.reserve(static_cast<size_t>(part_capacity) * d.models *
+                                              std::max(d.rank, 1));
+            float* d_vectors = ctx.vectors.reserve(static_cast<size_t>(part_capacity) * d.models *
+                                                   std::max(d.rank, 1) * n_stride);
+            int* d_status = ctx.status.reserve(static_cast<size_t>(part_capacity) * d.models);
             d_dims_last = d_dims;
             d_problems_last = d_problems;
             int* d_work = ctx.work_counters.reserve(2);
             check(vnlbhip::launch_fill_zero_int(d_work, 2, stream), "clear work counters");
             timed(ctx, VNLBHIP_KERNEL_PREPARE, stream, "prepare kernel", [&]() {
                 return vnlbhip::launch_prepare(d, noisy, basic, d_anchors, d_counts, d_positions,
-                                               d_part_list, groups, n_stride, d_flags, d_dims, d_means,
-                                               d_problems, ctx.form == VNLBHIP_FORM_FAST, d_work, stream);
+                                               d_part_list, d_count, capacity, n_stride, d_flags, d_dims,
+                                               d_means, d_problems, ctx.form == VNLBHIP_FORM_FAST, d_work,
+                                               stream);
             });
             if (d.rank > 0) {
                 const bool parallel_jacobi =
@@ -844,44 +852,51 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
                 int* d_refine = nullptr;
                 if (parallel_jacobi && ctx.refine_gap > 0.0F) {
                     // refine_list[0] counts, the list follows.
-                    d_refine = ctx.refine_list.reserve(static_cast<size_t>(problems) + 1);
+                    d_refine = ctx.refine_list.reserve(static_cast<size_t>(part_capacity) * d.models + 1);
                     check(vnlbhip::launch_fill_zero_int(d_refine, 1, stream), "clear refine count");
                 }
                 timed(ctx, VNLBHIP_KERNEL_JACOBI, stream, "jacobi kernel", [&]() {
                     if (parallel_jacobi) {
                         return vnlbhip::launch_jacobi_parallel(
-                            d, problems, n_stride, d_flags, d_dims, d_problems, d_eigen, d_vectors, d_status,
+                            d, d_count, problem_capacity, n_stride, d_flags, d_dims, d_problems, d_eigen,
+                            d_vectors, d_status,
                             ctx.refine_gap, d_refine != nullptr ? d_refine + 1 : nullptr, d_refine,
                             summary + 4, stream);
                     }
-                    return vnlbhip::launch_jacobi(d, problems, n_stride, d_flags, d_dims, d_problems,
-                                                  d_eigen, d_vectors, d_status, stream);
+                    return vnlbhip::launch_jacobi(d, d_count, problem_capacity, n_stride, d_flags, d_dims,
+                                                  d_problems, d_eigen, d_vectors, d_status, stream);
                 });
             }
             const bool fast_filter = acc_f32;
             timed(ctx, VNLBHIP_KERNEL_FILTER, stream, "filter kernel", [&]() {
                 if (fast_filter) {
                     return vnlbhip::launch_fast_filter(d, noisy, basic, d_anchors, d_counts, d_positions,
-                                                       d_part_list, groups, n_stride, d_flags, d_means,
-                                                       d_eigen, d_vectors, d_problems, acc32, d_weights,
-                                                       d_work + 1, stream);
+                                                       d_part_list, d_count, capacity, n_stride, d_flags,
+                                                       d_means, d_eigen, d_vectors, d_problems, acc32,
+                                                       d_weights, d_work + 1, stream);
                 }
                 return vnlbhip::launch_filter(d, noisy, basic, d_anchors, d_counts, d_positions,
-                                              d_part_list, groups, n_stride, d_flags, d_dims, d_means,
-                                              d_eigen, d_vectors, acc, d_weights, d_work + 1, stream);
+                                              d_part_list, d_count, capacity, n_stride, d_flags, d_dims,
+                                              d_means, d_eigen, d_vectors, acc, d_weights, d_work + 1,
+                                              stream);
             });
             check(vnlbhip::launch_summarize(d_flags, d.rank > 0 ? d_status : nullptr, d_counts,
-                                            d_part_list, groups, d.models, summary, stream),
+                                            d_part_list, d_count, capacity, d.models, summary, stream),
                   "summarize kernel");
         };
         for (int h = 0; h < parts; ++h) {
-            check(hipEventSynchronize(ctx.paste_event[h]), "hipEventSynchronize");
-            groups = ctx.host_counts[h];
-            ctx.stats.groups += groups;
             check(hipStreamWaitEvent(stream, ctx.paste_event[h], 0), "hipStreamWaitEvent");
-            run_model(d_list + grid.offsets[static_cast<size_t>(part_frame[h])]);
+            const int ab = grid.offsets[static_cast<size_t>(part_frame[h])];
+            const int ae = grid.offsets[static_cast<size_t>(part_frame[h + 1])];
+            run_model(d_list + ab, d_group_count + h, ae - ab);
         }
         if (debug != nullptr) {
+            // One part: its group count, read back for the diagnostics.
+            check(hipMemcpyAsync(ctx.host_counts, d_group_count, sizeof(int), hipMemcpyDeviceToHost, stream),
+                  "read group count");
+            check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+            groups = ctx.host_counts[0];
+            problems = groups * d.models;
             require(anchor_count <= debug->capacity_anchors, "debug capacity too small");
             std::vector<int> counts(anchor_count);
             std::vector<uint32_t> packed(static_cast<size_t>(anchor_count) * cap);
@@ -1010,6 +1025,7 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
     std::memcpy(&retained, host_summary + 2, sizeof(retained));
     ctx.stats.retained += retained;
     ctx.stats.refined = host_summary[4];
+    ctx.stats.groups = host_summary[5];
     if (host_summary[0] != 0) {
         throw ModelFailure("group_size must cover the effective VNLB rank");
     }
