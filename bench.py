@@ -161,9 +161,11 @@ def main() -> int:
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
     parser.add_argument("--steps", type=int, default=3)
-    parser.add_argument("--warmup", type=int, default=1)
+    parser.add_argument("--warmup", type=int, default=2)
     parser.add_argument("--workload", default="yuv444ps_1080p_16f", choices=sorted(WORKLOADS))
     parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--no-graph", action="store_true",
+                        help="launch every kernel eagerly instead of replaying captured stage graphs")
     parser.add_argument("--no-autotune", action="store_true",
                         help="keep the default pipeline part count instead of timing the candidates")
     parser.add_argument("--form", default="fast", choices=("fast", "plain"),
@@ -194,11 +196,17 @@ def main() -> int:
     # The first warmup step picks each stage's pipeline part count by timing
     # (without warmup steps the default count is used).
     ctx.set_autotune(args.warmup > 0 and not args.no_autotune)
+    # Stage graphs: the first call of each stage runs eagerly and captures its
+    # launch sequence; later calls with the same tensors replay it (the kernel
+    # timing step below runs eagerly).
+    ctx.set_graph(not args.no_graph)
+    basic_out = torch.empty_like(noisy)
+    final_out = torch.empty_like(noisy)
 
     def step():
-        basic = ctx.stage(BASIC, bp, noisy)
+        basic = ctx.stage(BASIC, bp, noisy, out=basic_out)
         stats_b = ctx.stats()
-        final = ctx.stage(FINAL, fp, noisy, basic)
+        final = ctx.stage(FINAL, fp, noisy, basic, out=final_out)
         stats_f = ctx.stats()
         return basic, final, stats_b, stats_f
 
@@ -281,6 +289,7 @@ def main() -> int:
             "kernel_ms": {k: round(v["ms"], 3) for k, v in times.items()},
             "retained_basic": sb["retained"], "retained_final": sf["retained"],
             "pipeline_parts": {"basic": sb["parts"], "final": sf["parts"]},
+            "stage_graphs": not args.no_graph, "graph_replays": stats_f["graph_replays"],
             "rooflines": rooflines,
         },
         "roofline": roofline,

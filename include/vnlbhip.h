@@ -75,6 +75,7 @@ typedef struct vnlbhip_stats {
     int32_t parts;                /* pipeline parts per batch of the last call */
     int64_t retained;             /* retained samples summed over the groups */
     int64_t refined;              /* FAST form: eigen problems re-solved in fp64 */
+    int64_t graph_replays;        /* stage graphs replayed by the context so far */
 } vnlbhip_stats;
 
 typedef struct vnlbhip_context vnlbhip_context;
@@ -201,6 +202,16 @@ int vnlbhip_set_form(vnlbhip_context* context, int form);
  * Default 1e-3; 0 disables; negative or non-finite values are rejected with
  * VNLBHIP_ERROR_INVALID.  Only models with rank < rows have a cutoff. */
 int vnlbhip_set_eigen_refine(vnlbhip_context* context, float relative_gap);
+
+/* 1: vnlbhip_stage captures a stage's launch sequence (every kernel, both
+ * streams) into a hipGraph after an eager run and replays it on later calls
+ * with the same stage, form, pipeline parts, geometry, parameters and
+ * noisy / ref / out addresses; a replay is skipped (eager run and a new
+ * capture) after any device buffer of the context was reallocated.  Only
+ * stages whose anchor frames fit one batch are captured.  Off while kernel
+ * profiling is on.  0 (default): eager launches.  No counterpart in the
+ * reference.  Values other than 0 and 1 are rejected. */
+int vnlbhip_set_graph(vnlbhip_context* context, int enable);
 
 /* 1: vnlbhip_stage times the stage once per pipeline part count (1..4
  * parts of consecutive anchor frames per batch; a part's serial paste scan

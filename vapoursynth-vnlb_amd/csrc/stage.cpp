@@ -353,6 +353,11 @@ struct BatchPlan {
     Tiles tiles;
 };
 
+// Bumped whenever a device buffer is reallocated or the anchor plans are
+// rebuilt: captured stage graphs hold raw addresses and plan data, so a
+// graph captured at an older generation is not replayed.
+unsigned long long g_buffer_generation = 0;
+
 template <class T> struct DeviceBuffer {
     T* ptr = nullptr;
     size_t capacity = 0;
@@ -377,6 +382,7 @@ template <class T> struct DeviceBuffer {
             release();
             check(hipMalloc(reinterpret_cast<void**>(&ptr), count * sizeof(T)), "hipMalloc");
             capacity = count;
+            ++g_buffer_generation;
         }
         return ptr;
     }
@@ -453,18 +459,39 @@ struct vnlbhip_context {
     vnlbhip_kernel_times times{};
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
     std::vector<hipEvent_t> event_pool;
-    // Side stream of the paste scans (default priority), its events and the
-    // pinned group counts read back per part; created on first use.
+    // Side stream of the paste scans (default priority), its events and
+    // pinned host words (kMaxParts group counts, then the 8-word summary);
+    // created on first use.
     hipStream_t side = nullptr;
     hipEvent_t ready_event = nullptr;
     hipEvent_t match_event[kMaxParts] = {};
     hipEvent_t paste_event[kMaxParts] = {};
     int parts = kDefaultParts;  // pipeline parts of the next stage
     bool autotune = false;
+    // Stage graphs: the launch sequence of a stage captured once and
+    // replayed (same stage, form, parts, geometry, parameters and pointers).
+    struct StageGraph {
+        hipGraphExec_t exec = nullptr;
+        unsigned long long generation = 0;
+        vnlbhip_stats stats{};
+    };
+    bool use_graph = false;
+    int64_t graph_replays = 0;
+    bool capturing = false;  // run_stage enqueues only (no uploads, no read-back)
+    hipStream_t capture = nullptr;
+    std::map<std::vector<long long>, StageGraph> graphs;
     // (stage, form, geometry, parameters) -> the part count timed fastest
     std::map<std::vector<long long>, int> tuned_parts;
     int* host_counts = nullptr;
     ~vnlbhip_context() {
+        for (auto& entry : graphs) {
+            if (entry.second.exec != nullptr) {
+                (void)hipGraphExecDestroy(entry.second.exec);
+            }
+        }
+        if (capture != nullptr) {
+            (void)hipStreamDestroy(capture);
+        }
         if (host_counts != nullptr) {
             (void)hipHostFree(host_counts);
         }
@@ -541,6 +568,25 @@ void collect_times(vnlbhip_context& ctx) {
     ctx.pending.clear();
 }
 
+// Waits for the stage and folds the device summary into the stats: model
+// failures, unconverged problems, retained samples, fp64 re-solves, groups.
+void finish_stage(vnlbhip_context& ctx, hipStream_t stream) {
+    int* host_summary = ctx.host_counts + kMaxParts;  // pinned, 8 ints
+    check(hipMemcpyAsync(host_summary, ctx.summary.ptr, 8 * sizeof(int), hipMemcpyDeviceToHost, stream),
+          "read summary");
+    check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    collect_times(ctx);
+    ctx.stats.unconverged_groups = host_summary[1];
+    int64_t retained = 0;
+    std::memcpy(&retained, host_summary + 2, sizeof(retained));
+    ctx.stats.retained = retained;
+    ctx.stats.refined = host_summary[4];
+    ctx.stats.groups = host_summary[5];
+    if (host_summary[0] != 0) {
+        throw ModelFailure("group_size must cover the effective VNLB rank");
+    }
+}
+
 // stack (with only_frame): that anchor frame's contribution stack, packed.
 void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const float* noisy,
                const float* ref, float* out, const Geometry& g, hipStream_t stream,
@@ -552,6 +598,7 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
     }
     check(hipSetDevice(ctx.device), "hipSetDevice");
     ctx.stats = vnlbhip_stats{};
+    ctx.stats.graph_replays = ctx.graph_replays;
 
     const int last_anchor_frame = g.frames - p.patch_time;
     std::vector<int> frames;
@@ -602,6 +649,8 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
         return key;
     }();
     if (ctx.plan_key != plan_key) {
+        require(!ctx.capturing, "the anchor plan changed during capture");
+        ++g_buffer_generation;
         ctx.plan_key.clear();
         ctx.plans.clear();
         size_t next = 0;
@@ -637,7 +686,7 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
             check(hipEventCreateWithFlags(&ctx.match_event[h], hipEventDisableTiming), "hipEventCreate");
             check(hipEventCreateWithFlags(&ctx.paste_event[h], hipEventDisableTiming), "hipEventCreate");
         }
-        check(hipHostMalloc(reinterpret_cast<void**>(&ctx.host_counts), kMaxParts * sizeof(int)),
+        check(hipHostMalloc(reinterpret_cast<void**>(&ctx.host_counts), (kMaxParts + 8) * sizeof(int)),
               "hipHostMalloc");
     }
     for (BatchPlan& plan : ctx.plans) {
@@ -652,14 +701,18 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
         int* d_offsets = ctx.offsets.reserve(grid.offsets.size());
         int* d_ids = ctx.ids.reserve(grid.ids.size());
         int* d_rows = ctx.rows.reserve(grid.rows.size());
-        check(hipMemcpyAsync(d_anchors, grid.anchors.data(), grid.anchors.size() * sizeof(uint32_t),
-                             hipMemcpyHostToDevice, stream), "upload anchors");
-        check(hipMemcpyAsync(d_offsets, grid.offsets.data(), grid.offsets.size() * sizeof(int),
-                             hipMemcpyHostToDevice, stream), "upload offsets");
-        check(hipMemcpyAsync(d_ids, grid.ids.data(), grid.ids.size() * sizeof(int),
-                             hipMemcpyHostToDevice, stream), "upload frame ids");
-        check(hipMemcpyAsync(d_rows, grid.rows.data(), grid.rows.size() * sizeof(int),
-                             hipMemcpyHostToDevice, stream), "upload row tables");
+        if (!ctx.capturing) {
+            // (A captured stage is single-batch and reuses the tables the
+            // eager run before the capture uploaded.)
+            check(hipMemcpyAsync(d_anchors, grid.anchors.data(), grid.anchors.size() * sizeof(uint32_t),
+                                 hipMemcpyHostToDevice, stream), "upload anchors");
+            check(hipMemcpyAsync(d_offsets, grid.offsets.data(), grid.offsets.size() * sizeof(int),
+                                 hipMemcpyHostToDevice, stream), "upload offsets");
+            check(hipMemcpyAsync(d_ids, grid.ids.data(), grid.ids.size() * sizeof(int),
+                                 hipMemcpyHostToDevice, stream), "upload frame ids");
+            check(hipMemcpyAsync(d_rows, grid.rows.data(), grid.rows.size() * sizeof(int),
+                                 hipMemcpyHostToDevice, stream), "upload row tables");
+        }
 
         int* d_counts = ctx.counts.reserve(anchor_count);
         uint32_t* d_positions = ctx.positions.reserve(static_cast<size_t>(anchor_count) * cap);
@@ -703,8 +756,10 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
                 // The tile list lives in the cached plan, so the upload may
                 // complete asynchronously.
                 d_tiles = ctx.fast_tiles.reserve(tiles->slots.size());
-                check(hipMemcpyAsync(d_tiles, tiles->slots.data(), tiles->slots.size() * sizeof(int),
-                                     hipMemcpyHostToDevice, stream), "upload tiles");
+                if (!ctx.capturing) {
+                    check(hipMemcpyAsync(d_tiles, tiles->slots.data(), tiles->slots.size() * sizeof(int),
+                                         hipMemcpyHostToDevice, stream), "upload tiles");
+                }
             }
         }
         const int tile_count = fast_matcher ? static_cast<int>(tiles->slots.size() / 16) : 0;
@@ -1015,20 +1070,49 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
                                  : vnlbhip::launch_normalize(d, acc, noisy, out, stream);
               });
     }
-    int host_summary[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    check(hipMemcpyAsync(host_summary, summary, sizeof(host_summary), hipMemcpyDeviceToHost, stream),
-          "read summary");
-    check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-    collect_times(ctx);
-    ctx.stats.unconverged_groups = host_summary[1];
-    int64_t retained = 0;
-    std::memcpy(&retained, host_summary + 2, sizeof(retained));
-    ctx.stats.retained += retained;
-    ctx.stats.refined = host_summary[4];
-    ctx.stats.groups = host_summary[5];
-    if (host_summary[0] != 0) {
-        throw ModelFailure("group_size must cover the effective VNLB rank");
+    if (ctx.capturing) {
+        return;  // the summary is read after each replay
     }
+    finish_stage(ctx, stream);
+}
+
+// Captures the launch sequence enqueue(stream) enqueues on an internal
+// stream (the side stream joins through its events) and keeps the
+// instantiated graph under key.  Nothing is allocated or uploaded while
+// capturing: the eager run before it sized every buffer.
+template <class Fn>
+void capture_stage(vnlbhip_context& ctx, const std::vector<long long>& key, const vnlbhip_stats& stats,
+                   Fn&& enqueue) {
+    if (ctx.capture == nullptr) {
+        check(hipStreamCreateWithFlags(&ctx.capture, hipStreamNonBlocking), "hipStreamCreate");
+    }
+    auto old = ctx.graphs.find(key);
+    if (old != ctx.graphs.end()) {
+        check(hipGraphExecDestroy(old->second.exec), "hipGraphExecDestroy");
+        ctx.graphs.erase(old);
+    }
+    const unsigned long long generation = g_buffer_generation;
+    check(hipStreamBeginCapture(ctx.capture, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+    ctx.capturing = true;
+    hipGraph_t graph = nullptr;
+    try {
+        enqueue(ctx.capture);
+    } catch (...) {
+        ctx.capturing = false;
+        (void)hipStreamEndCapture(ctx.capture, &graph);
+        if (graph != nullptr) {
+            (void)hipGraphDestroy(graph);
+        }
+        throw;
+    }
+    ctx.capturing = false;
+    check(hipStreamEndCapture(ctx.capture, &graph), "hipStreamEndCapture");
+    require(generation == g_buffer_generation, "device buffers changed during capture");
+    hipGraphExec_t exec = nullptr;
+    const hipError_t status = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    check(status, "hipGraphInstantiate");
+    ctx.graphs[key] = vnlbhip_context::StageGraph{exec, generation, stats};
 }
 
 int fail(vnlbhip_context* ctx, int code, const char* message) {
@@ -1194,6 +1278,14 @@ int vnlbhip_set_eigen_refine(vnlbhip_context* context, float relative_gap) {
     return VNLBHIP_OK;
 }
 
+int vnlbhip_set_graph(vnlbhip_context* context, int enable) {
+    if (context == nullptr || (enable != 0 && enable != 1)) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    context->use_graph = enable != 0;
+    return VNLBHIP_OK;
+}
+
 int vnlbhip_set_autotune(vnlbhip_context* context, int enable) {
     if (context == nullptr || (enable != 0 && enable != 1)) {
         return VNLBHIP_ERROR_INVALID;
@@ -1259,6 +1351,34 @@ int vnlbhip_stage(vnlbhip_context* context, int stage, const vnlbhip_params* par
                 it = context->tuned_parts.emplace(key, best).first;
             }
             context->parts = it->second;
+        }
+        if (context->use_graph && !context->profiling) {
+            // Key: the stage's launch sequence and every address it holds.
+            std::vector<long long> key = {stage, context->form, context->parts, width, height, frames,
+                                          channels, reinterpret_cast<long long>(noisy),
+                                          reinterpret_cast<long long>(ref), reinterpret_cast<long long>(out)};
+            const unsigned char* raw = reinterpret_cast<const unsigned char*>(params);
+            key.insert(key.end(), raw, raw + sizeof(vnlbhip_params));
+            auto it = context->graphs.find(key);
+            if (it != context->graphs.end() && it->second.generation == g_buffer_generation) {
+                check(hipSetDevice(context->device), "hipSetDevice");
+                check(hipGraphLaunch(it->second.exec, s), "hipGraphLaunch");
+                context->stats = it->second.stats;
+                finish_stage(*context, s);
+                context->stats.graph_replays = ++context->graph_replays;
+                return;
+            }
+            // Eager run (the output), then the capture for the next calls:
+            // single-batch stages only (one upload of the anchor tables).
+            run_stage(*context, stage, *params, noisy, ref, out, g, s, -1, nullptr);
+            if (context->plans.size() == 1) {
+                const vnlbhip_stats stats = context->stats;
+                capture_stage(*context, key, stats, [&](hipStream_t cs) {
+                    run_stage(*context, stage, *params, noisy, ref, out, g, cs, -1, nullptr);
+                });
+                context->stats = stats;
+            }
+            return;
         }
         run_stage(*context, stage, *params, noisy, ref, out, g, s, -1, nullptr);
     });

@@ -71,7 +71,8 @@ class Context:
         self._lib.vnlbhip_get_stats(self._handle, ctypes.byref(s))
         return {"anchors": s.anchors, "groups": s.groups,
                 "unconverged_groups": s.unconverged_groups, "batches": s.batches,
-                "retained": s.retained, "refined": s.refined, "parts": s.parts}
+                "retained": s.retained, "refined": s.refined, "parts": s.parts,
+                "graph_replays": s.graph_replays}
 
     def set_form(self, form: str) -> None:
         """'plain' (fp32 in the CPU path's order, the default) or 'fast'
@@ -90,6 +91,14 @@ class Context:
         rc = self._lib.vnlbhip_set_eigen_refine(self._handle, float(relative_gap))
         if rc != 0:
             raise VNLBError("VNLB HIP: relative_gap must be finite and non-negative")
+
+    def set_graph(self, enable: bool) -> None:
+        """Capture each stage's launch sequence into a hipGraph after an eager
+        run and replay it on calls with the same stage, parameters, geometry
+        and tensor addresses (pass out= to keep the output address)."""
+        rc = self._lib.vnlbhip_set_graph(self._handle, int(enable))
+        if rc != 0:
+            self._raise(rc)
 
     def set_autotune(self, enable: bool) -> None:
         """Time the stage once per pipeline part count (1..4) on the first
@@ -142,8 +151,10 @@ class Context:
             self._raise(rc)
         return stack
 
-    def stage(self, stage: int, params: StageParameters, noisy, ref=None):
-        """Run one stage over a whole clip and return the aggregated frames."""
+    def stage(self, stage: int, params: StageParameters, noisy, ref=None, out=None):
+        """Run one stage over a whole clip and return the aggregated frames
+        (device clips: into out when given, a float32 tensor of the clip's
+        shape on the same device)."""
         frames, channels, height, width = _canonical_shape(tuple(noisy.shape))
         if channels not in (1, 3):
             raise VNLBError("VNLB HIP: only constant GrayS and YUV444PS clips are supported")
@@ -160,7 +171,11 @@ class Context:
                 raise VNLBError("VNLB HIP: device clips must be float32 tensors on the GPU")
             src = noisy.contiguous()
             guide = ref.contiguous() if ref is not None else None
-            out = torch.empty_like(src)
+            if out is None:
+                out = torch.empty_like(src)
+            elif (tuple(out.shape) != tuple(src.shape) or out.dtype != torch.float32
+                  or out.device != src.device or not out.is_contiguous()):
+                raise VNLBError("VNLB HIP: out must be a contiguous float32 tensor like the clip")
             stream = torch.cuda.current_stream(src.device).cuda_stream
             rc = self._lib.vnlbhip_stage(
                 self._handle, stage, ctypes.byref(p), ctypes.c_void_p(src.data_ptr()),

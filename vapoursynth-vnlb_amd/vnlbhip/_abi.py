@@ -30,6 +30,7 @@ EXPORTED_SYMBOLS = (
     "vnlbhip_set_eigen_refine",
     "vnlbhip_set_fused_selection",
     "vnlbhip_set_autotune",
+    "vnlbhip_set_graph",
 )
 
 KERNEL_NAMES = ("match", "paste", "prepare", "jacobi", "filter", "normalize", "select", "energy")
@@ -72,6 +73,7 @@ class Stats(ctypes.Structure):
         ("parts", ctypes.c_int32),
         ("retained", ctypes.c_int64),
         ("refined", ctypes.c_int64),
+        ("graph_replays", ctypes.c_int64),
     ]
 
 
@@ -150,6 +152,8 @@ def load() -> ctypes.CDLL:
     lib.vnlbhip_set_fused_selection.restype = ctypes.c_int
     lib.vnlbhip_set_autotune.argtypes = [vp, ctypes.c_int]
     lib.vnlbhip_set_autotune.restype = ctypes.c_int
+    lib.vnlbhip_set_graph.argtypes = [vp, ctypes.c_int]
+    lib.vnlbhip_set_graph.restype = ctypes.c_int
     lib.vnlbhip_get_kernel_times.argtypes = [vp, P(KernelTimes)]
     lib.vnlbhip_get_kernel_times.restype = ctypes.c_int
     _lib = lib
