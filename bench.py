@@ -33,6 +33,17 @@ WORKLOADS = {
         clip=dict(width=1920, height=1080, frames=16, channels=1, sigma8=20, seed=2026, motion="subpixel"),
         args=dict(sigma=20, block_size=8, block_step=4, group_size=16, bm_range=9, patch_time=2, radius=2,
                   rank=16, tau=0)),
+    # The matching shape of the learned-proposal configuration (bm_range 12)
+    # without the learned part: exhaustive search.
+    "grays_1080p_16f_bm12": dict(
+        clip=dict(width=1920, height=1080, frames=16, channels=1, sigma8=20, seed=2026, motion="subpixel"),
+        args=dict(sigma=20, block_size=8, block_step=4, group_size=16, bm_range=12, patch_time=2, radius=2,
+                  rank=16, tau=0)),
+    # The full 4K configuration (rank clamped to K = 16), exhaustive search.
+    "yuv444ps_2160p_8f": dict(
+        clip=dict(width=3840, height=2160, frames=8, channels=3, sigma8=25, seed=2026, motion="subpixel"),
+        args=dict(sigma=25, block_size=8, block_step=4, group_size=16, bm_range=12, patch_time=2, radius=3,
+                  rank=16, tau=200, cap_factor=4.0, model_cap_factor=1.0, chroma=1)),
     "grays_256_5f": dict(
         clip=dict(width=256, height=256, frames=5, channels=1, sigma8=10, seed=2026, motion="integer"),
         args=dict(sigma=10, block_size=8, block_step=8, group_size=8, bm_range=9, patch_time=1, radius=1,

@@ -181,8 +181,8 @@ int vnlbhip_get_kernel_times(const vnlbhip_context* context, vnlbhip_kernel_time
 /* Arithmetic form of the stages run on a context (no counterpart in the
  * reference).  PLAIN, the default, sums every distance and model product in
  * fp32 in the CPU path's order.  FAST takes the candidate distances from
- * bf16x3 products on the matrix cores (8x8 patches; other shapes, and
- * windows too wide for its tiles, keep the PLAIN matcher), solves group
+ * bf16x3 products on the matrix cores (8x8 patches, windows up to 25 wide;
+ * other shapes keep the PLAIN matcher), solves group
  * models of at most 16 rows with a parallel-order fp32 Jacobi, and filters
  * dual-PCA groups (one model, at most 16 model rows, at most 64 retained
  * samples, 8x8 patches) through their bf16x3 cross-Gram; the selection

@@ -74,8 +74,11 @@ def test_fast_distances_track_exact(contexts, stage):
     assert same_sets >= 0.9 * len(want["counts"]), (same_sets, len(want["counts"]))
 
 
-@pytest.mark.parametrize("stage", [BASIC, FINAL])
-def test_fused_selection_matches_staged(stage):
+WIDE = dict(ARGS, bm_range=12, radius=3)  # 25 x 25 windows, 7 candidate frames: two-row anchor tiles
+
+
+@pytest.mark.parametrize("stage,args", [(BASIC, ARGS), (FINAL, ARGS), (BASIC, WIDE), (FINAL, WIDE)])
+def test_fused_selection_matches_staged(stage, args):
     """The k-nearest selection inside the MFMA matcher keeps exactly the
     lists that the plain form's selection (select.h) makes from the same
     staged distances: counts, positions and distances are equal on every
@@ -87,12 +90,12 @@ def test_fused_selection_matches_staged(stage):
         fused.set_form("fast")
         staged.set_form("fast")
         staged.set_fused_selection(False)
-        noisy, _ = synth.make_clip(width=128, height=96, frames=6, channels=3, sigma8=15, seed=5,
+        noisy, _ = synth.make_clip(width=128, height=96, frames=8, channels=3, sigma8=15, seed=5,
                                    motion="subpixel")
-        params = parse_stage_parameters(stage, ARGS)
+        params = parse_stage_parameters(stage, args)
         ref = None
         if stage == FINAL:
-            ref = fused.stage(BASIC, parse_stage_parameters(BASIC, ARGS), _dev(noisy))
+            ref = fused.stage(BASIC, parse_stage_parameters(BASIC, args), _dev(noisy))
         for frame in (0, 2, 4):
             print(f"[fast] fused vs staged selection stage={stage} frame={frame}", flush=True)
             a = fused.debug_frame(stage, params, _dev(noisy), ref, frame)
@@ -297,3 +300,21 @@ def test_autotune_picks_a_part_count_and_keeps_results():
     finally:
         tuned.close()
         default.close()
+
+
+@pytest.mark.parametrize("name", ["gray_bm12_radius2", "yuv_bm12_radius3"])
+def test_fast_form_on_wide_windows(contexts, name):
+    """25 x 25 windows (bm_range 12, radius 2 and 3): the MFMA matcher on
+    two-row anchor tiles, the fast model and filter; Basic + Final quality
+    within the fast-form bound."""
+    import workloads
+
+    plain, fast = contexts
+    noisy, clean, bp, fp = workloads.make(name)
+    x = _dev(noisy)
+    print(f"[fast] wide window {name}", flush=True)
+    want = plain.stage(FINAL, fp, x, plain.stage(BASIC, bp, x)).cpu().numpy()
+    got = fast.stage(FINAL, fp, x, fast.stage(BASIC, bp, x)).cpu().numpy()
+    assert np.isfinite(got).all()
+    clean = np.asarray(clean, np.float32).reshape(want.shape)
+    assert abs(synth.psnr(got, clean) - synth.psnr(want, clean)) <= PSNR_TOL

@@ -39,6 +39,22 @@ WORKLOADS = {
         dict(sigma=15, block_size=8, block_step=4, group_size=16, bm_range=9, patch_time=2, radius=2,
              rank=16, tau=200, cap_factor=4.0, model_cap_factor=1.0, chroma=1),
         None, None),
+    # GrayS, 25x25 window (bm_range 12, 625 candidates per temporal slot),
+    # radius 2: the matching shape of the learned-proposal configuration
+    # (crop; the MFMA matcher's tiles take windows up to 19, so both forms
+    # match with the plain matcher here).
+    "gray_bm12_radius2": (
+        dict(width=88, height=72, frames=6, channels=1, sigma8=20, seed=15, motion="subpixel"),
+        dict(sigma=20, block_size=8, block_step=4, group_size=16, bm_range=12, patch_time=2, radius=2,
+             rank=16, tau=0),
+        None, None),
+    # YUV444, bm_range 12, radius 3 (4375 candidates), tau 200, cap 4, sigma 25:
+    # the shape of the full 4K configuration (crop; rank clamped to K = 16).
+    "yuv_bm12_radius3": (
+        dict(width=80, height=64, frames=8, channels=3, sigma8=25, seed=16, motion="subpixel"),
+        dict(sigma=25, block_size=8, block_step=4, group_size=16, bm_range=12, patch_time=2, radius=3,
+             rank=16, tau=200, cap_factor=4.0, model_cap_factor=1.0, chroma=1),
+        None, None),
     # YUV444 with uncoupled channels (three per-channel models).
     "yuv_uncoupled": (
         dict(width=64, height=56, frames=4, channels=3, sigma8=15, seed=14, motion="integer"),

@@ -13,7 +13,8 @@
 // (select.h), so only the distance values differ from it.
 //
 // Anchors are matched in tiles of up to 16 anchors of one frame (4 grid rows
-// x 4 anchors): the tile's candidates are the union of the anchors' windows.
+// x 4 anchors; 2 x 4 where the union of wider windows would not fit): the
+// tile's candidates are the union of the anchors' windows.
 // One v_mfma_f32_16x16x32_bf16 multiplies 16 candidates of a column
 // (consecutive rows) by the 16 anchors over 4 patch rows.  The search region
 // of one (frame, channel) plane is staged in LDS as bf16 hi / lo, four
@@ -966,7 +967,7 @@ bool fast_match_supported(const StageDesc& d, int max_cols, int max_rows) {
            S.region_rows * (S.pitch / 4) <= kFillPerThread * kFastThreads &&
            static_cast<long long>(max_cols) * max_rows <= static_cast<long long>(kEnergyPerThread) * kFastThreads &&
            static_cast<long long>(S.ebox_words) * 4 <= static_cast<long long>(2) * kCopies * S.copy_elems * 2 &&
-           fast_match_shared_bytes(d, max_cols, max_rows) <= 80 * 1024;
+           fast_match_shared_bytes(d, max_cols, max_rows) <= 150 * 1024;  // of 160 KB per CU
 }
 
 bool fast_match_fused(const StageDesc& d, int max_cols, int max_rows) {

@@ -285,16 +285,18 @@ void append_grid_frame(const StageDesc& d, int anchor_frame, Grid& grid) {
     }
 }
 
-// Anchor tiles of the MFMA matcher: 4 consecutive grid rows x 4 anchors of
-// one frame (slot rr * 4 + ii, -1 when a row is shorter), and the largest
-// union of the tiles' search windows.
+// Anchor tiles of the MFMA matcher: `rows` (4 or 2) consecutive grid rows x
+// 4 anchors of one frame (slot rr * 4 + ii; -1 when a row is shorter and for
+// rows past `rows`), and the largest union of the tiles' search windows.
+// Two-row tiles keep the union of 25-wide windows within the kernel's
+// limits, at half the anchors per matrix product.
 struct Tiles {
     std::vector<int> slots;
     int max_cols = 0;
     int max_rows = 0;
 };
 
-Tiles build_tiles(const StageDesc& d, const Grid& grid, int height) {
+Tiles build_tiles(const StageDesc& d, const Grid& grid, int height, int rows) {
     Tiles tiles;
     const int frame_count = static_cast<int>(grid.ids.size());
     for (int fi = 0; fi < frame_count; ++fi) {
@@ -307,9 +309,9 @@ Tiles build_tiles(const StageDesc& d, const Grid& grid, int height) {
                 ys.push_back(y);
             }
         }
-        for (size_t r0 = 0; r0 < ys.size(); r0 += 4) {
+        for (size_t r0 = 0; r0 < ys.size(); r0 += static_cast<size_t>(rows)) {
             int widest = 0;
-            for (size_t rr = r0; rr < r0 + 4 && rr < ys.size(); ++rr) {
+            for (size_t rr = r0; rr < r0 + static_cast<size_t>(rows) && rr < ys.size(); ++rr) {
                 widest = std::max(widest, count[ys[rr]]);
             }
             for (int i0 = 0; i0 < widest; i0 += 4) {
@@ -321,7 +323,7 @@ Tiles build_tiles(const StageDesc& d, const Grid& grid, int height) {
                     for (int ii = 0; ii < 4; ++ii) {
                         int index = -1;
                         const size_t row = r0 + static_cast<size_t>(rr);
-                        if (row < ys.size() && i0 + ii < count[ys[row]]) {
+                        if (rr < rows && row < ys.size() && i0 + ii < count[ys[row]]) {
                             index = frame_begin + first[ys[row]] + i0 + ii;
                             const uint32_t p = grid.anchors[static_cast<size_t>(index)];
                             int lo = 0;
@@ -747,7 +749,12 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
         bool fast_matcher = false;
         if (fast_energy != nullptr) {
             if (!plan.tiles_built) {
-                plan.tiles = build_tiles(d, grid, g.height);
+                // Four grid rows per tile where the window union fits the
+                // kernel, else two.
+                plan.tiles = build_tiles(d, grid, g.height, 4);
+                if (!vnlbhip::fast_match_supported(d, plan.tiles.max_cols, plan.tiles.max_rows)) {
+                    plan.tiles = build_tiles(d, grid, g.height, 2);
+                }
                 plan.tiles_built = true;
             }
             tiles = &plan.tiles;
