@@ -65,16 +65,24 @@ def algorithmic_match(params, width, height, frames, channels, anchors, stage):
     return 2.0 * nx * ny * nt * d * anchors
 
 
-def measured_traffic(kernel: str):
+def measured_traffic(kernel: str, launches_per_step: int):
     """HBM bytes per launch of `kernel` from the committed counter summary
-    (rocprofv3 FETCH_SIZE / WRITE_SIZE passes, copy-calibrated), or None."""
+    (rocprofv3 FETCH_SIZE / WRITE_SIZE passes, copy-calibrated), rescaled
+    from the profiled run's launches per step (default pipeline parts) to
+    this run's, or None."""
     path = REPO / "profiles" / "hbm_traffic_fast_kernels_yuv444ps_1080p_16f.md"
     if not path.exists():
         return None
+    per_launch = None
+    profiled = None
     for line in path.read_text().splitlines():
         if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
-            return float(line.split(":", 1)[1])
-    return None
+            per_launch = float(line.split(":", 1)[1])
+        if line.startswith(f"traffic_launches_per_step {kernel}:"):
+            profiled = int(line.split(":", 1)[1])
+    if per_launch is None or profiled is None or launches_per_step <= 0:
+        return None
+    return per_launch * profiled / launches_per_step
 
 
 def match_roofline(form, times, bp, fp, clip, sb, sf) -> dict:
@@ -89,7 +97,7 @@ def match_roofline(form, times, bp, fp, clip, sb, sf) -> dict:
     achieved = flops / launches / avg_s / 1e12 if avg_s > 0 else 0.0
     if form == "fast":
         kernel, peak = "match_fast_kernel", MI355X_PEAK["bf16_tflops"]
-        traffic = measured_traffic("match_fast_kernel")
+        traffic = measured_traffic("match_fast_kernel", launches)
     else:
         kernel, peak, traffic = "match_kernel", MI355X_PEAK["fp32_tflops"], None
     return {
@@ -131,7 +139,7 @@ def filter_roofline(form, times, bp, fp, clip, sb, sf) -> dict:
         "peak": MI355X_PEAK["hbm_gbs"],
         "unit": "GB/s",
         "frac": round(achieved / MI355X_PEAK["hbm_gbs"], 5),
-        "traffic": measured_traffic(kernel) if form == "fast" else None,
+        "traffic": measured_traffic(kernel, launches) if form == "fast" else None,
         "launch_ms": round(avg_s * 1000.0, 4),
         "algorithmic_per_launch": round(total / launches, 1),
     }
