@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define VNLBHIP_ABI_VERSION 4
+#define VNLBHIP_ABI_VERSION 5
 
 enum {
     VNLBHIP_STAGE_BASIC = 0,
@@ -221,6 +221,59 @@ int vnlbhip_set_graph(vnlbhip_context* context, int enable);
  * vnlbhip_stats.parts reports the count used.  No counterpart in the
  * reference.  Values other than 0 and 1 are rejected. */
 int vnlbhip_set_autotune(vnlbhip_context* context, int enable);
+
+/* Per-anchor search centres from a motion field (MVToolsFlowProvider's
+ * centres, src/flow/flow.hpp:321-390, as a dense input; the reference
+ * centres each scheduled frame's window on them, src/core/core.cpp:1328-1340).
+ * offsets is a device array of int16 pairs (dx, dy), laid out
+ * [frames][temporal][height][width][2] for the clip of the next
+ * vnlbhip_stage / vnlbhip_anchor_stack / vnlbhip_debug_frame calls, with
+ * temporal = search_bwd + search_fwd + 1; index ti is the candidate frame's
+ * place in the anchor's schedule (the anchor frame, then later frames
+ * ascending, then earlier frames descending).  For ti > 0 the window is
+ * centred on (x + dx, y + dy) clamped to the frame and then shifted inside
+ * it like an unguided window; ti = 0 keeps the anchor.  Guided stages run
+ * the plain matcher and filter in either form.  NULL (default) clears the
+ * field.  The array must stay valid while it is set; temporal <= 0 with a
+ * non-NULL array is rejected with VNLBHIP_ERROR_INVALID, and a stage whose
+ * search window does not match temporal fails with VNLBHIP_ERROR_INVALID. */
+int vnlbhip_set_search_offsets(vnlbhip_context* context, const int16_t* offsets, int temporal);
+
+/* The grid of one MVTools vector clip: the MVTools_MVAnalysisData fields
+ * that MVToolsVectorGrid::nearest_vector reads (src/flow/flow.hpp:38-56,
+ * 176-222; parsed by parse_mvtools_analysis, :196-258). */
+typedef struct vnlbhip_mv_shape {
+    int32_t block_size_x;
+    int32_t block_size_y;
+    int32_t overlap_x;
+    int32_t overlap_y;
+    int32_t block_count_x;
+    int32_t block_count_y;
+    int32_t pel;
+} vnlbhip_mv_shape;
+
+/* Fills the offset field of vnlbhip_set_search_offsets from MVTools vector
+ * clips, as MVToolsFlowProvider::center_for chains them
+ * (src/flow/flow.hpp:340-390): towards an earlier candidate frame the grids
+ * of the forward clip (mvfw) of the frames from the anchor frame down to one
+ * after the candidate, towards a later one the grids of the backward clip
+ * (mvbw) of the frames from the anchor frame up to one before the
+ * candidate (the provider as src/vs/plugin.cpp:700-726 builds it); each
+ * usable grid moves the centre by the vector of the block nearest to
+ * centre + patch_size / 2, rounded from 1/pel pixel units, and clamps it to
+ * the frame.  Device arrays: *_vectors [frames][block_count_y]
+ * [block_count_x][2] int32 (x, y) of the finest level, *_valid [frames]
+ * uint8 (0: the frame's grid is not used -- an invalid vector frame or a
+ * scene change, src/vs/plugin.cpp:476-485,516-519); NULL vectors: no clip
+ * for that direction.  offsets: device, 4-byte aligned,
+ * [frames][search_bwd + search_fwd + 1][height][width][2] int16, written on
+ * stream; the schedule follows params (patch_size, patch_time, search_bwd,
+ * search_fwd).  Invalid arguments: VNLBHIP_ERROR_INVALID. */
+int vnlbhip_mvtools_offsets(vnlbhip_context* context, const vnlbhip_params* params,
+                            const vnlbhip_mv_shape* fw_shape, const int32_t* fw_vectors,
+                            const uint8_t* fw_valid, const vnlbhip_mv_shape* bw_shape,
+                            const int32_t* bw_vectors, const uint8_t* bw_valid, int16_t* offsets,
+                            int width, int height, int frames, void* stream);
 
 /* FAST form: 1 (default) runs the k-nearest selection inside the MFMA
  * matcher where the retained cap allows (at most 64); 0 stages every

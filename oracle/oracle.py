@@ -36,6 +36,61 @@ class VoParams(ctypes.Structure):
     _fields_ = _FIELDS
 
 
+MV_SHAPE_FIELDS = ("block_size_x", "block_size_y", "overlap_x", "overlap_y", "block_count_x",
+                   "block_count_y", "pel")
+
+
+class MvShape(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_int) for name in MV_SHAPE_FIELDS]
+
+
+def _vector_clip(clip):
+    """(shape dict, vectors [frames, by, bx, 2] int32, valid [frames] uint8) or
+    None -> ctypes arguments (kept alive by the returned tuple)."""
+    P = ctypes.POINTER
+    if clip is None:
+        return None, None, None, None
+    shape, vectors, valid = clip
+    s = MvShape(*[int(shape[name]) for name in MV_SHAPE_FIELDS])
+    vectors = np.ascontiguousarray(vectors, np.int32)
+    valid = np.ascontiguousarray(valid, np.uint8)
+    return (ctypes.byref(s), vectors.ctypes.data_as(P(ctypes.c_int32)),
+            valid.ctypes.data_as(P(ctypes.c_ubyte)), (s, vectors, valid))
+
+
+def center_for(fw, bw, anchor_x, anchor_y, anchor_frame, candidate_frame, patch_size, width, height,
+               grid_frames=None):
+    """MVToolsFlowProvider::center_for over dense vector clips (fw: the
+    mvfw clip, walked towards earlier frames; bw: the mvbw clip)."""
+    lib = load()
+    a = _vector_clip(fw)
+    b = _vector_clip(bw)
+    if grid_frames is None:
+        grid_frames = len((fw or bw)[2]) if (fw or bw) is not None else 0
+    out = (ctypes.c_int * 2)()
+    lib.vo_center_for(a[0], a[1], a[2], b[0], b[1], b[2], int(grid_frames), int(anchor_x),
+                      int(anchor_y), int(anchor_frame), int(candidate_frame), int(patch_size),
+                      int(width), int(height), out)
+    return int(out[0]), int(out[1])
+
+
+def mvtools_offsets(params, fw, bw, width, height, frames):
+    """Dense search offsets [frames, temporal, height, width, 2] int16 from
+    the vector clips, in the layout of set_search_offsets."""
+    lib = load()
+    struct = to_struct(params)
+    temporal = struct.search_bwd + struct.search_fwd + 1
+    out = np.zeros((frames, temporal, height, width, 2), np.int16)
+    a = _vector_clip(fw)
+    b = _vector_clip(bw)
+    rc = lib.vo_mvtools_offsets(ctypes.byref(struct), a[0], a[1], a[2], b[0], b[1], b[2], int(width),
+                                int(height), int(frames),
+                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_int16)))
+    if rc != 0:
+        raise ValueError("vo_mvtools_offsets: invalid arguments")
+    return out
+
+
 def to_struct(params) -> VoParams:
     s = VoParams()
     source = params if isinstance(params, dict) else params.as_dict()
@@ -70,6 +125,14 @@ def load() -> ctypes.CDLL:
                                        fp, fp, ip, ctypes.c_int, ctypes.c_int, dp]
         lib.vo_anchor_stack.argtypes = [ctypes.c_int, P(VoParams), fp, fp, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, fp, ctypes.c_int]
+        lib.vo_set_search_offsets.argtypes = [P(ctypes.c_int16), ctypes.c_int]
+        lib.vo_set_search_offsets.restype = None
+        mv = P(MvShape)
+        i32, u8 = P(ctypes.c_int32), P(ctypes.c_ubyte)
+        lib.vo_center_for.argtypes = [mv, i32, u8, mv, i32, u8] + [ctypes.c_int] * 8 + [ip]
+        lib.vo_center_for.restype = None
+        lib.vo_mvtools_offsets.argtypes = [P(VoParams), mv, i32, u8, mv, i32, u8, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_int, P(ctypes.c_int16)]
         lib.vo_retained_cap.argtypes = [P(VoParams)]
         lib.vo_grid_count.argtypes = [P(VoParams), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int]
@@ -109,6 +172,22 @@ def stage(stage_id: int, params, noisy, ref=None, flags: int = ACCUM_DOUBLE, thr
     if rc != 0:
         raise ValueError(lib.vo_last_error().decode())
     return out, int(groups.value)
+
+
+_offsets = None  # keeps the array the oracle points at alive
+
+
+def set_search_offsets(offsets, temporal: int = 0) -> None:
+    """Search-centre offsets [frames, temporal, height, width, 2] int16 (dx, dy)
+    per anchor position and scheduled candidate frame, or None to clear."""
+    global _offsets
+    lib = load()
+    if offsets is None:
+        _offsets = None
+        lib.vo_set_search_offsets(None, 0)
+        return
+    _offsets = np.ascontiguousarray(offsets, dtype=np.int16)
+    lib.vo_set_search_offsets(_offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int16)), int(temporal))
 
 
 def anchor_stack(stage_id: int, params, noisy, ref, anchor_frame: int, flags: int = ACCUM_DOUBLE):

@@ -525,6 +525,29 @@ void spatial_range(const vo_params& p, int center, int axis, int& low,
     high = std::min(max_origin, end - shift);
 }
 
+// Search-centre guidance (MVToolsFlowProvider::center_for, src/flow/flow.hpp:340-390,
+// as a dense input): offsets[((anchor_frame * temporal + ti) * height + y) *
+// width + x] holds (dx, dy) for the anchor at (x, y) and the ti-th frame of
+// its schedule; the centre is the anchor plus the offset, clamped to the
+// frame, and the anchor frame itself (ti = 0) keeps the anchor.  Null: the
+// anchor is the centre of every window.
+const int16_t* g_offsets = nullptr;
+int g_offsets_temporal = 0;
+
+void search_window(const vo_params& p, const Geometry& g, int ax, int ay, int af, int ti,
+                   int& x_low, int& x_high, int& y_low, int& y_high) {
+    int cx = ax;
+    int cy = ay;
+    if (g_offsets != nullptr && ti > 0) {
+        const std::size_t at =
+            ((static_cast<std::size_t>(af) * g_offsets_temporal + ti) * g.height + ay) * g.width + ax;
+        cx = std::clamp(ax + g_offsets[2 * at], 0, g.width - 1);
+        cy = std::clamp(ay + g_offsets[2 * at + 1], 0, g.height - 1);
+    }
+    spatial_range(p, cx, g.width, x_low, x_high);
+    spatial_range(p, cy, g.height, y_low, y_high);
+}
+
 // reference_processing_mask_selected (core.cpp:934-950).
 bool grid_selected(const vo_params& p, const Geometry& g, int anchor_frame,
                    int x, int y) {
@@ -671,17 +694,16 @@ void enumerate_candidates(const StageSetup& s, int ax, int ay, int af,
     int high = 0;
     temporal_range(p, s.g, af, low, high);
     schedule_frames(af, low, high, ws.scheduled);
-    int x_low = 0;
-    int x_high = 0;
-    int y_low = 0;
-    int y_high = 0;
-    spatial_range(p, ax, s.g.width, x_low, x_high);
-    spatial_range(p, ay, s.g.height, y_low, y_high);
-
     ws.candidates.clear();
     ws.row_buffer.resize(static_cast<std::size_t>(P));
     int scan = 0;
-    for (int frame : ws.scheduled) {
+    for (std::size_t ti = 0; ti < ws.scheduled.size(); ++ti) {
+        const int frame = ws.scheduled[ti];
+        int x_low = 0;
+        int x_high = 0;
+        int y_low = 0;
+        int y_high = 0;
+        search_window(p, s.g, ax, ay, af, static_cast<int>(ti), x_low, x_high, y_low, y_high);
         for (int cy = y_low; cy <= y_high; ++cy) {
             for (int cx = x_low; cx <= x_high; ++cx) {
                 float distance = 0.0F;
@@ -814,13 +836,6 @@ int select_faithful(const StageSetup& s, int ax, int ay, int af,
     int high = 0;
     temporal_range(p, s.g, af, low, high);
     schedule_frames(af, low, high, ws.scheduled);
-    int x_low = 0;
-    int x_high = 0;
-    int y_low = 0;
-    int y_high = 0;
-    spatial_range(p, ax, s.g.width, x_low, x_high);
-    spatial_range(p, ay, s.g.height, y_low, y_high);
-
     const long long window_cap = static_cast<long long>(p.search_window) *
                                  p.search_window *
                                  static_cast<long long>(ws.scheduled.size());
@@ -841,7 +856,13 @@ int select_faithful(const StageSetup& s, int ax, int ay, int af,
         soft_worst = worst->distance;
     };
     int scan = 0;
-    for (int frame : ws.scheduled) {
+    for (std::size_t ti = 0; ti < ws.scheduled.size(); ++ti) {
+        const int frame = ws.scheduled[ti];
+        int x_low = 0;
+        int x_high = 0;
+        int y_low = 0;
+        int y_high = 0;
+        search_window(p, s.g, ax, ay, af, static_cast<int>(ti), x_low, x_high, y_low, y_high);
         for (int cy = y_low; cy <= y_high; ++cy) {
             for (int cx = x_low; cx <= x_high; ++cx, ++scan) {
                 float limit = static_cast<int>(top.size()) >= p.similar
@@ -1689,6 +1710,116 @@ int vo_grid_count(const vo_params* params, int width, int height, int frames,
         }
     }
     return count;
+}
+
+void vo_set_search_offsets(const int16_t* offsets, int temporal) {
+    g_offsets = offsets;
+    g_offsets_temporal = offsets != nullptr ? temporal : 0;
+}
+
+namespace {
+
+// MVToolsVectorGrid::nearest_block_index (src/flow/flow.hpp:208-222): the
+// block whose centre first_center + i * step is nearest, halves rounded
+// away from the first block, clamped to the grid.
+int nearest_block(int coordinate, int first_center, int step, int count) {
+    if (count <= 1) {
+        return 0;
+    }
+    const int shifted = coordinate - first_center;
+    const int index = shifted >= 0 ? (shifted + step / 2) / step : -((-shifted + step / 2) / step);
+    return std::clamp(index, 0, count - 1);
+}
+
+// detail::round_subpixel_to_pixel (src/flow/flow.hpp:185-193): nearest
+// whole pixel, halves away from zero.
+int pel_to_pixel(int value, int pel) {
+    if (pel <= 1) {
+        return value;
+    }
+    return value >= 0 ? (value + pel / 2) / pel : -((-value + pel / 2) / pel);
+}
+
+// One step of the chain: the grid of `frame` moves the centre by the vector
+// of the block nearest to the centre's patch middle
+// (center_for's loop bodies, src/flow/flow.hpp:359-386).
+void follow_vector(const vo_mv_shape& s, const int32_t* vectors, int frame, int patch_center,
+                   int width, int height, int& x, int& y) {
+    const int step_x = std::max(1, s.block_size_x - std::max(0, s.overlap_x));
+    const int step_y = std::max(1, s.block_size_y - std::max(0, s.overlap_y));
+    const int bx = nearest_block(x + patch_center, s.block_size_x / 2, step_x, s.block_count_x);
+    const int by = nearest_block(y + patch_center, s.block_size_y / 2, step_y, s.block_count_y);
+    const std::size_t at =
+        ((static_cast<std::size_t>(frame) * s.block_count_y + by) * s.block_count_x + bx) * 2;
+    x += pel_to_pixel(vectors[at], s.pel);
+    y += pel_to_pixel(vectors[at + 1], s.pel);
+    if (width > 0) {
+        x = std::clamp(x, 0, width - 1);
+    }
+    if (height > 0) {
+        y = std::clamp(y, 0, height - 1);
+    }
+}
+
+}  // namespace
+
+void vo_center_for(const vo_mv_shape* fw, const int32_t* fw_vectors, const unsigned char* fw_valid,
+                   const vo_mv_shape* bw, const int32_t* bw_vectors, const unsigned char* bw_valid,
+                   int grid_frames, int anchor_x, int anchor_y, int anchor_frame,
+                   int candidate_frame, int patch_size, int width, int height, int* out_xy) {
+    int x = anchor_x;
+    int y = anchor_y;
+    const int patch_center = std::max(0, patch_size / 2);
+    if (candidate_frame < anchor_frame) {
+        for (int frame = anchor_frame; frame > candidate_frame; --frame) {
+            if (fw_vectors != nullptr && frame >= 0 && frame < grid_frames && fw_valid[frame] != 0) {
+                follow_vector(*fw, fw_vectors, frame, patch_center, width, height, x, y);
+            }
+        }
+    } else {
+        for (int frame = anchor_frame; frame < candidate_frame; ++frame) {
+            if (bw_vectors != nullptr && frame >= 0 && frame < grid_frames && bw_valid[frame] != 0) {
+                follow_vector(*bw, bw_vectors, frame, patch_center, width, height, x, y);
+            }
+        }
+    }
+    out_xy[0] = x;
+    out_xy[1] = y;
+}
+
+int vo_mvtools_offsets(const vo_params* params, const vo_mv_shape* fw, const int32_t* fw_vectors,
+                       const unsigned char* fw_valid, const vo_mv_shape* bw,
+                       const int32_t* bw_vectors, const unsigned char* bw_valid, int width,
+                       int height, int frames, int16_t* offsets) {
+    if (params == nullptr || offsets == nullptr || width <= 0 || height <= 0 || frames <= 0) {
+        return -1;
+    }
+    const vo_params& p = *params;
+    const Geometry g{width, height, frames, 1};
+    const int temporal = p.search_bwd + p.search_fwd + 1;
+    std::vector<int> schedule;
+    for (int f = 0; f < frames; ++f) {
+        int low = 0;
+        int high = 0;
+        temporal_range(p, g, f, low, high);
+        schedule_frames(f, low, high, schedule);
+        for (int ti = 0; ti < temporal; ++ti) {
+            for (int y = 0; y < height; ++y) {
+                for (int x = 0; x < width; ++x) {
+                    int c[2] = {x, y};
+                    if (ti < static_cast<int>(schedule.size())) {
+                        vo_center_for(fw, fw_vectors, fw_valid, bw, bw_vectors, bw_valid, frames, x,
+                                      y, f, schedule[ti], p.patch_size, width, height, c);
+                    }
+                    const std::size_t at =
+                        ((static_cast<std::size_t>(f) * temporal + ti) * height + y) * width + x;
+                    offsets[2 * at] = static_cast<int16_t>(c[0] - x);
+                    offsets[2 * at + 1] = static_cast<int16_t>(c[1] - y);
+                }
+            }
+        }
+    }
+    return 0;
 }
 
 int vo_anchor_stack(int stage, const vo_params* params, const float* noisy,

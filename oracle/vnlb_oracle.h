@@ -95,6 +95,50 @@ int vo_frame_debug(int stage, const vo_params* params, const float* noisy,
                    unsigned char* active, float* weights, float* eigenvalues,
                    int* group_flags, int flags, int gram_dim, double* grams);
 
+/* Search-centre guidance for the following calls (MVToolsFlowProvider's
+ * centres, src/flow/flow.hpp:321-390, as a dense input): offsets
+ * [frames][temporal][height][width][2] int16 (dx, dy) per anchor position and
+ * scheduled candidate frame (anchor, future ascending, past descending);
+ * temporal = search_bwd + search_fwd + 1.  The window of candidate frame ti
+ * is centred on the anchor plus the offset clamped to the frame; ti = 0
+ * keeps the anchor.  NULL clears it.  The pointer must stay valid while set. */
+void vo_set_search_offsets(const int16_t* offsets, int temporal);
+
+/* The fields of one MVTools vector clip that MVToolsVectorGrid::nearest_vector
+ * reads (src/flow/flow.hpp:38-56,176-222). */
+typedef struct vo_mv_shape {
+    int block_size_x;
+    int block_size_y;
+    int overlap_x;
+    int overlap_y;
+    int block_count_x;
+    int block_count_y;
+    int pel;
+} vo_mv_shape;
+
+/* MVToolsFlowProvider::center_for (src/flow/flow.hpp:340-390) over dense
+ * vector clips: fw_vectors / bw_vectors [grid_frames][block_count_y]
+ * [block_count_x][2] int32 (x, y) in 1/pel pixel units, fw_valid / bw_valid
+ * [grid_frames] (0: the frame's grid is unusable, as for an invalid vector
+ * frame or a scene change).  The forward clip (mvfw) supplies the grids
+ * walked towards earlier frames, the backward clip (mvbw) those walked
+ * towards later frames (src/vs/plugin.cpp:700-726).  NULL vectors: no clip
+ * for that direction.  Writes the centre to out_xy[0..1]. */
+void vo_center_for(const vo_mv_shape* fw, const int32_t* fw_vectors, const unsigned char* fw_valid,
+                   const vo_mv_shape* bw, const int32_t* bw_vectors, const unsigned char* bw_valid,
+                   int grid_frames, int anchor_x, int anchor_y, int anchor_frame,
+                   int candidate_frame, int patch_size, int width, int height, int* out_xy);
+
+/* The dense offset field of vo_set_search_offsets for a clip of `frames`
+ * frames: offsets[((f * temporal + ti) * height + y) * width + x] = centre -
+ * anchor for anchor (x, y, f) and the ti-th frame of its schedule (0 where
+ * ti is past the schedule).  temporal = search_bwd + search_fwd + 1.
+ * Returns 0, or -1 on invalid arguments. */
+int vo_mvtools_offsets(const vo_params* params, const vo_mv_shape* fw, const int32_t* fw_vectors,
+                       const unsigned char* fw_valid, const vo_mv_shape* bw,
+                       const int32_t* bw_vectors, const unsigned char* bw_valid, int width,
+                       int height, int frames, int16_t* offsets);
+
 /* Contribution stack of one anchor frame in the vnlb.Basic / vnlb.Final
  * output-frame layout (src/vs/plugin.cpp:264-271,625-641; the CUDA
  * pack_kernel, src/cuda/aggregation.cu:508-528): per channel c and slot s =

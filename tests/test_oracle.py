@@ -269,3 +269,71 @@ def test_frame_debug_grams_are_centered_xxt(oracle_lib):
         if checked >= 40:
             break
     assert checked >= 20, checked
+
+
+def _offset_field(frames, temporal, height, width, seed, span=6):
+    rng = np.random.default_rng(seed)
+    return rng.integers(-span, span + 1, size=(frames, temporal, height, width, 2)).astype(np.int16)
+
+
+def test_search_offsets_zero_field_is_no_guidance(oracle_lib):
+    """Zero offsets centre every window on the anchor: identical to no field."""
+    noisy, _, bp, _ = workloads.make("gray_pt2_step4")
+    clip = np.ascontiguousarray(noisy, np.float32)
+    frames = clip.shape[0]
+    height, width = clip.shape[-2:]
+    temporal = bp.search_bwd + bp.search_fwd + 1
+    want, _ = oracle_lib.stage(0, bp, noisy)
+    try:
+        oracle_lib.set_search_offsets(np.zeros((frames, temporal, height, width, 2), np.int16), temporal)
+        got, _ = oracle_lib.stage(0, bp, noisy)
+    finally:
+        oracle_lib.set_search_offsets(None)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_search_offsets_move_the_windows(oracle_lib):
+    """Every match of candidate frame ti lies in the window centred on the
+    anchor plus its offset (clamped to the frame), shifted inside the frame
+    by compute_spatial_range; the anchor frame keeps the anchor's window."""
+    noisy, _, bp, _ = workloads.make("gray_pt2_step4")
+    clip = np.ascontiguousarray(noisy, np.float32)
+    frames = clip.shape[0]
+    height, width = clip.shape[-2:]
+    temporal = bp.search_bwd + bp.search_fwd + 1
+    field = _offset_field(frames, temporal, height, width, seed=3)
+    anchor_frame = 2
+    try:
+        oracle_lib.set_search_offsets(field, temporal)
+        dbg = oracle_lib.frame_debug(0, bp, noisy, None, anchor_frame)
+    finally:
+        oracle_lib.set_search_offsets(None)
+    P, half = bp.patch_size, (bp.search_window - 1) // 2
+    shift_t = min(0, anchor_frame - bp.search_bwd) + max(0, anchor_frame + bp.search_fwd - (frames - bp.patch_time))
+    low_t = max(0, anchor_frame - bp.search_bwd - shift_t)
+    high_t = min(frames - bp.patch_time, anchor_frame + bp.search_fwd - shift_t)
+    schedule = [anchor_frame] + list(range(anchor_frame + 1, high_t + 1)) + list(range(anchor_frame - 1, low_t - 1, -1))
+
+    def window(center, axis):
+        max_origin = axis - P
+        start, end = center - half, center + half
+        shift = min(0, start) + max(0, end - max_origin)
+        return max(0, start - shift), min(max_origin, end - shift)
+
+    moved = 0
+    for a, n in enumerate(dbg["counts"]):
+        ax, ay = int(dbg["anchor_x"][a]), int(dbg["anchor_y"][a])
+        for pos in dbg["positions"][a, :n]:
+            f, rem = divmod(int(pos), height * width)
+            y, x = divmod(rem, width)
+            ti = schedule.index(f)
+            cx, cy = ax, ay
+            if ti > 0:
+                dx, dy = field[anchor_frame, ti, ay, ax]
+                cx = min(max(ax + int(dx), 0), width - 1)
+                cy = min(max(ay + int(dy), 0), height - 1)
+                moved += (cx, cy) != (ax, ay)
+            xl, xh = window(cx, width)
+            yl, yh = window(cy, height)
+            assert xl <= x <= xh and yl <= y <= yh, (a, f, x, y, cx, cy)
+    assert moved > 100, moved

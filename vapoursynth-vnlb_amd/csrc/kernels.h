@@ -7,10 +7,29 @@
 
 namespace vnlbhip {
 
-size_t match_shared_bytes(const StageDesc& d);
+// guided: per-frame search windows from offsets ([frames][offsets_temporal]
+// [height][width][2] int16 (dx, dy) per anchor and scheduled frame), else
+// NULL and every window centred on the anchor.
+size_t match_shared_bytes(const StageDesc& d, bool guided = false);
+
+// The grid fields of an MVTools vector clip (flow.hip).
+struct MvShape {
+    int block_size_x;
+    int block_size_y;
+    int overlap_x;
+    int overlap_y;
+    int block_count_x;
+    int block_count_y;
+    int pel;
+};
+// Dense offsets [frames][temporal][H][W][2] from vector clips (either
+// vectors pointer may be NULL: no clip for that direction).
+hipError_t launch_mvtools_offsets(const StageDesc& d, const MvShape& fw, const int32_t* fw_vectors,
+                                  const uint8_t* fw_valid, const MvShape& bw, const int32_t* bw_vectors,
+                                  const uint8_t* bw_valid, int16_t* offsets, hipStream_t stream);
 hipError_t launch_match(const StageDesc& d, const float* source, const uint32_t* anchors,
                         int anchor_count, int* counts, uint32_t* positions, float* distances,
-                        hipStream_t stream);
+                        const int16_t* offsets, int offsets_temporal, hipStream_t stream);
 
 // MFMA form of the matcher (match_fast.hip).  Tiles hold 16 anchor indices
 // (-1 for empty slots) of one frame; max_cols / max_rows bound the union of

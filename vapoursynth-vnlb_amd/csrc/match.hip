@@ -72,9 +72,13 @@ struct Region {
     int floats;    // frames * rows * cols (+ padding)
 };
 
-__host__ __device__ inline int region_floats(int window, int patch, int temporal, int patch_time) {
+// Unguided: the scheduled frames share one window, frames t_low ..
+// t_high + Pt - 1.  Guided: every scheduled frame has its own window, so
+// each (frame, patch frame) pair has its own plane.
+__host__ __device__ inline int region_floats(int window, int patch, int temporal, int patch_time,
+                                             bool guided) {
     const int side = window + patch - 1;
-    return (temporal + patch_time - 1) * side * side + 8;
+    return (guided ? temporal * patch_time : temporal + patch_time - 1) * side * side + 8;
 }
 
 // Distances (scan order) with the search region staged in LDS one channel
@@ -86,8 +90,11 @@ template <int kP>
 __global__ __launch_bounds__(kMatchThreads) void match_kernel(
     StageDesc d, const float* __restrict__ source, const uint32_t* __restrict__ anchors,
     int anchor_count, int* __restrict__ counts, uint32_t* __restrict__ positions,
-    float* __restrict__ distances) {
+    float* __restrict__ distances, const int16_t* __restrict__ offsets, int offsets_temporal) {
     extern __shared__ float s_dyn[];
+    // Guided search: the window origin of each scheduled frame.
+    __shared__ int s_frame_x_low[kMaxTemporal];
+    __shared__ int s_frame_y_low[kMaxTemporal];
     __shared__ float s_ref[kMaxDim];
     __shared__ unsigned s_hist[256];
     __shared__ int s_int[16];
@@ -136,13 +143,37 @@ __global__ __launch_bounds__(kMatchThreads) void match_kernel(
     const int n = nt * per_frame;
     const int rcols = nx + P - 1;
     const int rrows = ny + P - 1;
-    const int rframes = nt + d.patch_time - 1;
+    // Guided search (MVToolsFlowProvider::center_for, src/flow/flow.hpp:340-390, from
+    // a dense offset field): the window of scheduled frame ti is centred on
+    // the anchor plus its offset clamped to the frame (the anchor frame keeps
+    // the anchor); the window size does not depend on the centre, since
+    // compute_spatial_range shifts it inside the frame.
+    const bool guided = offsets != nullptr;
+    if (guided) {
+        for (int ti = threadIdx.x; ti < nt; ti += blockDim.x) {
+            int cx = ax;
+            int cy = ay;
+            if (ti > 0) {
+                const size_t at =
+                    ((static_cast<size_t>(af) * offsets_temporal + ti) * d.height + ay) * d.width + ax;
+                cx = min(max(ax + static_cast<int>(offsets[2 * at]), 0), d.width - 1);
+                cy = min(max(ay + static_cast<int>(offsets[2 * at + 1]), 0), d.height - 1);
+            }
+            int low = 0;
+            int high = 0;
+            spatial_range(d, cx, d.width, low, high);
+            s_frame_x_low[ti] = low;
+            spatial_range(d, cy, d.height, low, high);
+            s_frame_y_low[ti] = low;
+        }
+    }
+    const int rframes = guided ? nt * d.patch_time : nt + d.patch_time - 1;
     const int plane = rrows * rcols;
     const int region_size = rframes * plane;
 
     float* s_region = s_dyn;
     float* s_dist = s_dyn + region_floats(d.search_window, P, d.search_bwd + d.search_fwd + 1,
-                                          d.patch_time);
+                                          d.patch_time, guided);
 
     // Reference patch [c][t][y][x] (load_reference_patch_* core.cpp:1152-1198).
     const int ref_len = match_channels * d.patch_time * area;
@@ -168,7 +199,14 @@ __global__ __launch_bounds__(kMatchThreads) void match_kernel(
             const int rem = i - fr * plane;
             const int ry = rem / rcols;
             const int rx = rem - ry * rcols;
-            s_region[i] = video_at(source, d, x_low + rx, y_low + ry, t_low + fr, ch);
+            if (guided) {
+                const int ti = fr / d.patch_time;
+                const int t = fr - ti * d.patch_time;
+                s_region[i] = video_at(source, d, s_frame_x_low[ti] + rx, s_frame_y_low[ti] + ry,
+                                       scheduled_frame(af, t_low, t_high, ti) + t, ch);
+            } else {
+                s_region[i] = video_at(source, d, x_low + rx, y_low + ry, t_low + fr, ch);
+            }
         }
         __syncthreads();
         for (int item = threadIdx.x; item < items; item += blockDim.x) {
@@ -186,7 +224,8 @@ __global__ __launch_bounds__(kMatchThreads) void match_kernel(
                 dist[j] = (ch == 0 || j >= valid) ? 0.0F : s_dist[base_c + j];
             }
             for (int t = 0; t < d.patch_time; ++t) {
-                const float* region_plane = s_region + (frame - t_low + t) * plane;
+                const float* region_plane =
+                    s_region + (guided ? ti * d.patch_time + t : frame - t_low + t) * plane;
                 const float* ref_plane = s_ref + (ch * d.patch_time + t) * area;
 #pragma unroll
                 for (int y = 0; y < P; ++y) {
@@ -205,34 +244,36 @@ __global__ __launch_bounds__(kMatchThreads) void match_kernel(
     __syncthreads();
     select_and_write(d, s_dist, n, af, t_low, t_high, x_low, y_low, nx, per_frame, nt, s_hist,
                      s_int, s_scan, s_sel_dist, s_sel_index, s_sel_rank, a, out_base, counts,
-                     positions, distances);
+                     positions, distances, guided ? s_frame_x_low : nullptr,
+                     guided ? s_frame_y_low : nullptr);
 }
 
 }  // namespace
 
-size_t match_shared_bytes(const StageDesc& d) {
+size_t match_shared_bytes(const StageDesc& d, bool guided) {
     const int temporal = d.search_bwd + d.search_fwd + 1;
     const long long candidates = static_cast<long long>(d.search_window) * d.search_window * temporal;
-    return (static_cast<size_t>(region_floats(d.search_window, d.patch, temporal, d.patch_time)) +
+    return (static_cast<size_t>(region_floats(d.search_window, d.patch, temporal, d.patch_time, guided)) +
             static_cast<size_t>(candidates)) *
            sizeof(float);
 }
 
 hipError_t launch_match(const StageDesc& d, const float* source,
                         const uint32_t* anchors, int anchor_count, int* counts,
-                        uint32_t* positions, float* distances,
-                        hipStream_t stream) {
+                        uint32_t* positions, float* distances, const int16_t* offsets,
+                        int offsets_temporal, hipStream_t stream) {
     if (anchor_count <= 0) {
         return hipSuccess;
     }
     const dim3 grid(static_cast<unsigned>(anchor_count));
     const dim3 block(kMatchThreads);
-    const size_t shared = match_shared_bytes(d);
+    const size_t shared = match_shared_bytes(d, offsets != nullptr);
     switch (d.patch) {
 #define VNLBHIP_MATCH_CASE(PV)                                                                  \
     case PV:                                                                                    \
         hipLaunchKernelGGL(match_kernel<PV>, grid, block, shared, stream, d, source, anchors,  \
-                           anchor_count, counts, positions, distances);                         \
+                           anchor_count, counts, positions, distances, offsets,                \
+                           offsets_temporal);                                                   \
         break;
         VNLBHIP_MATCH_CASE(1)
         VNLBHIP_MATCH_CASE(2)

@@ -287,7 +287,8 @@ __device__ inline void write_selected(const float* s_sel_dist, const int* s_sel_
                                       int t_high, int x_low, int y_low, int nx, int per_frame, int a,
                                       size_t out_base, int* __restrict__ counts,
                                       uint32_t* __restrict__ positions,
-                                      float* __restrict__ distances) {
+                                      float* __restrict__ distances,
+                                      const int* frame_x_low, const int* frame_y_low) {
     const int k_out = k < selected ? k : selected;
     for (int e = threadIdx.x; e < selected; e += blockDim.x) {
         const float de = s_sel_dist[e];
@@ -313,8 +314,9 @@ __device__ inline void write_selected(const float* s_sel_dist, const int* s_sel_
         const int ti = c / per_frame;
         const int rem = c - ti * per_frame;
         const int yy = rem / nx;
-        const int cx = x_low + (rem - yy * nx);
-        const int cy = y_low + yy;
+        // Guided search: each scheduled frame has its own window origin.
+        const int cx = (frame_x_low != nullptr ? frame_x_low[ti] : x_low) + (rem - yy * nx);
+        const int cy = (frame_y_low != nullptr ? frame_y_low[ti] : y_low) + yy;
         const int frame = scheduled_frame(af, t_low, t_high, ti);
         positions[out_base + slot] = pack_position(cx, cy, frame);
         distances[out_base + slot] = s_sel_dist[e];
@@ -331,7 +333,9 @@ __device__ inline void select_and_write(const StageDesc& d, const float* s_dist,
                                  float* s_sel_dist, int* s_sel_index, int* s_sel_rank, int a,
                                  size_t out_base, int* __restrict__ counts,
                                  uint32_t* __restrict__ positions,
-                                 float* __restrict__ distances) {
+                                 float* __restrict__ distances,
+                                 const int* frame_x_low = nullptr,
+                                 const int* frame_y_low = nullptr) {
     int valid_local = 0;
     for (int c = threadIdx.x; c < n; c += blockDim.x) {
         const float v = s_dist[c];
@@ -365,7 +369,8 @@ __device__ inline void select_and_write(const StageDesc& d, const float* s_dist,
                                   s_sel_index);
     }
     write_selected(s_sel_dist, s_sel_index, s_sel_rank, selected, k, af, t_low, t_high, x_low,
-                   y_low, nx, per_frame, a, out_base, counts, positions, distances);
+                   y_low, nx, per_frame, a, out_base, counts, positions, distances, frame_x_low,
+                   frame_y_low);
 }
 
 }  // namespace vnlbhip

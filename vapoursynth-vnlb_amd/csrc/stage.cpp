@@ -439,6 +439,10 @@ struct vnlbhip_context {
     DeviceBuffer<int> status;
     DeviceBuffer<int> refine_list;
     DeviceBuffer<int> work_counters;  // prepare, filter: next group to take
+    // Search-centre offsets (device, [frames][temporal][H][W][2] int16) or
+    // null: every window centred on its anchor.
+    const int16_t* search_offsets = nullptr;
+    int search_offsets_temporal = 0;
     float refine_gap = 1e-3F;  // FAST form: relative eigen gap below which groups are re-solved in fp64
     bool fused_selection = true;  // FAST form: k-nearest selection inside the MFMA matcher
     DeviceBuffer<float> weights;
@@ -618,7 +622,8 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
     const size_t acc_count = plane * (g.channels + 1) * g.frames;
     // The FAST filter aggregates into fp32 accumulators (same layout, in
     // the same buffer); every other path into fp64 ones.
-    const bool acc_f32 = ctx.form == VNLBHIP_FORM_FAST && vnlbhip::fast_filter_supported(d);
+    const bool acc_f32 = ctx.form == VNLBHIP_FORM_FAST && vnlbhip::fast_filter_supported(d) &&
+                         ctx.search_offsets == nullptr;
     double* acc = ctx.accumulators.reserve(acc_count);
     float* acc32 = reinterpret_cast<float*>(acc);
     check(vnlbhip::launch_fill_zero(acc, acc_f32 ? (acc_count + 1) / 2 : acc_count, stream),
@@ -630,7 +635,17 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
     // MFMA matcher: patch energies of the matched source, once per stage.
     float* fast_offsets = nullptr;
     float* fast_energy = nullptr;
-    if (ctx.form == VNLBHIP_FORM_FAST && d.patch == 8 && d.similar > 1) {
+    const bool guided = ctx.search_offsets != nullptr;
+    if (guided) {
+        require(ctx.search_offsets_temporal == p.search_bwd + p.search_fwd + 1,
+                "search offsets were set for another temporal window (search_bwd + search_fwd + 1)");
+        require(vnlbhip::match_shared_bytes(d, true) <= 140 * 1024,
+                "search window too large for guided matching's LDS budget");
+    }
+    // The MFMA matcher's anchor tiles and the fast filter's aggregation tile
+    // assume windows centred on the anchor: guided stages keep the plain
+    // matcher and filter.
+    if (ctx.form == VNLBHIP_FORM_FAST && d.patch == 8 && d.similar > 1 && !guided) {
         const int match_channels = stage == VNLBHIP_STAGE_FINAL ? g.channels : 1;
         fast_offsets = ctx.fast_offsets.reserve(4);
         float* rows = ctx.fast_rows.reserve(plane * match_channels * g.frames);
@@ -847,7 +862,8 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
                 timed(ctx, VNLBHIP_KERNEL_MATCH, stream, "match kernel", [&]() {
                     return vnlbhip::launch_match(d, source, d_anchors + ab, ae - ab, d_counts + ab,
                                                  d_positions + static_cast<size_t>(ab) * cap,
-                                                 d_distances + static_cast<size_t>(ab) * cap, stream);
+                                                 d_distances + static_cast<size_t>(ab) * cap,
+                                                 ctx.search_offsets, ctx.search_offsets_temporal, stream);
                 });
             }
             check(hipEventRecord(ctx.match_event[h], stream), "hipEventRecord");
@@ -1285,6 +1301,57 @@ int vnlbhip_set_eigen_refine(vnlbhip_context* context, float relative_gap) {
     return VNLBHIP_OK;
 }
 
+int vnlbhip_set_search_offsets(vnlbhip_context* context, const int16_t* offsets, int temporal) {
+    if (context == nullptr || (offsets != nullptr && temporal <= 0)) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    context->search_offsets = offsets;
+    context->search_offsets_temporal = offsets != nullptr ? temporal : 0;
+    return VNLBHIP_OK;
+}
+
+int vnlbhip_mvtools_offsets(vnlbhip_context* context, const vnlbhip_params* params,
+                            const vnlbhip_mv_shape* fw_shape, const int32_t* fw_vectors,
+                            const uint8_t* fw_valid, const vnlbhip_mv_shape* bw_shape,
+                            const int32_t* bw_vectors, const uint8_t* bw_valid, int16_t* offsets,
+                            int width, int height, int frames, void* stream) {
+    if (context == nullptr) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    return guarded(context, [&]() {
+        require(params != nullptr && offsets != nullptr, "params and offsets must not be null");
+        require(reinterpret_cast<uintptr_t>(offsets) % 4 == 0, "offsets must be 4-byte aligned");
+        const Geometry g{width, height, frames, 1};
+        const StageDesc d = make_desc(VNLBHIP_STAGE_BASIC, *params, g);
+        // MVToolsVectorGrid::usable and parse_mvtools_analysis's checks
+        // (src/flow/flow.hpp:176-181,196-258) on a clip's grid shape.
+        const auto shape_of = [](const vnlbhip_mv_shape* in, const int32_t* vectors, const uint8_t* valid,
+                                 const char* name) {
+            vnlbhip::MvShape s{1, 1, 0, 0, 1, 1, 1};
+            if (vectors == nullptr) {
+                return s;
+            }
+            require(in != nullptr && valid != nullptr,
+                    (std::string(name) + ": vectors need a shape and per-frame valid flags").c_str());
+            require(in->block_size_x > 0 && in->block_size_y > 0 && in->pel > 0 && in->block_count_x > 0 &&
+                        in->block_count_y > 0,
+                    (std::string(name) + ": block sizes, block counts and pel must be positive").c_str());
+            require(in->overlap_x >= 0 && in->overlap_y >= 0 && in->overlap_x < in->block_size_x &&
+                        in->overlap_y < in->block_size_y,
+                    (std::string(name) + ": MVTools overlap must be smaller than block size").c_str());
+            s = vnlbhip::MvShape{in->block_size_x, in->block_size_y, in->overlap_x, in->overlap_y,
+                                 in->block_count_x, in->block_count_y, in->pel};
+            return s;
+        };
+        const vnlbhip::MvShape fw = shape_of(fw_shape, fw_vectors, fw_valid, "mvfw");
+        const vnlbhip::MvShape bw = shape_of(bw_shape, bw_vectors, bw_valid, "mvbw");
+        check(hipSetDevice(context->device), "hipSetDevice");
+        check(vnlbhip::launch_mvtools_offsets(d, fw, fw_vectors, fw_valid, bw, bw_vectors, bw_valid, offsets,
+                                              reinterpret_cast<hipStream_t>(stream)),
+              "mvtools offsets kernel");
+    });
+}
+
 int vnlbhip_set_graph(vnlbhip_context* context, int enable) {
     if (context == nullptr || (enable != 0 && enable != 1)) {
         return VNLBHIP_ERROR_INVALID;
@@ -1330,7 +1397,8 @@ int vnlbhip_stage(vnlbhip_context* context, int stage, const vnlbhip_params* par
         context->parts = kDefaultParts;
         if (context->autotune) {
             // Key: everything the stage's kernels and grids depend on.
-            std::vector<long long> key = {stage, context->form, width, height, frames, channels};
+            std::vector<long long> key = {stage, context->form, width, height, frames, channels,
+                                          context->search_offsets != nullptr};
             const unsigned char* raw = reinterpret_cast<const unsigned char*>(params);
             key.insert(key.end(), raw, raw + sizeof(vnlbhip_params));
             auto it = context->tuned_parts.find(key);
@@ -1363,7 +1431,9 @@ int vnlbhip_stage(vnlbhip_context* context, int stage, const vnlbhip_params* par
             // Key: the stage's launch sequence and every address it holds.
             std::vector<long long> key = {stage, context->form, context->parts, width, height, frames,
                                           channels, reinterpret_cast<long long>(noisy),
-                                          reinterpret_cast<long long>(ref), reinterpret_cast<long long>(out)};
+                                          reinterpret_cast<long long>(ref), reinterpret_cast<long long>(out),
+                                          reinterpret_cast<long long>(context->search_offsets),
+                                          context->search_offsets_temporal};
             const unsigned char* raw = reinterpret_cast<const unsigned char*>(params);
             key.insert(key.end(), raw, raw + sizeof(vnlbhip_params));
             auto it = context->graphs.find(key);

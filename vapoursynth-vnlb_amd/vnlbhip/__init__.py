@@ -50,6 +50,7 @@ class Context:
             raise RuntimeError(f"VNLB HIP: cannot create a context on device {device} ({_ERROR_NAMES.get(rc, rc)})")
         self.device = int(device)
         self.form = "plain"
+        self._search_offsets = None
 
     def close(self) -> None:
         if self._handle:
@@ -116,6 +117,71 @@ class Context:
         if rc != 0:
             self._raise(rc)
 
+    def set_search_offsets(self, offsets) -> None:
+        """Centre each scheduled frame's search window on the anchor plus a
+        per-anchor offset (MVToolsFlowProvider::center_for,
+        src/flow/flow.hpp:340-390, from a dense field): an int16 GPU tensor
+        [frames, search_bwd + search_fwd + 1, height, width, 2] of (dx, dy)
+        per anchor position and schedule index (the anchor frame, later
+        frames ascending, earlier frames descending), or None to clear it.
+        The tensor is kept alive while it is set."""
+        import torch
+
+        if offsets is None:
+            rc = self._lib.vnlbhip_set_search_offsets(self._handle, None, 0)
+            self._search_offsets = None
+        else:
+            if (not offsets.is_cuda or offsets.dtype != torch.int16 or offsets.dim() != 5
+                    or offsets.shape[-1] != 2 or not offsets.is_contiguous()):
+                raise VNLBError("VNLB HIP: offsets must be a contiguous int16 GPU tensor "
+                                "[frames, temporal, height, width, 2]")
+            rc = self._lib.vnlbhip_set_search_offsets(self._handle, ctypes.c_void_p(offsets.data_ptr()),
+                                                      int(offsets.shape[1]))
+            self._search_offsets = offsets
+        if rc != 0:
+            self._raise(rc)
+
+    def mvtools_offsets(self, params: StageParameters, frames: int, height: int, width: int,
+                        mvfw=None, mvbw=None):
+        """The search-offset field of set_search_offsets from MVTools vector
+        clips (vnlbhip.flow.VectorClip; mvfw is walked towards earlier
+        frames, mvbw towards later ones, as MVToolsFlowProvider::center_for
+        does, src/flow/flow.hpp:340-390): an int16 GPU tensor [frames,
+        search_bwd + search_fwd + 1, height, width, 2], computed on the GPU."""
+        import torch
+
+        device = torch.device("cuda", self.device)
+        temporal = params.search_bwd + params.search_fwd + 1
+        out = torch.empty((frames, temporal, height, width, 2), dtype=torch.int16, device=device)
+        keep = []
+
+        def clip_args(clip, name, backwards):
+            if clip is None:
+                return None, None, None
+            clip.check(name, frames, width, height, backwards)
+            shape = _abi.MvShape(**clip.analysis.grid_shape())
+            vectors = torch.from_numpy(clip.vectors).to(device)
+            valid = torch.from_numpy(clip.valid).to(device)
+            keep.extend((shape, vectors, valid))
+            return (ctypes.byref(shape), ctypes.c_void_p(vectors.data_ptr()),
+                    ctypes.c_void_p(valid.data_ptr()))
+
+        fw = clip_args(mvfw, "mvfw", False)
+        bw = clip_args(mvbw, "mvbw", True)
+        stream = torch.cuda.current_stream(device).cuda_stream
+        rc = self._lib.vnlbhip_mvtools_offsets(
+            self._handle, ctypes.byref(_abi.to_struct(params)), fw[0], fw[1], fw[2], bw[0], bw[1], bw[2],
+            ctypes.c_void_p(out.data_ptr()), int(width), int(height), int(frames), ctypes.c_void_p(stream))
+        if rc != 0:
+            self._raise(rc)
+        return out
+
+    def _check_offsets(self, frames: int, height: int, width: int) -> None:
+        field = self._search_offsets
+        if field is not None and (field.shape[0] != frames or tuple(field.shape[2:4]) != (height, width)):
+            raise VNLBError(f"VNLB HIP: search offsets {tuple(field.shape)} do not cover a clip of "
+                            f"{frames} frames of {width}x{height}")
+
     def set_profiling(self, enable: bool) -> None:
         self._lib.vnlbhip_set_profiling(self._handle, 1 if enable else 0)
 
@@ -132,6 +198,7 @@ class Context:
         import torch
 
         frames, channels, height, width = _canonical_shape(tuple(noisy.shape))
+        self._check_offsets(frames, height, width)
         if not _is_torch(noisy) or noisy.dtype != torch.float32 or not noisy.is_cuda:
             raise VNLBError("VNLB HIP: device clips must be float32 tensors on the GPU")
         if stage == FINAL and (ref is None or tuple(ref.shape) != tuple(noisy.shape)):
@@ -156,6 +223,7 @@ class Context:
         (device clips: into out when given, a float32 tensor of the clip's
         shape on the same device)."""
         frames, channels, height, width = _canonical_shape(tuple(noisy.shape))
+        self._check_offsets(frames, height, width)
         if channels not in (1, 3):
             raise VNLBError("VNLB HIP: only constant GrayS and YUV444PS clips are supported")
         if stage == FINAL:
@@ -205,6 +273,7 @@ class Context:
         import torch
 
         frames, channels, height, width = _canonical_shape(tuple(noisy.shape))
+        self._check_offsets(frames, height, width)
         p = _abi.to_struct(params)
         cap = self._lib.vnlbhip_retained_cap(ctypes.byref(p))
         if cap <= 0:
@@ -260,6 +329,22 @@ def _context(device: int) -> Context:
     return contexts[device]
 
 
+def _guided(context: Context, stage: int, params: StageParameters, clip, ref, mvfw, mvbw):
+    """A stage, with search centres from mvfw / mvbw when either is given
+    (process_*_anchor_mvtools, src/vs/plugin.cpp:700-737)."""
+    if mvfw is None and mvbw is None:
+        return context.stage(stage, params, clip, ref)
+    if not _is_torch(clip):
+        raise VNLBError("VNLB HIP: mvfw/mvbw guidance needs the clip as a GPU tensor")
+    frames, _, height, width = _canonical_shape(tuple(clip.shape))
+    field = context.mvtools_offsets(params, frames, height, width, mvfw, mvbw)
+    context.set_search_offsets(field)
+    try:
+        return context.stage(stage, params, clip, ref)
+    finally:
+        context.set_search_offsets(None)
+
+
 def Basic(clip, sigma, block_size=None, block_step=None, group_size=None, bm_range=None,
           patch_time=None, radius=None, search_bwd=None, search_fwd=None, rank=None,
           cap_factor=None, model_cap_factor=None, beta=None, tau=None,
@@ -274,7 +359,7 @@ def Basic(clip, sigma, block_size=None, block_step=None, group_size=None, bm_ran
     params = parse_stage_parameters(BASIC, args)
     context = _context(int(device_id or 0))
     context.set_form(form)
-    return context.stage(BASIC, params, clip)
+    return _guided(context, BASIC, params, clip, None, mvfw, mvbw)
 
 
 def Final(clip, ref, sigma, block_size=None, block_step=None, group_size=None, bm_range=None,
@@ -293,4 +378,4 @@ def Final(clip, ref, sigma, block_size=None, block_step=None, group_size=None, b
     params = parse_stage_parameters(FINAL, args)
     context = _context(int(device_id or 0))
     context.set_form(form)
-    return context.stage(FINAL, params, clip, ref)
+    return _guided(context, FINAL, params, clip, ref, mvfw, mvbw)

@@ -31,9 +31,16 @@ EXPORTED_SYMBOLS = (
     "vnlbhip_set_fused_selection",
     "vnlbhip_set_autotune",
     "vnlbhip_set_graph",
+    "vnlbhip_set_search_offsets",
+    "vnlbhip_mvtools_offsets",
 )
 
 KERNEL_NAMES = ("match", "paste", "prepare", "jacobi", "filter", "normalize", "select", "energy")
+
+
+class MvShape(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_int32) for name in (
+        "block_size_x", "block_size_y", "overlap_x", "overlap_y", "block_count_x", "block_count_y", "pel")]
 
 
 class Params(ctypes.Structure):
@@ -154,6 +161,12 @@ def load() -> ctypes.CDLL:
     lib.vnlbhip_set_autotune.restype = ctypes.c_int
     lib.vnlbhip_set_graph.argtypes = [vp, ctypes.c_int]
     lib.vnlbhip_set_graph.restype = ctypes.c_int
+    lib.vnlbhip_set_search_offsets.argtypes = [vp, vp, ctypes.c_int]
+    lib.vnlbhip_set_search_offsets.restype = ctypes.c_int
+    mv = P(MvShape)
+    lib.vnlbhip_mvtools_offsets.argtypes = [vp, P(Params), mv, vp, vp, mv, vp, vp, vp, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int, vp]
+    lib.vnlbhip_mvtools_offsets.restype = ctypes.c_int
     lib.vnlbhip_get_kernel_times.argtypes = [vp, P(KernelTimes)]
     lib.vnlbhip_get_kernel_times.restype = ctypes.c_int
     _lib = lib

@@ -162,8 +162,10 @@ def parse_stage_parameters(stage: int, args: dict) -> StageParameters:
             setattr(p, name, _float(name, get(name)))
     if get("chroma") is not None:
         p.couple_channels = _int("chroma", get("chroma")) != 0
-    if get("mvfw") is not None or get("mvbw") is not None:
-        raise VNLBError("VNLB HIP: mvfw/mvbw motion guidance is not supported by this build")
+    for name in ("mvfw", "mvbw"):
+        clip = get(name)
+        if clip is not None and type(clip).__name__ != "VectorClip":
+            raise VNLBError(f"VNLB HIP: mvfw/mvbw must be vnlbhip.flow.VectorClip objects ({name})")
     return p
 
 
