@@ -239,6 +239,38 @@ int vnlbhip_set_autotune(vnlbhip_context* context, int enable);
  * search window does not match temporal fails with VNLBHIP_ERROR_INVALID. */
 int vnlbhip_set_search_offsets(vnlbhip_context* context, const int16_t* offsets, int temporal);
 
+/* Per-output-frame rendering, the frame-request model of the VapourSynth
+ * plugins (vnlb.Basic / vnlb.Final produce one anchor frame's contribution
+ * stack per request, src/vs/plugin.cpp:616-758; vnlb.Aggregate sums the
+ * stacks of the anchors around an output frame, :966-1045; the CUDA plugin
+ * keeps device frames in a FrameCache, src/cuda/frame_cache.hpp:73-177).
+ * A cache holds `capacity` anchor stacks in device memory (each
+ * channels * (search_bwd + search_fwd + patch_time) * 2 * width * height
+ * floats); rendering output frame n computes the missing stacks of anchors
+ * [n - search_fwd - patch_time + 1, n + search_bwd] (replacing the least
+ * recently used stacks of other anchors) and aggregates them into out
+ * ([channels][height][width], device).  noisy / ref are the whole clip on
+ * the device, as for vnlbhip_stage; calls with other clip pointers or after
+ * vnlbhip_set_form changed the form start from an empty cache.  capacity
+ * must be at least search_bwd + search_fwd + patch_time (at most 32 stacks
+ * contribute to a frame).  Rendering every frame in order computes each
+ * anchor stack once.  The cache belongs to its context (destroy it first). */
+typedef struct vnlbhip_frame_cache vnlbhip_frame_cache;
+
+typedef struct vnlbhip_frame_cache_stats {
+    int64_t hits;       /* stacks found in the cache */
+    int64_t misses;     /* stacks computed */
+    int64_t evictions;  /* cached stacks replaced */
+} vnlbhip_frame_cache_stats;
+
+int vnlbhip_frame_cache_create(vnlbhip_context* context, int stage, const vnlbhip_params* params,
+                               int width, int height, int frames, int channels, int capacity,
+                               vnlbhip_frame_cache** out);
+int vnlbhip_frame_cache_render(vnlbhip_frame_cache* cache, int frame, const float* noisy,
+                               const float* ref, float* out, void* stream);
+int vnlbhip_frame_cache_get_stats(const vnlbhip_frame_cache* cache, vnlbhip_frame_cache_stats* out);
+void vnlbhip_frame_cache_destroy(vnlbhip_frame_cache* cache);
+
 /* The grid of one MVTools vector clip: the MVTools_MVAnalysisData fields
  * that MVToolsVectorGrid::nearest_vector reads (src/flow/flow.hpp:38-56,
  * 176-222; parsed by parse_mvtools_analysis, :196-258). */

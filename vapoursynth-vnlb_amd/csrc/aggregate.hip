@@ -9,6 +9,7 @@
 // time.  Normalisation: numerator / weight, or the source sample where no
 // weight landed.
 #include "common.h"
+#include "kernels.h"
 
 namespace vnlbhip {
 namespace {
@@ -80,6 +81,35 @@ __global__ __launch_bounds__(256) void pack_stack_kernel(StageDesc d, const T* _
         }
         stack[plane * 2 * pixels + pixel] = numerator;
         stack[(plane * 2 + 1) * pixels + pixel] = weight;
+    }
+}
+
+// vnlb.Aggregate of one output frame from anchor stacks
+// (render_aggregate_frame, src/vs/plugin.cpp:966-1045; aggregate_frame,
+// src/aggregate/aggregate.cpp:95-163): the channel-c numerators and the
+// weights at each contributing anchor's slot, summed in double, then
+// numerator / weight, or the source sample where no weight landed.
+__global__ __launch_bounds__(256) void render_frame_kernel(StageDesc d, StackSet set,
+                                                           const float* __restrict__ source,
+                                                           float* __restrict__ out) {
+    const size_t pixels = static_cast<size_t>(d.width) * d.height;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t pixel = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; pixel < pixels;
+         pixel += stride) {
+        double weight = 0.0;
+        for (int k = 0; k < set.count; ++k) {
+            weight += set.stack[k][(static_cast<size_t>(set.slot[k]) * 2 + 1) * pixels + pixel];
+        }
+        const float w = static_cast<float>(weight);
+        for (int c = 0; c < d.channels; ++c) {
+            double numerator = 0.0;
+            for (int k = 0; k < set.count; ++k) {
+                const size_t plane = static_cast<size_t>(c) * d.slots + set.slot[k];
+                numerator += set.stack[k][plane * 2 * pixels + pixel];
+            }
+            const size_t index = static_cast<size_t>(c) * pixels + pixel;
+            out[index] = w > 0.0F ? static_cast<float>(numerator) / w : source[index];
+        }
     }
 }
 
@@ -173,6 +203,14 @@ hipError_t launch_pack_stack(const StageDesc& d, const double* acc, int first_fr
 hipError_t launch_pack_stack(const StageDesc& d, const float* acc, int first_frame, int slots,
                              float* stack, hipStream_t stream) {
     return launch_pack(d, acc, first_frame, slots, stack, stream);
+}
+
+hipError_t launch_render_frame(const StageDesc& d, const StackSet& set, const float* source, float* out,
+                               hipStream_t stream) {
+    const size_t pixels = static_cast<size_t>(d.width) * d.height;
+    hipLaunchKernelGGL(render_frame_kernel, dim3(grid_for(pixels)), dim3(256), 0, stream, d, set, source,
+                       out);
+    return hipGetLastError();
 }
 
 hipError_t launch_normalize(const StageDesc& d, const double* acc, const float* source,

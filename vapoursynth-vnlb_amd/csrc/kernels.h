@@ -12,6 +12,19 @@ namespace vnlbhip {
 // NULL and every window centred on the anchor.
 size_t match_shared_bytes(const StageDesc& d, bool guided = false);
 
+// Anchor stacks that contribute to one output frame: stack k's slot[k]
+// holds that frame (render_frame_kernel).
+constexpr int kMaxStackSet = 32;
+struct StackSet {
+    const float* stack[kMaxStackSet];
+    int slot[kMaxStackSet];
+    int count;
+};
+// One output frame [C][H][W] from the stacks; source: that frame of the
+// clip (fallback where no weight landed).
+hipError_t launch_render_frame(const StageDesc& d, const StackSet& set, const float* source, float* out,
+                               hipStream_t stream);
+
 // The grid fields of an MVTools vector clip (flow.hip).
 struct MvShape {
     int block_size_x;

@@ -25,6 +25,7 @@
 #include <stdexcept>
 #include <string>
 #include <map>
+#include <memory>
 #include <vector>
 
 namespace {
@@ -1474,6 +1475,154 @@ int vnlbhip_anchor_stack(vnlbhip_context* context, int stage, const vnlbhip_para
                   Geometry{width, height, frames, channels},
                   reinterpret_cast<hipStream_t>(stream), anchor_frame, nullptr, stack);
     });
+}
+
+// Per-output-frame rendering over a device cache of anchor stacks (the
+// frame-request front end of the VapourSynth plugins: vnlb.Basic /
+// vnlb.Final produce one anchor stack per frame, src/vs/plugin.cpp:616-758,
+// and vnlb.Aggregate sums the stacks of the anchors around an output frame,
+// :966-1045).  Each anchor frame's stack is computed once while it stays
+// cached; the least recently used stack outside the current output frame's
+// anchors is replaced.
+struct vnlbhip_frame_cache {
+    vnlbhip_context* ctx = nullptr;
+    int stage = 0;
+    vnlbhip_params params{};
+    Geometry g{};
+    StageDesc d{};
+    size_t stack_floats = 0;
+    std::vector<float*> stacks;
+    std::vector<int> anchor;          // -1: empty
+    std::vector<uint64_t> used;
+    uint64_t tick = 0;
+    const float* noisy = nullptr;     // clip the cached stacks belong to
+    const float* ref = nullptr;
+    int form = -1;
+    vnlbhip_frame_cache_stats stats{};
+
+    ~vnlbhip_frame_cache() {
+        if (!stacks.empty()) {
+            (void)hipSetDevice(ctx->device);
+            (void)hipDeviceSynchronize();
+            for (float* p : stacks) {
+                (void)hipFree(p);
+            }
+        }
+    }
+};
+
+int vnlbhip_frame_cache_create(vnlbhip_context* context, int stage, const vnlbhip_params* params,
+                               int width, int height, int frames, int channels, int capacity,
+                               vnlbhip_frame_cache** out) {
+    if (context == nullptr || out == nullptr) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    *out = nullptr;
+    return guarded(context, [&]() {
+        require(params != nullptr, "params must not be null");
+        const Geometry g{width, height, frames, channels};
+        const StageDesc d = make_desc(stage, *params, g);
+        const int needed = d.search_bwd + d.search_fwd + d.patch_time;
+        require(capacity >= needed, "frame cache capacity must cover search_bwd + search_fwd + patch_time stacks");
+        require(needed <= vnlbhip::kMaxStackSet, "more than 32 anchor stacks per output frame are not supported");
+        check(hipSetDevice(context->device), "hipSetDevice");
+        auto cache = std::make_unique<vnlbhip_frame_cache>();
+        cache->ctx = context;
+        cache->stage = stage;
+        cache->params = *params;
+        cache->g = g;
+        cache->d = d;
+        cache->stack_floats = static_cast<size_t>(width) * height * channels * d.slots * 2;
+        for (int i = 0; i < capacity; ++i) {
+            float* p = nullptr;
+            check(hipMalloc(reinterpret_cast<void**>(&p), cache->stack_floats * sizeof(float)),
+                  "hipMalloc frame cache stack");
+            cache->stacks.push_back(p);
+            cache->anchor.push_back(-1);
+            cache->used.push_back(0);
+        }
+        *out = cache.release();
+    });
+}
+
+int vnlbhip_frame_cache_render(vnlbhip_frame_cache* cache, int frame, const float* noisy, const float* ref,
+                               float* out, void* stream) {
+    if (cache == nullptr) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    vnlbhip_context* ctx = cache->ctx;
+    return guarded(ctx, [&]() {
+        const Geometry& g = cache->g;
+        const StageDesc& d = cache->d;
+        require(noisy != nullptr && out != nullptr, "noisy and out must not be null");
+        require(frame >= 0 && frame < g.frames, "frame must be a frame of the clip");
+        const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+        if (noisy != cache->noisy || ref != cache->ref || ctx->form != cache->form) {
+            // Another clip or arithmetic form: nothing cached applies.
+            std::fill(cache->anchor.begin(), cache->anchor.end(), -1);
+            cache->noisy = noisy;
+            cache->ref = ref;
+            cache->form = ctx->form;
+        }
+        // Anchors whose stacks hold this frame (anchors past frames - Pt
+        // have empty stacks).
+        const int first = std::max(0, frame - (d.search_fwd + d.patch_time - 1));
+        const int last = std::min(g.frames - d.patch_time, frame + d.search_bwd);
+        ++cache->tick;
+        vnlbhip::StackSet set{};
+        for (int a = first; a <= last; ++a) {
+            int slot = -1;
+            for (size_t i = 0; i < cache->anchor.size(); ++i) {
+                if (cache->anchor[i] == a) {
+                    slot = static_cast<int>(i);
+                    break;
+                }
+            }
+            if (slot >= 0) {
+                ++cache->stats.hits;
+            } else {
+                // Least recently used entry not holding this frame's anchors.
+                for (size_t i = 0; i < cache->anchor.size(); ++i) {
+                    if (cache->used[i] != cache->tick &&
+                        (slot < 0 || cache->anchor[i] < 0 ||
+                         (cache->anchor[slot] >= 0 && cache->used[i] < cache->used[slot]))) {
+                        slot = static_cast<int>(i);
+                        if (cache->anchor[i] < 0) {
+                            break;
+                        }
+                    }
+                }
+                require(slot >= 0, "frame cache has no free entry");
+                if (cache->anchor[slot] >= 0) {
+                    ++cache->stats.evictions;
+                }
+                cache->anchor[slot] = -1;
+                run_stage(*ctx, cache->stage, cache->params, noisy, ref, nullptr, g, s, a, nullptr,
+                          cache->stacks[slot]);
+                cache->anchor[slot] = a;
+                ++cache->stats.misses;
+            }
+            cache->used[slot] = cache->tick;
+            set.stack[set.count] = cache->stacks[slot];
+            set.slot[set.count] = frame - a + d.search_bwd;
+            ++set.count;
+        }
+        const size_t plane = static_cast<size_t>(g.width) * g.height * g.channels;
+        check(vnlbhip::launch_render_frame(d, set, noisy + static_cast<size_t>(frame) * plane, out, s),
+              "render frame kernel");
+    });
+}
+
+int vnlbhip_frame_cache_get_stats(const vnlbhip_frame_cache* cache, vnlbhip_frame_cache_stats* out) {
+    if (cache == nullptr || out == nullptr) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    *out = cache->stats;
+    return VNLBHIP_OK;
+}
+
+void vnlbhip_frame_cache_destroy(vnlbhip_frame_cache* cache) {
+    delete cache;
 }
 
 int vnlbhip_stage_host(vnlbhip_context* context, int stage, const vnlbhip_params* params,

@@ -176,6 +176,13 @@ class Context:
             self._raise(rc)
         return out
 
+    def frame_cache(self, stage: int, params: StageParameters, frames: int, channels: int, height: int,
+                    width: int, capacity: int | None = None) -> "FrameCache":
+        """A per-output-frame renderer over a device cache of `capacity`
+        anchor stacks (default: the search_bwd + search_fwd + patch_time
+        stacks one output frame needs)."""
+        return FrameCache(self, stage, params, frames, channels, height, width, capacity)
+
     def _check_offsets(self, frames: int, height: int, width: int) -> None:
         field = self._search_offsets
         if field is not None and (field.shape[0] != frames or tuple(field.shape[2:4]) != (height, width)):
@@ -315,6 +322,61 @@ class Context:
             "group_flags": flags[:n],
             "grams": grams[: n * gram_dim * gram_dim].reshape(n, gram_dim, gram_dim),
         }
+
+
+class FrameCache:
+    """Output frames one at a time, as a VapourSynth frame request makes them
+    (vnlb.Basic / vnlb.Final stacks aggregated by vnlb.Aggregate), with each
+    anchor frame's stack computed once while it stays in the device cache."""
+
+    def __init__(self, context: Context, stage: int, params: StageParameters, frames: int, channels: int,
+                 height: int, width: int, capacity: int | None = None):
+        self._context = context
+        self._lib = context._lib
+        self._shape = (frames, channels, height, width)
+        if capacity is None:
+            capacity = params.search_bwd + params.search_fwd + params.patch_time
+        self._handle = ctypes.c_void_p()
+        rc = self._lib.vnlbhip_frame_cache_create(context._handle, int(stage), ctypes.byref(_abi.to_struct(params)),
+                                                  int(width), int(height), int(frames), int(channels),
+                                                  int(capacity), ctypes.byref(self._handle))
+        if rc != 0:
+            context._raise(rc)
+
+    def render(self, frame: int, noisy, ref=None, out=None):
+        """Output frame `frame` [channels, height, width] of the clip
+        (a float32 GPU tensor [frames, channels, height, width]; the same
+        tensors on every call keep the cache)."""
+        import torch
+
+        frames, channels, height, width = self._shape
+        src = noisy if noisy.dim() == 4 else noisy.unsqueeze(1)
+        if tuple(src.shape) != self._shape or src.dtype != torch.float32 or not src.is_cuda:
+            raise VNLBError(f"VNLB HIP: the clip must be a float32 GPU tensor of shape {self._shape}")
+        src = src.contiguous()
+        guide = None
+        if ref is not None:
+            guide = (ref if ref.dim() == 4 else ref.unsqueeze(1)).contiguous()
+        if out is None:
+            out = torch.empty((channels, height, width), dtype=torch.float32, device=src.device)
+        stream = torch.cuda.current_stream(src.device).cuda_stream
+        rc = self._lib.vnlbhip_frame_cache_render(
+            self._handle, int(frame), ctypes.c_void_p(src.data_ptr()),
+            ctypes.c_void_p(guide.data_ptr()) if guide is not None else None,
+            ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream))
+        if rc != 0:
+            self._context._raise(rc)
+        return out
+
+    def stats(self) -> dict:
+        s = _abi.FrameCacheStats()
+        self._lib.vnlbhip_frame_cache_get_stats(self._handle, ctypes.byref(s))
+        return {"hits": s.hits, "misses": s.misses, "evictions": s.evictions}
+
+    def close(self) -> None:
+        if self._handle:
+            self._lib.vnlbhip_frame_cache_destroy(self._handle)
+            self._handle = ctypes.c_void_p()
 
 
 _local = threading.local()

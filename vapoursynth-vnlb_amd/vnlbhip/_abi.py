@@ -33,9 +33,17 @@ EXPORTED_SYMBOLS = (
     "vnlbhip_set_graph",
     "vnlbhip_set_search_offsets",
     "vnlbhip_mvtools_offsets",
+    "vnlbhip_frame_cache_create",
+    "vnlbhip_frame_cache_render",
+    "vnlbhip_frame_cache_get_stats",
+    "vnlbhip_frame_cache_destroy",
 )
 
 KERNEL_NAMES = ("match", "paste", "prepare", "jacobi", "filter", "normalize", "select", "energy")
+
+
+class FrameCacheStats(ctypes.Structure):
+    _fields_ = [("hits", ctypes.c_int64), ("misses", ctypes.c_int64), ("evictions", ctypes.c_int64)]
 
 
 class MvShape(ctypes.Structure):
@@ -167,6 +175,15 @@ def load() -> ctypes.CDLL:
     lib.vnlbhip_mvtools_offsets.argtypes = [vp, P(Params), mv, vp, vp, mv, vp, vp, vp, ctypes.c_int,
                                             ctypes.c_int, ctypes.c_int, vp]
     lib.vnlbhip_mvtools_offsets.restype = ctypes.c_int
+    ci = ctypes.c_int
+    lib.vnlbhip_frame_cache_create.argtypes = [vp, ci, P(Params), ci, ci, ci, ci, ci, P(vp)]
+    lib.vnlbhip_frame_cache_create.restype = ci
+    lib.vnlbhip_frame_cache_render.argtypes = [vp, ci, vp, vp, vp, vp]
+    lib.vnlbhip_frame_cache_render.restype = ci
+    lib.vnlbhip_frame_cache_get_stats.argtypes = [vp, P(FrameCacheStats)]
+    lib.vnlbhip_frame_cache_get_stats.restype = ci
+    lib.vnlbhip_frame_cache_destroy.argtypes = [vp]
+    lib.vnlbhip_frame_cache_destroy.restype = None
     lib.vnlbhip_get_kernel_times.argtypes = [vp, P(KernelTimes)]
     lib.vnlbhip_get_kernel_times.restype = ctypes.c_int
     _lib = lib
