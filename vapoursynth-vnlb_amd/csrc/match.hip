@@ -86,7 +86,7 @@ __host__ __device__ inline int region_floats(int window, int patch, int temporal
 // row so every loaded sample feeds 4 distances.  The per-candidate order of
 // additions is exactly that of the CPU scan: channel, frame offset, row,
 // column.
-template <int kP>
+template <int kP, bool kGuided>
 __global__ __launch_bounds__(kMatchThreads) void match_kernel(
     StageDesc d, const float* __restrict__ source, const uint32_t* __restrict__ anchors,
     int anchor_count, int* __restrict__ counts, uint32_t* __restrict__ positions,
@@ -148,7 +148,7 @@ __global__ __launch_bounds__(kMatchThreads) void match_kernel(
     // the anchor plus its offset clamped to the frame (the anchor frame keeps
     // the anchor); the window size does not depend on the centre, since
     // compute_spatial_range shifts it inside the frame.
-    const bool guided = offsets != nullptr;
+    constexpr bool guided = kGuided;
     if (guided) {
         for (int ti = threadIdx.x; ti < nt; ti += blockDim.x) {
             int cx = ax;
@@ -271,9 +271,15 @@ hipError_t launch_match(const StageDesc& d, const float* source,
     switch (d.patch) {
 #define VNLBHIP_MATCH_CASE(PV)                                                                  \
     case PV:                                                                                    \
-        hipLaunchKernelGGL(match_kernel<PV>, grid, block, shared, stream, d, source, anchors,  \
-                           anchor_count, counts, positions, distances, offsets,                \
-                           offsets_temporal);                                                   \
+        if (offsets != nullptr) {                                                               \
+            hipLaunchKernelGGL((match_kernel<PV, true>), grid, block, shared, stream, d, source, \
+                               anchors, anchor_count, counts, positions, distances, offsets,   \
+                               offsets_temporal);                                               \
+        } else {                                                                                \
+            hipLaunchKernelGGL((match_kernel<PV, false>), grid, block, shared, stream, d, source,\
+                               anchors, anchor_count, counts, positions, distances, offsets,   \
+                               offsets_temporal);                                               \
+        }                                                                                       \
         break;
         VNLBHIP_MATCH_CASE(1)
         VNLBHIP_MATCH_CASE(2)
