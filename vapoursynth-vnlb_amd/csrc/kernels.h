@@ -69,14 +69,20 @@ hipError_t launch_fast_select(const StageDesc& d, const float* staged, const uin
 
 size_t paste_shared_bytes(const StageDesc& d, int max_anchors);
 int paste_target_stride(const StageDesc& d);
+// Entries of the targets buffer for anchor_count anchors (rows plus the
+// padding the walk reads past the last row).
+size_t paste_targets_size(const StageDesc& d, int anchor_count);
 // frame_offsets holds frame_count + 1 clip-wide anchor indices (the range
 // may start past anchor 0); anchor_count is their span.  dep (64-bit, indexed
-// by clip-wide anchor) must be zero on entry over that span.
+// by clip-wide anchor) must be zero on entry over that span.  active holds
+// at least active_pad + 64 bytes: bytes from active_pad on are scratch for
+// the walk's stores past a frame's last anchor (active_pad >= every anchor
+// index of the batch).
 hipError_t launch_paste(const StageDesc& d, int frame_count, const int* frame_offsets,
                         const int* frame_ids, const int* row_tables, const int* counts,
                         const uint32_t* positions, unsigned char* active, int max_anchors,
                         int anchor_count, unsigned long long* dep, int* target_counts,
-                        int* targets, hipStream_t stream);
+                        int* targets, int active_pad, hipStream_t stream);
 // Appends the active anchors of [anchor_begin, anchor_begin + anchor_count)
 // to group_list at atomically reserved places of *group_count.
 hipError_t launch_compact(const unsigned char* active, int anchor_begin, int anchor_count,

@@ -138,13 +138,16 @@ __global__ __launch_bounds__(256) void paste_targets_kernel(
 // The serial raster walk, 64 anchors at a time: a scalar scan over the
 // chunk's predecessor masks makes the sequential decisions exactly, then the
 // survivors clear their targets beyond the chunk.  The inputs of the next
-// two chunks are in flight while the current one is decided.
+// kPasteDepth - 1 chunks are in flight while the current one is decided.
 constexpr int kPrefetchTargets = 24;
+constexpr int kPasteDepth = 4;
+static_assert(kPasteThreads == 64, "the paste walk is one wavefront per frame");
 
 __global__ __launch_bounds__(kPasteThreads) void paste_kernel(
     StageDesc d, const int* __restrict__ frame_offsets,
     const unsigned long long* __restrict__ dep, const int* __restrict__ target_counts,
-    const int* __restrict__ targets, int target_stride, unsigned char* __restrict__ active) {
+    const int* __restrict__ targets, int target_stride, unsigned char* __restrict__ active,
+    int active_pad) {
     extern __shared__ unsigned s_mask[];
     const int slot = blockIdx.x;
     const int first = frame_offsets[slot];
@@ -162,77 +165,92 @@ __global__ __launch_bounds__(kPasteThreads) void paste_kernel(
         int n;
         int t[kPrefetchTargets];
     };
+    // Every load and store of the walk issues unconditionally (indices
+    // clamped into the frame, rows read past their targets into the next row
+    // or the buffer's padding, stores past the frame into active's padding),
+    // so the wait for a chunk counts exactly the operations issued after it.
     const auto load = [&](int base, Chunk& c) {
         const int a = base + lane;
         const bool valid = a < count;
-        c.dep = valid ? dep[first + a] : 0ull;
-        c.n = valid ? target_counts[first + a] : 0;
-        const int4* list = reinterpret_cast<const int4*>(targets + static_cast<size_t>(first + (valid ? a : 0)) * target_stride);
+        const int at = first + (valid ? a : count - 1);
+        // Raw values: lanes past the frame are never read (decide uses lanes
+        // below n only), and a select here would wait for the load.
+        c.dep = dep[at];
+        c.n = target_counts[at];
+        const int4* list = reinterpret_cast<const int4*>(targets + static_cast<size_t>(at) * target_stride);
 #pragma unroll
         for (int h = 0; h < kPrefetchTargets / 4; ++h) {
-            int4 v = int4{-1, -1, -1, -1};
-            if (valid && 4 * h < target_stride) {
-                v = list[h];
-            }
+            const int4 v = list[h];
             c.t[4 * h + 0] = v.x;
             c.t[4 * h + 1] = v.y;
             c.t[4 * h + 2] = v.z;
             c.t[4 * h + 3] = v.w;
         }
     };
-    Chunk next1;
-    Chunk next2;
-    load(0, next1);
-    load(64, next2);
-    for (int base = 0; base < count; base += 64) {
-        const int n = count - base < 64 ? count - base : 64;
-        const int a = base + lane;
-        const unsigned long long my_dep = next1.dep;
-        const int my_n = next1.n;
-        int my_t[kPrefetchTargets];
+    // A ring of kPasteDepth chunks in flight: chunk c is decided from
+    // ring[c % kPasteDepth], which is then refilled with chunk
+    // c + kPasteDepth.  The ring is indexed by unrolled constants (registers,
+    // no copies), and the block is one wavefront, so the end of a chunk
+    // needs only a wave barrier: LDS operations of one wave complete in
+    // order, and no fence drains the loads in flight.
+    if (count <= 0) {
+        return;
+    }
+    Chunk ring[kPasteDepth];
 #pragma unroll
-        for (int i = 0; i < kPrefetchTargets; ++i) {
-            my_t[i] = next1.t[i];
-        }
-        next1 = next2;
-        if (base + 128 < count) {
-            load(base + 128, next2);
-        }
-
-        const bool start = lane < n && my_n >= 0 && ((s_mask[a >> 5] >> (a & 31)) & 1u) != 0;
+    for (int k = 0; k < kPasteDepth; ++k) {
+        load(64 * k, ring[k]);
+    }
+    const auto decide = [&](const Chunk& c, int base) {
+        const int n = min(max(count - base, 0), 64);
+        const int a = base + lane;
+        const bool start = lane < n && c.n >= 0 && ((s_mask[a >> 5] >> (a & 31)) & 1u) != 0;
         const unsigned long long start_bits = __ballot(start);
-        const unsigned dep_lo = static_cast<unsigned>(my_dep);
-        const unsigned dep_hi = static_cast<unsigned>(my_dep >> 32);
+        const unsigned dep_lo = static_cast<unsigned>(c.dep);
+        const unsigned dep_hi = static_cast<unsigned>(c.dep >> 32);
+        // In raster order over the chunk's candidates only (anchors whose
+        // bit was still set): one is processed unless an earlier processed
+        // anchor of the chunk clears it.
         unsigned long long active_bits = 0ull;
-        for (int i = 0; i < n; ++i) {
+        unsigned long long pending = start_bits;
+        while (pending != 0ull) {
+            const int i = __builtin_ctzll(pending);
+            pending &= pending - 1ull;
             const unsigned long long dep_i =
                 (static_cast<unsigned long long>(
                      static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(dep_hi), i)))
                  << 32) |
                 static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(dep_lo), i));
-            if (((start_bits >> i) & 1ull) != 0 && (dep_i & active_bits) == 0) {
+            if ((dep_i & active_bits) == 0) {
                 active_bits |= 1ull << i;
             }
         }
         const bool is_active = ((active_bits >> lane) & 1ull) != 0;
-        if (lane < n) {
-            active[first + a] = is_active ? 1 : 0;
-        }
+        active[lane < n ? first + a : active_pad + lane] = is_active ? 1 : 0;
         if (is_active) {
 #pragma unroll
             for (int i = 0; i < kPrefetchTargets; ++i) {
-                if (i < my_n) {
-                    const int t = my_t[i];
+                if (i < c.n) {
+                    const int t = c.t[i];
                     atomicAnd(&s_mask[t >> 5], ~(1u << (t & 31)));
                 }
             }
             const int* list = targets + static_cast<size_t>(first + a) * target_stride;
-            for (int i = kPrefetchTargets; i < my_n; ++i) {
+            for (int i = kPrefetchTargets; i < c.n; ++i) {
                 const int t = list[i];
                 atomicAnd(&s_mask[t >> 5], ~(1u << (t & 31)));
             }
         }
-        __syncthreads();
+        __builtin_amdgcn_wave_barrier();
+    };
+    for (int base = 0; base < count; base += 64 * kPasteDepth) {
+#pragma unroll
+        for (int k = 0; k < kPasteDepth; ++k) {
+            // Chunks past the frame decide nothing (n = 0).
+            const int chunk = base + 64 * k;
+            decide(ring[k], chunk);
+            load(chunk + 64 * kPasteDepth, ring[k]);
+        }
     }
 }
 
@@ -269,11 +287,17 @@ int paste_target_stride(const StageDesc& d) {
     return (5 * d.cap + 3) / 4 * 4;  // rows of 16-byte multiples
 }
 
+size_t paste_targets_size(const StageDesc& d, int anchor_count) {
+    // The walk reads kPrefetchTargets entries of every row, the last row's
+    // past the end of the rows.
+    return static_cast<size_t>(anchor_count) * paste_target_stride(d) + kPrefetchTargets;
+}
+
 hipError_t launch_paste(const StageDesc& d, int frame_count, const int* frame_offsets,
                         const int* frame_ids, const int* row_tables, const int* counts,
                         const uint32_t* positions, unsigned char* active, int max_anchors,
                         int anchor_count, unsigned long long* dep, int* target_counts,
-                        int* targets, hipStream_t stream) {
+                        int* targets, int active_pad, hipStream_t stream) {
     if (frame_count <= 0 || anchor_count <= 0) {
         return hipSuccess;
     }
@@ -287,7 +311,7 @@ hipError_t launch_paste(const StageDesc& d, int frame_count, const int* frame_of
     }
     hipLaunchKernelGGL(paste_kernel, dim3(static_cast<unsigned>(frame_count)), dim3(kPasteThreads),
                        paste_shared_bytes(d, max_anchors), stream, d, frame_offsets, dep,
-                       target_counts, targets, stride, active);
+                       target_counts, targets, stride, active, active_pad);
     return hipGetLastError();
 }
 

@@ -735,13 +735,13 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
         int* d_counts = ctx.counts.reserve(anchor_count);
         uint32_t* d_positions = ctx.positions.reserve(static_cast<size_t>(anchor_count) * cap);
         float* d_distances = ctx.distances.reserve(static_cast<size_t>(anchor_count) * cap);
-        unsigned char* d_active = ctx.active.reserve(anchor_count);
+        // 64 bytes past the batch's anchors: scratch of the paste walk.
+        unsigned char* d_active = ctx.active.reserve(static_cast<size_t>(anchor_count) + 64);
         int* d_list = ctx.group_list.reserve(anchor_count);
         int* d_group_count = ctx.group_count.reserve(kMaxParts);
         unsigned long long* d_dep = ctx.paste_dep.reserve(anchor_count);
         int* d_tcount = ctx.paste_target_counts.reserve(anchor_count);
-        int* d_targets = ctx.paste_targets.reserve(static_cast<size_t>(anchor_count) *
-                                                   vnlbhip::paste_target_stride(d));
+        int* d_targets = ctx.paste_targets.reserve(vnlbhip::paste_targets_size(d, anchor_count));
         check(vnlbhip::launch_fill_zero(reinterpret_cast<double*>(d_dep), anchor_count, stream),
               "clear paste masks");
         check(vnlbhip::launch_fill_zero_int(d_group_count, kMaxParts, stream), "clear group counts");
@@ -874,7 +874,7 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
                 return vnlbhip::launch_paste(d, fe - fb, d_offsets + fb, d_ids + fb,
                                              d_rows + static_cast<size_t>(fb) * 3 * g.height, d_counts,
                                              d_positions, d_active, max_anchors, ae - ab, d_dep, d_tcount,
-                                             d_targets, ctx.side);
+                                             d_targets, anchor_count, ctx.side);
             });
             check(vnlbhip::launch_compact(d_active, ab, ae - ab, d_list + ab, d_group_count + h, ctx.side),
                   "compact kernel");
