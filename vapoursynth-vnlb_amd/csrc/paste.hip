@@ -135,8 +135,8 @@ __global__ __launch_bounds__(256) void paste_targets_kernel(
     target_counts[a] = similar > 0 ? n : -1;
 }
 
-// The serial raster walk, 64 anchors at a time: a scalar scan over the
-// chunk's predecessor masks makes the sequential decisions exactly, then the
+// The serial raster walk, 64 anchors at a time: wave-wide rounds over the
+// chunk's predecessor masks make the sequential decisions exactly, then the
 // survivors clear their targets beyond the chunk.  The inputs of the next
 // kPasteDepth - 1 chunks are in flight while the current one is decided.
 constexpr int kPrefetchTargets = 24;
@@ -206,24 +206,25 @@ __global__ __launch_bounds__(kPasteThreads) void paste_kernel(
         const int a = base + lane;
         const bool start = lane < n && c.n >= 0 && ((s_mask[a >> 5] >> (a & 31)) & 1u) != 0;
         const unsigned long long start_bits = __ballot(start);
-        const unsigned dep_lo = static_cast<unsigned>(c.dep);
-        const unsigned dep_hi = static_cast<unsigned>(c.dep >> 32);
-        // In raster order over the chunk's candidates only (anchors whose
-        // bit was still set): one is processed unless an earlier processed
-        // anchor of the chunk clears it.
+        // The serial rule -- a candidate (bit still set) is processed unless
+        // an earlier processed anchor of the chunk clears it -- resolved in
+        // rounds across the wave: a candidate is dropped once an earlier
+        // processed one clears it, and processed once every earlier
+        // candidate clearing it is decided and none was processed.  Each
+        // round decides at least the lowest undecided candidate, so this
+        // equals the raster-order walk; rounds follow the longest chain of
+        // clears inside the chunk.
+        const unsigned long long preds = c.dep & start_bits;
         unsigned long long active_bits = 0ull;
-        unsigned long long pending = start_bits;
-        while (pending != 0ull) {
-            const int i = __builtin_ctzll(pending);
-            pending &= pending - 1ull;
-            const unsigned long long dep_i =
-                (static_cast<unsigned long long>(
-                     static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(dep_hi), i)))
-                 << 32) |
-                static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(dep_lo), i));
-            if ((dep_i & active_bits) == 0) {
-                active_bits |= 1ull << i;
-            }
+        unsigned long long undecided = start_bits;
+        while (undecided != 0ull) {
+            const bool mine = ((undecided >> lane) & 1ull) != 0;
+            const bool drop = mine && (preds & active_bits) != 0;
+            const bool take = mine && !drop && (preds & undecided) == 0;
+            const unsigned long long taken = __ballot(take);
+            const unsigned long long dropped = __ballot(drop);
+            active_bits |= taken;
+            undecided &= ~(taken | dropped);
         }
         const bool is_active = ((active_bits >> lane) & 1ull) != 0;
         active[lane < n ? first + a : active_pad + lane] = is_active ? 1 : 0;
