@@ -98,3 +98,16 @@ def test_fast_form_and_errors():
             cache.close()
     finally:
         ctx.close()
+
+
+def test_closing_the_context_frees_its_caches():
+    noisy, _, bp, _ = workloads.make("gray_pt2_step4")
+    x = _dev(noisy)
+    frames, channels, height, width = x.shape
+    ctx = vnlbhip.Context(0)
+    cache = ctx.frame_cache(BASIC, bp, frames, channels, height, width)
+    first = cache.render(1, x).cpu().numpy()
+    assert np.isfinite(first).all()
+    ctx.close()
+    assert not cache._handle
+    cache.close()  # already freed with its context: a no-op

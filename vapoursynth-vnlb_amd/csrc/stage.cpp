@@ -1486,6 +1486,7 @@ int vnlbhip_anchor_stack(vnlbhip_context* context, int stage, const vnlbhip_para
 // anchors is replaced.
 struct vnlbhip_frame_cache {
     vnlbhip_context* ctx = nullptr;
+    int device = 0;                   // freeing needs no context
     int stage = 0;
     vnlbhip_params params{};
     Geometry g{};
@@ -1502,7 +1503,7 @@ struct vnlbhip_frame_cache {
 
     ~vnlbhip_frame_cache() {
         if (!stacks.empty()) {
-            (void)hipSetDevice(ctx->device);
+            (void)hipSetDevice(device);
             (void)hipDeviceSynchronize();
             for (float* p : stacks) {
                 (void)hipFree(p);
@@ -1528,6 +1529,7 @@ int vnlbhip_frame_cache_create(vnlbhip_context* context, int stage, const vnlbhi
         check(hipSetDevice(context->device), "hipSetDevice");
         auto cache = std::make_unique<vnlbhip_frame_cache>();
         cache->ctx = context;
+        cache->device = context->device;
         cache->stage = stage;
         cache->params = *params;
         cache->g = g;

@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import ctypes
 import threading
+import weakref
 
 import numpy as np
 
@@ -51,9 +52,13 @@ class Context:
         self.device = int(device)
         self.form = "plain"
         self._search_offsets = None
+        self._caches = weakref.WeakSet()
 
     def close(self) -> None:
         if self._handle:
+            # Frame caches hold device stacks of this context: free them first.
+            for cache in list(getattr(self, "_caches", ())):
+                cache.close()
             self._lib.vnlbhip_destroy(self._handle)
             self._handle = ctypes.c_void_p()
 
@@ -181,7 +186,9 @@ class Context:
         """A per-output-frame renderer over a device cache of `capacity`
         anchor stacks (default: the search_bwd + search_fwd + patch_time
         stacks one output frame needs)."""
-        return FrameCache(self, stage, params, frames, channels, height, width, capacity)
+        cache = FrameCache(self, stage, params, frames, channels, height, width, capacity)
+        self._caches.add(cache)
+        return cache
 
     def _check_offsets(self, frames: int, height: int, width: int) -> None:
         field = self._search_offsets
@@ -377,6 +384,12 @@ class FrameCache:
         if self._handle:
             self._lib.vnlbhip_frame_cache_destroy(self._handle)
             self._handle = ctypes.c_void_p()
+
+    def __del__(self):  # pragma: no cover - best effort
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 _local = threading.local()
