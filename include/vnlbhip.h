@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define VNLBHIP_ABI_VERSION 5
+#define VNLBHIP_ABI_VERSION 6
 
 enum {
     VNLBHIP_STAGE_BASIC = 0,
@@ -76,6 +76,7 @@ typedef struct vnlbhip_stats {
     int64_t retained;             /* retained samples summed over the groups */
     int64_t refined;              /* FAST form: eigen problems re-solved in fp64 */
     int64_t graph_replays;        /* stage graphs replayed by the context so far */
+    int64_t reallocations;        /* device buffers the last call (re)allocated */
 } vnlbhip_stats;
 
 typedef struct vnlbhip_context vnlbhip_context;
@@ -92,10 +93,21 @@ int vnlbhip_validate(int stage, const vnlbhip_params* params, int width,
                      size_t message_size);
 
 /* One context per stream/thread (the reference's reserve/enqueue objects are
- * likewise not thread-safe, src/cuda/pipeline.hpp:82-126). */
+ * likewise not thread-safe, src/cuda/pipeline.hpp:82-126).  Contexts share
+ * nothing: each owns its device buffers, plans, graphs and streams. */
 int vnlbhip_create(int device, vnlbhip_context** out);
 void vnlbhip_destroy(vnlbhip_context* context);
 const char* vnlbhip_last_error(const vnlbhip_context* context);
+
+/* Sizes every device buffer a vnlbhip_stage call of this stage, parameter
+ * set, geometry and the context's current form needs, launching nothing
+ * (StagePipeline::reserve, src/cuda/pipeline.hpp:82-107).  Later stage,
+ * stack and debug calls of that shape or smaller then neither allocate nor
+ * synchronise the device for buffers (vnlbhip_stats.reallocations stays 0);
+ * without it the first call of a larger shape allocates and waits for the
+ * device.  Blocks until done. */
+int vnlbhip_reserve(vnlbhip_context* context, int stage, const vnlbhip_params* params, int width,
+                    int height, int frames, int channels);
 
 /* Whole-clip stage on device memory: every anchor frame's matching, paste
  * mask, group model and weighted aggregation, then normalisation with the
@@ -122,6 +134,23 @@ int vnlbhip_anchor_stack(vnlbhip_context* context, int stage,
                          const float* ref, int width, int height, int frames,
                          int channels, int anchor_frame, float* stack,
                          void* stream);
+
+/* The same over a window of a longer clip, as the reference's frame
+ * requests run it (render_stage_frame, src/vs/plugin.cpp:616-758, with
+ * make_geometry(vi, range.first, local_frames) and VideoGeometry's
+ * first_frame / total_frames, src/core/video.hpp:19-47): noisy / ref hold
+ * `frames` frames starting at absolute frame first_frame of a clip of
+ * total_frames; anchor_frame is absolute.  Grid phases and temporal ranges
+ * follow the absolute indices and the clip length.  The window must hold
+ * the anchor frame's input frames (input_frame_range_for_anchor,
+ * src/core/core.cpp:2026-2034), else VNLBHIP_ERROR_INVALID; an anchor past
+ * total_frames - patch_time gives a zero stack.  At most 256 frames per
+ * window; the clip may be of any length. */
+int vnlbhip_anchor_stack_window(vnlbhip_context* context, int stage,
+                                const vnlbhip_params* params, const float* noisy,
+                                const float* ref, int width, int height, int frames,
+                                int channels, int first_frame, int total_frames,
+                                int anchor_frame, float* stack, void* stream);
 
 /* Same with host buffers: copies in, runs, copies out. */
 int vnlbhip_stage_host(vnlbhip_context* context, int stage,
@@ -234,10 +263,13 @@ int vnlbhip_set_autotune(vnlbhip_context* context, int enable);
  * centred on (x + dx, y + dy) clamped to the frame and then shifted inside
  * it like an unguided window; ti = 0 keeps the anchor.  Guided stages run
  * the plain matcher and filter in either form.  NULL (default) clears the
- * field.  The array must stay valid while it is set; temporal <= 0 with a
- * non-NULL array is rejected with VNLBHIP_ERROR_INVALID, and a stage whose
- * search window does not match temporal fails with VNLBHIP_ERROR_INVALID. */
-int vnlbhip_set_search_offsets(vnlbhip_context* context, const int16_t* offsets, int temporal);
+ * field.  The array must stay valid while it is set; frames, height and
+ * width are the shape of the clip it was made for.  Non-positive sizes with
+ * a non-NULL array are rejected with VNLBHIP_ERROR_INVALID, and a stage
+ * whose search window or clip does not match the field fails with
+ * VNLBHIP_ERROR_INVALID before any kernel reads it. */
+int vnlbhip_set_search_offsets(vnlbhip_context* context, const int16_t* offsets, int temporal,
+                               int frames, int height, int width);
 
 /* Per-output-frame rendering, the frame-request model of the VapourSynth
  * plugins (vnlb.Basic / vnlb.Final produce one anchor frame's contribution
@@ -249,12 +281,25 @@ int vnlbhip_set_search_offsets(vnlbhip_context* context, const int16_t* offsets,
  * floats); rendering output frame n computes the missing stacks of anchors
  * [n - search_fwd - patch_time + 1, n + search_bwd] (replacing the least
  * recently used stacks of other anchors) and aggregates them into out
- * ([channels][height][width], device).  noisy / ref are the whole clip on
- * the device, as for vnlbhip_stage; calls with other clip pointers or after
- * vnlbhip_set_form changed the form start from an empty cache.  capacity
- * must be at least search_bwd + search_fwd + patch_time (at most 32 stacks
- * contribute to a frame).  Rendering every frame in order computes each
- * anchor stack once.  The cache belongs to its context (destroy it first). */
+ * ([channels][height][width], device).  `frames` at creation is the
+ * clip's length.  vnlbhip_frame_cache_render takes the whole clip on the
+ * device, as vnlbhip_stage does (at most 256 frames); calls with other clip
+ * pointers start from an empty cache.  vnlbhip_frame_cache_render_window
+ * takes a window of the clip instead: noisy / ref hold window_frames frames
+ * from absolute frame first_frame, which must include frame n and the
+ * input frames of every anchor stack the call computes (a clip of any
+ * length; cached stacks are keyed by frame number, so a frame's content
+ * must not change while its stacks are cached -- VapourSynth frames do not;
+ * vnlbhip_frame_cache_invalidate drops every stack).  A change of the
+ * context's form, eigen re-solve gap, selection placement or search offsets
+ * also starts from an empty cache.  capacity must be at least search_bwd +
+ * search_fwd + patch_time (at most 32 stacks contribute to a frame).
+ * Rendering every frame in order computes each anchor stack once.  Renders
+ * of one cache are serialised by a mutex inside it, but its context must
+ * not be used by another thread meanwhile.  A render returns before its
+ * kernel ends; a later render on another stream that replaces a stack waits
+ * for the renders that read it.  The cache belongs to its context (destroy
+ * it first). */
 typedef struct vnlbhip_frame_cache vnlbhip_frame_cache;
 
 typedef struct vnlbhip_frame_cache_stats {
@@ -268,6 +313,10 @@ int vnlbhip_frame_cache_create(vnlbhip_context* context, int stage, const vnlbhi
                                vnlbhip_frame_cache** out);
 int vnlbhip_frame_cache_render(vnlbhip_frame_cache* cache, int frame, const float* noisy,
                                const float* ref, float* out, void* stream);
+int vnlbhip_frame_cache_render_window(vnlbhip_frame_cache* cache, int frame, const float* noisy,
+                                      const float* ref, int first_frame, int window_frames,
+                                      float* out, void* stream);
+int vnlbhip_frame_cache_invalidate(vnlbhip_frame_cache* cache);
 int vnlbhip_frame_cache_get_stats(const vnlbhip_frame_cache* cache, vnlbhip_frame_cache_stats* out);
 void vnlbhip_frame_cache_destroy(vnlbhip_frame_cache* cache);
 

@@ -125,6 +125,8 @@ def load() -> ctypes.CDLL:
                                        fp, fp, ip, ctypes.c_int, ctypes.c_int, dp]
         lib.vo_anchor_stack.argtypes = [ctypes.c_int, P(VoParams), fp, fp, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, fp, ctypes.c_int]
+        lib.vo_anchor_stack_window.argtypes = [ctypes.c_int, P(VoParams), fp, fp] + [ctypes.c_int] * 7 + [
+            fp, ctypes.c_int]
         lib.vo_set_search_offsets.argtypes = [P(ctypes.c_int16), ctypes.c_int]
         lib.vo_set_search_offsets.restype = None
         mv = P(MvShape)
@@ -202,6 +204,25 @@ def anchor_stack(stage_id: int, params, noisy, ref, anchor_frame: int, flags: in
     packed = np.empty((c, slots * 2 * h, w), np.float32)
     rc = lib.vo_anchor_stack(stage_id, ctypes.byref(p), _fp(src), _fp(guide), w, h, f, c,
                              anchor_frame, _fp(packed), flags)
+    if rc != 0:
+        raise ValueError(lib.vo_last_error().decode())
+    return packed
+
+
+def anchor_stack_window(stage_id: int, params, noisy, ref, first_frame: int, total_frames: int,
+                        anchor_frame: int, flags: int = ACCUM_DOUBLE):
+    """anchor_stack over a window: noisy / ref hold the frames [first_frame,
+    first_frame + len(noisy)) of a clip of total_frames; anchor_frame is
+    absolute."""
+    lib = load()
+    src, (w, h, f, c) = _video(noisy)
+    guide = _video(ref)[0] if ref is not None else None
+    p = to_struct(params)
+    slots = int(p.search_bwd) + int(p.search_fwd) + int(p.patch_time)
+    packed = np.empty((c, slots * 2 * h, w), np.float32)
+    rc = lib.vo_anchor_stack_window(stage_id, ctypes.byref(p), _fp(src), _fp(guide), w, h, f, c,
+                                    int(first_frame), int(total_frames), int(anchor_frame), _fp(packed),
+                                    flags)
     if rc != 0:
         raise ValueError(lib.vo_last_error().decode())
     return packed

@@ -395,11 +395,18 @@ void dual_basis(const T* centered, int rows, int cols, const T* values,
 // Stage definitions (src/core/core.cpp).
 // ---------------------------------------------------------------------------
 
+// VideoGeometry (src/core/video.hpp:19-47): `frames` frames in memory, the
+// window [first_frame, first_frame + frames) of a clip of total_frames (0:
+// the window is the clip).  Frame arguments below are absolute indices.
 struct Geometry {
     int width = 0;
     int height = 0;
     int frames = 0;
     int channels = 0;
+    int first_frame = 0;
+    int total_frames = 0;
+    int source_frames() const { return total_frames > 0 ? total_frames : frames; }
+    std::size_t local(int frame) const { return static_cast<std::size_t>(frame - first_frame); }
 };
 
 // soft_similar_cap / model_similar_cap (core.cpp:139-172).
@@ -476,9 +483,12 @@ void validate(const vo_params& p, const Geometry& g) {
     require(p.proc_step >= 0, "block_step must be non-negative; use 0 for auto");
     require(g.width > 0 && g.height > 0 && g.frames > 0 && g.channels > 0,
             "video geometry must be non-empty");
+    require(g.first_frame >= 0, "first frame must be non-negative");
+    require(static_cast<long long>(g.source_frames()) >= static_cast<long long>(g.frames) + g.first_frame,
+            "local frame window must fit inside the source frame count");
     require(p.patch_size <= g.width && p.patch_size <= g.height,
             "block_size must fit inside the frame");
-    require(p.patch_time <= g.frames, "patch_time must fit inside the clip");
+    require(p.patch_time <= g.source_frames(), "patch_time must fit inside the clip");
     const long long estimator_dim =
         static_cast<long long>(p.patch_size) * p.patch_size * p.patch_time *
         ((p.couple_channels != 0) ? g.channels : 1);
@@ -493,12 +503,12 @@ void validate(const vo_params& p, const Geometry& g) {
 // compute_temporal_range (core.cpp:873-896).
 void temporal_range(const vo_params& p, const Geometry& g, int anchor, int& low,
                     int& high) {
-    const int max_origin = g.frames - p.patch_time;
-    const int start = anchor - p.search_bwd;
-    const int end = anchor + p.search_fwd;
-    const int shift = std::min(0, start) + std::max(0, end - max_origin);
-    low = std::max(0, start - shift);
-    high = std::min(max_origin, end - shift);
+    const int max_origin = g.source_frames() - p.patch_time;
+    const long long start = static_cast<long long>(anchor) - p.search_bwd;
+    const long long end = static_cast<long long>(anchor) + p.search_fwd;
+    const long long shift = std::min(0LL, start) + std::max(0LL, end - max_origin);
+    low = static_cast<int>(std::max(0LL, start - shift));
+    high = static_cast<int>(std::min<long long>(max_origin, end - shift));
 }
 
 // schedule_temporal_range (core.cpp:898-908): anchor, future, past.
@@ -554,7 +564,7 @@ bool grid_selected(const vo_params& p, const Geometry& g, int anchor_frame,
     const int step = normalized_proc_step(p);
     const int max_x = g.width - p.patch_size;
     const int max_y = g.height - p.patch_size;
-    const int max_frame = g.frames - p.patch_time;
+    const int max_frame = g.source_frames() - p.patch_time;
     const int phase_y = anchor_frame == max_frame ? 0 : anchor_frame;
     if (y != 0 && y != max_y && y % step != phase_y % step) {
         return false;
@@ -617,7 +627,7 @@ struct StageSetup {
     int flags = 0;
 
     float sample(const float* video, int x, int y, int f, int c) const {
-        return video[((static_cast<std::size_t>(f) * g.channels + c) * g.height +
+        return video[((g.local(f) * g.channels + c) * g.height +
                       y) *
                          g.width +
                      x];
@@ -716,7 +726,7 @@ void enumerate_candidates(const StageSetup& s, int ax, int ay, int af,
                                 (t * P + y) * P;
                             const float* candidate_row =
                                 source +
-                                ((static_cast<std::size_t>(frame + t) *
+                                ((s.g.local(frame + t) *
                                       s.g.channels +
                                   c) *
                                      s.g.height +
@@ -880,7 +890,7 @@ int select_faithful(const StageSetup& s, int ax, int ay, int af,
                                 static_cast<std::size_t>(c) * s.patch_dim + (t * P + y) * P;
                             const float* candidate_row =
                                 source +
-                                ((static_cast<std::size_t>(frame + t) * s.g.channels + c) *
+                                ((s.g.local(frame + t) * s.g.channels + c) *
                                      s.g.height +
                                  cy + y) *
                                     s.g.width +
@@ -1297,11 +1307,20 @@ long long process_anchor_frame(const StageSetup& s, int anchor_frame, Stack<Acc>
     const Geometry& g = s.g;
     const int P = p.patch_size;
     stack.reset(s.slots, g.channels, g.width, g.height);  // clear_contributions
-    if (anchor_frame < 0 || anchor_frame > g.frames - p.patch_time) {
+    if (anchor_frame < 0 || anchor_frame > g.source_frames() - p.patch_time) {
         if (debug != nullptr) {
             debug->count = 0;
         }
         return 0;
+    }
+    {
+        // The window must hold the anchor's input frames
+        // (input_frame_range_for_anchor, core.cpp:2026-2034).
+        int low = 0;
+        int high = 0;
+        temporal_range(p, g, anchor_frame, low, high);
+        require(low >= g.first_frame && high + p.patch_time - 1 < g.first_frame + g.frames,
+                "the frame window does not hold the anchor frame's input frames");
     }
     const int max_x = g.width - P;
     const int max_y = g.height - P;
@@ -1825,11 +1844,19 @@ int vo_mvtools_offsets(const vo_params* params, const vo_mv_shape* fw, const int
 int vo_anchor_stack(int stage, const vo_params* params, const float* noisy,
                     const float* ref, int width, int height, int frames,
                     int channels, int anchor_frame, float* packed, int flags) {
+    return vo_anchor_stack_window(stage, params, noisy, ref, width, height, frames, channels, 0,
+                                  frames, anchor_frame, packed, flags);
+}
+
+int vo_anchor_stack_window(int stage, const vo_params* params, const float* noisy,
+                           const float* ref, int width, int height, int frames,
+                           int channels, int first_frame, int total_frames, int anchor_frame,
+                           float* packed, int flags) {
     try {
         require(params != nullptr && packed != nullptr, "null argument");
         require(stage == VO_STAGE_BASIC || stage == VO_STAGE_FINAL, "bad stage");
-        require(anchor_frame >= 0 && anchor_frame < frames, "anchor frame out of range");
-        const Geometry g{width, height, frames, channels};
+        require(anchor_frame >= 0 && anchor_frame < total_frames, "anchor frame out of range");
+        const Geometry g{width, height, frames, channels, first_frame, total_frames};
         const StageSetup s = make_setup(stage, *params, noisy, ref, g, flags);
         const auto pack = [&](auto& stack) {
             const std::size_t pixels = static_cast<std::size_t>(width) * height;

@@ -150,6 +150,14 @@ int vo_anchor_stack(int stage, const vo_params* params, const float* noisy,
                     const float* ref, int width, int height, int frames,
                     int channels, int anchor_frame, float* packed, int flags);
 
+/* The same over a window: noisy / ref hold `frames` frames from absolute
+ * frame first_frame of a clip of total_frames; anchor_frame is absolute
+ * (VideoGeometry::first_frame / total_frames, src/core/video.hpp:19-47). */
+int vo_anchor_stack_window(int stage, const vo_params* params, const float* noisy,
+                           const float* ref, int width, int height, int frames,
+                           int channels, int first_frame, int total_frames,
+                           int anchor_frame, float* packed, int flags);
+
 /* Retained group cap (max(K, ceil(K * cap_factor)) limited by the window). */
 int vo_retained_cap(const vo_params* params);
 

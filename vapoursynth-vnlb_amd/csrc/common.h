@@ -2,7 +2,8 @@
 //
 // Video layout: contiguous float32 [frames][channels][height][width].
 // Patch positions are packed as x | y << 12 | frame << 24 (width, height up to
-// 4096, at most 256 frames per stage call).
+// 4096, at most 256 frames in the device window of one stage call; the clip
+// itself may be longer, see StageDesc::first_frame).
 //
 // Floating-point order: every decision-relevant quantity follows the 16-lane
 // order the CPU restatement fixes for the reference's Highway kernels
@@ -40,8 +41,10 @@ struct StageDesc {
     int stage;
     int width;
     int height;
-    int frames;
+    int frames;         // frames held in device memory (the local window)
     int channels;
+    int first_frame;    // absolute clip index of local frame 0
+    int total_frames;   // frames of the whole clip (>= first_frame + frames)
     int patch;          // P
     int patch_time;     // Pt
     int patch_area;     // P^2
@@ -52,7 +55,7 @@ struct StageDesc {
     int model_dim;      // coupled ? dim : patch_dim
     int use_dual;       // model_dim >= 64 && model rows < model_dim
     int search_window;
-    int search_bwd;
+    int search_bwd;     // clamped to total_frames - Pt (same ranges, no overflow)
     int search_fwd;
     int similar;        // K
     int cap;            // retained cap (output stride)

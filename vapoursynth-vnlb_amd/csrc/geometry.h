@@ -5,14 +5,20 @@
 
 namespace vnlbhip {
 
-// compute_temporal_range (reference src/core/core.cpp:873-896).
+// compute_temporal_range (reference src/core/core.cpp:873-896) on a window
+// of the clip: `anchor`, `low` and `high` are local frame indices (absolute
+// index - first_frame); the range is shifted into the clip's patch origins
+// [0, total_frames - Pt] in absolute terms, as VideoGeometry::first_frame /
+// source_frames make the reference do (src/core/video.hpp:19-47).
 __host__ __device__ inline void temporal_range(const StageDesc& d, int anchor,
                                                int& low, int& high) {
-    const int max_origin = d.frames - d.patch_time;
+    const int min_origin = -d.first_frame;
+    const int max_origin = d.total_frames - d.patch_time - d.first_frame;
     const int start = anchor - d.search_bwd;
     const int end = anchor + d.search_fwd;
-    const int shift = (start < 0 ? start : 0) + (end - max_origin > 0 ? end - max_origin : 0);
-    low = start - shift > 0 ? start - shift : 0;
+    const int shift = (start < min_origin ? start - min_origin : 0) +
+                      (end - max_origin > 0 ? end - max_origin : 0);
+    low = start - shift > min_origin ? start - shift : min_origin;
     high = end - shift < max_origin ? end - shift : max_origin;
 }
 

@@ -26,6 +26,7 @@
 #include <string>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 namespace {
@@ -56,11 +57,17 @@ void check(hipError_t status, const char* what) {
     }
 }
 
+// A clip, or a window of it held in device memory: `frames` local frames
+// starting at absolute frame `first_frame` of a clip of `total_frames`
+// (VideoGeometry, src/core/video.hpp:19-47; 0 means the window is the clip).
 struct Geometry {
     int width = 0;
     int height = 0;
     int frames = 0;
     int channels = 0;
+    int first_frame = 0;
+    int total_frames = 0;
+    int source_frames() const { return total_frames > 0 ? total_frames : frames; }
 };
 
 // soft_similar_cap / model_similar_cap (src/core/core.cpp:139-172).
@@ -119,9 +126,13 @@ void validate_reference(const vnlbhip_params& p, const Geometry& g) {
     require(p.proc_step >= 0, "block_step must be non-negative; use 0 for auto");
     require(g.width > 0 && g.height > 0 && g.frames > 0 && g.channels > 0,
             "video geometry must be non-empty");
+    require(g.first_frame >= 0, "first frame must be non-negative");
+    require(g.total_frames >= 0 && static_cast<long long>(g.source_frames()) >=
+                                       static_cast<long long>(g.frames) + g.first_frame,
+            "local frame window must fit inside the source frame count");
     require(p.patch_size <= g.width && p.patch_size <= g.height,
             "block_size must fit inside the frame");
-    require(p.patch_time <= g.frames, "patch_time must fit inside the clip");
+    require(p.patch_time <= g.source_frames(), "patch_time must fit inside the clip");
     const long long estimator_dim = static_cast<long long>(p.patch_size) * p.patch_size *
                                     p.patch_time * (p.couple_channels != 0 ? g.channels : 1);
     require(estimator_dim <= INT_MAX, "estimator dimension overflows int");
@@ -169,6 +180,8 @@ StageDesc make_desc(int stage, const vnlbhip_params& p, const Geometry& g) {
     d.height = g.height;
     d.frames = g.frames;
     d.channels = g.channels;
+    d.first_frame = g.first_frame;
+    d.total_frames = g.source_frames();
     d.patch = p.patch_size;
     d.patch_time = p.patch_time;
     d.patch_area = p.patch_size * p.patch_size;
@@ -178,14 +191,18 @@ StageDesc make_desc(int stage, const vnlbhip_params& p, const Geometry& g) {
     d.models = d.coupled ? 1 : g.channels;
     d.model_dim = d.coupled ? d.dim : d.patch_dim;
     d.search_window = p.search_window;
-    d.search_bwd = p.search_bwd;
-    d.search_fwd = p.search_fwd;
+    // A radius beyond the clip's last patch origin gives the same temporal
+    // ranges as that origin (compute_temporal_range shifts the window back
+    // inside), so the kernels see radii that cannot overflow.
+    const int max_origin = g.source_frames() - p.patch_time;
+    d.search_bwd = std::min(p.search_bwd, max_origin);
+    d.search_fwd = std::min(p.search_fwd, max_origin);
     d.similar = p.similar;
     d.cap = retained_cap(p);
     d.soft_cap = capped_similar(p.similar, p.similar_cap_factor);
     d.model_cap = capped_similar(p.similar, p.model_cap_factor);
     d.rank = p.rank;
-    d.slots = p.search_bwd + p.search_fwd + p.patch_time;
+    d.slots = d.search_bwd + d.search_fwd + p.patch_time;
     d.step = p.proc_step > 0 ? p.proc_step : (p.patch_size <= 1 ? 1 : p.patch_size);
     d.step_magic = (1u << 18) / static_cast<uint32_t>(d.step) + 1u;
     require(d.step <= 64, "block_step above 64 is not supported");
@@ -214,7 +231,8 @@ StageDesc make_desc(int stage, const vnlbhip_params& p, const Geometry& g) {
     // Kernel limits of this build.
     require(g.width <= vnlbhip::kMaxSide && g.height <= vnlbhip::kMaxSide,
             "frame width and height must be at most 4096");
-    require(g.frames <= vnlbhip::kMaxFrames, "at most 256 frames per stage call");
+    require(g.frames <= vnlbhip::kMaxFrames,
+            "at most 256 frames in device memory per stage call (pass longer clips as windows)");
     require(g.channels == 1 || g.channels == 3, "only GrayS and YUV444PS clips are supported");
     require(p.patch_size <= vnlbhip::kMaxPatch, "block_size above 16 is not supported");
     require(d.dim <= vnlbhip::kMaxDim, "patch dimension above 1024 is not supported");
@@ -223,11 +241,14 @@ StageDesc make_desc(int stage, const vnlbhip_params& p, const Geometry& g) {
     require(d.jacobi_dim <= 64,
             "eigen problems above 64x64 are not supported (covariance path with a large patch)");
     require(d.max_rows <= vnlbhip::kMaxModelRows, "more than 64 model rows is not supported");
-    const long long candidates = static_cast<long long>(p.search_window) * p.search_window *
-                                 (static_cast<long long>(p.search_bwd) + p.search_fwd + 1);
+    // Candidates per anchor: the temporal range holds at most
+    // min(bwd + fwd + 1, total_frames - Pt + 1) frames.
+    const long long temporal = std::min<long long>(static_cast<long long>(d.search_bwd) + d.search_fwd + 1,
+                                                   static_cast<long long>(max_origin) + 1);
+    const long long candidates = static_cast<long long>(p.search_window) * p.search_window * temporal;
     require(candidates <= vnlbhip::kMaxCandidates,
             "search window times temporal slots above 8192 candidates is not supported");
-    require(p.search_bwd + p.search_fwd + 1 <= vnlbhip::kMaxTemporal,
+    require(d.search_bwd + d.search_fwd + 1 <= vnlbhip::kMaxTemporal,
             "temporal search radius too large");
     require(vnlbhip::prepare_shared_bytes(d) <= 96 * 1024,
             "model rows times patch dimension exceed the prepare kernel's LDS budget");
@@ -249,12 +270,15 @@ struct Grid {
     std::vector<int> rows;     // [frame][3][height]
 };
 
+// anchor_frame is local; the phases follow its absolute index and the
+// clip's last patch origin, as in the reference.
 void append_grid_frame(const StageDesc& d, int anchor_frame, Grid& grid) {
     const int step = d.step;
     const int max_x = d.width - d.patch;
     const int max_y = d.height - d.patch;
-    const int max_frame = d.frames - d.patch_time;
-    const int phase_y = anchor_frame == max_frame ? 0 : anchor_frame;
+    const int max_frame = d.total_frames - d.patch_time;
+    const int absolute_frame = anchor_frame + d.first_frame;
+    const int phase_y = absolute_frame == max_frame ? 0 : absolute_frame;
     const size_t row_base = grid.rows.size();
     grid.rows.resize(row_base + static_cast<size_t>(3) * d.height, -1);
     int* first = grid.rows.data() + row_base;
@@ -356,15 +380,19 @@ struct BatchPlan {
     Tiles tiles;
 };
 
-// Bumped whenever a device buffer is reallocated or the anchor plans are
-// rebuilt: captured stage graphs hold raw addresses and plan data, so a
-// graph captured at an older generation is not replayed.
-unsigned long long g_buffer_generation = 0;
+// Reallocation count of one context.  Captured stage graphs hold raw
+// addresses and plan data, so a graph captured at an older generation is
+// not replayed; each context counts its own buffers and plans.
+struct Generation {
+    unsigned long long value = 0;   // bumped on any reallocation or plan rebuild
+    int64_t reallocations = 0;      // device buffers (re)allocated so far
+};
 
 template <class T> struct DeviceBuffer {
     T* ptr = nullptr;
     size_t capacity = 0;
-    DeviceBuffer() = default;
+    Generation* generation;
+    explicit DeviceBuffer(Generation* owner) : generation(owner) {}
     DeviceBuffer(const DeviceBuffer&) = delete;
     DeviceBuffer& operator=(const DeviceBuffer&) = delete;
     ~DeviceBuffer() { release(); }
@@ -381,11 +409,14 @@ template <class T> struct DeviceBuffer {
         }
         if (count > capacity) {
             // Work queued on any stream may still use the old allocation.
+            // vnlbhip_reserve sizes every buffer up front so that later
+            // stage calls of that shape never get here.
             check(hipDeviceSynchronize(), "hipDeviceSynchronize");
             release();
             check(hipMalloc(reinterpret_cast<void**>(&ptr), count * sizeof(T)), "hipMalloc");
             capacity = count;
-            ++g_buffer_generation;
+            ++generation->value;
+            ++generation->reallocations;
         }
         return ptr;
     }
@@ -416,50 +447,54 @@ constexpr int kDefaultParts = 2;
 
 struct vnlbhip_context {
     int device = 0;
+    Generation generation;
     std::string error;
     vnlbhip_stats stats{};
-    DeviceBuffer<uint32_t> anchors;
-    DeviceBuffer<int> offsets;
-    DeviceBuffer<int> ids;
-    DeviceBuffer<int> rows;
-    DeviceBuffer<int> counts;
-    DeviceBuffer<uint32_t> positions;
-    DeviceBuffer<float> distances;
-    DeviceBuffer<unsigned char> active;
-    DeviceBuffer<int> group_list;
-    DeviceBuffer<unsigned long long> paste_dep;
-    DeviceBuffer<int> paste_target_counts;
-    DeviceBuffer<int> paste_targets;
-    DeviceBuffer<int> group_count;
-    DeviceBuffer<int> group_flags;
-    DeviceBuffer<int> problem_dims;
-    DeviceBuffer<float> means;
-    DeviceBuffer<float> problems;
-    DeviceBuffer<float> eigenvalues;
-    DeviceBuffer<float> vectors;
-    DeviceBuffer<int> status;
-    DeviceBuffer<int> refine_list;
-    DeviceBuffer<int> work_counters;  // prepare, filter: next group to take
+    DeviceBuffer<uint32_t> anchors{&generation};
+    DeviceBuffer<int> offsets{&generation};
+    DeviceBuffer<int> ids{&generation};
+    DeviceBuffer<int> rows{&generation};
+    DeviceBuffer<int> counts{&generation};
+    DeviceBuffer<uint32_t> positions{&generation};
+    DeviceBuffer<float> distances{&generation};
+    DeviceBuffer<unsigned char> active{&generation};
+    DeviceBuffer<int> group_list{&generation};
+    DeviceBuffer<unsigned long long> paste_dep{&generation};
+    DeviceBuffer<int> paste_target_counts{&generation};
+    DeviceBuffer<int> paste_targets{&generation};
+    DeviceBuffer<int> group_count{&generation};
+    DeviceBuffer<int> group_flags{&generation};
+    DeviceBuffer<int> problem_dims{&generation};
+    DeviceBuffer<float> means{&generation};
+    DeviceBuffer<float> problems{&generation};
+    DeviceBuffer<float> eigenvalues{&generation};
+    DeviceBuffer<float> vectors{&generation};
+    DeviceBuffer<int> status{&generation};
+    DeviceBuffer<int> refine_list{&generation};
+    DeviceBuffer<int> work_counters{&generation};  // prepare, filter: next group to take
     // Search-centre offsets (device, [frames][temporal][H][W][2] int16) or
     // null: every window centred on its anchor.
     const int16_t* search_offsets = nullptr;
     int search_offsets_temporal = 0;
+    int search_offsets_frames = 0;  // the field's clip shape, checked by every stage
+    int search_offsets_height = 0;
+    int search_offsets_width = 0;
     float refine_gap = 1e-3F;  // FAST form: relative eigen gap below which groups are re-solved in fp64
     bool fused_selection = true;  // FAST form: k-nearest selection inside the MFMA matcher
-    DeviceBuffer<float> weights;
-    DeviceBuffer<double> accumulators;
-    DeviceBuffer<int> summary;
-    DeviceBuffer<float> host_in;
-    DeviceBuffer<float> host_ref;
-    DeviceBuffer<float> host_out;
+    DeviceBuffer<float> weights{&generation};
+    DeviceBuffer<double> accumulators{&generation};
+    DeviceBuffer<int> summary{&generation};
+    DeviceBuffer<float> host_in{&generation};
+    DeviceBuffer<float> host_ref{&generation};
+    DeviceBuffer<float> host_out{&generation};
     size_t batch_entry_budget = static_cast<size_t>(1) << 28;  // anchors * cap
     size_t fast_stage_budget = static_cast<size_t>(1) << 30;  // anchors * window slots
     int form = VNLBHIP_FORM_PLAIN;
-    DeviceBuffer<float> fast_offsets;
-    DeviceBuffer<float> fast_rows;
-    DeviceBuffer<float> fast_energy;
-    DeviceBuffer<int> fast_tiles;
-    DeviceBuffer<float> fast_staged;
+    DeviceBuffer<float> fast_offsets{&generation};
+    DeviceBuffer<float> fast_rows{&generation};
+    DeviceBuffer<float> fast_energy{&generation};
+    DeviceBuffer<int> fast_tiles{&generation};
+    DeviceBuffer<float> fast_staged{&generation};
     std::vector<long long> plan_key;  // see run_stage
     std::vector<BatchPlan> plans;
     bool profiling = false;
@@ -485,6 +520,7 @@ struct vnlbhip_context {
     bool use_graph = false;
     int64_t graph_replays = 0;
     bool capturing = false;  // run_stage enqueues only (no uploads, no read-back)
+    bool sizing = false;     // run_stage only sizes the buffers (vnlbhip_reserve)
     hipStream_t capture = nullptr;
     std::map<std::vector<long long>, StageGraph> graphs;
     // (stage, form, geometry, parameters) -> the part count timed fastest
@@ -550,6 +586,9 @@ hipEvent_t take_event(vnlbhip_context& ctx) {
 // Records events around one launch when profiling is on.
 template <class Fn>
 void timed(vnlbhip_context& ctx, int kernel, hipStream_t stream, const char* what, Fn&& launch) {
+    if (ctx.sizing) {
+        return;
+    }
     if (!ctx.profiling) {
         check(launch(), what);
         return;
@@ -560,6 +599,13 @@ void timed(vnlbhip_context& ctx, int kernel, hipStream_t stream, const char* wha
     check(launch(), what);
     check(hipEventRecord(stop, stream), "hipEventRecord");
     ctx.pending.push_back({kernel, {start, stop}});
+}
+
+// A launch outside the per-kernel timings (fills, compaction, summary).
+template <class Fn> void untimed(vnlbhip_context& ctx, const char* what, Fn&& launch) {
+    if (!ctx.sizing) {
+        check(launch(), what);
+    }
 }
 
 // Folds completed launch timings into the totals (stream synchronised).
@@ -594,29 +640,46 @@ void finish_stage(vnlbhip_context& ctx, hipStream_t stream) {
     }
 }
 
-// stack (with only_frame): that anchor frame's contribution stack, packed.
+// only_frame: one anchor frame (local index) or kAllFrames; stack (with
+// only_frame): that anchor frame's contribution stack, packed.
+constexpr int kAllFrames = INT_MIN;
+
 void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const float* noisy,
                const float* ref, float* out, const Geometry& g, hipStream_t stream,
                int only_frame, DebugOutput* debug, float* stack = nullptr) {
     const StageDesc d = make_desc(stage, p, g);
-    require(noisy != nullptr, "video data pointer must not be null");
+    require(ctx.sizing || noisy != nullptr, "video data pointer must not be null");
     if (stage == VNLBHIP_STAGE_FINAL) {
-        require(ref != nullptr, "Final needs the ref (basic) clip");
+        require(ctx.sizing || ref != nullptr, "Final needs the ref (basic) clip");
     }
     check(hipSetDevice(ctx.device), "hipSetDevice");
     ctx.stats = vnlbhip_stats{};
     ctx.stats.graph_replays = ctx.graph_replays;
+    const int64_t reallocations_before = ctx.generation.reallocations;
 
-    const int last_anchor_frame = g.frames - p.patch_time;
+    // Anchor frames (local indices): only_frame, or every frame of the
+    // window; frames past the clip's last patch origin have empty stacks
+    // (process_anchor_impl, src/core/core.cpp:1839-1843).
+    const int last_anchor_frame = g.source_frames() - p.patch_time - g.first_frame;
     std::vector<int> frames;
-    if (only_frame >= 0) {
-        if (only_frame <= last_anchor_frame) {
+    if (only_frame != kAllFrames) {
+        if (only_frame + g.first_frame >= 0 && only_frame <= last_anchor_frame) {
             frames.push_back(only_frame);
         }
     } else {
-        for (int f = 0; f <= last_anchor_frame; ++f) {
+        for (int f = 0; f <= std::min(last_anchor_frame, g.frames - 1); ++f) {
             frames.push_back(f);
         }
+    }
+    // Each anchor frame reads the frames of its temporal range
+    // (input_frame_range_for_anchor, src/core/core.cpp:2026-2034): the
+    // window must hold them.
+    for (const int f : frames) {
+        int low = 0;
+        int high = 0;
+        vnlbhip::temporal_range(d, f, low, high);
+        require(low >= 0 && high + p.patch_time - 1 < g.frames,
+                "the frame window does not hold the anchor frame's input frames");
     }
 
     const size_t plane = static_cast<size_t>(g.width) * g.height;
@@ -627,10 +690,9 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
                          ctx.search_offsets == nullptr;
     double* acc = ctx.accumulators.reserve(acc_count);
     float* acc32 = reinterpret_cast<float*>(acc);
-    check(vnlbhip::launch_fill_zero(acc, acc_f32 ? (acc_count + 1) / 2 : acc_count, stream),
-          "fill accumulators");
+    untimed(ctx, "fill accumulators", [&]() { return vnlbhip::launch_fill_zero(acc, acc_f32 ? (acc_count + 1) / 2 : acc_count, stream); });
     int* summary = ctx.summary.reserve(8);
-    check(vnlbhip::launch_fill_zero_int(summary, 8, stream), "clear summary");
+    untimed(ctx, "clear summary", [&]() { return vnlbhip::launch_fill_zero_int(summary, 8, stream); });
 
     const float* source = stage == VNLBHIP_STAGE_FINAL ? ref : noisy;
     // MFMA matcher: patch energies of the matched source, once per stage.
@@ -638,8 +700,13 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
     float* fast_energy = nullptr;
     const bool guided = ctx.search_offsets != nullptr;
     if (guided) {
-        require(ctx.search_offsets_temporal == p.search_bwd + p.search_fwd + 1,
+        require(static_cast<long long>(ctx.search_offsets_temporal) ==
+                    static_cast<long long>(p.search_bwd) + p.search_fwd + 1 &&
+                    d.search_bwd == p.search_bwd && d.search_fwd == p.search_fwd,
                 "search offsets were set for another temporal window (search_bwd + search_fwd + 1)");
+        require(ctx.search_offsets_frames == g.frames && ctx.search_offsets_height == g.height &&
+                    ctx.search_offsets_width == g.width,
+                "search offsets were set for a clip of another shape");
         require(vnlbhip::match_shared_bytes(d, true) <= 140 * 1024,
                 "search window too large for guided matching's LDS budget");
     }
@@ -661,14 +728,15 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
     // only on the geometry, the grid step, the search window, the cap and the
     // frame list: Basic and Final, and repeated calls, reuse them.
     const std::vector<long long> plan_key = [&]() {
-        std::vector<long long> key = {g.width, g.height, g.frames, d.step, d.patch, d.patch_time,
-                                      d.search_window, cap, static_cast<long long>(ctx.batch_entry_budget)};
+        std::vector<long long> key = {g.width, g.height, g.frames, d.first_frame, d.total_frames, d.step,
+                                      d.patch, d.patch_time, d.search_window, cap,
+                                      static_cast<long long>(ctx.batch_entry_budget)};
         key.insert(key.end(), frames.begin(), frames.end());
         return key;
     }();
     if (ctx.plan_key != plan_key) {
         require(!ctx.capturing, "the anchor plan changed during capture");
-        ++g_buffer_generation;
+        ++ctx.generation.value;
         ctx.plan_key.clear();
         ctx.plans.clear();
         size_t next = 0;
@@ -719,7 +787,7 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
         int* d_offsets = ctx.offsets.reserve(grid.offsets.size());
         int* d_ids = ctx.ids.reserve(grid.ids.size());
         int* d_rows = ctx.rows.reserve(grid.rows.size());
-        if (!ctx.capturing) {
+        if (!ctx.capturing && !ctx.sizing) {
             // (A captured stage is single-batch and reuses the tables the
             // eager run before the capture uploaded.)
             check(hipMemcpyAsync(d_anchors, grid.anchors.data(), grid.anchors.size() * sizeof(uint32_t),
@@ -742,9 +810,8 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
         unsigned long long* d_dep = ctx.paste_dep.reserve(anchor_count);
         int* d_tcount = ctx.paste_target_counts.reserve(anchor_count);
         int* d_targets = ctx.paste_targets.reserve(vnlbhip::paste_targets_size(d, anchor_count));
-        check(vnlbhip::launch_fill_zero(reinterpret_cast<double*>(d_dep), anchor_count, stream),
-              "clear paste masks");
-        check(vnlbhip::launch_fill_zero_int(d_group_count, kMaxParts, stream), "clear group counts");
+        untimed(ctx, "clear paste masks", [&]() { return vnlbhip::launch_fill_zero(reinterpret_cast<double*>(d_dep), anchor_count, stream); });
+        untimed(ctx, "clear group counts", [&]() { return vnlbhip::launch_fill_zero_int(d_group_count, kMaxParts, stream); });
 
         // The batch runs as parts of consecutive anchor frames: the serial
         // paste scan of a part (a few wavefronts) runs on the side stream
@@ -779,7 +846,7 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
                 // The tile list lives in the cached plan, so the upload may
                 // complete asynchronously.
                 d_tiles = ctx.fast_tiles.reserve(tiles->slots.size());
-                if (!ctx.capturing) {
+                if (!ctx.capturing && !ctx.sizing) {
                     check(hipMemcpyAsync(d_tiles, tiles->slots.data(), tiles->slots.size() * sizeof(int),
                                          hipMemcpyHostToDevice, stream), "upload tiles");
                 }
@@ -876,8 +943,7 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
                                              d_positions, d_active, max_anchors, ae - ab, d_dep, d_tcount,
                                              d_targets, anchor_count, ctx.side);
             });
-            check(vnlbhip::launch_compact(d_active, ab, ae - ab, d_list + ab, d_group_count + h, ctx.side),
-                  "compact kernel");
+            untimed(ctx, "compact kernel", [&]() { return vnlbhip::launch_compact(d_active, ab, ae - ab, d_list + ab, d_group_count + h, ctx.side); });
             check(hipEventRecord(ctx.paste_event[h], ctx.side), "hipEventRecord");
         }
 
@@ -918,7 +984,7 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
             d_dims_last = d_dims;
             d_problems_last = d_problems;
             int* d_work = ctx.work_counters.reserve(2);
-            check(vnlbhip::launch_fill_zero_int(d_work, 2, stream), "clear work counters");
+            untimed(ctx, "clear work counters", [&]() { return vnlbhip::launch_fill_zero_int(d_work, 2, stream); });
             timed(ctx, VNLBHIP_KERNEL_PREPARE, stream, "prepare kernel", [&]() {
                 return vnlbhip::launch_prepare(d, noisy, basic, d_anchors, d_counts, d_positions,
                                                d_part_list, d_count, capacity, n_stride, d_flags, d_dims,
@@ -932,7 +998,7 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
                 if (parallel_jacobi && ctx.refine_gap > 0.0F) {
                     // refine_list[0] counts, the list follows.
                     d_refine = ctx.refine_list.reserve(static_cast<size_t>(part_capacity) * d.models + 1);
-                    check(vnlbhip::launch_fill_zero_int(d_refine, 1, stream), "clear refine count");
+                    untimed(ctx, "clear refine count", [&]() { return vnlbhip::launch_fill_zero_int(d_refine, 1, stream); });
                 }
                 timed(ctx, VNLBHIP_KERNEL_JACOBI, stream, "jacobi kernel", [&]() {
                     if (parallel_jacobi) {
@@ -959,9 +1025,8 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
                                               d_means, d_eigen, d_vectors, acc, d_weights, d_work + 1,
                                               stream);
             });
-            check(vnlbhip::launch_summarize(d_flags, d.rank > 0 ? d_status : nullptr, d_counts,
-                                            d_part_list, d_count, capacity, d.models, summary, stream),
-                  "summarize kernel");
+            untimed(ctx, "summarize kernel", [&]() { return vnlbhip::launch_summarize(d_flags, d.rank > 0 ? d_status : nullptr, d_counts,
+                                            d_part_list, d_count, capacity, d.models, summary, stream); });
         };
         for (int h = 0; h < parts; ++h) {
             check(hipStreamWaitEvent(stream, ctx.paste_event[h], 0), "hipStreamWaitEvent");
@@ -1081,8 +1146,8 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
 
     if (stack != nullptr) {
         // Slot s of the stack is output frame only_frame - search_bwd + s.
-        const int slots = p.search_bwd + p.search_fwd + p.patch_time;
-        const int first = only_frame - p.search_bwd;
+        const int slots = d.slots;
+        const int first = only_frame - d.search_bwd;
         check(acc_f32 ? vnlbhip::launch_pack_stack(d, acc32, first, slots, stack, stream)
                       : vnlbhip::launch_pack_stack(d, acc, first, slots, stack, stream),
               "pack kernel");
@@ -1094,8 +1159,9 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
                                  : vnlbhip::launch_normalize(d, acc, noisy, out, stream);
               });
     }
-    if (ctx.capturing) {
-        return;  // the summary is read after each replay
+    ctx.stats.reallocations = ctx.generation.reallocations - reallocations_before;
+    if (ctx.capturing || ctx.sizing) {
+        return;  // the summary is read after each replay; a sizing pass launches nothing
     }
     finish_stage(ctx, stream);
 }
@@ -1115,7 +1181,7 @@ void capture_stage(vnlbhip_context& ctx, const std::vector<long long>& key, cons
         check(hipGraphExecDestroy(old->second.exec), "hipGraphExecDestroy");
         ctx.graphs.erase(old);
     }
-    const unsigned long long generation = g_buffer_generation;
+    const unsigned long long generation = ctx.generation.value;
     check(hipStreamBeginCapture(ctx.capture, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
     ctx.capturing = true;
     hipGraph_t graph = nullptr;
@@ -1131,12 +1197,26 @@ void capture_stage(vnlbhip_context& ctx, const std::vector<long long>& key, cons
     }
     ctx.capturing = false;
     check(hipStreamEndCapture(ctx.capture, &graph), "hipStreamEndCapture");
-    require(generation == g_buffer_generation, "device buffers changed during capture");
+    require(generation == ctx.generation.value, "device buffers changed during capture");
     hipGraphExec_t exec = nullptr;
     const hipError_t status = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     check(status, "hipGraphInstantiate");
     ctx.graphs[key] = vnlbhip_context::StageGraph{exec, generation, stats};
+}
+
+// Context settings that change what a stage computes or which kernels it
+// launches: the arithmetic form, the fp64 re-solve gap (a kernel argument),
+// the selection placement and the search-offset field.  Keys of captured
+// graphs, tuned part counts and cached stacks include them.
+void append_settings(const vnlbhip_context& ctx, std::vector<long long>& key) {
+    uint32_t gap_bits = 0;
+    std::memcpy(&gap_bits, &ctx.refine_gap, sizeof(gap_bits));
+    key.push_back(ctx.form);
+    key.push_back(gap_bits);
+    key.push_back(ctx.fused_selection ? 1 : 0);
+    key.push_back(reinterpret_cast<long long>(ctx.search_offsets));
+    key.push_back(ctx.search_offsets_temporal);
 }
 
 int fail(vnlbhip_context* ctx, int code, const char* message) {
@@ -1302,12 +1382,16 @@ int vnlbhip_set_eigen_refine(vnlbhip_context* context, float relative_gap) {
     return VNLBHIP_OK;
 }
 
-int vnlbhip_set_search_offsets(vnlbhip_context* context, const int16_t* offsets, int temporal) {
-    if (context == nullptr || (offsets != nullptr && temporal <= 0)) {
+int vnlbhip_set_search_offsets(vnlbhip_context* context, const int16_t* offsets, int temporal, int frames,
+                               int height, int width) {
+    if (context == nullptr || (offsets != nullptr && (temporal <= 0 || frames <= 0 || height <= 0 || width <= 0))) {
         return VNLBHIP_ERROR_INVALID;
     }
     context->search_offsets = offsets;
     context->search_offsets_temporal = offsets != nullptr ? temporal : 0;
+    context->search_offsets_frames = offsets != nullptr ? frames : 0;
+    context->search_offsets_height = offsets != nullptr ? height : 0;
+    context->search_offsets_width = offsets != nullptr ? width : 0;
     return VNLBHIP_OK;
 }
 
@@ -1324,6 +1408,8 @@ int vnlbhip_mvtools_offsets(vnlbhip_context* context, const vnlbhip_params* para
         require(reinterpret_cast<uintptr_t>(offsets) % 4 == 0, "offsets must be 4-byte aligned");
         const Geometry g{width, height, frames, 1};
         const StageDesc d = make_desc(VNLBHIP_STAGE_BASIC, *params, g);
+        require(d.search_bwd == params->search_bwd && d.search_fwd == params->search_fwd,
+                "search offsets need search_bwd and search_fwd within frames - patch_time");
         // MVToolsVectorGrid::usable and parse_mvtools_analysis's checks
         // (src/flow/flow.hpp:176-181,196-258) on a clip's grid shape.
         const auto shape_of = [](const vnlbhip_mv_shape* in, const int32_t* vectors, const uint8_t* valid,
@@ -1398,8 +1484,9 @@ int vnlbhip_stage(vnlbhip_context* context, int stage, const vnlbhip_params* par
         context->parts = kDefaultParts;
         if (context->autotune) {
             // Key: everything the stage's kernels and grids depend on.
-            std::vector<long long> key = {stage, context->form, width, height, frames, channels,
+            std::vector<long long> key = {stage, width, height, frames, channels,
                                           context->search_offsets != nullptr};
+            append_settings(*context, key);
             const unsigned char* raw = reinterpret_cast<const unsigned char*>(params);
             key.insert(key.end(), raw, raw + sizeof(vnlbhip_params));
             auto it = context->tuned_parts.find(key);
@@ -1409,13 +1496,13 @@ int vnlbhip_stage(vnlbhip_context* context, int stage, const vnlbhip_params* par
                 // time plus the fixed launch work.
                 const bool profiling = context->profiling;
                 context->profiling = false;
-                run_stage(*context, stage, *params, noisy, ref, out, g, s, -1, nullptr);
+                run_stage(*context, stage, *params, noisy, ref, out, g, s, kAllFrames, nullptr);
                 int best = kDefaultParts;
                 double best_ms = 0.0;
                 for (int parts = 1; parts <= kMaxParts; ++parts) {
                     context->parts = parts;
                     const auto t0 = std::chrono::steady_clock::now();
-                    run_stage(*context, stage, *params, noisy, ref, out, g, s, -1, nullptr);
+                    run_stage(*context, stage, *params, noisy, ref, out, g, s, kAllFrames, nullptr);
                     const double ms =
                         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
                     if (parts == 1 || ms < best_ms) {
@@ -1430,15 +1517,14 @@ int vnlbhip_stage(vnlbhip_context* context, int stage, const vnlbhip_params* par
         }
         if (context->use_graph && !context->profiling) {
             // Key: the stage's launch sequence and every address it holds.
-            std::vector<long long> key = {stage, context->form, context->parts, width, height, frames,
+            std::vector<long long> key = {stage, context->parts, width, height, frames,
                                           channels, reinterpret_cast<long long>(noisy),
-                                          reinterpret_cast<long long>(ref), reinterpret_cast<long long>(out),
-                                          reinterpret_cast<long long>(context->search_offsets),
-                                          context->search_offsets_temporal};
+                                          reinterpret_cast<long long>(ref), reinterpret_cast<long long>(out)};
+            append_settings(*context, key);
             const unsigned char* raw = reinterpret_cast<const unsigned char*>(params);
             key.insert(key.end(), raw, raw + sizeof(vnlbhip_params));
             auto it = context->graphs.find(key);
-            if (it != context->graphs.end() && it->second.generation == g_buffer_generation) {
+            if (it != context->graphs.end() && it->second.generation == context->generation.value) {
                 check(hipSetDevice(context->device), "hipSetDevice");
                 check(hipGraphLaunch(it->second.exec, s), "hipGraphLaunch");
                 context->stats = it->second.stats;
@@ -1448,17 +1534,57 @@ int vnlbhip_stage(vnlbhip_context* context, int stage, const vnlbhip_params* par
             }
             // Eager run (the output), then the capture for the next calls:
             // single-batch stages only (one upload of the anchor tables).
-            run_stage(*context, stage, *params, noisy, ref, out, g, s, -1, nullptr);
+            run_stage(*context, stage, *params, noisy, ref, out, g, s, kAllFrames, nullptr);
             if (context->plans.size() == 1) {
                 const vnlbhip_stats stats = context->stats;
                 capture_stage(*context, key, stats, [&](hipStream_t cs) {
-                    run_stage(*context, stage, *params, noisy, ref, out, g, cs, -1, nullptr);
+                    run_stage(*context, stage, *params, noisy, ref, out, g, cs, kAllFrames, nullptr);
                 });
                 context->stats = stats;
             }
             return;
         }
-        run_stage(*context, stage, *params, noisy, ref, out, g, s, -1, nullptr);
+        run_stage(*context, stage, *params, noisy, ref, out, g, s, kAllFrames, nullptr);
+    });
+}
+
+}  // extern "C"
+
+namespace {
+
+// One anchor frame's stack over a window of the clip.  The stack layout
+// has search_bwd + search_fwd + patch_time slots, so radii that reach past
+// the clip (which the stage itself clamps) are refused here.
+void anchor_stack_window(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const float* noisy,
+                         const float* ref, const Geometry& g, int anchor_frame, float* stack, hipStream_t stream) {
+    require(stack != nullptr, "stack must not be null");
+    const StageDesc d = make_desc(stage, p, g);
+    require(d.search_bwd == p.search_bwd && d.search_fwd == p.search_fwd,
+            "contribution stacks need search_bwd and search_fwd within frames - patch_time");
+    run_stage(ctx, stage, p, noisy, ref, nullptr, g, stream, anchor_frame - g.first_frame, nullptr, stack);
+}
+
+}  // namespace
+
+extern "C" {
+
+int vnlbhip_reserve(vnlbhip_context* context, int stage, const vnlbhip_params* params, int width, int height,
+                    int frames, int channels) {
+    if (context == nullptr) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    return guarded(context, [&]() {
+        require(params != nullptr, "params must not be null");
+        context->sizing = true;
+        try {
+            run_stage(*context, stage, *params, nullptr, nullptr, nullptr,
+                      Geometry{width, height, frames, channels}, nullptr, kAllFrames, nullptr);
+        } catch (...) {
+            context->sizing = false;
+            throw;
+        }
+        context->sizing = false;
+        check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     });
 }
 
@@ -1469,11 +1595,27 @@ int vnlbhip_anchor_stack(vnlbhip_context* context, int stage, const vnlbhip_para
         return VNLBHIP_ERROR_INVALID;
     }
     return guarded(context, [&]() {
-        require(params != nullptr && stack != nullptr, "params and stack must not be null");
+        require(params != nullptr, "params must not be null");
         require(anchor_frame >= 0 && anchor_frame < frames, "anchor_frame must be a frame of the clip");
-        run_stage(*context, stage, *params, noisy, ref, nullptr,
-                  Geometry{width, height, frames, channels},
-                  reinterpret_cast<hipStream_t>(stream), anchor_frame, nullptr, stack);
+        anchor_stack_window(*context, stage, *params, noisy, ref, Geometry{width, height, frames, channels},
+                            anchor_frame, stack, reinterpret_cast<hipStream_t>(stream));
+    });
+}
+
+int vnlbhip_anchor_stack_window(vnlbhip_context* context, int stage, const vnlbhip_params* params,
+                                const float* noisy, const float* ref, int width, int height, int frames,
+                                int channels, int first_frame, int total_frames, int anchor_frame, float* stack,
+                                void* stream) {
+    if (context == nullptr) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    return guarded(context, [&]() {
+        require(params != nullptr, "params must not be null");
+        require(total_frames > 0, "total_frames must be positive");
+        require(anchor_frame >= 0 && anchor_frame < total_frames, "anchor_frame must be a frame of the clip");
+        anchor_stack_window(*context, stage, *params, noisy, ref,
+                            Geometry{width, height, frames, channels, first_frame, total_frames}, anchor_frame,
+                            stack, reinterpret_cast<hipStream_t>(stream));
     });
 }
 
@@ -1483,30 +1625,44 @@ int vnlbhip_anchor_stack(vnlbhip_context* context, int stage, const vnlbhip_para
 // and vnlb.Aggregate sums the stacks of the anchors around an output frame,
 // :966-1045).  Each anchor frame's stack is computed once while it stays
 // cached; the least recently used stack outside the current output frame's
-// anchors is replaced.
+// anchors is replaced.  Renders of one cache are serialised by its mutex.
 struct vnlbhip_frame_cache {
     vnlbhip_context* ctx = nullptr;
     int device = 0;                   // freeing needs no context
     int stage = 0;
     vnlbhip_params params{};
-    Geometry g{};
+    int width = 0;
+    int height = 0;
+    int channels = 0;
+    int total_frames = 0;             // frames of the clip
     StageDesc d{};
     size_t stack_floats = 0;
     std::vector<float*> stacks;
+    std::vector<hipEvent_t> read_done;  // per stack: the last render that read it
     std::vector<int> anchor;          // -1: empty
     std::vector<uint64_t> used;
     uint64_t tick = 0;
-    const float* noisy = nullptr;     // clip the cached stacks belong to
+    // What the cached stacks were computed from: whole-clip renders key on
+    // the clip pointers; windowed renders on frame numbers alone (a clip's
+    // frames do not change, as in VapourSynth).
+    int mode = -1;                    // 0: whole clip, 1: windows
+    const float* noisy = nullptr;
     const float* ref = nullptr;
-    int form = -1;
+    std::vector<long long> settings;  // append_settings of the context
+    std::mutex mutex;
     vnlbhip_frame_cache_stats stats{};
 
+    void invalidate() { std::fill(anchor.begin(), anchor.end(), -1); }
+
     ~vnlbhip_frame_cache() {
-        if (!stacks.empty()) {
+        if (!stacks.empty() || !read_done.empty()) {
             (void)hipSetDevice(device);
             (void)hipDeviceSynchronize();
             for (float* p : stacks) {
                 (void)hipFree(p);
+            }
+            for (hipEvent_t e : read_done) {
+                (void)hipEventDestroy(e);
             }
         }
     }
@@ -1521,9 +1677,11 @@ int vnlbhip_frame_cache_create(vnlbhip_context* context, int stage, const vnlbhi
     *out = nullptr;
     return guarded(context, [&]() {
         require(params != nullptr, "params must not be null");
-        const Geometry g{width, height, frames, channels};
+        require(frames > 0, "video geometry must be non-empty");
+        // Every limit of a stage over a window of the clip.
+        const Geometry g{width, height, std::min(frames, vnlbhip::kMaxFrames), channels, 0, frames};
         const StageDesc d = make_desc(stage, *params, g);
-        const int needed = d.search_bwd + d.search_fwd + d.patch_time;
+        const int needed = d.slots;
         require(capacity >= needed, "frame cache capacity must cover search_bwd + search_fwd + patch_time stacks");
         require(needed <= vnlbhip::kMaxStackSet, "more than 32 anchor stacks per output frame are not supported");
         check(hipSetDevice(context->device), "hipSetDevice");
@@ -1532,7 +1690,10 @@ int vnlbhip_frame_cache_create(vnlbhip_context* context, int stage, const vnlbhi
         cache->device = context->device;
         cache->stage = stage;
         cache->params = *params;
-        cache->g = g;
+        cache->width = width;
+        cache->height = height;
+        cache->channels = channels;
+        cache->total_frames = frames;
         cache->d = d;
         cache->stack_floats = static_cast<size_t>(width) * height * channels * d.slots * 2;
         for (int i = 0; i < capacity; ++i) {
@@ -1540,6 +1701,9 @@ int vnlbhip_frame_cache_create(vnlbhip_context* context, int stage, const vnlbhi
             check(hipMalloc(reinterpret_cast<void**>(&p), cache->stack_floats * sizeof(float)),
                   "hipMalloc frame cache stack");
             cache->stacks.push_back(p);
+            hipEvent_t e = nullptr;
+            check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+            cache->read_done.push_back(e);
             cache->anchor.push_back(-1);
             cache->used.push_back(0);
         }
@@ -1547,72 +1711,126 @@ int vnlbhip_frame_cache_create(vnlbhip_context* context, int stage, const vnlbhi
     });
 }
 
+namespace {
+
+// Renders output frame `frame` from the cached stacks, computing the missing
+// ones over the window [first_frame, first_frame + window_frames) that
+// noisy / ref hold.
+void render_frame(vnlbhip_frame_cache& cache, int frame, const float* noisy, const float* ref,
+                  int first_frame, int window_frames, float* out, hipStream_t s, int mode) {
+    vnlbhip_context& ctx = *cache.ctx;
+    const StageDesc& d = cache.d;
+    require(noisy != nullptr && out != nullptr, "noisy and out must not be null");
+    require(frame >= 0 && frame < cache.total_frames, "frame must be a frame of the clip");
+    require(first_frame >= 0 && window_frames > 0 &&
+                static_cast<long long>(first_frame) + window_frames <= cache.total_frames,
+            "the frame window must lie inside the clip");
+    require(frame >= first_frame && frame < first_frame + window_frames,
+            "the frame window must hold the output frame");
+    const Geometry g{cache.width, cache.height, window_frames, cache.channels, first_frame, cache.total_frames};
+    std::vector<long long> settings;
+    append_settings(ctx, settings);
+    if (mode != cache.mode || settings != cache.settings ||
+        (mode == 0 && (noisy != cache.noisy || ref != cache.ref))) {
+        // Another clip, arithmetic form, search field or re-solve gap:
+        // nothing cached applies.
+        cache.invalidate();
+        cache.mode = mode;
+        cache.settings = settings;
+        cache.noisy = mode == 0 ? noisy : nullptr;
+        cache.ref = mode == 0 ? ref : nullptr;
+    }
+    // Anchors whose stacks hold this frame (anchors past the last patch
+    // origin have empty stacks).
+    const int first = std::max(0, frame - (d.search_fwd + d.patch_time - 1));
+    const int last = std::min(cache.total_frames - d.patch_time, frame + d.search_bwd);
+    ++cache.tick;
+    vnlbhip::StackSet set{};
+    for (int a = first; a <= last; ++a) {
+        int slot = -1;
+        for (size_t i = 0; i < cache.anchor.size(); ++i) {
+            if (cache.anchor[i] == a) {
+                slot = static_cast<int>(i);
+                break;
+            }
+        }
+        if (slot >= 0) {
+            ++cache.stats.hits;
+        } else {
+            // Least recently used entry not holding this frame's anchors.
+            for (size_t i = 0; i < cache.anchor.size(); ++i) {
+                if (cache.used[i] != cache.tick &&
+                    (slot < 0 || cache.anchor[i] < 0 ||
+                     (cache.anchor[slot] >= 0 && cache.used[i] < cache.used[slot]))) {
+                    slot = static_cast<int>(i);
+                    if (cache.anchor[i] < 0) {
+                        break;
+                    }
+                }
+            }
+            require(slot >= 0, "frame cache has no free entry");
+            if (cache.anchor[slot] >= 0) {
+                ++cache.stats.evictions;
+            }
+            cache.anchor[slot] = -1;
+            // A render on another stream may still read the old stack.
+            check(hipStreamWaitEvent(s, cache.read_done[static_cast<size_t>(slot)], 0), "hipStreamWaitEvent");
+            run_stage(ctx, cache.stage, cache.params, noisy, ref, nullptr, g, s, a - first_frame, nullptr,
+                      cache.stacks[static_cast<size_t>(slot)]);
+            cache.anchor[slot] = a;
+            ++cache.stats.misses;
+        }
+        cache.used[slot] = cache.tick;
+        set.stack[set.count] = cache.stacks[static_cast<size_t>(slot)];
+        set.slot[set.count] = frame - a + d.search_bwd;
+        ++set.count;
+    }
+    const size_t plane = static_cast<size_t>(cache.width) * cache.height * cache.channels;
+    check(vnlbhip::launch_render_frame(d, set, noisy + static_cast<size_t>(frame - first_frame) * plane, out, s),
+          "render frame kernel");
+    for (int a = first; a <= last; ++a) {
+        for (size_t i = 0; i < cache.anchor.size(); ++i) {
+            if (cache.anchor[i] == a) {
+                check(hipEventRecord(cache.read_done[i], s), "hipEventRecord");
+            }
+        }
+    }
+}
+
+}  // namespace
+
 int vnlbhip_frame_cache_render(vnlbhip_frame_cache* cache, int frame, const float* noisy, const float* ref,
                                float* out, void* stream) {
     if (cache == nullptr) {
         return VNLBHIP_ERROR_INVALID;
     }
-    vnlbhip_context* ctx = cache->ctx;
-    return guarded(ctx, [&]() {
-        const Geometry& g = cache->g;
-        const StageDesc& d = cache->d;
-        require(noisy != nullptr && out != nullptr, "noisy and out must not be null");
-        require(frame >= 0 && frame < g.frames, "frame must be a frame of the clip");
-        const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-        if (noisy != cache->noisy || ref != cache->ref || ctx->form != cache->form) {
-            // Another clip or arithmetic form: nothing cached applies.
-            std::fill(cache->anchor.begin(), cache->anchor.end(), -1);
-            cache->noisy = noisy;
-            cache->ref = ref;
-            cache->form = ctx->form;
-        }
-        // Anchors whose stacks hold this frame (anchors past frames - Pt
-        // have empty stacks).
-        const int first = std::max(0, frame - (d.search_fwd + d.patch_time - 1));
-        const int last = std::min(g.frames - d.patch_time, frame + d.search_bwd);
-        ++cache->tick;
-        vnlbhip::StackSet set{};
-        for (int a = first; a <= last; ++a) {
-            int slot = -1;
-            for (size_t i = 0; i < cache->anchor.size(); ++i) {
-                if (cache->anchor[i] == a) {
-                    slot = static_cast<int>(i);
-                    break;
-                }
-            }
-            if (slot >= 0) {
-                ++cache->stats.hits;
-            } else {
-                // Least recently used entry not holding this frame's anchors.
-                for (size_t i = 0; i < cache->anchor.size(); ++i) {
-                    if (cache->used[i] != cache->tick &&
-                        (slot < 0 || cache->anchor[i] < 0 ||
-                         (cache->anchor[slot] >= 0 && cache->used[i] < cache->used[slot]))) {
-                        slot = static_cast<int>(i);
-                        if (cache->anchor[i] < 0) {
-                            break;
-                        }
-                    }
-                }
-                require(slot >= 0, "frame cache has no free entry");
-                if (cache->anchor[slot] >= 0) {
-                    ++cache->stats.evictions;
-                }
-                cache->anchor[slot] = -1;
-                run_stage(*ctx, cache->stage, cache->params, noisy, ref, nullptr, g, s, a, nullptr,
-                          cache->stacks[slot]);
-                cache->anchor[slot] = a;
-                ++cache->stats.misses;
-            }
-            cache->used[slot] = cache->tick;
-            set.stack[set.count] = cache->stacks[slot];
-            set.slot[set.count] = frame - a + d.search_bwd;
-            ++set.count;
-        }
-        const size_t plane = static_cast<size_t>(g.width) * g.height * g.channels;
-        check(vnlbhip::launch_render_frame(d, set, noisy + static_cast<size_t>(frame) * plane, out, s),
-              "render frame kernel");
+    return guarded(cache->ctx, [&]() {
+        std::lock_guard<std::mutex> lock(cache->mutex);
+        render_frame(*cache, frame, noisy, ref, 0, cache->total_frames, out, reinterpret_cast<hipStream_t>(stream),
+                     0);
     });
+}
+
+int vnlbhip_frame_cache_render_window(vnlbhip_frame_cache* cache, int frame, const float* noisy,
+                                      const float* ref, int first_frame, int window_frames, float* out,
+                                      void* stream) {
+    if (cache == nullptr) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    return guarded(cache->ctx, [&]() {
+        std::lock_guard<std::mutex> lock(cache->mutex);
+        render_frame(*cache, frame, noisy, ref, first_frame, window_frames, out,
+                     reinterpret_cast<hipStream_t>(stream), 1);
+    });
+}
+
+int vnlbhip_frame_cache_invalidate(vnlbhip_frame_cache* cache) {
+    if (cache == nullptr) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    std::lock_guard<std::mutex> lock(cache->mutex);
+    cache->invalidate();
+    return VNLBHIP_OK;
 }
 
 int vnlbhip_frame_cache_get_stats(const vnlbhip_frame_cache* cache, vnlbhip_frame_cache_stats* out) {
@@ -1650,7 +1868,7 @@ int vnlbhip_stage_host(vnlbhip_context* context, int stage, const vnlbhip_params
             check(hipMemcpy(d_ref, ref, count * sizeof(float), hipMemcpyHostToDevice), "upload ref");
         }
         run_stage(*context, stage, *params, d_in, d_ref, d_out,
-                  Geometry{width, height, frames, channels}, nullptr, -1, nullptr);
+                  Geometry{width, height, frames, channels}, nullptr, kAllFrames, nullptr);
         check(hipMemcpy(out, d_out, count * sizeof(float), hipMemcpyDeviceToHost), "download output");
     });
 }

@@ -78,7 +78,7 @@ class Context:
         return {"anchors": s.anchors, "groups": s.groups,
                 "unconverged_groups": s.unconverged_groups, "batches": s.batches,
                 "retained": s.retained, "refined": s.refined, "parts": s.parts,
-                "graph_replays": s.graph_replays}
+                "graph_replays": s.graph_replays, "reallocations": s.reallocations}
 
     def set_form(self, form: str) -> None:
         """'plain' (fp32 in the CPU path's order, the default) or 'fast'
@@ -133,15 +133,16 @@ class Context:
         import torch
 
         if offsets is None:
-            rc = self._lib.vnlbhip_set_search_offsets(self._handle, None, 0)
+            rc = self._lib.vnlbhip_set_search_offsets(self._handle, None, 0, 0, 0, 0)
             self._search_offsets = None
         else:
             if (not offsets.is_cuda or offsets.dtype != torch.int16 or offsets.dim() != 5
                     or offsets.shape[-1] != 2 or not offsets.is_contiguous()):
                 raise VNLBError("VNLB HIP: offsets must be a contiguous int16 GPU tensor "
                                 "[frames, temporal, height, width, 2]")
+            frames, temporal, height, width = (int(s) for s in offsets.shape[:4])
             rc = self._lib.vnlbhip_set_search_offsets(self._handle, ctypes.c_void_p(offsets.data_ptr()),
-                                                      int(offsets.shape[1]))
+                                                      temporal, frames, height, width)
             self._search_offsets = offsets
         if rc != 0:
             self._raise(rc)
@@ -231,6 +232,45 @@ class Context:
         if rc != 0:
             self._raise(rc)
         return stack
+
+    def anchor_stack_window(self, stage: int, params: StageParameters, noisy, ref, first_frame: int,
+                            total_frames: int, anchor_frame: int):
+        """anchor_stack over a window of a longer clip: noisy / ref hold the
+        frames [first_frame, first_frame + len(noisy)) of a clip of
+        total_frames; anchor_frame is absolute (the reference's frame
+        requests, src/vs/plugin.cpp:616-758)."""
+        import torch
+
+        frames, channels, height, width = _canonical_shape(tuple(noisy.shape))
+        self._check_offsets(frames, height, width)
+        if not _is_torch(noisy) or noisy.dtype != torch.float32 or not noisy.is_cuda:
+            raise VNLBError("VNLB HIP: device clips must be float32 tensors on the GPU")
+        if stage == FINAL and (ref is None or tuple(ref.shape) != tuple(noisy.shape)):
+            raise VNLBError("VNLB HIP: ref clip properties must match input clip")
+        p = _abi.to_struct(params)
+        slots = int(params.search_bwd) + int(params.search_fwd) + int(params.patch_time)
+        src = noisy.contiguous()
+        guide = ref.contiguous() if ref is not None else None
+        stack = torch.empty((channels, slots * 2 * height, width), dtype=torch.float32, device=src.device)
+        stream = torch.cuda.current_stream(src.device).cuda_stream
+        rc = self._lib.vnlbhip_anchor_stack_window(
+            self._handle, stage, ctypes.byref(p), ctypes.c_void_p(src.data_ptr()),
+            ctypes.c_void_p(guide.data_ptr()) if guide is not None else None, width, height,
+            frames, channels, int(first_frame), int(total_frames), int(anchor_frame),
+            ctypes.c_void_p(stack.data_ptr()), ctypes.c_void_p(stream))
+        if rc != 0:
+            self._raise(rc)
+        return stack
+
+    def reserve(self, stage: int, params: StageParameters, frames: int, channels: int, height: int,
+                width: int) -> None:
+        """Size every device buffer a stage of this shape needs in the
+        current form, so that later calls of that shape allocate nothing
+        (StagePipeline::reserve, src/cuda/pipeline.hpp:82-107)."""
+        rc = self._lib.vnlbhip_reserve(self._handle, int(stage), ctypes.byref(_abi.to_struct(params)),
+                                       int(width), int(height), int(frames), int(channels))
+        if rc != 0:
+            self._raise(rc)
 
     def stage(self, stage: int, params: StageParameters, noisy, ref=None, out=None):
         """Run one stage over a whole clip and return the aggregated frames
@@ -357,6 +397,7 @@ class FrameCache:
         import torch
 
         frames, channels, height, width = self._shape
+        self._context._check_offsets(frames, height, width)
         src = noisy if noisy.dim() == 4 else noisy.unsqueeze(1)
         if tuple(src.shape) != self._shape or src.dtype != torch.float32 or not src.is_cuda:
             raise VNLBError(f"VNLB HIP: the clip must be a float32 GPU tensor of shape {self._shape}")
@@ -374,6 +415,41 @@ class FrameCache:
         if rc != 0:
             self._context._raise(rc)
         return out
+
+    def render_window(self, frame: int, noisy, ref, first_frame: int, out=None):
+        """Output frame `frame` from a window of the clip: noisy / ref hold the
+        frames [first_frame, first_frame + len(noisy)) (float32 GPU tensors),
+        which must include `frame` and the input frames of every stack the
+        call computes.  Stacks are cached by frame number."""
+        import torch
+
+        frames, channels, height, width = self._shape
+        src = noisy if noisy.dim() == 4 else noisy.unsqueeze(1)
+        window = int(src.shape[0])
+        self._context._check_offsets(window, height, width)
+        if (tuple(src.shape[1:]) != (channels, height, width) or src.dtype != torch.float32
+                or not src.is_cuda):
+            raise VNLBError("VNLB HIP: the window must be a float32 GPU tensor of the clip's frame shape")
+        src = src.contiguous()
+        guide = None
+        if ref is not None:
+            guide = (ref if ref.dim() == 4 else ref.unsqueeze(1)).contiguous()
+            if tuple(guide.shape) != tuple(src.shape):
+                raise VNLBError("VNLB HIP: ref clip properties must match input clip")
+        if out is None:
+            out = torch.empty((channels, height, width), dtype=torch.float32, device=src.device)
+        stream = torch.cuda.current_stream(src.device).cuda_stream
+        rc = self._lib.vnlbhip_frame_cache_render_window(
+            self._handle, int(frame), ctypes.c_void_p(src.data_ptr()),
+            ctypes.c_void_p(guide.data_ptr()) if guide is not None else None, int(first_frame), window,
+            ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream))
+        if rc != 0:
+            self._context._raise(rc)
+        return out
+
+    def invalidate(self) -> None:
+        """Drop every cached stack (the clip's content changed)."""
+        self._lib.vnlbhip_frame_cache_invalidate(self._handle)
 
     def stats(self) -> dict:
         s = _abi.FrameCacheStats()

@@ -18,9 +18,11 @@ EXPORTED_SYMBOLS = (
     "vnlbhip_create",
     "vnlbhip_destroy",
     "vnlbhip_last_error",
+    "vnlbhip_reserve",
     "vnlbhip_stage",
     "vnlbhip_stage_host",
     "vnlbhip_anchor_stack",
+    "vnlbhip_anchor_stack_window",
     "vnlbhip_debug_frame",
     "vnlbhip_retained_cap",
     "vnlbhip_get_stats",
@@ -35,6 +37,8 @@ EXPORTED_SYMBOLS = (
     "vnlbhip_mvtools_offsets",
     "vnlbhip_frame_cache_create",
     "vnlbhip_frame_cache_render",
+    "vnlbhip_frame_cache_render_window",
+    "vnlbhip_frame_cache_invalidate",
     "vnlbhip_frame_cache_get_stats",
     "vnlbhip_frame_cache_destroy",
 )
@@ -89,6 +93,7 @@ class Stats(ctypes.Structure):
         ("retained", ctypes.c_int64),
         ("refined", ctypes.c_int64),
         ("graph_replays", ctypes.c_int64),
+        ("reallocations", ctypes.c_int64),
     ]
 
 
@@ -138,6 +143,9 @@ def load() -> ctypes.CDLL:
     lib.vnlbhip_destroy.restype = None
     lib.vnlbhip_last_error.argtypes = [vp]
     lib.vnlbhip_last_error.restype = ctypes.c_char_p
+    lib.vnlbhip_reserve.argtypes = [vp, ctypes.c_int, P(Params), ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int]
+    lib.vnlbhip_reserve.restype = ctypes.c_int
     lib.vnlbhip_stage.argtypes = [vp, ctypes.c_int, P(Params), vp, vp, vp, ctypes.c_int,
                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
     lib.vnlbhip_stage.restype = ctypes.c_int
@@ -145,6 +153,8 @@ def load() -> ctypes.CDLL:
                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          vp, vp]
     lib.vnlbhip_anchor_stack.restype = ctypes.c_int
+    lib.vnlbhip_anchor_stack_window.argtypes = [vp, ctypes.c_int, P(Params), vp, vp] + [ctypes.c_int] * 7 + [vp, vp]
+    lib.vnlbhip_anchor_stack_window.restype = ctypes.c_int
     lib.vnlbhip_stage_host.argtypes = [vp, ctypes.c_int, P(Params), fp, fp, fp, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.vnlbhip_stage_host.restype = ctypes.c_int
@@ -169,7 +179,7 @@ def load() -> ctypes.CDLL:
     lib.vnlbhip_set_autotune.restype = ctypes.c_int
     lib.vnlbhip_set_graph.argtypes = [vp, ctypes.c_int]
     lib.vnlbhip_set_graph.restype = ctypes.c_int
-    lib.vnlbhip_set_search_offsets.argtypes = [vp, vp, ctypes.c_int]
+    lib.vnlbhip_set_search_offsets.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.vnlbhip_set_search_offsets.restype = ctypes.c_int
     mv = P(MvShape)
     lib.vnlbhip_mvtools_offsets.argtypes = [vp, P(Params), mv, vp, vp, mv, vp, vp, vp, ctypes.c_int,
@@ -180,6 +190,10 @@ def load() -> ctypes.CDLL:
     lib.vnlbhip_frame_cache_create.restype = ci
     lib.vnlbhip_frame_cache_render.argtypes = [vp, ci, vp, vp, vp, vp]
     lib.vnlbhip_frame_cache_render.restype = ci
+    lib.vnlbhip_frame_cache_render_window.argtypes = [vp, ci, vp, vp, ci, ci, vp, vp]
+    lib.vnlbhip_frame_cache_render_window.restype = ci
+    lib.vnlbhip_frame_cache_invalidate.argtypes = [vp]
+    lib.vnlbhip_frame_cache_invalidate.restype = ci
     lib.vnlbhip_frame_cache_get_stats.argtypes = [vp, P(FrameCacheStats)]
     lib.vnlbhip_frame_cache_get_stats.restype = ci
     lib.vnlbhip_frame_cache_destroy.argtypes = [vp]
