@@ -180,6 +180,19 @@ int vnlbhip_debug_frame(vnlbhip_context* context, int stage,
                         float* grams);
 
 int vnlbhip_retained_cap(const vnlbhip_params* params);
+
+/* The staggered anchor grid of one anchor frame (absolute index) of a clip
+ * of total_frames, in raster order (reference_processing_mask_selected /
+ * initialize_frame_processing_mask, src/core/core.cpp:934-967): writes up
+ * to capacity positions to xs / ys (either may be NULL) and returns the
+ * anchor count.  Needs no GPU. */
+int vnlbhip_anchor_grid(const vnlbhip_params* params, int width, int height, int total_frames,
+                        int anchor_frame, int capacity, int* xs, int* ys);
+
+/* The aggregation window a(u) of prepare_aggregation_window
+ * (src/core/core.cpp:296-324), patch_size^2 floats, row-major.  Needs no
+ * GPU. */
+int vnlbhip_aggregation_window(const vnlbhip_params* params, float* window);
 int vnlbhip_get_stats(const vnlbhip_context* context, vnlbhip_stats* out);
 
 /* Per-kernel device time, measured with hipEvents recorded on the stage's

@@ -127,6 +127,11 @@ def load() -> ctypes.CDLL:
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, fp, ctypes.c_int]
         lib.vo_anchor_stack_window.argtypes = [ctypes.c_int, P(VoParams), fp, fp] + [ctypes.c_int] * 7 + [
             fp, ctypes.c_int]
+        lib.vo_aggregation_weights.argtypes = [ctypes.c_int, P(VoParams), dp, ctypes.c_int, ctypes.c_int, dp, dp,
+                                               ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, fp]
+        lib.vo_aggregation_window.argtypes = [P(VoParams), fp]
+        lib.vo_grid_mask.argtypes = [P(VoParams), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     P(ctypes.c_ubyte)]
         lib.vo_set_search_offsets.argtypes = [P(ctypes.c_int16), ctypes.c_int]
         lib.vo_set_search_offsets.restype = None
         mv = P(MvShape)
@@ -270,6 +275,42 @@ def frame_debug(stage_id: int, params, noisy, ref, anchor_frame: int, flags: int
 
 def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def aggregation_weights(stage_id: int, params, rows, eigenvalues=None, basis=None, basis_similar=None,
+                        models: int = 1, precision: int = 1):
+    """Weights of one group (reset, `models` x accumulate_aggregation_model,
+    finalize) from its centred membership rows [S, D] and rank eigen pairs."""
+    lib = load()
+    r = np.ascontiguousarray(rows, np.float64)
+    similar, dim = r.shape
+    e = np.ascontiguousarray(np.zeros(0) if eigenvalues is None else eigenvalues, np.float64)
+    b = np.ascontiguousarray(np.zeros((0, dim)) if basis is None else basis, np.float64)
+    rank = e.shape[0]
+    out = np.zeros(similar, np.float32)
+    rc = lib.vo_aggregation_weights(stage_id, ctypes.byref(to_struct(params)), _dp(r), similar, dim, _dp(e),
+                                    _dp(b), rank, int(basis_similar or similar), int(models), int(precision),
+                                    _fp(out))
+    if rc != 0:
+        raise ValueError(lib.vo_last_error().decode())
+    return out
+
+
+def aggregation_window(params):
+    lib = load()
+    p = to_struct(params)
+    out = np.zeros(p.patch_size * p.patch_size, np.float32)
+    assert lib.vo_aggregation_window(ctypes.byref(p), _fp(out)) == 0
+    return out.reshape(p.patch_size, p.patch_size)
+
+
+def grid_mask(params, width: int, height: int, total_frames: int, anchor_frame: int):
+    lib = load()
+    mask = np.zeros((height, width), np.uint8)
+    count = lib.vo_grid_mask(ctypes.byref(to_struct(params)), width, height, total_frames, anchor_frame,
+                             mask.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)))
+    assert count >= 0
+    return mask
 
 
 def linalg_mean(samples):

@@ -1005,6 +1005,111 @@ struct ModelInfo {
     std::vector<double> gram;  // in the model's precision, widened
 };
 
+// accumulate_aggregation_model (core.cpp:326-412): adds one model's log
+// weights to log_weights[0, similar).  rows: the S centred membership rows
+// (the basic rows in Final, the noisy rows in Basic), d values each;
+// eigenvalues / basis: the model's `rank` leading pairs (basis rows of d).
+// Group precision tau = eps + d / B + sum lambda^ / (lambda^ + beta sigma^2)
+// with lambda^ = max(lambda - model noise, 0) (group_precision_tau,
+// :222-242); membership scores Z and the volume term as in README.md:122-185.
+template <class T>
+void accumulate_model_weights(const vo_params& p, bool final_stage, const T* rows, int similar, int d,
+                              const T* eigenvalues, const T* basis, int rank, int basis_similar,
+                              std::vector<float>& log_weights, std::vector<T>& scores) {
+    if (p.weight_alpha == 0.0F && p.weight_beta == 0.0F) {
+        return;
+    }
+    const T sigma = static_cast<T>(p.sigma);
+    const T sigma_basic = static_cast<T>(p.sigma_basic);
+    const T beta = static_cast<T>(p.beta);
+    const T model_noise2 = final_stage ? (beta * sigma_basic) * sigma_basic : (beta * sigma) * sigma;
+    const T estimate_noise2 = (beta * sigma) * sigma;
+    const T alpha = static_cast<T>(p.weight_alpha);
+    T trace{};
+    for (int j = 0; j < rank; ++j) {
+        const T lambda = positive_or_zero(eigenvalues[j] - model_noise2);
+        if (lambda > T{}) {
+            trace = trace + lambda / (lambda + estimate_noise2);
+        }
+    }
+    const T mean_term = static_cast<T>(d) / static_cast<T>(basis_similar);
+    const T tau = (static_cast<T>(p.weight_epsilon) + mean_term) + trace;
+    const T log_group = p.weight_alpha == 0.0F ? T{} : -alpha * t_log(tau);
+    if (p.weight_beta == 0.0F) {
+        for (int r = 0; r < similar; ++r) {
+            log_weights[static_cast<std::size_t>(r)] =
+                static_cast<float>(static_cast<T>(log_weights[static_cast<std::size_t>(r)]) + log_group);
+        }
+        return;
+    }
+    const T match_sigma = final_stage ? sigma_basic : sigma;
+    const T match_noise2 = match_sigma * match_sigma;
+    const T floor_noise2 = (static_cast<T>(p.membership_noise_floor) * sigma) * sigma;
+    const T membership_noise2 = std::max(match_noise2, floor_noise2);
+    T volume{};
+    for (int j = 0; j < rank; ++j) {
+        const T lambda = positive_or_zero(eigenvalues[j] - model_noise2);
+        if (lambda > T{}) {
+            volume = volume + t_log1p(lambda / membership_noise2);
+        }
+    }
+    volume = volume / static_cast<T>(d);
+    T min_score = std::numeric_limits<T>::infinity();
+    const T score_scale = std::sqrt(static_cast<T>(2) * static_cast<T>(d));
+    scores.resize(static_cast<std::size_t>(similar));
+    for (int r = 0; r < similar; ++r) {
+        const T* row = rows + static_cast<std::size_t>(r) * d;
+        const T norm2 = dot(row, row, static_cast<std::size_t>(d));
+        T distance{};
+        T projected_norm2{};
+        for (int j = 0; j < rank; ++j) {
+            const T projection = dot(row, basis + static_cast<std::size_t>(j) * d, static_cast<std::size_t>(d));
+            const T projection2 = projection * projection;
+            const T lambda = positive_or_zero(eigenvalues[j] - model_noise2);
+            distance = distance + projection2 / (lambda + membership_noise2);
+            projected_norm2 = projected_norm2 + projection2;
+        }
+        const T residual = positive_or_zero(norm2 - projected_norm2);
+        distance = distance + residual / membership_noise2;
+        const T score = (distance - static_cast<T>(d)) / score_scale;
+        scores[static_cast<std::size_t>(r)] = score;
+        min_score = std::min(min_score, score);
+    }
+    const T weight_beta = static_cast<T>(p.weight_beta);
+    for (int r = 0; r < similar; ++r) {
+        const T membership_value = positive_or_zero(scores[static_cast<std::size_t>(r)] - min_score) + volume;
+        const T delta = log_group - (weight_beta * membership_value);
+        log_weights[static_cast<std::size_t>(r)] =
+            static_cast<float>(static_cast<T>(log_weights[static_cast<std::size_t>(r)]) + delta);
+    }
+}
+
+// finalize_aggregation_weights (core.cpp:276-294) with
+// accumulate_default_aggregation_model (:245-274) when no model added log
+// weights (equal groups): w = exp(clamp(mean log weight, +-80)), all 1 when
+// alpha = beta_w = gamma_w = 0.
+void finalize_weights(const vo_params& p, int model_dim, int basis_similar, int models,
+                      std::vector<float>& log_weights, std::vector<float>& weights, int similar) {
+    const bool enabled = p.weight_alpha != 0.0F || p.weight_beta != 0.0F || p.weight_gamma != 0.0F;
+    weights.assign(static_cast<std::size_t>(similar), 1.0F);
+    if (!enabled) {
+        return;
+    }
+    if (models == 0) {
+        const float tau = p.weight_epsilon + (static_cast<float>(model_dim) / static_cast<float>(basis_similar));
+        const float log_group = p.weight_alpha == 0.0F ? 0.0F : -p.weight_alpha * t_log(tau);
+        for (int r = 0; r < similar; ++r) {
+            log_weights[static_cast<std::size_t>(r)] = log_weights[static_cast<std::size_t>(r)] + log_group;
+        }
+        models = 1;
+    }
+    const float inv_models = 1.0F / static_cast<float>(models);
+    for (int r = 0; r < similar; ++r) {
+        const float value = log_weights[static_cast<std::size_t>(r)] * inv_models;
+        weights[static_cast<std::size_t>(r)] = t_exp(std::clamp(value, -80.0F, 80.0F));
+    }
+}
+
 // filter_vnlb_samples (core.cpp:638-792) including compute_filter_coefficients
 // (:616-636), accumulate_aggregation_model (:326-412) and
 // project_bayes_estimate (:578-614).  Inputs are the S rows of one model;
@@ -1060,9 +1165,6 @@ void filter_samples(const StageSetup& s, ModelScratch<T>& m, int similar,
     const T sigma = static_cast<T>(p.sigma);
     const T sigma_basic = static_cast<T>(p.sigma_basic);
     const T beta = static_cast<T>(p.beta);
-    const T model_noise2 = final_stage ? (beta * sigma_basic) * sigma_basic
-                                       : (beta * sigma) * sigma;
-    const T estimate_noise2 = (beta * sigma) * sigma;
 
     m.filtered.resize(sd);
     m.eigenvalues.assign(static_cast<std::size_t>(std::max(rank, 1)), T{});
@@ -1109,78 +1211,9 @@ void filter_samples(const StageSetup& s, ModelScratch<T>& m, int similar,
     }
 
     // accumulate_aggregation_model (core.cpp:326-412).
-    if (p.weight_alpha == 0.0F && p.weight_beta == 0.0F) {
-        ++model_count;
-    } else {
-        const T alpha = static_cast<T>(p.weight_alpha);
-        T trace{};
-        for (int j = 0; j < rank; ++j) {
-            const T lambda = positive_or_zero(m.eigenvalues[static_cast<std::size_t>(j)] -
-                                              model_noise2);
-            if (lambda > T{}) {
-                trace = trace + lambda / (lambda + estimate_noise2);
-            }
-        }
-        const T mean_term = static_cast<T>(d) / static_cast<T>(basis_similar);
-        const T tau = (static_cast<T>(p.weight_epsilon) + mean_term) + trace;
-        const T log_group = p.weight_alpha == 0.0F ? T{} : -alpha * t_log(tau);
-        if (p.weight_beta != 0.0F) {
-            const T match_sigma = final_stage ? sigma_basic : sigma;
-            const T match_noise2 = match_sigma * match_sigma;
-            const T floor_noise2 =
-                (static_cast<T>(p.membership_noise_floor) * sigma) * sigma;
-            const T membership_noise2 = std::max(match_noise2, floor_noise2);
-            T volume{};
-            for (int j = 0; j < rank; ++j) {
-                const T lambda = positive_or_zero(
-                    m.eigenvalues[static_cast<std::size_t>(j)] - model_noise2);
-                if (lambda > T{}) {
-                    volume = volume + t_log1p(lambda / membership_noise2);
-                }
-            }
-            volume = volume / static_cast<T>(d);
-            T min_score = std::numeric_limits<T>::infinity();
-            const T score_scale = std::sqrt(static_cast<T>(2) * static_cast<T>(d));
-            m.scores.resize(static_cast<std::size_t>(similar));
-            for (int r = 0; r < similar; ++r) {
-                const T* row = membership.data() + static_cast<std::size_t>(r) * d;
-                const T norm2 = dot(row, row, static_cast<std::size_t>(d));
-                T distance{};
-                T projected_norm2{};
-                for (int j = 0; j < rank; ++j) {
-                    const T projection =
-                        dot(row, m.basis.data() + static_cast<std::size_t>(j) * d,
-                            static_cast<std::size_t>(d));
-                    const T projection2 = projection * projection;
-                    const T lambda = positive_or_zero(
-                        m.eigenvalues[static_cast<std::size_t>(j)] - model_noise2);
-                    distance = distance + projection2 / (lambda + membership_noise2);
-                    projected_norm2 = projected_norm2 + projection2;
-                }
-                const T residual = positive_or_zero(norm2 - projected_norm2);
-                distance = distance + residual / membership_noise2;
-                const T score = (distance - static_cast<T>(d)) / score_scale;
-                m.scores[static_cast<std::size_t>(r)] = score;
-                min_score = std::min(min_score, score);
-            }
-            const T weight_beta = static_cast<T>(p.weight_beta);
-            for (int r = 0; r < similar; ++r) {
-                const T membership_value =
-                    positive_or_zero(m.scores[static_cast<std::size_t>(r)] - min_score) +
-                    volume;
-                const T delta = log_group - (weight_beta * membership_value);
-                log_weights[static_cast<std::size_t>(r)] = static_cast<float>(
-                    static_cast<T>(log_weights[static_cast<std::size_t>(r)]) + delta);
-            }
-        } else {
-            for (int r = 0; r < similar; ++r) {
-                log_weights[static_cast<std::size_t>(r)] = static_cast<float>(
-                    static_cast<T>(log_weights[static_cast<std::size_t>(r)]) +
-                    log_group);
-            }
-        }
-        ++model_count;
-    }
+    accumulate_model_weights<T>(p, final_stage, membership.data(), similar, d, m.eigenvalues.data(),
+                                m.basis.data(), rank, basis_similar, log_weights, m.scores);
+    ++model_count;
 
     // project_bayes_estimate (core.cpp:578-614), or the centre for rank 0.
     for (int r = 0; r < similar; ++r) {
@@ -1386,8 +1419,6 @@ long long process_anchor_frame(const StageSetup& s, int anchor_frame, Stack<Acc>
     ModelScratch<T> scratch;
     ModelInfo info;
     long long groups = 0;
-    const bool accum_weights = p.weight_alpha != 0.0F || p.weight_beta != 0.0F ||
-                               p.weight_gamma != 0.0F;
     const bool final_stage = s.stage == VO_STAGE_FINAL;
 
     for (int ay = 0; ay <= max_y; ++ay) {
@@ -1479,31 +1510,8 @@ long long process_anchor_frame(const StageSetup& s, int anchor_frame, Stack<Acc>
                 }
             }
 
-            // finalize_aggregation_weights (core.cpp:276-294) with
-            // accumulate_default_aggregation_model (:245-274).
-            if (!accum_weights) {
-                std::fill(gw.weights.begin(), gw.weights.end(), 1.0F);
-            } else {
-                if (gw.models == 0) {
-                    const int sample_dim = s.model_dim;
-                    const int basis_similar = std::min(similar, s.model_cap);
-                    const float tau = p.weight_epsilon +
-                                      (static_cast<float>(sample_dim) /
-                                       static_cast<float>(basis_similar));
-                    const float log_group =
-                        p.weight_alpha == 0.0F ? 0.0F : -p.weight_alpha * t_log(tau);
-                    for (float& lw : gw.log_weights) {
-                        lw = lw + log_group;
-                    }
-                    ++gw.models;
-                }
-                const float inv_models = 1.0F / static_cast<float>(gw.models);
-                for (int r = 0; r < similar; ++r) {
-                    const float value = gw.log_weights[static_cast<std::size_t>(r)] * inv_models;
-                    gw.weights[static_cast<std::size_t>(r)] =
-                        t_exp(std::clamp(value, -80.0F, 80.0F));
-                }
-            }
+            finalize_weights(p, s.model_dim, std::min(similar, s.model_cap), gw.models, gw.log_weights,
+                             gw.weights, similar);
 
             // Contributions (write_group_contributions core.cpp:1587-1636,
             // write_sample_contributions_coupled_* :1638-1795).
@@ -1711,6 +1719,72 @@ int vo_retained_cap(const vo_params* params) {
         return -1;
     }
     return retained_cap(*params);
+}
+
+int vo_aggregation_weights(int stage, const vo_params* params, const double* rows, int similar,
+                           int dim, const double* eigenvalues, const double* basis, int rank,
+                           int basis_similar, int models, int precision, float* weights) {
+    try {
+        require(params != nullptr && weights != nullptr && similar > 0 && dim > 0 && rank >= 0 &&
+                    basis_similar > 0 && models >= 0,
+                "bad arguments");
+        require(rank == 0 || (eigenvalues != nullptr && basis != nullptr), "rank needs eigen pairs");
+        require(rows != nullptr, "rows must not be null");
+        const bool final_stage = stage == VO_STAGE_FINAL;
+        std::vector<float> log_weights(static_cast<std::size_t>(similar), 0.0F);
+        std::vector<float> out;
+        const auto run = [&](auto zero) {
+            using T = decltype(zero);
+            const std::size_t sd = static_cast<std::size_t>(similar) * dim;
+            std::vector<T> r(rows, rows + sd);
+            std::vector<T> e(eigenvalues, eigenvalues + rank);
+            std::vector<T> b(basis, basis + static_cast<std::size_t>(rank) * dim);
+            std::vector<T> scores;
+            for (int m = 0; m < models; ++m) {
+                accumulate_model_weights<T>(*params, final_stage, r.data(), similar, dim, e.data(), b.data(),
+                                            rank, basis_similar, log_weights, scores);
+            }
+        };
+        if (precision == 0) {
+            run(0.0F);
+        } else {
+            run(0.0);
+        }
+        finalize_weights(*params, dim, basis_similar, models, log_weights, out, similar);
+        std::copy(out.begin(), out.end(), weights);
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int vo_aggregation_window(const vo_params* params, float* window) {
+    if (params == nullptr || window == nullptr || params->patch_size <= 0) {
+        return -1;
+    }
+    const std::vector<float> w = aggregation_window(*params);
+    std::copy(w.begin(), w.end(), window);
+    return 0;
+}
+
+int vo_grid_mask(const vo_params* params, int width, int height, int total_frames, int anchor_frame,
+                 unsigned char* mask) {
+    if (params == nullptr || mask == nullptr || width < params->patch_size || height < params->patch_size ||
+        total_frames < params->patch_time) {
+        return -1;
+    }
+    const Geometry g{width, height, total_frames, 1, 0, total_frames};
+    int count = 0;
+    for (int y = 0; y < height; ++y) {
+        for (int x = 0; x < width; ++x) {
+            const bool inside = x <= width - params->patch_size && y <= height - params->patch_size;
+            const bool on = inside && grid_selected(*params, g, anchor_frame, x, y);
+            mask[static_cast<std::size_t>(y) * width + x] = on ? 1 : 0;
+            count += on ? 1 : 0;
+        }
+    }
+    return count;
 }
 
 int vo_grid_count(const vo_params* params, int width, int height, int frames,

@@ -158,6 +158,26 @@ int vo_anchor_stack_window(int stage, const vo_params* params, const float* nois
                            int channels, int first_frame, int total_frames,
                            int anchor_frame, float* packed, int flags);
 
+/* One group's aggregation weights from its centred membership rows
+ * (similar x dim, row-major) and `rank` eigen pairs (basis rows of dim):
+ * reset, `models` times accumulate_aggregation_model, then
+ * finalize_aggregation_weights (src/core/core.cpp:235-294,326-412);
+ * models = 0 takes the default model of equal groups.  precision 0 runs the
+ * model arithmetic in float (the stage's), 1 in double. */
+int vo_aggregation_weights(int stage, const vo_params* params, const double* rows, int similar,
+                           int dim, const double* eigenvalues, const double* basis, int rank,
+                           int basis_similar, int models, int precision, float* weights);
+
+/* prepare_aggregation_window (src/core/core.cpp:296-324): patch_size^2
+ * values, row-major. */
+int vo_aggregation_window(const vo_params* params, float* window);
+
+/* The staggered anchor grid of one frame (reference_processing_mask_selected,
+ * src/core/core.cpp:934-950) as a width x height byte mask; returns the
+ * anchor count. */
+int vo_grid_mask(const vo_params* params, int width, int height, int total_frames, int anchor_frame,
+                 unsigned char* mask);
+
 /* Retained group cap (max(K, ceil(K * cap_factor)) limited by the window). */
 int vo_retained_cap(const vo_params* params);
 

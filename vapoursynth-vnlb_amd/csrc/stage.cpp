@@ -1307,6 +1307,46 @@ int vnlbhip_validate(int stage, const vnlbhip_params* params, int width, int hei
     }
 }
 
+int vnlbhip_anchor_grid(const vnlbhip_params* params, int width, int height, int total_frames, int anchor_frame,
+                        int capacity, int* xs, int* ys) {
+    if (params == nullptr || params->patch_size <= 0 || params->patch_time <= 0 || params->proc_step < 0 ||
+        width < params->patch_size || height < params->patch_size || width > vnlbhip::kMaxSide ||
+        height > vnlbhip::kMaxSide || total_frames < params->patch_time || anchor_frame < 0 ||
+        anchor_frame >= total_frames || capacity < 0) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    StageDesc d{};
+    d.width = width;
+    d.height = height;
+    d.frames = 1;
+    d.first_frame = anchor_frame;
+    d.total_frames = total_frames;
+    d.patch = params->patch_size;
+    d.patch_time = params->patch_time;
+    d.step = params->proc_step > 0 ? params->proc_step : (params->patch_size <= 1 ? 1 : params->patch_size);
+    Grid grid;
+    append_grid_frame(d, 0, grid);
+    const int count = static_cast<int>(grid.anchors.size());
+    for (int i = 0; i < count && i < capacity; ++i) {
+        if (xs != nullptr) {
+            xs[i] = vnlbhip::position_x(grid.anchors[static_cast<size_t>(i)]);
+        }
+        if (ys != nullptr) {
+            ys[i] = vnlbhip::position_y(grid.anchors[static_cast<size_t>(i)]);
+        }
+    }
+    return count;
+}
+
+int vnlbhip_aggregation_window(const vnlbhip_params* params, float* window) {
+    if (params == nullptr || window == nullptr || params->patch_size <= 0 ||
+        params->patch_size > vnlbhip::kMaxPatch) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    aggregation_window(*params, window);
+    return VNLBHIP_OK;
+}
+
 int vnlbhip_retained_cap(const vnlbhip_params* params) {
     if (params == nullptr || params->similar <= 0) {
         return VNLBHIP_ERROR_INVALID;

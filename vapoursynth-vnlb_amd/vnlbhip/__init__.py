@@ -23,7 +23,7 @@ from . import _abi
 from .params import BASIC, FINAL, StageParameters, VNLBError, parse_stage_parameters
 
 __all__ = ["Basic", "Final", "Context", "StageParameters", "VNLBError", "parse_stage_parameters",
-           "BASIC", "FINAL"]
+           "BASIC", "FINAL", "anchor_grid", "aggregation_window"]
 
 _ERROR_NAMES = {-1: "invalid argument", -2: "device error", -3: "model error"}
 
@@ -466,6 +466,36 @@ class FrameCache:
             self.close()
         except Exception:
             pass
+
+
+def anchor_grid(params: StageParameters, width: int, height: int, total_frames: int, anchor_frame: int):
+    """The staggered anchor grid of one anchor frame (absolute index) of a
+    clip of total_frames as (xs, ys) int32 arrays in raster order
+    (src/core/core.cpp:934-967), from the library's host code; needs no GPU."""
+    lib = _abi.load()
+    p = _abi.to_struct(params)
+    count = lib.vnlbhip_anchor_grid(ctypes.byref(p), int(width), int(height), int(total_frames),
+                                    int(anchor_frame), 0, None, None)
+    if count < 0:
+        raise VNLBError("VNLB HIP: invalid grid arguments")
+    xs = np.zeros(count, np.int32)
+    ys = np.zeros(count, np.int32)
+    ip = ctypes.POINTER(ctypes.c_int)
+    lib.vnlbhip_anchor_grid(ctypes.byref(p), int(width), int(height), int(total_frames), int(anchor_frame), count,
+                            xs.ctypes.data_as(ip), ys.ctypes.data_as(ip))
+    return xs, ys
+
+
+def aggregation_window(params: StageParameters):
+    """The aggregation window a(u) [patch_size, patch_size]
+    (src/core/core.cpp:296-324), from the library's host code; needs no GPU."""
+    lib = _abi.load()
+    out = np.zeros(params.patch_size * params.patch_size, np.float32)
+    rc = lib.vnlbhip_aggregation_window(ctypes.byref(_abi.to_struct(params)),
+                                        out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    if rc != 0:
+        raise VNLBError("VNLB HIP: invalid window arguments")
+    return out.reshape(params.patch_size, params.patch_size)
 
 
 _local = threading.local()

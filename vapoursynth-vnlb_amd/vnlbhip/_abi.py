@@ -25,6 +25,8 @@ EXPORTED_SYMBOLS = (
     "vnlbhip_anchor_stack_window",
     "vnlbhip_debug_frame",
     "vnlbhip_retained_cap",
+    "vnlbhip_anchor_grid",
+    "vnlbhip_aggregation_window",
     "vnlbhip_get_stats",
     "vnlbhip_set_profiling",
     "vnlbhip_get_kernel_times",
@@ -165,6 +167,10 @@ def load() -> ctypes.CDLL:
     lib.vnlbhip_debug_frame.restype = ctypes.c_int
     lib.vnlbhip_retained_cap.argtypes = [P(Params)]
     lib.vnlbhip_retained_cap.restype = ctypes.c_int
+    lib.vnlbhip_anchor_grid.argtypes = [P(Params)] + [ctypes.c_int] * 5 + [ip, ip]
+    lib.vnlbhip_anchor_grid.restype = ctypes.c_int
+    lib.vnlbhip_aggregation_window.argtypes = [P(Params), fp]
+    lib.vnlbhip_aggregation_window.restype = ctypes.c_int
     lib.vnlbhip_get_stats.argtypes = [vp, P(Stats)]
     lib.vnlbhip_get_stats.restype = ctypes.c_int
     lib.vnlbhip_set_profiling.argtypes = [vp, ctypes.c_int]
