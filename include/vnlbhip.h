@@ -119,6 +119,30 @@ int vnlbhip_stage(vnlbhip_context* context, int stage,
                   const float* ref, float* out, int width, int height,
                   int frames, int channels, void* stream);
 
+/* Size of the per-device scratch array of vnlbhip_stage_scratch. */
+#define VNLBHIP_SCRATCH_BYTES (128u * 1024u * 1024u)
+
+/* vnlbhip_stage without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as vnlbhip_stage; every internal buffer
+ * (match lists, paste tables, models, accumulators) is carved from one
+ * fixed scratch array of VNLBHIP_SCRATCH_BYTES per GPU, which each GPU the
+ * library is set up on holds until the process ends (its device address is
+ * taken once per device when a context is created).  The caller passes
+ * inputs and output.  A stage that needs more than the array fails with
+ * VNLBHIP_ERROR_INVALID before any kernel runs.  The call allocates no
+ * device or pinned memory, creates no stream or event, captures no graph,
+ * keeps no profiling timings and runs no autotuning; the context's own
+ * buffers are left as they were.  Every call on one GPU shares its array,
+ * which keeps the last call's contents (each call clears what it needs):
+ * calls that overlap on one GPU -- from two threads or two contexts --
+ * race, so serialise them.  Blocks until the stage has finished. */
+int vnlbhip_stage_scratch(vnlbhip_context* context, int stage,
+                          const vnlbhip_params* params, const float* noisy,
+                          const float* ref, float* out, int width, int height,
+                          int frames, int channels, void* stream);
+
 /* One anchor frame's contribution stack on device memory, as a vnlb.Basic /
  * vnlb.Final output frame holds it (src/vs/plugin.cpp:264-271,625-641; the
  * CUDA plugin's pack_kernel, src/cuda/aggregation.cu:508-528, for

@@ -149,4 +149,10 @@ hipError_t launch_normalize(const StageDesc& d, const double* acc, const float* 
 hipError_t launch_normalize(const StageDesc& d, const float* acc, const float* source, float* out,
                             hipStream_t stream);
 
+// The fixed per-device scratch array of vnlbhip_stage_scratch (scratch.hip):
+// its size, and its device address on `device`, which must be the current
+// device (looked up once per device, then cached).
+size_t scratch_bytes();
+hipError_t scratch_address(int device, void** out);
+
 }  // namespace vnlbhip

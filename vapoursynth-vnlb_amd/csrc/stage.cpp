@@ -383,29 +383,98 @@ struct BatchPlan {
 // Reallocation count of one context.  Captured stage graphs hold raw
 // addresses and plan data, so a graph captured at an older generation is
 // not replayed; each context counts its own buffers and plans.
+// A stage call that must not allocate (vnlbhip_stage_scratch) carves every
+// buffer, in call order, from the device's fixed scratch array instead.
+struct Arena {
+    unsigned char* base = nullptr;
+    size_t size = 0;
+    size_t used = 0;
+    bool active = false;
+};
+
+// Every device buffer of a context, so a scratch call can set the owned
+// allocations aside (park) and bring them back afterwards (unpark).
+struct BufferSlot {
+    virtual ~BufferSlot() = default;
+    virtual void park() = 0;
+    virtual void drop_carved() = 0;
+    virtual void unpark() = 0;
+};
+
 struct Generation {
     unsigned long long value = 0;   // bumped on any reallocation or plan rebuild
     int64_t reallocations = 0;      // device buffers (re)allocated so far
+    Arena arena;
+    std::vector<BufferSlot*> buffers;
 };
 
-template <class T> struct DeviceBuffer {
+template <class T> struct DeviceBuffer final : BufferSlot {
     T* ptr = nullptr;
     size_t capacity = 0;
+    T* owned_ptr = nullptr;  // the owned allocation while parked
+    size_t owned_capacity = 0;
+    bool parked = false;
     Generation* generation;
-    explicit DeviceBuffer(Generation* owner) : generation(owner) {}
+    explicit DeviceBuffer(Generation* owner) : generation(owner) { owner->buffers.push_back(this); }
     DeviceBuffer(const DeviceBuffer&) = delete;
     DeviceBuffer& operator=(const DeviceBuffer&) = delete;
-    ~DeviceBuffer() { release(); }
-    void release() {
-        if (ptr != nullptr) {
-            (void)hipFree(ptr);
+    ~DeviceBuffer() override {
+        unpark();
+        release();
+    }
+    void park() override {
+        if (!parked) {
+            owned_ptr = ptr;
+            owned_capacity = capacity;
             ptr = nullptr;
+            capacity = 0;
+            parked = true;
         }
+    }
+    void drop_carved() override {
+        if (parked) {
+            ptr = nullptr;
+            capacity = 0;
+        }
+    }
+    void unpark() override {
+        if (parked) {
+            ptr = owned_ptr;
+            capacity = owned_capacity;
+            owned_ptr = nullptr;
+            owned_capacity = 0;
+            parked = false;
+        }
+    }
+    void release() {
+        if (ptr != nullptr && !parked) {
+            (void)hipFree(ptr);
+        }
+        ptr = nullptr;
         capacity = 0;
     }
     T* reserve(size_t count) {
         if (count == 0) {
             count = 1;
+        }
+        Arena& arena = generation->arena;
+        if (arena.active) {
+            if (count > capacity) {
+                // 256-byte aligned pieces of the scratch array, never freed
+                // within the call (earlier pieces may still be read by
+                // queued kernels).
+                const size_t left = arena.size - arena.used;
+                if (count > left / sizeof(T)) {
+                    throw InvalidArgument(
+                        "the stage needs more device memory than the scratch array of "
+                        "vnlbhip_stage_scratch holds (VNLBHIP_SCRATCH_BYTES); use vnlbhip_stage");
+                }
+                const size_t bytes = (count * sizeof(T) + 255) & ~static_cast<size_t>(255);
+                ptr = reinterpret_cast<T*>(arena.base + arena.used);
+                capacity = count;
+                arena.used += std::min(bytes, left);
+            }
+            return ptr;
         }
         if (count > capacity) {
             // Work queued on any stream may still use the old allocation.
@@ -526,6 +595,9 @@ struct vnlbhip_context {
     // (stage, form, geometry, parameters) -> the part count timed fastest
     std::map<std::vector<long long>, int> tuned_parts;
     int* host_counts = nullptr;
+    // This device's copy of the scratch array (vnlbhip_stage_scratch), from
+    // hipGetSymbolAddress at set-up.
+    void* scratch = nullptr;
     ~vnlbhip_context() {
         for (auto& entry : graphs) {
             if (entry.second.exec != nullptr) {
@@ -571,6 +643,22 @@ hipError_t launch_summarize(const int* flags, const int* status, const int* coun
 }
 
 namespace {
+
+// The side stream of the paste scans, its events, the pinned host words and
+// this device's scratch array: set up once, when the context is created, so
+// that no stage call creates or allocates them.
+void set_up_context(vnlbhip_context& ctx) {
+    check(hipStreamCreateWithFlags(&ctx.side, hipStreamNonBlocking), "hipStreamCreate");
+    check(hipEventCreateWithFlags(&ctx.ready_event, hipEventDisableTiming), "hipEventCreate");
+    for (int h = 0; h < kMaxParts; ++h) {
+        check(hipEventCreateWithFlags(&ctx.match_event[h], hipEventDisableTiming), "hipEventCreate");
+        check(hipEventCreateWithFlags(&ctx.paste_event[h], hipEventDisableTiming), "hipEventCreate");
+    }
+    check(hipHostMalloc(reinterpret_cast<void**>(&ctx.host_counts), (kMaxParts + 8) * sizeof(int)),
+          "hipHostMalloc");
+    check(vnlbhip::scratch_address(ctx.device, &ctx.scratch), "hipGetSymbolAddress (scratch)");
+    require(vnlbhip::scratch_bytes() == VNLBHIP_SCRATCH_BYTES, "scratch array size mismatch");
+}
 
 hipEvent_t take_event(vnlbhip_context& ctx) {
     if (!ctx.event_pool.empty()) {
@@ -765,16 +853,7 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
         }
         ctx.plan_key = plan_key;
     }
-    if (ctx.side == nullptr) {
-        check(hipStreamCreateWithFlags(&ctx.side, hipStreamNonBlocking), "hipStreamCreate");
-        check(hipEventCreateWithFlags(&ctx.ready_event, hipEventDisableTiming), "hipEventCreate");
-        for (int h = 0; h < kMaxParts; ++h) {
-            check(hipEventCreateWithFlags(&ctx.match_event[h], hipEventDisableTiming), "hipEventCreate");
-            check(hipEventCreateWithFlags(&ctx.paste_event[h], hipEventDisableTiming), "hipEventCreate");
-        }
-        check(hipHostMalloc(reinterpret_cast<void**>(&ctx.host_counts), (kMaxParts + 8) * sizeof(int)),
-              "hipHostMalloc");
-    }
+    require(ctx.side != nullptr, "context set-up incomplete");
     for (BatchPlan& plan : ctx.plans) {
         const Grid& grid = plan.grid;
         const int max_anchors = plan.max_anchors;
@@ -1374,6 +1453,12 @@ int vnlbhip_create(int device, vnlbhip_context** out) {
         return VNLBHIP_ERROR_DEVICE;
     }
     ctx->device = device;
+    try {
+        set_up_context(*ctx);
+    } catch (const std::exception&) {
+        delete ctx;
+        return VNLBHIP_ERROR_DEVICE;
+    }
     *out = ctx;
     return VNLBHIP_OK;
 }
@@ -1584,6 +1669,81 @@ int vnlbhip_stage(vnlbhip_context* context, int stage, const vnlbhip_params* par
             }
             return;
         }
+        run_stage(*context, stage, *params, noisy, ref, out, g, s, kAllFrames, nullptr);
+    });
+}
+
+}  // extern "C"
+
+namespace {
+
+// The context state of one vnlbhip_stage_scratch call: owned buffers set
+// aside, every buffer carved from the scratch array, no graphs, no
+// profiling events, no autotuning runs; all restored on exit (also when the
+// call fails).
+struct ScratchCall {
+    vnlbhip_context& ctx;
+    bool profiling;
+    bool use_graph;
+    bool autotune;
+    explicit ScratchCall(vnlbhip_context& c)
+        : ctx(c), profiling(c.profiling), use_graph(c.use_graph), autotune(c.autotune) {
+        ctx.profiling = false;
+        ctx.use_graph = false;
+        ctx.autotune = false;
+        for (BufferSlot* b : ctx.generation.buffers) {
+            b->park();
+        }
+        Arena& arena = ctx.generation.arena;
+        arena.base = static_cast<unsigned char*>(ctx.scratch);
+        arena.size = VNLBHIP_SCRATCH_BYTES;
+        arena.used = 0;
+        arena.active = true;
+    }
+    // Back to an empty scratch array, for the pass after the sizing pass.
+    void rewind() {
+        for (BufferSlot* b : ctx.generation.buffers) {
+            b->drop_carved();
+        }
+        ctx.generation.arena.used = 0;
+    }
+    ~ScratchCall() {
+        ctx.sizing = false;
+        ctx.generation.arena = Arena{};
+        for (BufferSlot* b : ctx.generation.buffers) {
+            b->unpark();
+        }
+        ctx.profiling = profiling;
+        ctx.use_graph = use_graph;
+        ctx.autotune = autotune;
+    }
+    ScratchCall(const ScratchCall&) = delete;
+    ScratchCall& operator=(const ScratchCall&) = delete;
+};
+
+}  // namespace
+
+extern "C" {
+
+int vnlbhip_stage_scratch(vnlbhip_context* context, int stage, const vnlbhip_params* params,
+                          const float* noisy, const float* ref, float* out, int width, int height,
+                          int frames, int channels, void* stream) {
+    if (context == nullptr) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    return guarded(context, [&]() {
+        require(params != nullptr && out != nullptr, "params and out must not be null");
+        require(context->scratch != nullptr, "no scratch array on this device");
+        const Geometry g{width, height, frames, channels};
+        const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+        ScratchCall call(*context);
+        context->parts = kDefaultParts;
+        // Sizing pass: carves every buffer without launching anything, so a
+        // stage too large for the array is refused before any kernel runs.
+        context->sizing = true;
+        run_stage(*context, stage, *params, noisy, ref, out, g, s, kAllFrames, nullptr);
+        context->sizing = false;
+        call.rewind();
         run_stage(*context, stage, *params, noisy, ref, out, g, s, kAllFrames, nullptr);
     });
 }

@@ -27,6 +27,9 @@ __all__ = ["Basic", "Final", "Context", "StageParameters", "VNLBError", "parse_s
 
 _ERROR_NAMES = {-1: "invalid argument", -2: "device error", -3: "model error"}
 
+# VNLBHIP_SCRATCH_BYTES: the per-GPU scratch array of Context.stage_scratch.
+SCRATCH_BYTES = 128 * 1024 * 1024
+
 
 def _is_torch(x) -> bool:
     return type(x).__module__.startswith("torch")
@@ -271,6 +274,36 @@ class Context:
                                        int(width), int(height), int(frames), int(channels))
         if rc != 0:
             self._raise(rc)
+
+    def stage_scratch(self, stage: int, params: StageParameters, noisy, ref=None, out=None):
+        """stage() on device tensors without any device allocation inside
+        the call (vnlbhip_stage_scratch): the library's buffers come from a
+        fixed per-GPU scratch array of SCRATCH_BYTES; larger stages raise.
+        Calls that overlap on one GPU race."""
+        import torch
+
+        frames, channels, height, width = _canonical_shape(tuple(noisy.shape))
+        self._check_offsets(frames, height, width)
+        if not _is_torch(noisy) or noisy.dtype != torch.float32 or not noisy.is_cuda:
+            raise VNLBError("VNLB HIP: device clips must be float32 tensors on the GPU")
+        if stage == FINAL and (ref is None or tuple(ref.shape) != tuple(noisy.shape)):
+            raise VNLBError("VNLB HIP: ref clip properties must match input clip")
+        if out is None:
+            out = torch.empty_like(noisy)
+        for t in (noisy, ref, out):
+            if t is not None and not t.is_contiguous():
+                raise VNLBError("VNLB HIP: stage_scratch takes contiguous tensors (no copies are made)")
+        if tuple(out.shape) != tuple(noisy.shape) or out.dtype != torch.float32 or out.device != noisy.device:
+            raise VNLBError("VNLB HIP: out must be a contiguous float32 tensor like the clip")
+        p = _abi.to_struct(params)
+        stream = torch.cuda.current_stream(noisy.device).cuda_stream
+        rc = self._lib.vnlbhip_stage_scratch(
+            self._handle, stage, ctypes.byref(p), ctypes.c_void_p(noisy.data_ptr()),
+            ctypes.c_void_p(ref.data_ptr()) if ref is not None else None,
+            ctypes.c_void_p(out.data_ptr()), width, height, frames, channels, ctypes.c_void_p(stream))
+        if rc != 0:
+            self._raise(rc)
+        return out
 
     def stage(self, stage: int, params: StageParameters, noisy, ref=None, out=None):
         """Run one stage over a whole clip and return the aggregated frames

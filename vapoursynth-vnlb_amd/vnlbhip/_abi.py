@@ -20,6 +20,7 @@ EXPORTED_SYMBOLS = (
     "vnlbhip_last_error",
     "vnlbhip_reserve",
     "vnlbhip_stage",
+    "vnlbhip_stage_scratch",
     "vnlbhip_stage_host",
     "vnlbhip_anchor_stack",
     "vnlbhip_anchor_stack_window",
@@ -151,6 +152,8 @@ def load() -> ctypes.CDLL:
     lib.vnlbhip_stage.argtypes = [vp, ctypes.c_int, P(Params), vp, vp, vp, ctypes.c_int,
                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
     lib.vnlbhip_stage.restype = ctypes.c_int
+    lib.vnlbhip_stage_scratch.argtypes = lib.vnlbhip_stage.argtypes
+    lib.vnlbhip_stage_scratch.restype = ctypes.c_int
     lib.vnlbhip_anchor_stack.argtypes = [vp, ctypes.c_int, P(Params), vp, vp, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          vp, vp]
