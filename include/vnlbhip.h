@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define VNLBHIP_ABI_VERSION 6
+#define VNLBHIP_ABI_VERSION 7
 
 enum {
     VNLBHIP_STAGE_BASIC = 0,
@@ -77,6 +77,8 @@ typedef struct vnlbhip_stats {
     int64_t refined;              /* FAST form: eigen problems re-solved in fp64 */
     int64_t graph_replays;        /* stage graphs replayed by the context so far */
     int64_t reallocations;        /* device buffers the last call (re)allocated */
+    int64_t proposed;             /* anchors matched by the learned proposer */
+    int64_t exhaustive_fallbacks; /* of those, anchors whose score margin was too low */
 } vnlbhip_stats;
 
 typedef struct vnlbhip_context vnlbhip_context;
@@ -400,6 +402,84 @@ int vnlbhip_mvtools_offsets(vnlbhip_context* context, const vnlbhip_params* para
  * the same match lists; 0 exists to check that.  Values other than 0 and 1
  * are rejected with VNLBHIP_ERROR_INVALID. */
 int vnlbhip_set_fused_selection(vnlbhip_context* context, int enable);
+
+/* ---- Learned candidate proposal (no counterpart in the reference) ----
+ *
+ * The reference scans every candidate of the search window
+ * (find_similar_patches, src/core/core.cpp:1245-1362; the CUDA matcher
+ * src/cuda/block_match.cu:592-912).  A proposer is a small MLP (input 128
+ * features of a patch pair: 2x2 box means of the two 8x8 patches, their
+ * absolute differences, the offset, the box-mean SSD; two hidden layers of
+ * 256; one score) trained on the GPU on the plain matcher's exact k nearest.
+ * Set on a context, it scores every candidate of each window and only the
+ * top_k scored candidates get exact fp32 distances (each equal to the
+ * exhaustive scan's bit for bit); when the score margin between the K-th
+ * and the (top_k + 1)-th best is below `margin`, that anchor's window is
+ * scanned exhaustively.  The selection rules then run unchanged over the
+ * proposals.  Parameters are one flat fp32 vector of
+ * VNLBHIP_PROPOSER_PARAMS values: W1 [256][128], b1 [256], W2 [256][256],
+ * b2 [256], w3 [256], b3. */
+#define VNLBHIP_PROPOSER_INPUT 128
+#define VNLBHIP_PROPOSER_HIDDEN 256
+#define VNLBHIP_PROPOSER_PARAMS 99073
+#define VNLBHIP_PRECISION_BF16 0
+#define VNLBHIP_PRECISION_FP8 1
+
+typedef struct vnlbhip_proposer vnlbhip_proposer;
+
+/* A proposer on `device` with He-uniform weights from `seed` (bf16 copies
+ * made); errors are reported through the return code only. */
+int vnlbhip_proposer_create(int device, uint64_t seed, vnlbhip_proposer** out);
+void vnlbhip_proposer_destroy(vnlbhip_proposer* proposer);
+const char* vnlbhip_proposer_last_error(const vnlbhip_proposer* proposer);
+/* Host copies of the fp32 parameters (VNLBHIP_PROPOSER_PARAMS floats);
+ * set_params also refreshes the bf16 copies and drops fp8 calibration.
+ * Both synchronise the device. */
+int vnlbhip_proposer_get_params(const vnlbhip_proposer* proposer, float* params);
+int vnlbhip_proposer_set_params(vnlbhip_proposer* proposer, const float* params);
+/* Input rows of (anchor, candidate) pairs of a device clip: anchors and
+ * candidates are packed positions (x | y << 12 | frame << 24, frames of the
+ * clip, 8x8 patches of patch_time frames inside it), out: bf16
+ * [rows][VNLBHIP_PROPOSER_INPUT] (device, 16-byte aligned). */
+int vnlbhip_proposer_features(vnlbhip_proposer* proposer, const float* clip, int width, int height,
+                              int frames, int channels, int patch_time, const uint32_t* anchors,
+                              const uint32_t* candidates, int rows, void* out, void* stream);
+/* Forward, listwise loss and gradients of rows of inputs x (bf16, device)
+ * in groups of `group` consecutive rows with targets (device, summing to 1
+ * per group): grads (device, VNLBHIP_PROPOSER_PARAMS) and loss_groups
+ * (device, rows / group).  rows must be a multiple of group. */
+int vnlbhip_proposer_gradients(vnlbhip_proposer* proposer, const void* x, const float* targets, int rows,
+                               int group, float* grads, float* loss_groups, void* stream);
+/* The same, then one Adam step (beta 0.9 / 0.999, eps 1e-8) at
+ * learning_rate on the proposer's parameters and its bf16 copies. */
+int vnlbhip_proposer_train_step(vnlbhip_proposer* proposer, const void* x, const float* targets, int rows,
+                                int group, float learning_rate, float* loss_groups, void* stream);
+/* Scores of rows of inputs with the matcher's inference code (precision
+ * VNLBHIP_PRECISION_BF16, or _FP8 after calibration). */
+int vnlbhip_proposer_scores(vnlbhip_proposer* proposer, const void* x, int rows, int precision, float* scores,
+                            void* stream);
+/* FP8 (e4m3) inference copies and their scales from sample input rows
+ * (per-input and per-output-row scales, a static hidden scale). */
+int vnlbhip_proposer_calibrate(vnlbhip_proposer* proposer, const void* x, int rows, void* stream);
+/* Matching of the context's stages through the proposer (NULL clears):
+ * 8x8 patches, patch_time 1 or 2, unguided windows; top_k in [K, 255].
+ * The proposer must outlive its use by the context. */
+int vnlbhip_set_proposer(vnlbhip_context* context, const vnlbhip_proposer* proposer, int top_k, float margin,
+                         int precision);
+/* The match lists of one anchor frame (matching only, no paste or model),
+ * with the context's form and proposer: anchors (packed positions), counts
+ * and the retained matches [anchor][cap] (cap = vnlbhip_retained_cap;
+ * positions packed, distances) into device buffers of `capacity` anchors;
+ * *anchor_count (host) receives the frame's anchor count.  Blocks. */
+int vnlbhip_match_lists(vnlbhip_context* context, int stage, const vnlbhip_params* params, const float* noisy,
+                        const float* ref, int width, int height, int frames, int channels, int anchor_frame,
+                        int capacity, int* anchor_count, uint32_t* anchors, int* counts, uint32_t* positions,
+                        float* distances, void* stream);
+/* Search windows of anchors (host arrays; compute_temporal_range and
+ * compute_spatial_range, src/core/core.cpp:873-932): per anchor x_low,
+ * y_low, nx, ny, t_low, t_high (absolute frames).  Needs no GPU. */
+int vnlbhip_search_windows(const vnlbhip_params* params, int width, int height, int total_frames, int count,
+                           const uint32_t* anchors, int32_t* windows);
 
 #ifdef __cplusplus
 }

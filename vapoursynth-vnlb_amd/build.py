@@ -18,7 +18,7 @@ CSRC = PACKAGE_DIR / "csrc"
 BUILD = PACKAGE_DIR / "_build"
 LIBRARY = BUILD / "libvnlbhip.so"
 
-HIP_SOURCES = ["match.hip", "match_fast.hip", "paste.hip", "prepare.hip", "model.hip", "filter.hip", "filter_fast.hip", "aggregate.hip", "flow.hip", "scratch.hip"]
+HIP_SOURCES = ["match.hip", "match_fast.hip", "paste.hip", "prepare.hip", "model.hip", "filter.hip", "filter_fast.hip", "aggregate.hip", "flow.hip", "scratch.hip", "proposer.hip"]
 HOST_SOURCES = ["stage.cpp"]
 
 # -ffp-contract=off: only explicit fmaf calls fuse, so the summation order

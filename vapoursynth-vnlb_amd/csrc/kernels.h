@@ -2,6 +2,7 @@
 #pragma once
 
 #include "common.h"
+#include "proposer_types.h"
 
 #include <stddef.h>
 
@@ -148,6 +149,35 @@ hipError_t launch_normalize(const StageDesc& d, const double* acc, const float* 
 // FAST form accumulators (fp32, same layout).
 hipError_t launch_normalize(const StageDesc& d, const float* acc, const float* source, float* out,
                             hipStream_t stream);
+
+// Learned candidate proposer (proposer.hip; network in proposer.h).
+size_t proposer_workspace_floats(int rows);
+hipError_t launch_proposer_init(float* params, uint64_t seed, hipStream_t stream);
+hipError_t launch_proposer_refresh(const float* params, __bf16* w1, __bf16* w2, hipStream_t stream);
+hipError_t launch_proposer_features(const float* clip, int width, int height, int channels, int patch_time,
+                                    const uint32_t* anchors, const uint32_t* candidates, int rows, __bf16* out,
+                                    hipStream_t stream);
+hipError_t launch_proposer_scores(const __bf16* x, int rows, const ProposerWeights& wt, int precision,
+                                  float* scores, hipStream_t stream);
+// FP8 inference copies from sample rows: per-input scales (x_scale), W1 with
+// them folded in and per-row scales, the hidden scale, W2 per-row scales.
+hipError_t launch_proposer_calibrate(const float* params, const __bf16* w1, const __bf16* x, int rows,
+                                     float* workspace, uint8_t* w1_fp8, uint8_t* w2_fp8, float* scales,
+                                     hipStream_t stream);
+hipError_t launch_proposer_gradients(const float* params, const __bf16* w1, const __bf16* w2, const __bf16* x,
+                                     const float* targets, int rows, int group, float* workspace, float* grads,
+                                     float* loss_groups, hipStream_t stream);
+hipError_t launch_proposer_adam(float* params, const float* grads, float* m, float* v, float lr, int step,
+                                hipStream_t stream);
+// Proposing matcher (match.hip): 8x8 patches, unguided windows; the
+// network scores every candidate, exact distances are taken for the top_k
+// scored ones (or for the whole window when the score margin between the
+// K-th and the (top_k + 1)-th best is below `margin`), then the plain
+// selection.  fallbacks[0] counts the anchors searched exhaustively.
+bool learned_match_supported(const StageDesc& d);
+hipError_t launch_learned_match(const StageDesc& d, const float* source, const uint32_t* anchors, int anchor_count,
+                                int* counts, uint32_t* positions, float* distances, const ProposerWeights& wt,
+                                int top_k, float margin, int precision, int* fallbacks, hipStream_t stream);
 
 // The fixed per-device scratch array of vnlbhip_stage_scratch (scratch.hip):
 // its size, and its device address on `device`, which must be the current

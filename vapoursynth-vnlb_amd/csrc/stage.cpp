@@ -169,7 +169,7 @@ void aggregation_window(const vnlbhip_params& p, float* window) {
     }
 }
 
-StageDesc make_desc(int stage, const vnlbhip_params& p, const Geometry& g) {
+StageDesc make_desc(int stage, const vnlbhip_params& p, const Geometry& g, bool matching_only = false) {
     require(stage == VNLBHIP_STAGE_BASIC || stage == VNLBHIP_STAGE_FINAL,
             "stage must be 0 (Basic) or 1 (Final)");
     validate_reference(p, g);
@@ -237,10 +237,11 @@ StageDesc make_desc(int stage, const vnlbhip_params& p, const Geometry& g) {
     require(p.patch_size <= vnlbhip::kMaxPatch, "block_size above 16 is not supported");
     require(d.dim <= vnlbhip::kMaxDim, "patch dimension above 1024 is not supported");
     require(d.cap <= vnlbhip::kMaxCap, "retained group cap above 256 is not supported");
-    require(d.rank <= vnlbhip::kMaxRank, "rank above 64 is not supported");
-    require(d.jacobi_dim <= 64,
+    // Model limits do not bind a call that only matches (vnlbhip_match_lists).
+    require(matching_only || d.rank <= vnlbhip::kMaxRank, "rank above 64 is not supported");
+    require(matching_only || d.jacobi_dim <= 64,
             "eigen problems above 64x64 are not supported (covariance path with a large patch)");
-    require(d.max_rows <= vnlbhip::kMaxModelRows, "more than 64 model rows is not supported");
+    require(matching_only || d.max_rows <= vnlbhip::kMaxModelRows, "more than 64 model rows is not supported");
     // Candidates per anchor: the temporal range holds at most
     // min(bwd + fwd + 1, total_frames - Pt + 1) frames.
     const long long temporal = std::min<long long>(static_cast<long long>(d.search_bwd) + d.search_fwd + 1,
@@ -250,9 +251,9 @@ StageDesc make_desc(int stage, const vnlbhip_params& p, const Geometry& g) {
             "search window times temporal slots above 8192 candidates is not supported");
     require(d.search_bwd + d.search_fwd + 1 <= vnlbhip::kMaxTemporal,
             "temporal search radius too large");
-    require(vnlbhip::prepare_shared_bytes(d) <= 96 * 1024,
+    require(matching_only || vnlbhip::prepare_shared_bytes(d) <= 96 * 1024,
             "model rows times patch dimension exceed the prepare kernel's LDS budget");
-    require(vnlbhip::filter_shared_bytes(d) <= 112 * 1024,
+    require(matching_only || vnlbhip::filter_shared_bytes(d) <= 112 * 1024,
             "rank and model rows times patch dimension exceed the filter kernel's LDS budget");
     require(vnlbhip::match_shared_bytes(d) <= 96 * 1024,
             "search window too large for the matcher's LDS budget");
@@ -550,6 +551,12 @@ struct vnlbhip_context {
     int search_offsets_width = 0;
     float refine_gap = 1e-3F;  // FAST form: relative eigen gap below which groups are re-solved in fp64
     bool fused_selection = true;  // FAST form: k-nearest selection inside the MFMA matcher
+    // Learned candidate proposal (vnlbhip_set_proposer): the proposer, its
+    // top_k, score margin and inference precision.
+    const vnlbhip_proposer* proposer = nullptr;
+    int proposer_top_k = 48;
+    float proposer_margin = 0.0F;
+    int proposer_precision = VNLBHIP_PRECISION_BF16;
     DeviceBuffer<float> weights{&generation};
     DeviceBuffer<double> accumulators{&generation};
     DeviceBuffer<int> summary{&generation};
@@ -633,6 +640,53 @@ struct vnlbhip_context {
         for (hipEvent_t e : event_pool) {
             (void)hipEventDestroy(e);
         }
+    }
+};
+
+// The learned candidate proposer: fp32 master parameters, Adam moments, the
+// bf16 copies the matrix products read, the FP8 inference copies with their
+// scales, and the training workspace.
+struct vnlbhip_proposer {
+    int device = 0;
+    std::string error;
+    float* params = nullptr;   // kPropParams
+    float* grads = nullptr;
+    float* adam_m = nullptr;
+    float* adam_v = nullptr;
+    __bf16* w1 = nullptr;      // [256][128]
+    __bf16* w2 = nullptr;      // [256][256]
+    uint8_t* w1_fp8 = nullptr;
+    uint8_t* w2_fp8 = nullptr;
+    float* fp8_scales = nullptr;  // kPropScales (proposer_types.h)
+    bool fp8_ready = false;
+    float* workspace = nullptr;
+    size_t workspace_floats = 0;
+    float* loss_scratch = nullptr;  // train_step without a loss buffer
+    int loss_capacity = 0;
+    int step = 0;
+    ~vnlbhip_proposer() {
+        (void)hipSetDevice(device);
+        for (void* p : {static_cast<void*>(params), static_cast<void*>(grads), static_cast<void*>(adam_m),
+                        static_cast<void*>(adam_v), static_cast<void*>(w1), static_cast<void*>(w2),
+                        static_cast<void*>(w1_fp8), static_cast<void*>(w2_fp8), static_cast<void*>(fp8_scales),
+                        static_cast<void*>(workspace), static_cast<void*>(loss_scratch)}) {
+            if (p != nullptr) {
+                (void)hipFree(p);
+            }
+        }
+    }
+    vnlbhip::ProposerWeights weights() const {
+        vnlbhip::ProposerWeights wt{};
+        wt.w1 = w1;
+        wt.b1 = params + vnlbhip::kOffB1;
+        wt.w2 = w2;
+        wt.b2 = params + vnlbhip::kOffB2;
+        wt.w3 = params + vnlbhip::kOffW3;
+        wt.b3 = params + vnlbhip::kOffB3;
+        wt.fp8.w1 = w1_fp8;
+        wt.fp8.w2 = w2_fp8;
+        wt.fp8.scales = fp8_scales;
+        return wt;
     }
 };
 
@@ -723,19 +777,31 @@ void finish_stage(vnlbhip_context& ctx, hipStream_t stream) {
     ctx.stats.retained = retained;
     ctx.stats.refined = host_summary[4];
     ctx.stats.groups = host_summary[5];
+    ctx.stats.exhaustive_fallbacks = host_summary[6];
     if (host_summary[0] != 0) {
         throw ModelFailure("group_size must cover the effective VNLB rank");
     }
 }
 
+// Match lists of one anchor frame copied out of the stage (vnlbhip_match_lists).
+struct ListsOut {
+    int capacity = 0;
+    int* anchor_count = nullptr;  // host
+    uint32_t* anchors = nullptr;  // device
+    int* counts = nullptr;
+    uint32_t* positions = nullptr;
+    float* distances = nullptr;
+};
+
 // only_frame: one anchor frame (local index) or kAllFrames; stack (with
-// only_frame): that anchor frame's contribution stack, packed.
+// only_frame): that anchor frame's contribution stack, packed; lists (with
+// only_frame): matching only, the lists copied out.
 constexpr int kAllFrames = INT_MIN;
 
 void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const float* noisy,
                const float* ref, float* out, const Geometry& g, hipStream_t stream,
-               int only_frame, DebugOutput* debug, float* stack = nullptr) {
-    const StageDesc d = make_desc(stage, p, g);
+               int only_frame, DebugOutput* debug, float* stack = nullptr, ListsOut* lists = nullptr) {
+    const StageDesc d = make_desc(stage, p, g, lists != nullptr);
     require(ctx.sizing || noisy != nullptr, "video data pointer must not be null");
     if (stage == VNLBHIP_STAGE_FINAL) {
         require(ctx.sizing || ref != nullptr, "Final needs the ref (basic) clip");
@@ -798,10 +864,20 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
         require(vnlbhip::match_shared_bytes(d, true) <= 140 * 1024,
                 "search window too large for guided matching's LDS budget");
     }
+    // Learned candidate proposal: the proposing matcher replaces the
+    // matcher of either form.
+    const bool learned = ctx.proposer != nullptr && d.similar > 1;
+    if (learned) {
+        require(!guided && vnlbhip::learned_match_supported(d),
+                "the learned proposer needs 8x8 patches, patch_time 1 or 2 and unguided search windows");
+        require(ctx.proposer_top_k >= d.similar, "the proposer's top_k must be at least group_size");
+        require(ctx.proposer_precision == VNLBHIP_PRECISION_BF16 || ctx.proposer->fp8_ready,
+                "fp8 proposals need a calibrated proposer (vnlbhip_proposer_calibrate)");
+    }
     // The MFMA matcher's anchor tiles and the fast filter's aggregation tile
     // assume windows centred on the anchor: guided stages keep the plain
     // matcher and filter.
-    if (ctx.form == VNLBHIP_FORM_FAST && d.patch == 8 && d.similar > 1 && !guided) {
+    if (ctx.form == VNLBHIP_FORM_FAST && d.patch == 8 && d.similar > 1 && !guided && !learned) {
         const int match_channels = stage == VNLBHIP_STAGE_FINAL ? g.channels : 1;
         fast_offsets = ctx.fast_offsets.reserve(4);
         float* rows = ctx.fast_rows.reserve(plane * match_channels * g.frames);
@@ -861,6 +937,13 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
         const int anchor_count = static_cast<int>(grid.anchors.size());
         ctx.stats.anchors += anchor_count;
         ++ctx.stats.batches;
+        if (learned) {
+            ctx.stats.proposed += anchor_count;
+        }
+        if (lists != nullptr) {
+            require(ctx.plans.size() == 1 && anchor_count <= lists->capacity,
+                    "the list buffers hold fewer anchors than the frame has");
+        }
 
         uint32_t* d_anchors = ctx.anchors.reserve(grid.anchors.size());
         int* d_offsets = ctx.offsets.reserve(grid.offsets.size());
@@ -959,7 +1042,14 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
             const int ab = grid.offsets[static_cast<size_t>(fb)];
             const int ae = grid.offsets[static_cast<size_t>(fe)];
             // Matching of the part's anchors (stage stream).
-            if (fast_matcher && ctx.fused_selection &&
+            if (learned) {
+                timed(ctx, VNLBHIP_KERNEL_MATCH, stream, "learned match kernel", [&]() {
+                    return vnlbhip::launch_learned_match(
+                        d, source, d_anchors + ab, ae - ab, d_counts + ab, d_positions + static_cast<size_t>(ab) * cap,
+                        d_distances + static_cast<size_t>(ab) * cap, ctx.proposer->weights(), ctx.proposer_top_k,
+                        ctx.proposer_margin, ctx.proposer_precision, summary + 6, stream);
+                });
+            } else if (fast_matcher && ctx.fused_selection &&
                 vnlbhip::fast_match_fused(d, tiles->max_cols, tiles->max_rows)) {
                 const int tb = tile_at(ab);
                 const int te = tile_at(ae);
@@ -1012,6 +1102,9 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
                                                  d_distances + static_cast<size_t>(ab) * cap,
                                                  ctx.search_offsets, ctx.search_offsets_temporal, stream);
                 });
+            }
+            if (lists != nullptr) {
+                continue;  // matching only
             }
             check(hipEventRecord(ctx.match_event[h], stream), "hipEventRecord");
             // Paste mask and group list of the part (side stream).
@@ -1107,11 +1200,23 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
             untimed(ctx, "summarize kernel", [&]() { return vnlbhip::launch_summarize(d_flags, d.rank > 0 ? d_status : nullptr, d_counts,
                                             d_part_list, d_count, capacity, d.models, summary, stream); });
         };
-        for (int h = 0; h < parts; ++h) {
+        for (int h = 0; h < parts && lists == nullptr; ++h) {
             check(hipStreamWaitEvent(stream, ctx.paste_event[h], 0), "hipStreamWaitEvent");
             const int ab = grid.offsets[static_cast<size_t>(part_frame[h])];
             const int ae = grid.offsets[static_cast<size_t>(part_frame[h + 1])];
             run_model(d_list + ab, d_group_count + h, ae - ab);
+        }
+        if (lists != nullptr && !ctx.sizing) {
+            *lists->anchor_count = anchor_count;
+            check(hipMemcpyAsync(lists->anchors, d_anchors, static_cast<size_t>(anchor_count) * sizeof(uint32_t),
+                                 hipMemcpyDeviceToDevice, stream), "copy anchors");
+            check(hipMemcpyAsync(lists->counts, d_counts, static_cast<size_t>(anchor_count) * sizeof(int),
+                                 hipMemcpyDeviceToDevice, stream), "copy counts");
+            check(hipMemcpyAsync(lists->positions, d_positions,
+                                 static_cast<size_t>(anchor_count) * cap * sizeof(uint32_t),
+                                 hipMemcpyDeviceToDevice, stream), "copy positions");
+            check(hipMemcpyAsync(lists->distances, d_distances, static_cast<size_t>(anchor_count) * cap * sizeof(float),
+                                 hipMemcpyDeviceToDevice, stream), "copy distances");
         }
         if (debug != nullptr) {
             // One part: its group count, read back for the diagnostics.
@@ -1296,6 +1401,12 @@ void append_settings(const vnlbhip_context& ctx, std::vector<long long>& key) {
     key.push_back(ctx.fused_selection ? 1 : 0);
     key.push_back(reinterpret_cast<long long>(ctx.search_offsets));
     key.push_back(ctx.search_offsets_temporal);
+    uint32_t margin_bits = 0;
+    std::memcpy(&margin_bits, &ctx.proposer_margin, sizeof(margin_bits));
+    key.push_back(reinterpret_cast<long long>(ctx.proposer));
+    key.push_back(ctx.proposer_top_k);
+    key.push_back(margin_bits);
+    key.push_back(ctx.proposer_precision);
 }
 
 int fail(vnlbhip_context* ctx, int code, const char* message) {
@@ -2106,6 +2217,350 @@ int vnlbhip_debug_frame(vnlbhip_context* context, int stage, const vnlbhip_param
         run_stage(*context, stage, *params, noisy, ref, nullptr,
                   Geometry{width, height, frames, channels}, nullptr, anchor_frame, &debug);
     });
+}
+
+}  // extern "C"
+
+// ---- Learned candidate proposer ----
+
+namespace {
+
+template <class Fn> int proposer_guarded(vnlbhip_proposer* proposer, Fn&& fn) {
+    try {
+        fn();
+        if (proposer != nullptr) {
+            proposer->error.clear();
+        }
+        return VNLBHIP_OK;
+    } catch (const InvalidArgument& e) {
+        if (proposer != nullptr) {
+            proposer->error = e.what();
+        }
+        return VNLBHIP_ERROR_INVALID;
+    } catch (const std::exception& e) {
+        if (proposer != nullptr) {
+            proposer->error = e.what();
+        }
+        return VNLBHIP_ERROR_DEVICE;
+    }
+}
+
+// Training buffers for `rows` input rows.
+void reserve_training(vnlbhip_proposer& p, int rows) {
+    const size_t need = vnlbhip::proposer_workspace_floats(rows);
+    if (need > p.workspace_floats) {
+        check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        if (p.workspace != nullptr) {
+            (void)hipFree(p.workspace);
+            p.workspace = nullptr;
+        }
+        check(hipMalloc(reinterpret_cast<void**>(&p.workspace), need * sizeof(float)), "hipMalloc proposer workspace");
+        p.workspace_floats = need;
+    }
+}
+
+void check_rows(const void* x, int rows, int group) {
+    require(x != nullptr && reinterpret_cast<uintptr_t>(x) % 16 == 0, "input rows must be 16-byte aligned device memory");
+    require(rows > 0 && group > 0 && rows % group == 0, "rows must be a positive multiple of group");
+    require(rows <= (1 << 22), "at most 4194304 rows per call");
+}
+
+}  // namespace
+
+extern "C" {
+
+int vnlbhip_proposer_create(int device, uint64_t seed, vnlbhip_proposer** out) {
+    if (out == nullptr) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    auto* p = new (std::nothrow) vnlbhip_proposer();
+    if (p == nullptr) {
+        return VNLBHIP_ERROR_DEVICE;
+    }
+    p->device = device;
+    const int rc = proposer_guarded(p, [&]() {
+        check(hipSetDevice(device), "hipSetDevice");
+        const size_t n = vnlbhip::kPropParams;
+        for (float** buf : {&p->params, &p->grads, &p->adam_m, &p->adam_v}) {
+            check(hipMalloc(reinterpret_cast<void**>(buf), n * sizeof(float)), "hipMalloc proposer");
+        }
+        check(hipMalloc(reinterpret_cast<void**>(&p->w1), vnlbhip::kPropHidden * vnlbhip::kPropIn * sizeof(__bf16)),
+              "hipMalloc proposer");
+        check(hipMalloc(reinterpret_cast<void**>(&p->w2), vnlbhip::kPropHidden * vnlbhip::kPropHidden * sizeof(__bf16)),
+              "hipMalloc proposer");
+        check(hipMalloc(reinterpret_cast<void**>(&p->w1_fp8), vnlbhip::kPropHidden * vnlbhip::kPropIn), "hipMalloc proposer");
+        check(hipMalloc(reinterpret_cast<void**>(&p->w2_fp8), vnlbhip::kPropHidden * vnlbhip::kPropHidden),
+              "hipMalloc proposer");
+        check(hipMalloc(reinterpret_cast<void**>(&p->fp8_scales), vnlbhip::kPropScales * sizeof(float)),
+              "hipMalloc proposer");
+        check(hipMemset(p->adam_m, 0, n * sizeof(float)), "hipMemset");
+        check(hipMemset(p->adam_v, 0, n * sizeof(float)), "hipMemset");
+        check(vnlbhip::launch_proposer_init(p->params, seed, nullptr), "proposer init kernel");
+        check(vnlbhip::launch_proposer_refresh(p->params, p->w1, p->w2, nullptr), "proposer refresh kernel");
+        check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    });
+    if (rc != VNLBHIP_OK) {
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return VNLBHIP_OK;
+}
+
+void vnlbhip_proposer_destroy(vnlbhip_proposer* proposer) { delete proposer; }
+
+const char* vnlbhip_proposer_last_error(const vnlbhip_proposer* proposer) {
+    return proposer == nullptr ? "null proposer" : proposer->error.c_str();
+}
+
+int vnlbhip_proposer_get_params(const vnlbhip_proposer* proposer, float* params) {
+    if (proposer == nullptr || params == nullptr) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    if (hipSetDevice(proposer->device) != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(params, proposer->params, vnlbhip::kPropParams * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) {
+        return VNLBHIP_ERROR_DEVICE;
+    }
+    return VNLBHIP_OK;
+}
+
+int vnlbhip_proposer_set_params(vnlbhip_proposer* proposer, const float* params) {
+    if (proposer == nullptr || params == nullptr) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    return proposer_guarded(proposer, [&]() {
+        check(hipSetDevice(proposer->device), "hipSetDevice");
+        check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        check(hipMemcpy(proposer->params, params, vnlbhip::kPropParams * sizeof(float), hipMemcpyHostToDevice),
+              "hipMemcpy");
+        check(vnlbhip::launch_proposer_refresh(proposer->params, proposer->w1, proposer->w2, nullptr),
+              "proposer refresh kernel");
+        check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        proposer->fp8_ready = false;
+    });
+}
+
+int vnlbhip_proposer_features(vnlbhip_proposer* proposer, const float* clip, int width, int height, int frames,
+                              int channels, int patch_time, const uint32_t* anchors, const uint32_t* candidates,
+                              int rows, void* out, void* stream) {
+    if (proposer == nullptr) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    return proposer_guarded(proposer, [&]() {
+        require(clip != nullptr && anchors != nullptr && candidates != nullptr && out != nullptr,
+                "pointers must not be null");
+        require(reinterpret_cast<uintptr_t>(out) % 16 == 0, "out must be 16-byte aligned");
+        require(width >= 9 && height >= 9 && width <= vnlbhip::kMaxSide && height <= vnlbhip::kMaxSide,
+                "frame sides must be in [9, 4096]");
+        require(frames > 0 && frames <= vnlbhip::kMaxFrames && channels > 0 && channels <= vnlbhip::kMaxChannels,
+                "frames in [1, 256], channels in [1, 4]");
+        require(patch_time == 1 || patch_time == 2, "patch_time must be 1 or 2");
+        require(rows >= 0, "rows must be non-negative");
+        check(hipSetDevice(proposer->device), "hipSetDevice");
+        // Positions are trusted only after a bounds check on the host side
+        // of the caller's data: the kernel reads 8x8 patches at them, so
+        // they are validated here on a host copy.
+        std::vector<uint32_t> a(static_cast<size_t>(rows));
+        std::vector<uint32_t> c(static_cast<size_t>(rows));
+        if (rows > 0) {
+            check(hipMemcpy(a.data(), anchors, a.size() * sizeof(uint32_t), hipMemcpyDeviceToHost), "read anchors");
+            check(hipMemcpy(c.data(), candidates, c.size() * sizeof(uint32_t), hipMemcpyDeviceToHost),
+                  "read candidates");
+        }
+        for (int r = 0; r < rows; ++r) {
+            for (const uint32_t q : {a[static_cast<size_t>(r)], c[static_cast<size_t>(r)]}) {
+                require(vnlbhip::position_x(q) + 8 <= width && vnlbhip::position_y(q) + 8 <= height &&
+                            vnlbhip::position_frame(q) + patch_time <= frames,
+                        "a patch position lies outside the clip");
+            }
+        }
+        check(vnlbhip::launch_proposer_features(clip, width, height, channels, patch_time, anchors, candidates, rows,
+                                                static_cast<__bf16*>(out), reinterpret_cast<hipStream_t>(stream)),
+              "proposer features kernel");
+    });
+}
+
+int vnlbhip_proposer_gradients(vnlbhip_proposer* proposer, const void* x, const float* targets, int rows, int group,
+                               float* grads, float* loss_groups, void* stream) {
+    if (proposer == nullptr) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    return proposer_guarded(proposer, [&]() {
+        check_rows(x, rows, group);
+        require(targets != nullptr && grads != nullptr && loss_groups != nullptr, "pointers must not be null");
+        check(hipSetDevice(proposer->device), "hipSetDevice");
+        reserve_training(*proposer, rows);
+        check(vnlbhip::launch_proposer_gradients(proposer->params, proposer->w1, proposer->w2,
+                                                 static_cast<const __bf16*>(x), targets, rows, group,
+                                                 proposer->workspace, grads, loss_groups,
+                                                 reinterpret_cast<hipStream_t>(stream)),
+              "proposer gradient kernels");
+    });
+}
+
+int vnlbhip_proposer_train_step(vnlbhip_proposer* proposer, const void* x, const float* targets, int rows, int group,
+                                float learning_rate, float* loss_groups, void* stream) {
+    if (proposer == nullptr) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    return proposer_guarded(proposer, [&]() {
+        check_rows(x, rows, group);
+        require(targets != nullptr, "targets must not be null");
+        require(std::isfinite(learning_rate) && learning_rate > 0.0F, "learning_rate must be positive");
+        check(hipSetDevice(proposer->device), "hipSetDevice");
+        reserve_training(*proposer, rows);
+        float* losses = loss_groups;
+        if (losses == nullptr) {
+            const int groups = rows / group;
+            if (groups > proposer->loss_capacity) {
+                check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+                if (proposer->loss_scratch != nullptr) {
+                    (void)hipFree(proposer->loss_scratch);
+                    proposer->loss_scratch = nullptr;
+                }
+                check(hipMalloc(reinterpret_cast<void**>(&proposer->loss_scratch), groups * sizeof(float)),
+                      "hipMalloc proposer losses");
+                proposer->loss_capacity = groups;
+            }
+            losses = proposer->loss_scratch;
+        }
+        const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+        check(vnlbhip::launch_proposer_gradients(proposer->params, proposer->w1, proposer->w2,
+                                                 static_cast<const __bf16*>(x), targets, rows, group,
+                                                 proposer->workspace, proposer->grads, losses, s),
+              "proposer gradient kernels");
+        ++proposer->step;
+        check(vnlbhip::launch_proposer_adam(proposer->params, proposer->grads, proposer->adam_m, proposer->adam_v,
+                                            learning_rate, proposer->step, s),
+              "proposer adam kernel");
+        check(vnlbhip::launch_proposer_refresh(proposer->params, proposer->w1, proposer->w2, s),
+              "proposer refresh kernel");
+        proposer->fp8_ready = false;
+    });
+}
+
+int vnlbhip_proposer_scores(vnlbhip_proposer* proposer, const void* x, int rows, int precision, float* scores,
+                            void* stream) {
+    if (proposer == nullptr) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    return proposer_guarded(proposer, [&]() {
+        check_rows(x, rows, 1);
+        require(scores != nullptr, "scores must not be null");
+        require(precision == VNLBHIP_PRECISION_BF16 || precision == VNLBHIP_PRECISION_FP8,
+                "precision must be VNLBHIP_PRECISION_BF16 or VNLBHIP_PRECISION_FP8");
+        require(precision == VNLBHIP_PRECISION_BF16 || proposer->fp8_ready,
+                "fp8 scores need a calibrated proposer (vnlbhip_proposer_calibrate)");
+        check(hipSetDevice(proposer->device), "hipSetDevice");
+        check(vnlbhip::launch_proposer_scores(static_cast<const __bf16*>(x), rows, proposer->weights(), precision,
+                                              scores, reinterpret_cast<hipStream_t>(stream)),
+              "proposer scores kernel");
+    });
+}
+
+int vnlbhip_proposer_calibrate(vnlbhip_proposer* proposer, const void* x, int rows, void* stream) {
+    if (proposer == nullptr) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    return proposer_guarded(proposer, [&]() {
+        check_rows(x, rows, 1);
+        check(hipSetDevice(proposer->device), "hipSetDevice");
+        reserve_training(*proposer, rows);
+        const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+        check(vnlbhip::launch_proposer_calibrate(proposer->params, proposer->w1, static_cast<const __bf16*>(x), rows,
+                                                 proposer->workspace, proposer->w1_fp8, proposer->w2_fp8,
+                                                 proposer->fp8_scales, s),
+              "proposer calibration kernels");
+        check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        proposer->fp8_ready = true;
+    });
+}
+
+int vnlbhip_set_proposer(vnlbhip_context* context, const vnlbhip_proposer* proposer, int top_k, float margin,
+                         int precision) {
+    if (context == nullptr) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    return guarded(context, [&]() {
+        if (proposer == nullptr) {
+            context->proposer = nullptr;
+            return;
+        }
+        require(proposer->device == context->device, "the proposer lives on another device");
+        require(top_k >= 1 && top_k < vnlbhip::kPropMaxProposed, "top_k must be in [1, 255]");
+        require(std::isfinite(margin), "margin must be finite");
+        require(precision == VNLBHIP_PRECISION_BF16 || precision == VNLBHIP_PRECISION_FP8,
+                "precision must be VNLBHIP_PRECISION_BF16 or VNLBHIP_PRECISION_FP8");
+        require(precision == VNLBHIP_PRECISION_BF16 || proposer->fp8_ready,
+                "fp8 proposals need a calibrated proposer (vnlbhip_proposer_calibrate)");
+        context->proposer = proposer;
+        context->proposer_top_k = top_k;
+        context->proposer_margin = margin;
+        context->proposer_precision = precision;
+    });
+}
+
+int vnlbhip_match_lists(vnlbhip_context* context, int stage, const vnlbhip_params* params, const float* noisy,
+                        const float* ref, int width, int height, int frames, int channels, int anchor_frame,
+                        int capacity, int* anchor_count, uint32_t* anchors, int* counts, uint32_t* positions,
+                        float* distances, void* stream) {
+    if (context == nullptr) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    return guarded(context, [&]() {
+        require(params != nullptr && anchor_count != nullptr && anchors != nullptr && counts != nullptr &&
+                    positions != nullptr && distances != nullptr,
+                "pointers must not be null");
+        require(anchor_frame >= 0 && anchor_frame < frames, "anchor_frame must be a frame of the clip");
+        ListsOut lists;
+        lists.capacity = capacity;
+        lists.anchor_count = anchor_count;
+        lists.anchors = anchors;
+        lists.counts = counts;
+        lists.positions = positions;
+        lists.distances = distances;
+        *anchor_count = 0;
+        run_stage(*context, stage, *params, noisy, ref, nullptr, Geometry{width, height, frames, channels},
+                  reinterpret_cast<hipStream_t>(stream), anchor_frame, nullptr, nullptr, &lists);
+    });
+}
+
+int vnlbhip_search_windows(const vnlbhip_params* params, int width, int height, int total_frames, int count,
+                           const uint32_t* anchors, int32_t* windows) {
+    if (params == nullptr || anchors == nullptr || windows == nullptr || count < 0) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    try {
+        const Geometry g{width, height, total_frames, 1};
+        const vnlbhip::StageDesc d = make_desc(VNLBHIP_STAGE_BASIC, *params, g);
+        for (int i = 0; i < count; ++i) {
+            const uint32_t a = anchors[i];
+            int t_low = 0;
+            int t_high = 0;
+            int x_low = 0;
+            int x_high = 0;
+            int y_low = 0;
+            int y_high = 0;
+            vnlbhip::temporal_range(d, vnlbhip::position_frame(a), t_low, t_high);
+            vnlbhip::spatial_range(d, vnlbhip::position_x(a), width, x_low, x_high);
+            vnlbhip::spatial_range(d, vnlbhip::position_y(a), height, y_low, y_high);
+            int32_t* w = windows + static_cast<size_t>(i) * 6;
+            w[0] = x_low;
+            w[1] = y_low;
+            w[2] = x_high - x_low + 1;
+            w[3] = y_high - y_low + 1;
+            w[4] = t_low;
+            w[5] = t_high;
+        }
+    } catch (const std::exception&) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    return VNLBHIP_OK;
 }
 
 }  // extern "C"

@@ -55,6 +55,7 @@ class Context:
         self.device = int(device)
         self.form = "plain"
         self._search_offsets = None
+        self._proposer = None
         self._caches = weakref.WeakSet()
 
     def close(self) -> None:
@@ -81,7 +82,62 @@ class Context:
         return {"anchors": s.anchors, "groups": s.groups,
                 "unconverged_groups": s.unconverged_groups, "batches": s.batches,
                 "retained": s.retained, "refined": s.refined, "parts": s.parts,
-                "graph_replays": s.graph_replays, "reallocations": s.reallocations}
+                "graph_replays": s.graph_replays, "reallocations": s.reallocations,
+                "proposed": s.proposed, "exhaustive_fallbacks": s.exhaustive_fallbacks}
+
+    def set_proposer(self, proposer, top_k: int = 48, margin: float = 0.0, precision: str = "bf16") -> None:
+        """Match through a learned candidate proposer (vnlbhip.proposer.Proposer;
+        None restores exhaustive matching): only the top_k scored candidates of
+        each window get exact distances, unless the score margin between the
+        K-th and the (top_k + 1)-th best is below `margin` (then that window is
+        scanned whole).  precision 'bf16' or 'fp8' (after calibration).  The
+        proposer is kept alive while it is set."""
+        if precision not in _abi.PRECISIONS:
+            raise VNLBError(f"VNLB HIP: precision must be one of {sorted(_abi.PRECISIONS)}")
+        handle = None if proposer is None else proposer._handle
+        rc = self._lib.vnlbhip_set_proposer(self._handle, handle, int(top_k), float(margin),
+                                            _abi.PRECISIONS[precision])
+        if rc != 0:
+            self._raise(rc)
+        self._proposer = proposer
+
+    def match_lists(self, stage: int, params: StageParameters, noisy, ref, anchor_frame: int) -> dict:
+        """Match lists of one anchor frame (matching only) with the context's
+        form and proposer: GPU tensors anchors [A] (packed x | y << 12 |
+        frame << 24, int32 bits), counts [A], positions [A, cap] (packed),
+        distances [A, cap]."""
+        import torch
+
+        frames, channels, height, width = _canonical_shape(tuple(noisy.shape))
+        self._check_offsets(frames, height, width)
+        if not _is_torch(noisy) or noisy.dtype != torch.float32 or not noisy.is_cuda:
+            raise VNLBError("VNLB HIP: device clips must be float32 tensors on the GPU")
+        if stage == FINAL and (ref is None or tuple(ref.shape) != tuple(noisy.shape)):
+            raise VNLBError("VNLB HIP: ref clip properties must match input clip")
+        p = _abi.to_struct(params)
+        cap = int(self._lib.vnlbhip_retained_cap(ctypes.byref(p)))
+        step = max(1, int(params.proc_step))
+        capacity = (-(-width // step) + 2) * (-(-height // step) + 2)
+        src = noisy.contiguous()
+        guide = ref.contiguous() if ref is not None else None
+        dev = src.device
+        anchors = torch.empty(capacity, dtype=torch.int32, device=dev)
+        counts = torch.empty(capacity, dtype=torch.int32, device=dev)
+        positions = torch.empty((capacity, cap), dtype=torch.int32, device=dev)
+        distances = torch.empty((capacity, cap), dtype=torch.float32, device=dev)
+        count = ctypes.c_int(0)
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        rc = self._lib.vnlbhip_match_lists(
+            self._handle, stage, ctypes.byref(p), ctypes.c_void_p(src.data_ptr()),
+            ctypes.c_void_p(guide.data_ptr()) if guide is not None else None, width, height, frames, channels,
+            int(anchor_frame), capacity, ctypes.byref(count), ctypes.c_void_p(anchors.data_ptr()),
+            ctypes.c_void_p(counts.data_ptr()), ctypes.c_void_p(positions.data_ptr()),
+            ctypes.c_void_p(distances.data_ptr()), ctypes.c_void_p(stream))
+        if rc != 0:
+            self._raise(rc)
+        a = count.value
+        return {"anchors": anchors[:a], "counts": counts[:a], "positions": positions[:a],
+                "distances": distances[:a]}
 
     def set_form(self, form: str) -> None:
         """'plain' (fp32 in the CPU path's order, the default) or 'fast'

@@ -44,7 +44,25 @@ EXPORTED_SYMBOLS = (
     "vnlbhip_frame_cache_invalidate",
     "vnlbhip_frame_cache_get_stats",
     "vnlbhip_frame_cache_destroy",
+    "vnlbhip_proposer_create",
+    "vnlbhip_proposer_destroy",
+    "vnlbhip_proposer_last_error",
+    "vnlbhip_proposer_get_params",
+    "vnlbhip_proposer_set_params",
+    "vnlbhip_proposer_features",
+    "vnlbhip_proposer_gradients",
+    "vnlbhip_proposer_train_step",
+    "vnlbhip_proposer_scores",
+    "vnlbhip_proposer_calibrate",
+    "vnlbhip_set_proposer",
+    "vnlbhip_match_lists",
+    "vnlbhip_search_windows",
 )
+
+PROPOSER_INPUT = 128
+PROPOSER_HIDDEN = 256
+PROPOSER_PARAMS = 99073
+PRECISIONS = {"bf16": 0, "fp8": 1}
 
 KERNEL_NAMES = ("match", "paste", "prepare", "jacobi", "filter", "normalize", "select", "energy")
 
@@ -97,6 +115,8 @@ class Stats(ctypes.Structure):
         ("refined", ctypes.c_int64),
         ("graph_replays", ctypes.c_int64),
         ("reallocations", ctypes.c_int64),
+        ("proposed", ctypes.c_int64),
+        ("exhaustive_fallbacks", ctypes.c_int64),
     ]
 
 
@@ -209,5 +229,31 @@ def load() -> ctypes.CDLL:
     lib.vnlbhip_frame_cache_destroy.restype = None
     lib.vnlbhip_get_kernel_times.argtypes = [vp, P(KernelTimes)]
     lib.vnlbhip_get_kernel_times.restype = ctypes.c_int
+    lib.vnlbhip_proposer_create.argtypes = [ci, ctypes.c_uint64, P(vp)]
+    lib.vnlbhip_proposer_create.restype = ci
+    lib.vnlbhip_proposer_destroy.argtypes = [vp]
+    lib.vnlbhip_proposer_destroy.restype = None
+    lib.vnlbhip_proposer_last_error.argtypes = [vp]
+    lib.vnlbhip_proposer_last_error.restype = ctypes.c_char_p
+    lib.vnlbhip_proposer_get_params.argtypes = [vp, fp]
+    lib.vnlbhip_proposer_get_params.restype = ci
+    lib.vnlbhip_proposer_set_params.argtypes = [vp, fp]
+    lib.vnlbhip_proposer_set_params.restype = ci
+    lib.vnlbhip_proposer_features.argtypes = [vp, vp, ci, ci, ci, ci, ci, vp, vp, ci, vp, vp]
+    lib.vnlbhip_proposer_features.restype = ci
+    lib.vnlbhip_proposer_gradients.argtypes = [vp, vp, vp, ci, ci, vp, vp, vp]
+    lib.vnlbhip_proposer_gradients.restype = ci
+    lib.vnlbhip_proposer_train_step.argtypes = [vp, vp, vp, ci, ci, ctypes.c_float, vp, vp]
+    lib.vnlbhip_proposer_train_step.restype = ci
+    lib.vnlbhip_proposer_scores.argtypes = [vp, vp, ci, ci, vp, vp]
+    lib.vnlbhip_proposer_scores.restype = ci
+    lib.vnlbhip_proposer_calibrate.argtypes = [vp, vp, ci, vp]
+    lib.vnlbhip_proposer_calibrate.restype = ci
+    lib.vnlbhip_set_proposer.argtypes = [vp, vp, ci, ctypes.c_float, ci]
+    lib.vnlbhip_set_proposer.restype = ci
+    lib.vnlbhip_match_lists.argtypes = [vp, ci, P(Params), vp, vp, ci, ci, ci, ci, ci, ci, P(ci), vp, vp, vp, vp, vp]
+    lib.vnlbhip_match_lists.restype = ci
+    lib.vnlbhip_search_windows.argtypes = [P(Params), ci, ci, ci, ci, P(ctypes.c_uint32), P(ctypes.c_int32)]
+    lib.vnlbhip_search_windows.restype = ci
     _lib = lib
     return lib
