@@ -18,6 +18,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO))
 sys.path.insert(0, str(REPO / "vapoursynth-vnlb_amd"))
@@ -145,6 +147,103 @@ def filter_roofline(form, times, bp, fp, clip, sb, sf) -> dict:
     }
 
 
+def train_proposer(ctx, steps: int, seed: int = 7):
+    """The learned candidate proposer, trained on the GPU (its own forward,
+    backward and Adam kernels) on the plain matcher's exact 16 nearest over
+    fresh seeded clips of the precision configuration's kind (GrayS, sigma
+    20, sub-pixel motion, bm_range 12, radius 2): about 200k (anchor, window)
+    pairs from a 960x540x6 clip, matched on the noisy clip (Basic) and on its
+    Basic estimate (Final).  Returns (proposer, summary)."""
+    import torch
+
+    from vnlbhip import proposer as prop
+    from vnlbhip import synth
+    from vnlbhip.params import BASIC, parse_stage_parameters
+
+    args = dict(sigma=20, block_size=8, block_step=4, group_size=16, bm_range=12, patch_time=2, radius=2, rank=4,
+                tau=0)
+    params = parse_stage_parameters(BASIC, args)
+    t0 = time.perf_counter()
+    noisy, _ = synth.make_clip(960, 540, 6, 1, sigma8=20, seed=seed, motion="subpixel")
+    x = torch.from_numpy(noisy).to("cuda:0")
+    basic = ctx.stage(BASIC, params, x)
+    examples = [prop.make_examples(ctx, x, params, anchor_frames=(1, 2, 3), seed=seed, max_anchors=100_000),
+                prop.make_examples(ctx, basic, params, anchor_frames=(1, 2, 3), seed=seed + 1, max_anchors=100_000)]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    proposer = prop.Proposer(0, seed=seed)
+    history = prop.train(proposer, examples, steps=steps, batch_anchors=64, learning_rate=2e-3,
+                         log=lambda s, v: log(f"[bench] proposer step {s}: loss {v:.4f}") if s % 500 == 0 else None)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    # FP8 copies calibrated on a sample of the training rows.
+    rows, _ = prop.batch_rows(proposer, examples[0], torch.arange(64, device="cuda:0"))
+    proposer.calibrate(rows)
+    return proposer, {
+        "pairs": sum(len(e) for e in examples),
+        "rows_per_pair": examples[0].group,
+        "steps": steps,
+        "labels_s": round(t1 - t0, 2),
+        "train_s": round(t2 - t1, 2),
+        "loss_first": round(history[0][1], 4),
+        "loss_last": round(float(np.mean([v for _, v in history[-4:]])), 4),
+    }
+
+
+def learned_form(ctx, proposer, step, bp, fp, noisy, clean_np, final_plain_np, pixels, steps: int, margin: float,
+                 precision: str) -> dict:
+    """The combined form (the context's form with learned proposals): step
+    time, quality against the plain form, recall of one anchor frame's lists,
+    exhaustive-fallback share and the proposing matcher's time."""
+    import torch
+
+    from vnlbhip import synth
+    from vnlbhip.params import BASIC, FINAL
+
+    exact = {}
+    ctx.set_proposer(None)
+    form = ctx.form
+    ctx.set_form("plain")
+    frame = min(2, int(noisy.shape[0]) - 3)
+    exact[BASIC] = ctx.match_lists(BASIC, bp, noisy, None, frame)
+    ctx.set_form(form)
+    ctx.set_proposer(proposer, top_k=48, margin=margin, precision=precision)
+    got = ctx.match_lists(BASIC, bp, noisy, None, frame)
+    k = bp.similar
+    g = got["positions"][:, :k].cpu().numpy()
+    w = exact[BASIC]["positions"][:, :k].cpu().numpy()
+    recall = float(np.mean([len(set(a) & set(b)) / k for a, b in zip(g, w)]))
+    step()  # warm (the first call of a stage captures its graph)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, final, sb, sf = step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1000.0 / steps
+    ctx.set_profiling(True)
+    step()
+    torch.cuda.synchronize()
+    times = ctx.kernel_times()
+    ctx.set_profiling(False)
+    ctx.set_proposer(None)
+    final_np = final.cpu().numpy()
+    psnr = synth.psnr(final_np, clean_np)
+    share = (sb["exhaustive_fallbacks"] + sf["exhaustive_fallbacks"]) / max(1, sb["proposed"] + sf["proposed"])
+    return {
+        "precision": precision,
+        "top_k": 48,
+        "margin": margin,
+        "mpix_s": round(pixels / (ms / 1000.0) / 1e6, 4),
+        "ms_per_step": round(ms, 3),
+        "dpsnr_vs_plain_db": round(psnr - synth.psnr(final_plain_np, clean_np), 4),
+        "max_abs_vs_plain": float(np.abs(final_np - final_plain_np).max()),
+        "recall_at_16_basic": round(recall, 4),
+        "exhaustive_fallback_share": round(share, 4),
+        "match_ms": round(times["match"]["ms"], 3),
+        "match_launches": times["match"]["launches"],
+    }
+
+
 def cpu_baseline(workload: dict, seconds_target: float = 20.0) -> dict:
     """The CPU oracle on a crop of the same clip, frame-parallel over the
     host cores the box gives this job (at most 16)."""
@@ -189,6 +288,14 @@ def main() -> int:
                         help="keep the default pipeline part count instead of timing the candidates")
     parser.add_argument("--form", default="fast", choices=("fast", "plain"),
                         help="fast: MFMA candidate distances (the metric's form); plain: exact fp32 order")
+    parser.add_argument("--no-learned", action="store_true",
+                        help="skip the learned-proposal (combined) form's report")
+    parser.add_argument("--proposer-steps", type=int, default=3000,
+                        help="Adam steps of the proposer's training (64 anchors x 128 rows each)")
+    parser.add_argument("--learned-steps", type=int, default=1,
+                        help="timed steps of the combined form (the slowest form; after one warm step)")
+    parser.add_argument("--learned-margin", type=float, default=0.5,
+                        help="score margin below which an anchor's window is searched exhaustively")
     parser.add_argument("--frame-cache", action="store_true",
                         help="also time per-output-frame rendering over the anchor-stack cache "
                              "(Basic then Final, every frame once, in order)")
@@ -286,6 +393,23 @@ def main() -> int:
         log(f"[bench] plain form: {plain_ms:.1f} ms/step, dPSNR {psnr_final - psnr_plain:+.4f} dB")
         ctx.set_form(args.form)
 
+    learned = {}
+    if not args.no_learned and args.form != "plain":
+        # The combined form: this form's model and aggregation with candidates
+        # proposed by the learned network (bf16, and fp8 after calibration).
+        log(f"[bench] training the candidate proposer ({args.proposer_steps} steps)")
+        ctx.set_autotune(False)
+        proposer, training = train_proposer(ctx, args.proposer_steps)
+        log(f"[bench] proposer: {training}")
+        learned = {"learned_training": training}
+        for precision in ("bf16", "fp8"):
+            log(f"[bench] combined form, {precision} proposals")
+            entry = learned_form(ctx, proposer, step, bp, fp, noisy, clean_np, plain_np, pixels,
+                                 args.learned_steps, args.learned_margin, precision)
+            log(f"[bench] combined form {precision}: {entry}")
+            learned[f"learned_{precision}"] = entry
+        proposer.close()
+
     frame_cache = {}
     if args.frame_cache:
         # Frame-request rendering: each output frame from the cached anchor
@@ -336,6 +460,7 @@ def main() -> int:
             "psnr_noisy_db": round(synth.psnr(noisy_np, clean_np), 3),
             "psnr_final_db": round(psnr_final, 4),
             **quality,
+            **learned,
             **frame_cache,
             "kernel_ms": {k: round(v["ms"], 3) for k, v in times.items()},
             "retained_basic": sb["retained"], "retained_final": sf["retained"],
