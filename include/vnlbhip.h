@@ -403,6 +403,15 @@ int vnlbhip_mvtools_offsets(vnlbhip_context* context, const vnlbhip_params* para
  * are rejected with VNLBHIP_ERROR_INVALID. */
 int vnlbhip_set_fused_selection(vnlbhip_context* context, int enable);
 
+/* Eigen solver limits: max_sweeps (JacobiEigenOptions::max_sweeps,
+ * src/linalg/linalg.hpp:154-158, default 64; 1 .. 10000) for every Jacobi
+ * solver of the context, and what an unconverged problem does: 0 (default)
+ * its result is used and vnlbhip_stats.unconverged_groups counts it, as the
+ * CPU path does (topk_symmetric_eigen_jacobi's status is not checked by
+ * src/core/core.cpp); 1 the stage fails with VNLBHIP_ERROR_MODEL, as the
+ * CUDA plugin's solver status check does (src/vs/cuda_plugin.cpp:1831-1846). */
+int vnlbhip_set_jacobi(vnlbhip_context* context, int max_sweeps, int fail_unconverged);
+
 /* ---- Learned candidate proposal (no counterpart in the reference) ----
  *
  * The reference scans every candidate of the search window

@@ -134,6 +134,8 @@ def load() -> ctypes.CDLL:
                                      P(ctypes.c_ubyte)]
         lib.vo_set_search_offsets.argtypes = [P(ctypes.c_int16), ctypes.c_int]
         lib.vo_set_search_offsets.restype = None
+        lib.vo_set_max_sweeps.argtypes = [ctypes.c_int]
+        lib.vo_set_max_sweeps.restype = None
         mv = P(MvShape)
         i32, u8 = P(ctypes.c_int32), P(ctypes.c_ubyte)
         lib.vo_center_for.argtypes = [mv, i32, u8, mv, i32, u8] + [ctypes.c_int] * 8 + [ip]
@@ -195,6 +197,11 @@ def set_search_offsets(offsets, temporal: int = 0) -> None:
         return
     _offsets = np.ascontiguousarray(offsets, dtype=np.int16)
     lib.vo_set_search_offsets(_offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int16)), int(temporal))
+
+
+def set_max_sweeps(max_sweeps: int = 64) -> None:
+    """Jacobi sweep limit of every eigen solve (64 restores the default)."""
+    load().vo_set_max_sweeps(int(max_sweeps))
 
 
 def anchor_stack(stage_id: int, params, noisy, ref, anchor_frame: int, flags: int = ACCUM_DOUBLE):

@@ -290,9 +290,13 @@ struct EigenStatus {
     bool converged = true;
 };
 
+// JacobiEigenOptions::max_sweeps (linalg.hpp:153-158): 64 unless
+// vo_set_max_sweeps changed it.
+int g_max_sweeps = 64;
+
 // topk_symmetric_eigen_jacobi (linalg.cpp:447-519) with the default
-// JacobiEigenOptions (linalg.hpp:153-158): 64 sweeps, relative tolerance
-// 64*eps, absolute tolerance 0.  values[rank], vectors[rank][n].
+// JacobiEigenOptions (linalg.hpp:153-158): g_max_sweeps sweeps, relative
+// tolerance 64*eps, absolute tolerance 0.  values[rank], vectors[rank][n].
 template <class T>
 EigenStatus topk_eigen(const T* sym, int n, int rank, T* values, T* vectors,
                        std::vector<T>& m, std::vector<T>& v,
@@ -323,7 +327,7 @@ EigenStatus topk_eigen(const T* sym, int n, int rank, T* values, T* vectors,
     EigenStatus status;
     if (n > 1) {
         status.converged = false;
-        for (int sweep = 0; sweep < 64; ++sweep) {
+        for (int sweep = 0; sweep < g_max_sweeps; ++sweep) {
             status.sweeps = sweep + 1;
             if (max_offdiag() <= threshold) {
                 status.converged = true;
@@ -1809,6 +1813,8 @@ void vo_set_search_offsets(const int16_t* offsets, int temporal) {
     g_offsets = offsets;
     g_offsets_temporal = offsets != nullptr ? temporal : 0;
 }
+
+void vo_set_max_sweeps(int max_sweeps) { g_max_sweeps = max_sweeps > 0 ? max_sweeps : 64; }
 
 namespace {
 

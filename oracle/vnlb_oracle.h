@@ -104,6 +104,10 @@ int vo_frame_debug(int stage, const vo_params* params, const float* noisy,
  * keeps the anchor.  NULL clears it.  The pointer must stay valid while set. */
 void vo_set_search_offsets(const int16_t* offsets, int temporal);
 
+/* JacobiEigenOptions::max_sweeps of every eigen solve (src/linalg/linalg.hpp:
+ * 153-158); 0 or less restores the default 64. */
+void vo_set_max_sweeps(int max_sweeps);
+
 /* The fields of one MVTools vector clip that MVToolsVectorGrid::nearest_vector
  * reads (src/flow/flow.hpp:38-56,176-222). */
 typedef struct vo_mv_shape {

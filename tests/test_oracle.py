@@ -168,6 +168,24 @@ def test_set_rule_equals_reference_procedure(oracle_lib, name):
     np.testing.assert_allclose(out_a, out_b, atol=1e-6)
 
 
+def test_model_cap_factor_row_order_effect(oracle_lib):
+    """model_cap_factor 2 makes retained rows beyond the K nearest into model
+    rows.  The reference takes them in the order std::partial_sort's heap
+    leaves them (core.cpp:1212-1236, reproduced by SELECT_FAITHFUL); the set
+    rule (and the GPU) takes them in scan order.  Same sets and groups; the
+    measured effect on the output is recorded here."""
+    noisy, clean, bp, _ = workloads.make("gray_model_cap2")
+    set_rule, g1 = oracle_lib.stage(0, bp, noisy, flags=1, threads=4)
+    faithful, g2 = oracle_lib.stage(0, bp, noisy, flags=1 | oracle_lib.SELECT_FAITHFUL, threads=4)
+    counts = oracle_lib.frame_debug(0, bp, noisy, None, 2, flags=1)["counts"]
+    assert g1 == g2
+    assert counts.max() > 2 * bp.similar  # rows beyond K do become model rows
+    dpsnr = synth.psnr(set_rule, clean) - synth.psnr(faithful, clean)
+    max_abs = float(np.abs(set_rule - faithful).max())
+    print(f"model_cap_factor 2: set rule vs reference row order: max-abs {max_abs:.2e}, dPSNR {dpsnr:+.4f} dB")
+    assert abs(dpsnr) <= 0.02 and max_abs <= 1e-2
+
+
 def test_match_distances_brute_force(oracle_lib):
     """Distances and k-NN sets against a float64 numpy brute force over the
     temporal window and the boundary-shifted spatial window."""

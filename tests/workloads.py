@@ -55,6 +55,14 @@ WORKLOADS = {
         dict(sigma=25, block_size=8, block_step=4, group_size=16, bm_range=12, patch_time=2, radius=3,
              rank=16, tau=200, cap_factor=4.0, model_cap_factor=1.0, chroma=1),
         None, None),
+    # GrayS at low noise with a large tau: groups widen from K = 16 up to the
+    # cap of 64, and model_cap_factor 2 makes up to 32 of them model rows
+    # (rows beyond K included).
+    "gray_model_cap2": (
+        dict(width=80, height=64, frames=5, channels=1, sigma8=5, seed=17, motion="subpixel"),
+        dict(sigma=5, block_size=8, block_step=4, group_size=16, bm_range=7, patch_time=2, radius=1,
+             rank=16, tau=10000, cap_factor=4.0, model_cap_factor=2.0),
+        None, None),
     # YUV444 with uncoupled channels (three per-channel models).
     "yuv_uncoupled": (
         dict(width=64, height=56, frames=4, channels=3, sigma8=15, seed=14, motion="integer"),

@@ -23,9 +23,9 @@ constexpr int kLanes = 16;
 constexpr int kMaxPatch = 16;
 constexpr int kMaxDim = 1024;          // C * Pt * P^2
 constexpr int kMaxModelRows = 64;      // B = min(S, ceil(K * model_cap))
-constexpr int kMaxCap = 256;           // retained patches per group
+constexpr int kMaxCap = 1024;          // retained patches per group
 constexpr int kMaxRank = 64;
-constexpr int kMaxCandidates = 8192;   // window^2 * temporal slots
+constexpr int kMaxCandidates = 32768;  // window^2 * temporal slots (the matchers' LDS budget binds first)
 constexpr int kMaxChannels = 4;
 constexpr int kMaxTemporal = 64;
 constexpr int kMaxFrames = 256;

@@ -236,7 +236,7 @@ StageDesc make_desc(int stage, const vnlbhip_params& p, const Geometry& g, bool 
     require(g.channels == 1 || g.channels == 3, "only GrayS and YUV444PS clips are supported");
     require(p.patch_size <= vnlbhip::kMaxPatch, "block_size above 16 is not supported");
     require(d.dim <= vnlbhip::kMaxDim, "patch dimension above 1024 is not supported");
-    require(d.cap <= vnlbhip::kMaxCap, "retained group cap above 256 is not supported");
+    require(d.cap <= vnlbhip::kMaxCap, "retained group cap above 1024 is not supported");
     // Model limits do not bind a call that only matches (vnlbhip_match_lists).
     require(matching_only || d.rank <= vnlbhip::kMaxRank, "rank above 64 is not supported");
     require(matching_only || d.jacobi_dim <= 64,
@@ -248,7 +248,7 @@ StageDesc make_desc(int stage, const vnlbhip_params& p, const Geometry& g, bool 
                                                    static_cast<long long>(max_origin) + 1);
     const long long candidates = static_cast<long long>(p.search_window) * p.search_window * temporal;
     require(candidates <= vnlbhip::kMaxCandidates,
-            "search window times temporal slots above 8192 candidates is not supported");
+            "search window times temporal slots above 32768 candidates is not supported");
     require(d.search_bwd + d.search_fwd + 1 <= vnlbhip::kMaxTemporal,
             "temporal search radius too large");
     require(matching_only || vnlbhip::prepare_shared_bytes(d) <= 96 * 1024,
@@ -557,6 +557,11 @@ struct vnlbhip_context {
     int proposer_top_k = 48;
     float proposer_margin = 0.0F;
     int proposer_precision = VNLBHIP_PRECISION_BF16;
+    // Jacobi sweep limit (JacobiEigenOptions::max_sweeps, 64 by default) and
+    // whether unconverged problems fail the stage (the CUDA plugin) or are
+    // only counted (the CPU path).
+    int max_sweeps = 64;
+    bool fail_unconverged = false;
     DeviceBuffer<float> weights{&generation};
     DeviceBuffer<double> accumulators{&generation};
     DeviceBuffer<int> summary{&generation};
@@ -781,6 +786,12 @@ void finish_stage(vnlbhip_context& ctx, hipStream_t stream) {
     if (host_summary[0] != 0) {
         throw ModelFailure("group_size must cover the effective VNLB rank");
     }
+    if (ctx.fail_unconverged && host_summary[1] != 0) {
+        // The CUDA plugin's solver status check (src/vs/cuda_plugin.cpp:1831-1846).
+        throw ModelFailure(("the eigen solver did not converge within " + std::to_string(ctx.max_sweeps) +
+                            " sweeps for " + std::to_string(host_summary[1]) + " problems")
+                               .c_str());
+    }
 }
 
 // Match lists of one anchor frame copied out of the stage (vnlbhip_match_lists).
@@ -801,7 +812,11 @@ constexpr int kAllFrames = INT_MIN;
 void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const float* noisy,
                const float* ref, float* out, const Geometry& g, hipStream_t stream,
                int only_frame, DebugOutput* debug, float* stack = nullptr, ListsOut* lists = nullptr) {
-    const StageDesc d = make_desc(stage, p, g, lists != nullptr);
+    const StageDesc d = [&]() {
+        StageDesc desc = make_desc(stage, p, g, lists != nullptr);
+        desc.max_sweeps = ctx.max_sweeps;
+        return desc;
+    }();
     require(ctx.sizing || noisy != nullptr, "video data pointer must not be null");
     if (stage == VNLBHIP_STAGE_FINAL) {
         require(ctx.sizing || ref != nullptr, "Final needs the ref (basic) clip");
@@ -1407,6 +1422,7 @@ void append_settings(const vnlbhip_context& ctx, std::vector<long long>& key) {
     key.push_back(ctx.proposer_top_k);
     key.push_back(margin_bits);
     key.push_back(ctx.proposer_precision);
+    key.push_back(ctx.max_sweeps);
 }
 
 int fail(vnlbhip_context* ctx, int code, const char* message) {
@@ -1696,6 +1712,15 @@ int vnlbhip_set_fused_selection(vnlbhip_context* context, int enable) {
         return VNLBHIP_ERROR_INVALID;
     }
     context->fused_selection = enable != 0;
+    return VNLBHIP_OK;
+}
+
+int vnlbhip_set_jacobi(vnlbhip_context* context, int max_sweeps, int fail_unconverged) {
+    if (context == nullptr || max_sweeps < 1 || max_sweeps > 10000 || (fail_unconverged != 0 && fail_unconverged != 1)) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    context->max_sweeps = max_sweeps;
+    context->fail_unconverged = fail_unconverged != 0;
     return VNLBHIP_OK;
 }
 

@@ -149,6 +149,15 @@ class Context:
             self._raise(rc)
         self.form = form
 
+    def set_jacobi(self, max_sweeps: int = 64, fail_unconverged: bool = False) -> None:
+        """Jacobi sweep limit of every eigen solve (default 64) and whether a
+        problem that does not converge within it fails the stage (the CUDA
+        plugin's behaviour) or is used and counted in
+        stats()["unconverged_groups"] (the CPU path's, the default)."""
+        rc = self._lib.vnlbhip_set_jacobi(self._handle, int(max_sweeps), 1 if fail_unconverged else 0)
+        if rc != 0:
+            raise VNLBError("VNLB HIP: max_sweeps must be in [1, 10000]")
+
     def set_eigen_refine(self, relative_gap: float) -> None:
         """Fast form: re-solve in fp64 the group models whose eigenvalue gap at
         the rank cutoff is below relative_gap times the largest eigenvalue

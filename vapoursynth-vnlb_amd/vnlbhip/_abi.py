@@ -44,6 +44,7 @@ EXPORTED_SYMBOLS = (
     "vnlbhip_frame_cache_invalidate",
     "vnlbhip_frame_cache_get_stats",
     "vnlbhip_frame_cache_destroy",
+    "vnlbhip_set_jacobi",
     "vnlbhip_proposer_create",
     "vnlbhip_proposer_destroy",
     "vnlbhip_proposer_last_error",
@@ -229,6 +230,8 @@ def load() -> ctypes.CDLL:
     lib.vnlbhip_frame_cache_destroy.restype = None
     lib.vnlbhip_get_kernel_times.argtypes = [vp, P(KernelTimes)]
     lib.vnlbhip_get_kernel_times.restype = ctypes.c_int
+    lib.vnlbhip_set_jacobi.argtypes = [vp, ci, ci]
+    lib.vnlbhip_set_jacobi.restype = ci
     lib.vnlbhip_proposer_create.argtypes = [ci, ctypes.c_uint64, P(vp)]
     lib.vnlbhip_proposer_create.restype = ci
     lib.vnlbhip_proposer_destroy.argtypes = [vp]
