@@ -99,7 +99,7 @@ def test_validate_messages(change, message):
 def test_validate_reports_device_limits():
     p = parse_stage_parameters(BASIC, dict(sigma=10, bm_range=60, radius=1))
     rc, text = _validate(BASIC, p, 256, 256, 4, 1)
-    assert rc == -1 and "8192 candidates" in text
+    assert rc == -1 and "32768 candidates" in text
 
 
 def test_default_params_entry_point():
