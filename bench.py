@@ -244,19 +244,25 @@ def learned_form(ctx, proposer, step, bp, fp, noisy, clean_np, final_plain_np, p
     }
 
 
-def cpu_baseline(workload: dict, seconds_target: float = 20.0) -> dict:
-    """The CPU oracle on a crop of the same clip, frame-parallel over the
-    host cores the box gives this job (at most 16)."""
-    import numpy as np
+def _cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
+
+def _oracle_rate(workload: dict, width: int, height: int, threads: int) -> tuple[float, float]:
+    """(Mpix/s, seconds) of the oracle's Basic + Final on a width x height
+    crop of the workload's clip."""
     from oracle import oracle
     from vnlbhip import synth
     from vnlbhip.params import BASIC, FINAL, parse_stage_parameters
 
     clip = dict(workload["clip"])
-    threads = min(16, os.cpu_count() or 1)
-    crop_w, crop_h = 480, 270
-    clip.update(width=crop_w, height=crop_h)
+    clip.update(width=width, height=height)
     noisy, _ = synth.make_clip(**clip)
     bp = parse_stage_parameters(BASIC, workload["args"])
     fp = parse_stage_parameters(FINAL, workload["args"])
@@ -264,14 +270,34 @@ def cpu_baseline(workload: dict, seconds_target: float = 20.0) -> dict:
     basic, _ = oracle.stage(BASIC, bp, noisy, flags=oracle.SELECT_FAITHFUL, threads=threads)
     oracle.stage(FINAL, fp, noisy, basic, flags=oracle.SELECT_FAITHFUL, threads=threads)
     elapsed = time.perf_counter() - t0
-    pixels = crop_w * crop_h * clip["frames"]
+    return width * height * clip["frames"] / elapsed / 1e6, elapsed
+
+
+def cpu_baseline(workload: dict) -> dict:
+    """The CPU oracle (a C++ restatement of the reference's CPU path, with its
+    sequential early-exit selection), frame-parallel over the host cores this
+    job has (at most 16) on a 480x270 crop of the same clip, beside a
+    1-thread run on a 160x90 crop and the reference CPU configuration
+    (GrayS 256x256x5) in full at both thread counts."""
+    threads = min(16, os.cpu_count() or 1)
+    value, seconds = _oracle_rate(workload, 480, 270, threads)
+    one, one_s = _oracle_rate(workload, 160, 90, 1)
+    small = WORKLOADS["grays_256_5f"]
+    cfg1, cfg1_s = _oracle_rate(small, 256, 256, threads)
+    cfg1_one, cfg1_one_s = _oracle_rate(small, 256, 256, 1)
+    frames = workload["clip"]["frames"]
     return {
-        "value": pixels / elapsed / 1e6,
+        "value": round(value, 4),
         "unit": "Mpix/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"Basic+Final of the same workload on a {crop_w}x{crop_h}x{clip['frames']} crop "
-                  f"({elapsed:.1f} s, frame-parallel, reference sequential selection with early exit)",
+        "sample": f"Basic+Final of the same workload on a 480x270x{frames} crop ({seconds:.1f} s, frame-parallel, "
+                  f"reference sequential selection with early exit)",
+        "nproc": os.cpu_count(),
+        "cpu_model": _cpu_model(),
+        "one_thread": {"value": round(one, 4), "sample": f"160x90x{frames} crop ({one_s:.1f} s)"},
+        "grays_256_5f": {"threads": threads, "value": round(cfg1, 4), "seconds": round(cfg1_s, 2),
+                         "one_thread_value": round(cfg1_one, 4), "one_thread_seconds": round(cfg1_one_s, 2)},
     }
 
 
@@ -288,6 +314,8 @@ def main() -> int:
                         help="keep the default pipeline part count instead of timing the candidates")
     parser.add_argument("--form", default="fast", choices=("fast", "plain"),
                         help="fast: MFMA candidate distances (the metric's form); plain: exact fp32 order")
+    parser.add_argument("--plain-steps", type=int, default=5,
+                        help="timed steps of the plain form (after one warm step)")
     parser.add_argument("--no-learned", action="store_true",
                         help="skip the learned-proposal (combined) form's report")
     parser.add_argument("--proposer-steps", type=int, default=3000,
@@ -370,7 +398,7 @@ def main() -> int:
     roofline = rooflines[dominant]
 
     # The plain form on the same clip: quality reference of the metric and
-    # its own throughput (one untimed warm step, one timed step).
+    # its own throughput (one untimed warm step, then --plain-steps timed).
     quality = {}
     final_np = final.cpu().numpy()
     psnr_final = synth.psnr(final_np, clean_np)
@@ -379,9 +407,10 @@ def main() -> int:
         step()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        _, final_plain, _, _ = step()
+        for _ in range(args.plain_steps):
+            _, final_plain, _, _ = step()
         torch.cuda.synchronize()
-        plain_ms = (time.perf_counter() - t0) * 1000.0
+        plain_ms = (time.perf_counter() - t0) * 1000.0 / args.plain_steps
         plain_np = final_plain.cpu().numpy()
         psnr_plain = synth.psnr(plain_np, clean_np)
         quality = {
@@ -389,6 +418,8 @@ def main() -> int:
             "dpsnr_vs_plain_db": round(psnr_final - psnr_plain, 4),
             "max_abs_vs_plain": float(np.abs(final_np - plain_np).max()),
             "plain_mpix_s": round(pixels / (plain_ms / 1000.0) / 1e6, 4),
+            "plain_steps": args.plain_steps,
+            "fast_over_plain": round(plain_ms / ms_per_step, 4),
         }
         log(f"[bench] plain form: {plain_ms:.1f} ms/step, dPSNR {psnr_final - psnr_plain:+.4f} dB")
         ctx.set_form(args.form)
