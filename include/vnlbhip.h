@@ -79,6 +79,8 @@ typedef struct vnlbhip_stats {
     int64_t reallocations;        /* device buffers the last call (re)allocated */
     int64_t proposed;             /* anchors matched by the learned proposer */
     int64_t exhaustive_fallbacks; /* of those, anchors whose score margin was too low */
+    int32_t tile_rows;            /* FAST form: anchor tile of the MFMA matcher, grid rows x */
+    int32_t tile_cols;            /* anchors per row (0 x 0 when it did not run) */
 } vnlbhip_stats;
 
 typedef struct vnlbhip_context vnlbhip_context;
@@ -289,6 +291,16 @@ int vnlbhip_set_graph(vnlbhip_context* context, int enable);
  * vnlbhip_stats.parts reports the count used.  No counterpart in the
  * reference.  Values other than 0 and 1 are rejected. */
 int vnlbhip_set_autotune(vnlbhip_context* context, int enable);
+
+/* FAST form: anchor tiles of the MFMA matcher, `rows` grid rows x `cols`
+ * anchors per row (rows * cols <= 16; default 4 x 4), for every call the
+ * autotuner does not set (with autotuning on, the first vnlbhip_stage call
+ * of a stage, form, geometry and parameter set also times every shape of
+ * 4x4, 2x8, 8x2, 3x5, 2x6, 2x5, 4x2 and 2x4 whose window union fits the
+ * kernel and keeps the fastest).  A shape whose union does not fit falls
+ * back to 2 x 4.  vnlbhip_stats.tile_rows / tile_cols report the shape
+ * used.  No counterpart in the reference. */
+int vnlbhip_set_tile_shape(vnlbhip_context* context, int rows, int cols);
 
 /* Per-anchor search centres from a motion field (MVToolsFlowProvider's
  * centres, src/flow/flow.hpp:321-390, as a dense input; the reference

@@ -16,6 +16,7 @@ import pytest
 import torch
 
 import vnlbhip
+import workloads
 from vnlbhip import synth
 from vnlbhip.params import BASIC, FINAL, parse_stage_parameters
 
@@ -300,6 +301,40 @@ def test_autotune_picks_a_part_count_and_keeps_results():
     finally:
         tuned.close()
         default.close()
+
+
+@pytest.mark.parametrize("name", ["gray_pt2_step4", "gray_bm12_radius2"])
+def test_tile_shapes_give_the_same_lists(name):
+    """The MFMA matcher's anchor tiles only group anchors for the matrix
+    products: every shape gives the same distances (the same products in the
+    same order per anchor pair) and the same lists; the autotuner keeps one
+    of them."""
+    noisy, _, bp, _ = workloads.make(name)
+    x = _dev(noisy)
+    x = x[:, None].contiguous() if x.dim() == 3 else x
+    ctx = vnlbhip.Context(0)
+    try:
+        ctx.set_form("fast")
+        want = ctx.match_lists(BASIC, bp, x, None, 2)
+        used = set()
+        for rows, cols in ((4, 4), (2, 8), (8, 2), (3, 5), (2, 6), (2, 5), (4, 2), (2, 4), (1, 16)):
+            print(f"[fast] {name}: tile {rows}x{cols}", flush=True)
+            ctx.set_tile_shape(rows, cols)
+            got = ctx.match_lists(BASIC, bp, x, None, 2)
+            used.add(ctx.stats()["tile_shape"])
+            for key in ("anchors", "counts", "positions", "distances"):
+                torch.testing.assert_close(got[key], want[key], rtol=0, atol=0)
+        assert len(used) > 1
+        with pytest.raises(vnlbhip.VNLBError):
+            ctx.set_tile_shape(4, 5)
+        ctx.set_tile_shape(4, 4)
+        ctx.set_autotune(True)
+        ctx.stage(BASIC, bp, x)
+        shape = ctx.stats()["tile_shape"]
+        print(f"[fast] {name}: autotuned tile {shape}, parts {ctx.stats()['parts']}", flush=True)
+        assert shape[0] * shape[1] <= 16 and shape[0] >= 1
+    finally:
+        ctx.close()
 
 
 @pytest.mark.parametrize("name", ["gray_bm12_radius2", "yuv_bm12_radius3"])

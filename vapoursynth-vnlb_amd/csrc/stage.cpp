@@ -311,19 +311,32 @@ void append_grid_frame(const StageDesc& d, int anchor_frame, Grid& grid) {
     }
 }
 
-// Anchor tiles of the MFMA matcher: `rows` (4 or 2) consecutive grid rows x
-// 4 anchors of one frame (slot rr * 4 + ii; -1 when a row is shorter and for
-// rows past `rows`), and the largest union of the tiles' search windows.
-// Two-row tiles keep the union of 25-wide windows within the kernel's
-// limits, at half the anchors per matrix product.
+// Anchor tiles of the MFMA matcher: `rows` consecutive grid rows x `cols`
+// consecutive anchors of each row, of one frame (rows * cols <= 16; slot
+// rr * cols + ii, -1 where a row is shorter; the matrix products take the
+// 16 slots as their anchor columns whatever the shape), and the largest
+// union of the tiles' search windows.  The union per anchor is what the
+// shape trades: 4 x 4 is smallest for 19-wide windows; wider windows need
+// flatter tiles to stay within the kernel's limits.
 struct Tiles {
     std::vector<int> slots;
     int max_cols = 0;
     int max_rows = 0;
 };
 
-Tiles build_tiles(const StageDesc& d, const Grid& grid, int height, int rows) {
+struct TileShape {
+    int rows;
+    int cols;
+};
+
+// The shapes the autotuner tries, the default first.
+constexpr TileShape kTileShapes[] = {{4, 4}, {2, 8}, {8, 2}, {3, 5}, {2, 6}, {2, 5}, {4, 2}, {2, 4}};
+constexpr int kTileShapeCount = static_cast<int>(sizeof(kTileShapes) / sizeof(kTileShapes[0]));
+
+Tiles build_tiles(const StageDesc& d, const Grid& grid, int height, TileShape shape) {
     Tiles tiles;
+    const int rows = shape.rows;
+    const int cols = shape.cols;
     const int frame_count = static_cast<int>(grid.ids.size());
     for (int fi = 0; fi < frame_count; ++fi) {
         const int frame_begin = grid.offsets[static_cast<size_t>(fi)];
@@ -340,29 +353,29 @@ Tiles build_tiles(const StageDesc& d, const Grid& grid, int height, int rows) {
             for (size_t rr = r0; rr < r0 + static_cast<size_t>(rows) && rr < ys.size(); ++rr) {
                 widest = std::max(widest, count[ys[rr]]);
             }
-            for (int i0 = 0; i0 < widest; i0 += 4) {
+            for (int i0 = 0; i0 < widest; i0 += cols) {
                 int x0 = INT_MAX;
                 int x1 = INT_MIN;
                 int y0 = INT_MAX;
                 int y1 = INT_MIN;
-                for (int rr = 0; rr < 4; ++rr) {
-                    for (int ii = 0; ii < 4; ++ii) {
-                        int index = -1;
-                        const size_t row = r0 + static_cast<size_t>(rr);
-                        if (rr < rows && row < ys.size() && i0 + ii < count[ys[row]]) {
-                            index = frame_begin + first[ys[row]] + i0 + ii;
-                            const uint32_t p = grid.anchors[static_cast<size_t>(index)];
-                            int lo = 0;
-                            int hi = 0;
-                            vnlbhip::spatial_range(d, vnlbhip::position_x(p), d.width, lo, hi);
-                            x0 = std::min(x0, lo);
-                            x1 = std::max(x1, hi);
-                            vnlbhip::spatial_range(d, vnlbhip::position_y(p), d.height, lo, hi);
-                            y0 = std::min(y0, lo);
-                            y1 = std::max(y1, hi);
-                        }
-                        tiles.slots.push_back(index);
+                for (int slot = 0; slot < 16; ++slot) {
+                    const int rr = slot / cols;
+                    const int ii = slot - rr * cols;
+                    int index = -1;
+                    const size_t row = r0 + static_cast<size_t>(rr);
+                    if (rr < rows && row < ys.size() && i0 + ii < count[ys[row]]) {
+                        index = frame_begin + first[ys[row]] + i0 + ii;
+                        const uint32_t p = grid.anchors[static_cast<size_t>(index)];
+                        int lo = 0;
+                        int hi = 0;
+                        vnlbhip::spatial_range(d, vnlbhip::position_x(p), d.width, lo, hi);
+                        x0 = std::min(x0, lo);
+                        x1 = std::max(x1, hi);
+                        vnlbhip::spatial_range(d, vnlbhip::position_y(p), d.height, lo, hi);
+                        y0 = std::min(y0, lo);
+                        y1 = std::max(y1, hi);
                     }
+                    tiles.slots.push_back(index);
                 }
                 tiles.max_cols = std::max(tiles.max_cols, x1 - x0 + 1);
                 tiles.max_rows = std::max(tiles.max_rows, y1 - y0 + 1);
@@ -378,6 +391,8 @@ struct BatchPlan {
     Grid grid;  // offsets end with the batch's anchor count
     int max_anchors = 0;
     bool tiles_built = false;
+    TileShape tile_shape{4, 4};  // the shape requested when the tiles were built
+    TileShape used_shape{4, 4};  // the shape built (2 x 4 where the requested one does not fit)
     Tiles tiles;
 };
 
@@ -590,6 +605,9 @@ struct vnlbhip_context {
     hipEvent_t match_event[kMaxParts] = {};
     hipEvent_t paste_event[kMaxParts] = {};
     int parts = kDefaultParts;  // pipeline parts of the next stage
+    TileShape tile_shape{4, 4};        // MFMA matcher anchor tiles of a vnlbhip_stage call
+    TileShape fixed_tile_shape{4, 4};  // vnlbhip_set_tile_shape: every call without autotuning
+    bool stage_shape = false;          // run_stage takes tile_shape (vnlbhip_stage) not fixed_tile_shape
     bool autotune = false;
     // Stage graphs: the launch sequence of a stage captured once and
     // replayed (same stage, form, parts, geometry, parameters and pointers).
@@ -605,7 +623,11 @@ struct vnlbhip_context {
     hipStream_t capture = nullptr;
     std::map<std::vector<long long>, StageGraph> graphs;
     // (stage, form, geometry, parameters) -> the part count timed fastest
-    std::map<std::vector<long long>, int> tuned_parts;
+    struct Tuned {
+        int parts;
+        TileShape shape;
+    };
+    std::map<std::vector<long long>, Tuned> tuned;
     int* host_counts = nullptr;
     // This device's copy of the scratch array (vnlbhip_stage_scratch), from
     // hipGetSymbolAddress at set-up.
@@ -1008,18 +1030,24 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
         int* d_tiles = nullptr;
         bool fast_matcher = false;
         if (fast_energy != nullptr) {
-            if (!plan.tiles_built) {
-                // Four grid rows per tile where the window union fits the
-                // kernel, else two.
-                plan.tiles = build_tiles(d, grid, g.height, 4);
+            const TileShape want = ctx.stage_shape ? ctx.tile_shape : ctx.fixed_tile_shape;
+            if (!plan.tiles_built || plan.tile_shape.rows != want.rows || plan.tile_shape.cols != want.cols) {
+                // The requested tile shape where the window union fits the
+                // kernel, else 2 x 4.
+                plan.tiles = build_tiles(d, grid, g.height, want);
+                plan.used_shape = want;
                 if (!vnlbhip::fast_match_supported(d, plan.tiles.max_cols, plan.tiles.max_rows)) {
-                    plan.tiles = build_tiles(d, grid, g.height, 2);
+                    plan.tiles = build_tiles(d, grid, g.height, TileShape{2, 4});
+                    plan.used_shape = TileShape{2, 4};
                 }
                 plan.tiles_built = true;
+                plan.tile_shape = want;
             }
             tiles = &plan.tiles;
             fast_matcher = vnlbhip::fast_match_supported(d, tiles->max_cols, tiles->max_rows);
             if (fast_matcher) {
+                ctx.stats.tile_rows = plan.used_shape.rows;
+                ctx.stats.tile_cols = plan.used_shape.cols;
                 // The tile list lives in the cached plan, so the upload may
                 // complete asynchronously.
                 d_tiles = ctx.fast_tiles.reserve(tiles->slots.size());
@@ -1707,6 +1735,14 @@ int vnlbhip_set_autotune(vnlbhip_context* context, int enable) {
     return VNLBHIP_OK;
 }
 
+int vnlbhip_set_tile_shape(vnlbhip_context* context, int rows, int cols) {
+    if (context == nullptr || rows < 1 || cols < 1 || rows * cols > 16) {
+        return VNLBHIP_ERROR_INVALID;
+    }
+    context->fixed_tile_shape = TileShape{rows, cols};
+    return VNLBHIP_OK;
+}
+
 int vnlbhip_set_fused_selection(vnlbhip_context* context, int enable) {
     if (context == nullptr || (enable != 0 && enable != 1)) {
         return VNLBHIP_ERROR_INVALID;
@@ -1743,6 +1779,12 @@ int vnlbhip_stage(vnlbhip_context* context, int stage, const vnlbhip_params* par
         const Geometry g{width, height, frames, channels};
         const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
         context->parts = kDefaultParts;
+        context->tile_shape = context->fixed_tile_shape;
+        context->stage_shape = true;
+        struct ShapeScope {
+            vnlbhip_context* ctx;
+            ~ShapeScope() { ctx->stage_shape = false; }
+        } shape_scope{context};
         if (context->autotune) {
             // Key: everything the stage's kernels and grids depend on.
             std::vector<long long> key = {stage, width, height, frames, channels,
@@ -1750,35 +1792,65 @@ int vnlbhip_stage(vnlbhip_context* context, int stage, const vnlbhip_params* par
             append_settings(*context, key);
             const unsigned char* raw = reinterpret_cast<const unsigned char*>(params);
             key.insert(key.end(), raw, raw + sizeof(vnlbhip_params));
-            auto it = context->tuned_parts.find(key);
-            if (it == context->tuned_parts.end()) {
-                // One untimed run (plans, buffers), then every part count
-                // once; the stage blocks until done, so host time is device
-                // time plus the fixed launch work.
+            auto it = context->tuned.find(key);
+            if (it == context->tuned.end()) {
+                // One untimed run (plans, buffers), then every MFMA-matcher
+                // tile shape that fits the kernel (default part count), then
+                // every part count with the fastest shape; the stage blocks
+                // until done, so host time is device time plus the fixed
+                // launch work.
                 const bool profiling = context->profiling;
                 context->profiling = false;
                 run_stage(*context, stage, *params, noisy, ref, out, g, s, kAllFrames, nullptr);
-                int best = kDefaultParts;
-                double best_ms = 0.0;
-                for (int parts = 1; parts <= kMaxParts; ++parts) {
-                    context->parts = parts;
+                const auto timed_run = [&]() {
                     const auto t0 = std::chrono::steady_clock::now();
                     run_stage(*context, stage, *params, noisy, ref, out, g, s, kAllFrames, nullptr);
-                    const double ms =
-                        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-                    if (parts == 1 || ms < best_ms) {
-                        best = parts;
+                    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                };
+                vnlbhip_context::Tuned best{kDefaultParts, context->fixed_tile_shape};
+                double best_ms = timed_run();
+                const StageDesc d = make_desc(stage, *params, g);
+                const bool mfma_matcher = context->form == VNLBHIP_FORM_FAST && d.patch == 8 && d.similar > 1 &&
+                                          context->search_offsets == nullptr && context->proposer == nullptr &&
+                                          !context->plans.empty();
+                for (int i = 0; mfma_matcher && i < kTileShapeCount; ++i) {
+                    const TileShape shape = kTileShapes[i];
+                    if (shape.rows == best.shape.rows && shape.cols == best.shape.cols) {
+                        continue;  // timed above
+                    }
+                    const Tiles t = build_tiles(d, context->plans.front().grid, height, shape);
+                    if (!vnlbhip::fast_match_supported(d, t.max_cols, t.max_rows)) {
+                        continue;
+                    }
+                    context->tile_shape = shape;
+                    const double ms = timed_run();
+                    if (ms < best_ms) {
+                        best = vnlbhip_context::Tuned{kDefaultParts, shape};
+                        best_ms = ms;
+                    }
+                }
+                context->tile_shape = best.shape;
+                for (int parts = 1; parts <= kMaxParts; ++parts) {
+                    if (parts == kDefaultParts) {
+                        continue;
+                    }
+                    context->parts = parts;
+                    const double ms = timed_run();
+                    if (ms < best_ms) {
+                        best.parts = parts;
                         best_ms = ms;
                     }
                 }
                 context->profiling = profiling;
-                it = context->tuned_parts.emplace(key, best).first;
+                it = context->tuned.emplace(key, best).first;
             }
-            context->parts = it->second;
+            context->parts = it->second.parts;
+            context->tile_shape = it->second.shape;
         }
         if (context->use_graph && !context->profiling) {
             // Key: the stage's launch sequence and every address it holds.
-            std::vector<long long> key = {stage, context->parts, width, height, frames,
+            std::vector<long long> key = {stage, context->parts, context->tile_shape.rows,
+                                          context->tile_shape.cols, width, height, frames,
                                           channels, reinterpret_cast<long long>(noisy),
                                           reinterpret_cast<long long>(ref), reinterpret_cast<long long>(out)};
             append_settings(*context, key);

@@ -83,7 +83,8 @@ class Context:
                 "unconverged_groups": s.unconverged_groups, "batches": s.batches,
                 "retained": s.retained, "refined": s.refined, "parts": s.parts,
                 "graph_replays": s.graph_replays, "reallocations": s.reallocations,
-                "proposed": s.proposed, "exhaustive_fallbacks": s.exhaustive_fallbacks}
+                "proposed": s.proposed, "exhaustive_fallbacks": s.exhaustive_fallbacks,
+                "tile_shape": (s.tile_rows, s.tile_cols)}
 
     def set_proposer(self, proposer, top_k: int = 48, margin: float = 0.0, precision: str = "bf16") -> None:
         """Match through a learned candidate proposer (vnlbhip.proposer.Proposer;
@@ -148,6 +149,13 @@ class Context:
         if rc != 0:
             self._raise(rc)
         self.form = form
+
+    def set_tile_shape(self, rows: int, cols: int) -> None:
+        """Fast form: anchor tiles of the MFMA matcher (rows x cols anchors,
+        at most 16; default 4 x 4) for calls the autotuner does not tune."""
+        rc = self._lib.vnlbhip_set_tile_shape(self._handle, int(rows), int(cols))
+        if rc != 0:
+            raise VNLBError("VNLB HIP: a tile holds 1 .. 16 anchors (rows x cols)")
 
     def set_jacobi(self, max_sweeps: int = 64, fail_unconverged: bool = False) -> None:
         """Jacobi sweep limit of every eigen solve (default 64) and whether a

@@ -45,6 +45,7 @@ EXPORTED_SYMBOLS = (
     "vnlbhip_frame_cache_get_stats",
     "vnlbhip_frame_cache_destroy",
     "vnlbhip_set_jacobi",
+    "vnlbhip_set_tile_shape",
     "vnlbhip_proposer_create",
     "vnlbhip_proposer_destroy",
     "vnlbhip_proposer_last_error",
@@ -118,6 +119,8 @@ class Stats(ctypes.Structure):
         ("reallocations", ctypes.c_int64),
         ("proposed", ctypes.c_int64),
         ("exhaustive_fallbacks", ctypes.c_int64),
+        ("tile_rows", ctypes.c_int32),
+        ("tile_cols", ctypes.c_int32),
     ]
 
 
@@ -232,6 +235,8 @@ def load() -> ctypes.CDLL:
     lib.vnlbhip_get_kernel_times.restype = ctypes.c_int
     lib.vnlbhip_set_jacobi.argtypes = [vp, ci, ci]
     lib.vnlbhip_set_jacobi.restype = ci
+    lib.vnlbhip_set_tile_shape.argtypes = [vp, ci, ci]
+    lib.vnlbhip_set_tile_shape.restype = ci
     lib.vnlbhip_proposer_create.argtypes = [ci, ctypes.c_uint64, P(vp)]
     lib.vnlbhip_proposer_create.restype = ci
     lib.vnlbhip_proposer_destroy.argtypes = [vp]
