@@ -63,6 +63,37 @@ def test_search_fwd_int_max_minus_one(gpu, oracle_lib):
     _check_output(gpu.stage(BASIC, huge, x), out, clean)
 
 
+def test_tuned_rgb_profile_rank_39(gpu, oracle_lib):
+    """tests/cpp/test_cuda_group_filter.cpp:709-733: the tuned reference
+    profile's K = 100 (Basic) / 60 (Final) with rank 39 at sample dimension
+    128: dual Grams of 100 and 60 model rows, beyond one wavefront of the
+    cyclic Jacobi solver (two wavefronts per problem)."""
+    noisy, clean = synth.make_clip(64, 56, 5, 1, sigma8=20, seed=70, motion="subpixel")
+    common = dict(sigma=20, block_size=8, block_step=4, bm_range=9, patch_time=2, radius=1, rank=39, tau=0)
+    bp = parse_stage_parameters(BASIC, dict(common, group_size=100))
+    fp = parse_stage_parameters(FINAL, dict(common, group_size=60))
+    basic, _ = oracle_lib.stage(BASIC, bp, noisy, flags=oracle_lib.ACCUM_DOUBLE, threads=4)
+    for stage, params, ref in ((BASIC, bp, None), (FINAL, fp, basic)):
+        print(f"[limits] K {params.similar} rank 39, stage {stage}", flush=True)
+        want = _compare_frame(gpu, oracle_lib, stage, params, noisy, ref, 2)
+        assert ((want["group_flags"] & 4) != 0).any()
+        out, _ = oracle_lib.stage(stage, params, noisy, ref, flags=oracle_lib.ACCUM_DOUBLE, threads=4)
+        x = torch.from_numpy(noisy).to("cuda:0")
+        r = None if ref is None else torch.from_numpy(ref).to("cuda:0")
+        _check_output(gpu.stage(stage, params, x, r), out, clean)
+    # The fast form keeps its matrix-core matcher and takes the exact model
+    # kernels for groups this large.
+    fast = vnlbhip.Context(0)
+    try:
+        fast.set_form("fast")
+        x = torch.from_numpy(noisy).to("cuda:0")
+        got = fast.stage(FINAL, fp, x, fast.stage(BASIC, bp, x)).cpu().numpy()
+        plain = gpu.stage(FINAL, fp, x, gpu.stage(BASIC, bp, x)).cpu().numpy()
+        assert abs(synth.psnr(got, clean) - synth.psnr(plain, clean)) <= 0.05
+    finally:
+        fast.close()
+
+
 def test_retained_cap_above_256(gpu, oracle_lib):
     """cap_factor 20 with K = 16 (soft cap 320) and a tau that admits most
     of the window: groups retain more than 256 patches."""

@@ -22,7 +22,7 @@ namespace vnlbhip {
 constexpr int kLanes = 16;
 constexpr int kMaxPatch = 16;
 constexpr int kMaxDim = 1024;          // C * Pt * P^2
-constexpr int kMaxModelRows = 64;      // B = min(S, ceil(K * model_cap))
+constexpr int kMaxModelRows = 128;     // B = min(S, ceil(K * model_cap))
 constexpr int kMaxCap = 1024;          // retained patches per group
 constexpr int kMaxRank = 64;
 constexpr int kMaxCandidates = 32768;  // window^2 * temporal slots (the matchers' LDS budget binds first)

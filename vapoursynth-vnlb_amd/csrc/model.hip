@@ -26,18 +26,22 @@ namespace {
 // 8 * threshold; eigenpairs sorted descending (stable) and each vector made
 // unit length with its largest-magnitude entry positive
 // (canonicalize_vector_row :204-233).  Each lane owns one matrix row.
+// kLP = 128 (n up to 128, the dual Grams of up to 128 model rows): one
+// problem per block of two wavefronts; the segment reductions then go
+// through LDS and the rotation loops stop at the problem's n (uniform in the
+// block).
 template <int kLP>  // lanes per problem (power of two >= n)
-__global__ __launch_bounds__(64) void jacobi_kernel(
+__global__ __launch_bounds__(128) void jacobi_kernel(
     StageDesc d, const int* __restrict__ group_count_dev, int problem_stride,
     const int* __restrict__ group_flags, const int* __restrict__ problem_dims,
     const float* __restrict__ problems, float* __restrict__ eigenvalues,
     float* __restrict__ vectors, int* __restrict__ status) {
     const int problem_count = *group_count_dev * d.models;
-    constexpr int kPerWave = 64 / kLP;
+    constexpr int kPerWave = kLP >= 64 ? 1 : 64 / kLP;
     constexpr int kStride = kLP + 1;
-    __shared__ float s_m[kPerWave][kLP * kStride];
-    __shared__ float s_v[kPerWave][kLP * kStride];
+    extern __shared__ float s_jacobi[];
     __shared__ int s_order[kPerWave][kLP];
+    __shared__ float s_reduce[2];
 
     const int sub = threadIdx.x / kLP;
     const int r = threadIdx.x % kLP;
@@ -49,9 +53,24 @@ __global__ __launch_bounds__(64) void jacobi_kernel(
         valid = (group_flags[group] & kFlagModelled) != 0;
         n = problem_dims[group];
     }
-    float* m = s_m[sub];
-    float* v = s_v[sub];
+    float* m = s_jacobi + static_cast<size_t>(sub) * 2 * kLP * kStride;
+    float* v = m + kLP * kStride;
     const float* in = problems + static_cast<size_t>(problem) * problem_stride * problem_stride;
+    // Maximum over the problem's lanes (all threads of the block call it).
+    const auto problem_max = [&](float value) {
+        for (int offset = (kLP < 64 ? kLP : 64) / 2; offset >= 1; offset >>= 1) {
+            value = fmaxf(value, __shfl_xor(value, offset, kLP < 64 ? kLP : 64));
+        }
+        if constexpr (kLP > 64) {
+            __syncthreads();
+            if ((threadIdx.x & 63) == 0) {
+                s_reduce[threadIdx.x >> 6] = value;
+            }
+            __syncthreads();
+            value = fmaxf(s_reduce[0], s_reduce[1]);
+        }
+        return value;
+    };
 
     float row_max = 0.0F;
     for (int c = 0; c < kLP; ++c) {
@@ -63,10 +82,7 @@ __global__ __launch_bounds__(64) void jacobi_kernel(
         m[r * kStride + c] = value;
         v[r * kStride + c] = (r == c) ? 1.0F : 0.0F;
     }
-    // Segment max over the problem's lanes.
-    for (int offset = kLP / 2; offset >= 1; offset >>= 1) {
-        row_max = fmaxf(row_max, __shfl_xor(row_max, offset, kLP));
-    }
+    row_max = problem_max(row_max);
     const float matrix_scale = row_max > 1.0F ? row_max : 1.0F;
     const float threshold = (64.0F * FLT_EPSILON) * matrix_scale;
     __syncthreads();
@@ -78,10 +94,7 @@ __global__ __launch_bounds__(64) void jacobi_kernel(
                 value = fmaxf(value, fabsf(m[r * kStride + c]));
             }
         }
-        for (int offset = kLP / 2; offset >= 1; offset >>= 1) {
-            value = fmaxf(value, __shfl_xor(value, offset, kLP));
-        }
-        return value;
+        return problem_max(value);
     };
 
     bool done = !valid || n <= 1;
@@ -98,8 +111,11 @@ __global__ __launch_bounds__(64) void jacobi_kernel(
         if (__all(done)) {
             break;
         }
-        for (int p = 0; p + 1 < kLP; ++p) {
-            for (int q = p + 1; q < kLP; ++q) {
+        // With one problem per block its n bounds the pairs; with several
+        // per wavefront every lane walks all kLP (the barriers are shared).
+        const int pairs_n = kPerWave == 1 ? n : kLP;
+        for (int p = 0; p + 1 < pairs_n; ++p) {
+            for (int q = p + 1; q < pairs_n; ++q) {
                 bool rotate = !done && q < n;
                 float apq = 0.0F;
                 if (rotate) {
@@ -622,10 +638,18 @@ hipError_t launch_jacobi(const StageDesc& d, const int* group_count_dev, int pro
     }
     const int n = problem_stride;
 #define VNLBHIP_JACOBI(LP)                                                                      \
-    hipLaunchKernelGGL(jacobi_kernel<LP>,                                                       \
-                       dim3(static_cast<unsigned>((problem_capacity + (64 / LP) - 1) / (64 / LP))), \
-                       dim3(64), 0, stream, d, group_count_dev, problem_stride, group_flags,     \
-                       problem_dims, problems, eigenvalues, vectors, status)
+    do {                                                                                        \
+        constexpr int kPer = LP >= 64 ? 1 : 64 / LP;                                            \
+        const size_t shared = static_cast<size_t>(kPer) * 2 * LP * (LP + 1) * sizeof(float);    \
+        hipLaunchKernelGGL(jacobi_kernel<LP>,                                                   \
+                           dim3(static_cast<unsigned>((problem_capacity + kPer - 1) / kPer)),   \
+                           dim3(LP > 64 ? LP : 64), shared, stream, d, group_count_dev,         \
+                           problem_stride, group_flags, problem_dims, problems, eigenvalues,    \
+                           vectors, status);                                                    \
+    } while (0)
+    if (n > 128) {
+        return hipErrorInvalidValue;
+    }
     if (n <= 4) {
         VNLBHIP_JACOBI(4);
     } else if (n <= 8) {
@@ -634,8 +658,10 @@ hipError_t launch_jacobi(const StageDesc& d, const int* group_count_dev, int pro
         VNLBHIP_JACOBI(16);
     } else if (n <= 32) {
         VNLBHIP_JACOBI(32);
-    } else {
+    } else if (n <= 64) {
         VNLBHIP_JACOBI(64);
+    } else {
+        VNLBHIP_JACOBI(128);
     }
 #undef VNLBHIP_JACOBI
     return hipGetLastError();

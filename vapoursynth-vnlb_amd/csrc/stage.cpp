@@ -239,9 +239,9 @@ StageDesc make_desc(int stage, const vnlbhip_params& p, const Geometry& g, bool 
     require(d.cap <= vnlbhip::kMaxCap, "retained group cap above 1024 is not supported");
     // Model limits do not bind a call that only matches (vnlbhip_match_lists).
     require(matching_only || d.rank <= vnlbhip::kMaxRank, "rank above 64 is not supported");
-    require(matching_only || d.jacobi_dim <= 64,
-            "eigen problems above 64x64 are not supported (covariance path with a large patch)");
-    require(matching_only || d.max_rows <= vnlbhip::kMaxModelRows, "more than 64 model rows is not supported");
+    require(matching_only || d.jacobi_dim <= 128,
+            "eigen problems above 128x128 are not supported (covariance path with a large patch)");
+    require(matching_only || d.max_rows <= vnlbhip::kMaxModelRows, "more than 128 model rows is not supported");
     // Candidates per anchor: the temporal range holds at most
     // min(bwd + fwd + 1, total_frames - Pt + 1) frames.
     const long long temporal = std::min<long long>(static_cast<long long>(d.search_bwd) + d.search_fwd + 1,
