@@ -286,6 +286,19 @@ __device__ inline void wave_merge_lower64x4(uint64_t (&best)[4], const uint64_t 
 
 constexpr uint64_t kNoKey = ~static_cast<uint64_t>(0);
 
+// Keys per anchor up to which a frame's candidates are inserted one by one
+// rather than sorted and merged.
+constexpr int kInsertKeys = 24;
+
+// `key` into a list of distinct keys ascending over the lanes (the 64
+// smallest so far; kNoKey for empty places): the keys below it stay, the
+// rest move up one lane and the largest drops out.
+__device__ inline uint64_t wave_insert(uint64_t list, uint64_t key, int lane) {
+    const int place = __popcll(__ballot(list < key));
+    const uint64_t before = __shfl_up(list, 1, 64);
+    return lane < place ? list : (lane == place ? key : before);
+}
+
 __device__ inline uint64_t readlane64(uint64_t v, int lane) {
     const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v & 0xFFFFFFFFu), lane));
     const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v >> 32), lane));
@@ -780,9 +793,28 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                // Rounds of 64 compacted keys, the four anchors side by side;
-                // keys no longer below an anchor's 64th are dropped.
-                const int rounds = (max(max(m[0], m[1]), max(m[2], m[3])) + 63) / 64;
+                const int most = max(max(m[0], m[1]), max(m[2], m[3]));
+                if (most <= kInsertKeys) {
+                    // Few keys (the usual case once the lists hold a frame's
+                    // worth): each goes into its sorted list directly, a ballot
+                    // finding its place and one lane shift making room; the
+                    // lists end as the sort-and-merge rounds below leave them.
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int n = 4 * wave + q;
+                        const uint32_t base_index = static_cast<uint32_t>(ti * s_nx[n] * s_ny[n]);
+                        for (int j = 0; j < m[q]; ++j) {
+                            const int i = scratch[q * S.scratch_keys + j];
+                            const uint64_t key =
+                                (static_cast<uint64_t>(distance_key(s_stage[n * S.per_frame + i])) << 32) |
+                                (base_index + i);
+                            best[q] = wave_insert(best[q], key, lane);
+                        }
+                    }
+                }
+                // Otherwise rounds of 64 compacted keys, the four anchors side
+                // by side; keys no longer below an anchor's 64th are dropped.
+                const int rounds = most <= kInsertKeys ? 0 : (most + 63) / 64;
                 for (int r = 0; r < rounds; ++r) {
                     uint64_t key[4];
                     bool any = false;
