@@ -293,13 +293,13 @@ int vnlbhip_set_graph(vnlbhip_context* context, int enable);
 int vnlbhip_set_autotune(vnlbhip_context* context, int enable);
 
 /* FAST form: anchor tiles of the MFMA matcher, `rows` grid rows x `cols`
- * anchors per row (rows * cols <= 16; default 4 x 4), for every call the
+ * anchors per row (1 to 4 each; default 4 x 4), for every call the
  * autotuner does not set (with autotuning on, the first vnlbhip_stage call
  * of a stage, form, geometry and parameter set also times every shape of
- * 4x4, 2x8, 8x2, 3x5, 2x6, 2x5, 4x2 and 2x4 whose window union fits the
- * kernel and keeps the fastest).  A shape whose union does not fit falls
- * back to 2 x 4.  vnlbhip_stats.tile_rows / tile_cols report the shape
- * used.  No counterpart in the reference. */
+ * 4x4, 3x4, 2x4, 4x3, 4x2 and 1x4 whose window union fits the kernel and
+ * keeps the fastest).  A shape whose union does not fit falls back to
+ * 2 x 4.  vnlbhip_stats.tile_rows / tile_cols report the shape used.  No
+ * counterpart in the reference. */
 int vnlbhip_set_tile_shape(vnlbhip_context* context, int rows, int cols);
 
 /* Per-anchor search centres from a motion field (MVToolsFlowProvider's

@@ -317,7 +317,7 @@ def test_tile_shapes_give_the_same_lists(name):
         ctx.set_form("fast")
         want = ctx.match_lists(BASIC, bp, x, None, 2)
         used = set()
-        for rows, cols in ((4, 4), (2, 8), (8, 2), (3, 5), (2, 6), (2, 5), (4, 2), (2, 4), (1, 16)):
+        for rows, cols in ((4, 4), (3, 4), (2, 4), (1, 4), (4, 3), (4, 2), (2, 2), (4, 1)):
             print(f"[fast] {name}: tile {rows}x{cols}", flush=True)
             ctx.set_tile_shape(rows, cols)
             got = ctx.match_lists(BASIC, bp, x, None, 2)
@@ -325,14 +325,15 @@ def test_tile_shapes_give_the_same_lists(name):
             for key in ("anchors", "counts", "positions", "distances"):
                 torch.testing.assert_close(got[key], want[key], rtol=0, atol=0)
         assert len(used) > 1
-        with pytest.raises(vnlbhip.VNLBError):
-            ctx.set_tile_shape(4, 5)
+        for rows, cols in ((4, 5), (2, 8), (5, 2)):
+            with pytest.raises(vnlbhip.VNLBError):
+                ctx.set_tile_shape(rows, cols)
         ctx.set_tile_shape(4, 4)
         ctx.set_autotune(True)
         ctx.stage(BASIC, bp, x)
         shape = ctx.stats()["tile_shape"]
         print(f"[fast] {name}: autotuned tile {shape}, parts {ctx.stats()['parts']}", flush=True)
-        assert shape[0] * shape[1] <= 16 and shape[0] >= 1
+        assert 1 <= shape[0] <= 4 and 1 <= shape[1] <= 4
     finally:
         ctx.close()
 

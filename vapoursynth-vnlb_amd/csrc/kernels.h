@@ -46,21 +46,24 @@ hipError_t launch_match(const StageDesc& d, const float* source, const uint32_t*
                         const int16_t* offsets, int offsets_temporal, hipStream_t stream);
 
 // MFMA form of the matcher (match_fast.hip).  Tiles hold 16 anchor indices
-// (-1 for empty slots) of one frame; max_cols / max_rows bound the union of
-// the tiles' search windows.
+// (-1 for empty slots) of one frame, slot 4 * grid row + column (at most 4 x
+// 4); max_cols / max_rows bound the union of the tiles' search windows.
 bool fast_match_supported(const StageDesc& d, int max_cols, int max_rows);
 size_t fast_match_shared_bytes(const StageDesc& d, int max_cols, int max_rows);
 size_t fast_match_stride(const StageDesc& d);
-// offsets [4], rows [frames][match channels][H][W], energy [frames][H][W].
-hipError_t launch_fast_energy(const StageDesc& d, const float* source, float* offsets, float* rows,
-                              float* energy, hipStream_t stream);
+// offsets [4] channel offsets, flags [1] (set where an energy is not finite
+// or too large for the bounded epilogue), energy [frames][H][W], split
+// [frames][match channels][H][W] packed bf16 hi << 16 | lo of the offset
+// samples (its buffer holds the row sums on the way).
+hipError_t launch_fast_energy(const StageDesc& d, const float* source, float* offsets, unsigned* flags,
+                              float* energy, uint32_t* split, hipStream_t stream);
 // Distances of anchor index i land at staged[(i - anchor_base) * stride].
 // fused: k-nearest selection inside the matcher (fast_match_fused), results
 // straight into counts / positions / distances; otherwise distances are
 // staged for launch_fast_select.
 bool fast_match_fused(const StageDesc& d, int max_cols, int max_rows);
-hipError_t launch_fast_match(const StageDesc& d, int max_cols, int max_rows, const float* source,
-                             const float* offsets, const float* energy, const uint32_t* anchors,
+hipError_t launch_fast_match(const StageDesc& d, int max_cols, int max_rows, const uint32_t* split,
+                             const unsigned* flags, const float* energy, const uint32_t* anchors,
                              const int* tiles, int tile_count, int anchor_base, float* staged,
                              bool fused, int* counts, uint32_t* positions, float* distances,
                              hipStream_t stream);

@@ -6,26 +6,41 @@
 //
 // with E the patch energy (fp32, one value per patch origin) and the cross
 // term accumulated in fp32 on the matrix cores from three bf16 products
-// (x = hi + lo, <r, c> ~ hi.hi + hi.lo + lo.hi).  Samples are offset by a
+// (x = hi + lo, <r, c> ~ hi.hi + lo.hi + hi.lo).  Samples are offset by a
 // per-channel constant first, which leaves distances unchanged and keeps the
 // energies close to the distances they are subtracted from.  The candidates
 // kept are then chosen by the same k-nearest selection as the exact form
 // (select.h), so only the distance values differ from it.
 //
-// Anchors are matched in tiles of up to 16 anchors of one frame (4 grid rows
-// x 4 anchors; 2 x 4 where the union of wider windows would not fit): the
-// tile's candidates are the union of the anchors' windows.
-// One v_mfma_f32_16x16x32_bf16 multiplies 16 candidates of a column
-// (consecutive rows) by the 16 anchors over 4 patch rows.  The search region
-// of one (frame, channel) plane is staged in LDS as bf16 hi / lo, four
-// copies shifted by 0..3 samples so that every lane reads its 8 samples with
-// 8-byte aligned loads.  Supported for 8x8 patches.
+// Once per stage, every matched plane is split into packed bf16 pairs
+// (hi << 16 | lo of the offset sample), the format the matcher stages.
+//
+// Anchors are matched in tiles of up to 4 grid rows x 4 anchors of one frame
+// (slot 4 * row + column); the tile's candidates are the union of the
+// anchors' windows.  One v_mfma_f32_16x16x32_bf16 multiplies the 16 anchor
+// slots (A) by 16 candidates of a union column (B, consecutive rows) over 4
+// patch rows, so lane (n, kb) of the result holds candidate n against the
+// four anchors of grid row kb.  The search region of one (frame, channel)
+// plane is staged in LDS in groups of 4 columns: per row and group, four
+// copies shifted by 0..3 columns, each 4 bf16 hi then their 4 lo.
+// Wavefront w takes the union columns w mod 4, so it reads only copy w, at
+// group offsets fixed at compile time.
+//
+// After each candidate frame the wavefronts test their products against a
+// per-anchor bound (the largest key the running k-nearest lists can still
+// take) and write out only the candidates within it; the wavefront owning an
+// anchor merges them into its list.  The anchor frame, which comes first,
+// and clips whose energies are not all finite and small enough that no sum
+// overflows write every window distance and select from all of them.
+// Supported for 8x8 patches.
 #include "common.h"
 #include "device_util.h"
 #include "kernels.h"
 #include "select.h"
 
 #include <float.h>
+
+#include <algorithm>
 #include <hip/hip_bf16.h>
 
 namespace vnlbhip {
@@ -40,18 +55,26 @@ constexpr int kFastWaves = kFastThreads / 64;
 constexpr int kFastPatch = 8;
 constexpr int kTileAnchors = 16;
 constexpr int kCopies = 4;
-constexpr int kFillPerThread = 3;          // fill items (4 output columns) per thread
-constexpr int kFillSpan = 4 + kCopies - 1;  // samples behind one item
-constexpr int kEnergyPerThread = 5;         // energy box items per thread
+constexpr int kItemsPerThread = 3;    // staged (row, group) items per thread
+constexpr int kEnergyPerThread = 6;   // energy box items per thread
+// Energies above this bound (or not finite) switch a stage to the exhaustive
+// epilogue: below it no energy sum or doubled cross term overflows.
+constexpr float kEnergyBound = 8.0e37F;
 
 // Per-channel offset: the mean of the finite values of a strided sample of
 // the channel (any finite offset leaves the distances unchanged; non-finite
-// samples only poison the patches that hold them).
+// samples only poison the patches that hold them).  Block 0 also clears the
+// stage's energy flag.
 __global__ __launch_bounds__(256) void channel_offset_kernel(const float* __restrict__ source,
                                                              int channels, size_t plane,
-                                                             int frames, float* __restrict__ offsets) {
+                                                             int frames, float* __restrict__ offsets,
+                                                             unsigned* __restrict__ flags) {
     __shared__ double s_part[4];
+    __shared__ double s_count[4];
     const int c = blockIdx.x;
+    if (c == 0 && threadIdx.x == 0) {
+        flags[0] = 0u;
+    }
     const size_t per_channel = plane * frames;
     const size_t stride = per_channel / 65536 + 1;
     double sum = 0.0;
@@ -69,10 +92,6 @@ __global__ __launch_bounds__(256) void channel_offset_kernel(const float* __rest
     double n = wave_sum_double(static_cast<double>(count));
     if ((threadIdx.x & 63) == 0) {
         s_part[threadIdx.x >> 6] = sum;
-    }
-    __syncthreads();
-    __shared__ double s_count[4];
-    if ((threadIdx.x & 63) == 0) {
         s_count[threadIdx.x >> 6] = n;
     }
     __syncthreads();
@@ -116,9 +135,11 @@ __global__ __launch_bounds__(256) void row_energy_kernel(StageDesc d, const floa
 }
 
 // Patch energies: energy[f][y][x] = sum over channels, frames and rows of
-// the row sums, for every patch origin.
+// the row sums, for every patch origin.  flags[0] becomes 1 where an energy
+// is not finite or above kEnergyBound.
 __global__ __launch_bounds__(256) void patch_energy_kernel(StageDesc d, const float* __restrict__ rows,
-                                                           int match_channels, float* __restrict__ energy) {
+                                                           int match_channels, float* __restrict__ energy,
+                                                           unsigned* __restrict__ flags) {
     const size_t plane = static_cast<size_t>(d.width) * d.height;
     const size_t total = plane * d.frames;
     const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -139,93 +160,48 @@ __global__ __launch_bounds__(256) void patch_energy_kernel(StageDesc d, const fl
                 }
             }
         }
+        if (!(sum <= kEnergyBound)) {
+            atomicOr(flags, 1u);
+        }
     }
     energy[i] = sum;
 }
 
-struct FastShape {
-    int row_tiles;     // 16-row candidate tiles per column
-    int region_rows;   // 16 * row_tiles + P - 1
-    int pitch;         // bf16 samples per LDS copy row (multiple of 4)
-    int copy_elems;    // region_rows * pitch
-    int stride;        // distance slots per anchor (window^2 * temporal)
-    int per_frame;     // window^2
-    int ebox_pitch;    // energy box column stride: 16 * row_tiles + 4 (16-byte rows, banks spread)
-    int ebox_words;    // energy box, column-major: max cols x ebox_pitch
-    int scratch_keys;  // compaction list per anchor (window^2 16-bit entries, rounded to 2)
-};
-
-__device__ inline uint2 pack4(__bf16 a, __bf16 b, __bf16 c, __bf16 d) {
-    const unsigned lo = static_cast<unsigned>(__builtin_bit_cast(unsigned short, a)) |
-                        (static_cast<unsigned>(__builtin_bit_cast(unsigned short, b)) << 16);
-    const unsigned hi = static_cast<unsigned>(__builtin_bit_cast(unsigned short, c)) |
-                        (static_cast<unsigned>(__builtin_bit_cast(unsigned short, d)) << 16);
-    return uint2{lo, hi};
+// split[f][c][y][x] = bf16(v) << 16 | bf16(v - bf16(v)) of v = sample - offset.
+__global__ __launch_bounds__(256) void split_planes_kernel(StageDesc d, const float* __restrict__ source,
+                                                           const float* __restrict__ offsets,
+                                                           int match_channels, uint32_t* __restrict__ split) {
+    const size_t plane = static_cast<size_t>(d.width) * d.height;
+    const size_t total = plane * match_channels * d.frames;
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= total) {
+        return;
+    }
+    const size_t f = i / (plane * match_channels);
+    const size_t rem = i - f * plane * match_channels;
+    const int c = static_cast<int>(rem / plane);
+    const size_t p = rem - static_cast<size_t>(c) * plane;
+    const float v = source[(f * d.channels + c) * plane + p] - offsets[c];
+    const __bf16 hi = static_cast<__bf16>(v);
+    const __bf16 lo = static_cast<__bf16>(v - static_cast<float>(hi));
+    split[i] = (static_cast<uint32_t>(__builtin_bit_cast(unsigned short, hi)) << 16) |
+               static_cast<uint32_t>(__builtin_bit_cast(unsigned short, lo));
 }
 
-__device__ inline bf16x8 load_bf16x8(const __bf16* p) {
-    // 8-byte aligned: two 64-bit LDS reads.
-    const uint2* q = reinterpret_cast<const uint2*>(p);
-    const uint2 a = q[0];
-    const uint2 b = q[1];
-    const u32x4 v = {a.x, a.y, b.x, b.y};
-    return __builtin_bit_cast(bf16x8, v);
+// The bf16 hi (lo) halves of two packed samples as one dword, a first:
+// v_perm_b32 picks bytes 2, 3 (0, 1) of a, then of b.
+__device__ inline uint32_t hi_pair(uint32_t a, uint32_t b) {
+    return __builtin_amdgcn_perm(b, a, 0x07060302u);
+}
+__device__ inline uint32_t lo_pair(uint32_t a, uint32_t b) {
+    return __builtin_amdgcn_perm(b, a, 0x05040100u);
 }
 
 // One compare-exchange step of a bitonic network over the wavefront:
 // partner lane ^ kJ, blocks of kK lanes alternately ascending (kK = 64: the
-// whole wavefront ascending).
-template <int kK, int kJ>
-__device__ inline uint64_t bitonic_step(uint64_t key, int lane) {
-    const uint64_t other = xor64<kJ>(key);
-    // Keep the smaller key where the lane is the lower end of an ascending
-    // pair or the upper end of a descending one (keys are distinct).
-    const bool want_low = ((lane & kJ) == 0) == ((lane & kK) == 0);
-    return (other < key) == want_low ? other : key;
-}
-
-// Wave-wide bitonic sort of 64 keys, one per lane, ascending by lane.
-__device__ inline uint64_t wave_sort64(uint64_t key, int lane) {
-    key = bitonic_step<2, 1>(key, lane);
-    key = bitonic_step<4, 2>(key, lane);
-    key = bitonic_step<4, 1>(key, lane);
-    key = bitonic_step<8, 4>(key, lane);
-    key = bitonic_step<8, 2>(key, lane);
-    key = bitonic_step<8, 1>(key, lane);
-    key = bitonic_step<16, 8>(key, lane);
-    key = bitonic_step<16, 4>(key, lane);
-    key = bitonic_step<16, 2>(key, lane);
-    key = bitonic_step<16, 1>(key, lane);
-    key = bitonic_step<32, 16>(key, lane);
-    key = bitonic_step<32, 8>(key, lane);
-    key = bitonic_step<32, 4>(key, lane);
-    key = bitonic_step<32, 2>(key, lane);
-    key = bitonic_step<32, 1>(key, lane);
-    key = bitonic_step<64, 32>(key, lane);
-    key = bitonic_step<64, 16>(key, lane);
-    key = bitonic_step<64, 8>(key, lane);
-    key = bitonic_step<64, 4>(key, lane);
-    key = bitonic_step<64, 2>(key, lane);
-    key = bitonic_step<64, 1>(key, lane);
-    return key;
-}
-
-// The 64 smallest of two ascending wave-wide lists, ascending: the first
-// half-cleaner of the bitonic merge of r and reversed c, then the merge.
-__device__ inline uint64_t wave_merge_lower64(uint64_t r, uint64_t c, int lane) {
-    const uint64_t reversed = __shfl(c, 63 - lane, 64);
-    uint64_t key = reversed < r ? reversed : r;
-    key = bitonic_step<64, 32>(key, lane);
-    key = bitonic_step<64, 16>(key, lane);
-    key = bitonic_step<64, 8>(key, lane);
-    key = bitonic_step<64, 4>(key, lane);
-    key = bitonic_step<64, 2>(key, lane);
-    key = bitonic_step<64, 1>(key, lane);
-    return key;
-}
-
-// Four independent lists at once (one per anchor of a wavefront): the
-// exchanges of a step are issued together so their latencies overlap.
+// whole wavefront ascending).  Four independent lists at once (one per
+// anchor of a wavefront): the exchanges of a step are issued together so
+// their latencies overlap.
 template <int kK, int kJ>
 __device__ inline void bitonic_step4(uint64_t (&key)[4], int lane) {
     uint64_t other[4];
@@ -233,6 +209,8 @@ __device__ inline void bitonic_step4(uint64_t (&key)[4], int lane) {
     for (int q = 0; q < 4; ++q) {
         other[q] = xor64<kJ>(key[q]);
     }
+    // Keep the smaller key where the lane is the lower end of an ascending
+    // pair or the upper end of a descending one (keys are distinct).
     const bool want_low = ((lane & kJ) == 0) == ((lane & kK) == 0);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -240,6 +218,7 @@ __device__ inline void bitonic_step4(uint64_t (&key)[4], int lane) {
     }
 }
 
+// Wave-wide bitonic sort of 64 keys per list, one per lane, ascending by lane.
 __device__ inline void wave_sort64x4(uint64_t (&key)[4], int lane) {
     bitonic_step4<2, 1>(key, lane);
     bitonic_step4<4, 2>(key, lane);
@@ -264,7 +243,9 @@ __device__ inline void wave_sort64x4(uint64_t (&key)[4], int lane) {
     bitonic_step4<64, 1>(key, lane);
 }
 
-// best[q] <- the 64 smallest of best[q] and sorted[q], ascending.
+// best[q] <- the 64 smallest of best[q] and sorted[q], ascending: the first
+// half-cleaner of the bitonic merge of best and reversed sorted, then the
+// merge.
 __device__ inline void wave_merge_lower64x4(uint64_t (&best)[4], const uint64_t (&sorted)[4], int lane) {
     uint64_t key[4];
 #pragma unroll
@@ -290,12 +271,19 @@ constexpr uint64_t kNoKey = ~static_cast<uint64_t>(0);
 // rather than sorted and merged.
 constexpr int kInsertKeys = 24;
 
+// The value of lane l - 1 in lane l (DPP wave_shr:1; lane 0 keeps 0).
+__device__ inline uint64_t from_lane_below(uint64_t v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(v & 0xFFFFFFFFu), 0x138, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(v >> 32), 0x138, 0xF, 0xF, false);
+    return (static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) | static_cast<uint32_t>(lo);
+}
+
 // `key` into a list of distinct keys ascending over the lanes (the 64
 // smallest so far; kNoKey for empty places): the keys below it stay, the
 // rest move up one lane and the largest drops out.
 __device__ inline uint64_t wave_insert(uint64_t list, uint64_t key, int lane) {
     const int place = __popcll(__ballot(list < key));
-    const uint64_t before = __shfl_up(list, 1, 64);
+    const uint64_t before = from_lane_below(list);
     return lane < place ? list : (lane == place ? key : before);
 }
 
@@ -320,28 +308,63 @@ __device__ inline uint64_t admit_bound(uint64_t best, int similar, uint64_t tau_
     return open < last ? open : last;
 }
 
-// Cross terms of one anchor tile against its candidates, all candidate
-// frames.  kFused = false: distances land in `out` ([anchor][stride], scan
-// order) for select_fast_kernel.  kFused = true (retained cap <= 64): the
-// k-nearest selection runs here.  Each wavefront keeps, for four anchors of
+// LDS rows of one staged plane: the bf16 hi of groups of 4 columns, four
+// copies of each group (shifted by 0..3 columns, 8 bytes each), then the lo
+// in the same order, then 8 bytes: rows 2 x odd dwords apart, so the 8-byte
+// operand reads of 16 consecutive rows fall on distinct banks.
+__host__ __device__ constexpr int staged_row_bytes(int groups) {
+    return 2 * 8 * kCopies * groups + 8;
+}
+// The anchors' packed samples of one plane after its rows, 16-byte pieces
+// [patch row][half][slot] (16 slots read one piece each, on distinct banks).
+constexpr int kAnchorBytes = kTileAnchors * kFastPatch * kFastPatch * 4;
+
+// LDS layout of the matcher for one launch: every size follows from the
+// instantiation's row tiles (kR) and union columns per wavefront (kA), the
+// window and the widest union.  Two plane buffers; after a candidate frame's
+// products the epilogue takes the space of both.
+struct TileLayout {
+    int groups;        // 4-column groups per staged row: kA + 1
+    int row_bytes;     // staged_row_bytes(groups)
+    int rows_bytes;    // (16 kR + P - 1) rows
+    int buffer_bytes;  // rows, then the anchors' samples
+    int stride;        // distance slots per anchor (window^2 * temporal), staged mode
+    int per_frame;     // window^2
+    int ebox_pitch;    // floats per union column of the energy box (16 kR + 1: odd, columns on distinct banks)
+    int ebox_bytes;    // energy box, column-major, 16-byte aligned
+    int stage_bytes;   // window distances of the 16 anchor slots
+    int scratch_keys;  // candidate list entries per anchor (window^2 rounded to 2)
+    int sub_keys;      // entries of one wavefront's part of a list (its union columns of a window)
+    int shared_bytes;  // max(two buffers, energy box + distances + lists)
+};
+
+// Distances of one anchor tile against its candidates, all candidate frames.
+// fused = 0: distances land in `out` ([anchor][stride], scan order) for
+// select_fast_kernel.  fused = 1 (retained cap and K <= 64): the k-nearest
+// selection runs here.  Each wavefront keeps, for anchors 4 w .. 4 w + 3 of
 // the tile, the 64 smallest keys (distance key << 32 | scan position) seen
-// so far in lane order; after every candidate frame the candidates below the
-// 64th key are compacted, sorted and merged in.  The selected set of
-// select.h is a prefix of the (distance, scan position) order no longer
-// than the retained cap, so it lies within those 64 keys, and the lists
-// written are those of select_and_write.
-template <int kTilesPerWave, bool kFused>
-__global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
-    StageDesc d, FastShape S, const float* __restrict__ source, const float* __restrict__ offsets,
+// so far in lane order.  The selected set of select.h is a prefix of the
+// (distance, scan position) order no longer than the retained cap, so it
+// lies within those 64 keys, and the lists written are those of
+// select_and_write.
+template <int kR, int kA>
+__global__ __launch_bounds__(kFastThreads) __attribute__((amdgpu_waves_per_eu(kR * kA <= 18 ? 2 : 1)))
+void match_fast_kernel(
+    StageDesc d, TileLayout L, const uint32_t* __restrict__ split, const unsigned* __restrict__ flags,
     const float* __restrict__ energy, const uint32_t* __restrict__ anchors,
-    const int* __restrict__ tiles, int anchor_base, float* __restrict__ out, int* __restrict__ counts,
-    uint32_t* __restrict__ positions, float* __restrict__ distances) {
+    const int* __restrict__ tiles, int anchor_base, int fused, float* __restrict__ out,
+    int* __restrict__ counts, uint32_t* __restrict__ positions, float* __restrict__ distances) {
+    constexpr int kTiles = kR * kA;  // tile i: union column 4 (i / kR) + wave, row tile i % kR
+    static_assert(kTiles % 2 == 0, "tiles go through the matrix cores in pairs");
+    constexpr int kGroups = kA + 1;
+    constexpr int kRowBytes = staged_row_bytes(kGroups);
+    constexpr int kRegionRows = 16 * kR + kFastPatch - 1;
+    constexpr int kRowsBytes = kRegionRows * kRowBytes;
+    constexpr int kItems = (kRegionRows * kGroups + kFastThreads - 1) / kFastThreads;
+    static_assert(kItems <= kItemsPerThread, "staging items per thread");
+    static_assert(kAnchorBytes == 16 * kFastThreads, "one 16-byte anchor piece per thread");
     extern __shared__ uint4 s_dyn16[];
-    __bf16* s_copy = reinterpret_cast<__bf16*>(s_dyn16);  // [buffer][hi/lo][copy][rows][pitch]
-    // Epilogue staging after the two plane buffers.
-    float* s_stage = reinterpret_cast<float*>(s_copy + static_cast<size_t>(4) * kCopies * S.copy_elems);
-    // Patch energies of the candidate frame over the tile's window union
-    // go into the plane buffer that the epilogue's plane has finished with.
+    unsigned char* s_bytes = reinterpret_cast<unsigned char*>(s_dyn16);  // [buffer][rows | anchors]
     __shared__ int s_index[kTileAnchors];
     __shared__ int s_ax[kTileAnchors];
     __shared__ int s_ay[kTileAnchors];
@@ -350,7 +373,18 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
     __shared__ int s_nx[kTileAnchors];
     __shared__ int s_ny[kTileAnchors];
     __shared__ float s_energy[kTileAnchors];
+    // Largest distance that can still enter each anchor's list (NaN: every
+    // window distance is needed) and each wavefront's count of the frame's
+    // candidates per anchor.
+    __shared__ __align__(16) float s_bound[kTileAnchors];
+    __shared__ int s_parts[kTileAnchors * kFastWaves];
+    // Between candidate frames (fused): the running lists of the anchors of
+    // each wavefront and their counts of valid (non-NaN) distances.
+    __shared__ uint64_t s_best[kFastWaves][4][64];
+    __shared__ int s_valid[kTileAnchors];
     __shared__ int s_box[5];
+    __shared__ int s_vx[8];  // union x range of the anchors of each tile column
+    __shared__ int s_full;
 
     const int tid = threadIdx.x;
     // Uniform per wavefront: tile indices derived from it stay scalar.
@@ -359,28 +393,34 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
     const int* tile = tiles + static_cast<size_t>(blockIdx.x) * kTileAnchors;
     const bool final_stage = d.stage == kFinal;
     const int match_channels = final_stage ? d.channels : 1;
+    const bool finite = flags[0] == 0u;
 
     if (tid < kTileAnchors) {
         const int index = tile[tid];
         s_index[tid] = index;
+        int ax = 0;
+        int ay = 0;
+        int xl = 0;
+        int xh = -1;
+        int yl = 0;
+        int yh = -1;
+        float e = 0.0F;
         if (index >= 0) {
             const uint32_t p = anchors[index];
-            const int ax = position_x(p);
-            const int ay = position_y(p);
-            int xl = 0;
-            int xh = 0;
-            int yl = 0;
-            int yh = 0;
+            ax = position_x(p);
+            ay = position_y(p);
             spatial_range(d, ax, d.width, xl, xh);
             spatial_range(d, ay, d.height, yl, yh);
-            s_ax[tid] = ax;
-            s_ay[tid] = ay;
-            s_xl[tid] = xl;
-            s_yl[tid] = yl;
-            s_nx[tid] = xh - xl + 1;
-            s_ny[tid] = yh - yl + 1;
-            s_energy[tid] = energy[(static_cast<size_t>(position_frame(p)) * d.height + ay) * d.width + ax];
+            e = energy[(static_cast<size_t>(position_frame(p)) * d.height + ay) * d.width + ax];
         }
+        s_ax[tid] = ax;
+        s_ay[tid] = ay;
+        s_xl[tid] = xl;
+        s_yl[tid] = yl;
+        s_nx[tid] = xh - xl + 1;  // 0 for an empty slot
+        s_ny[tid] = yh - yl + 1;
+        s_energy[tid] = e;
+        s_bound[tid] = __builtin_nanf("");
     }
     __syncthreads();
     if (tid == 0) {
@@ -389,6 +429,10 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
         int y0 = 1 << 30;
         int y1 = -1;
         int first = -1;
+        for (int v = 0; v < 4; ++v) {
+            s_vx[v] = 1 << 30;
+            s_vx[4 + v] = -1;
+        }
         for (int n = 0; n < kTileAnchors; ++n) {
             if (s_index[n] >= 0) {
                 first = first < 0 ? s_index[n] : first;
@@ -396,6 +440,8 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
                 x1 = max(x1, s_xl[n] + s_nx[n] - 1);
                 y0 = min(y0, s_yl[n]);
                 y1 = max(y1, s_yl[n] + s_ny[n] - 1);
+                s_vx[n & 3] = min(s_vx[n & 3], s_xl[n]);
+                s_vx[4 + (n & 3)] = max(s_vx[4 + (n & 3)], s_xl[n] + s_nx[n] - 1);
             }
         }
         s_box[0] = x0;
@@ -411,327 +457,391 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
     const int af = position_frame(anchors[s_box[4]]);
     const int ux0 = s_box[0];
     const int uy0 = s_box[2];
-    const int ucols = s_box[1] - ux0 + 1;
+    const int ucols = s_box[1] - ux0 + 1;  // <= 4 kA (host checked)
     const int urows = s_box[3] - uy0 + 1;
-    const int row_tiles = (urows + 15) / 16;  // <= S.row_tiles (host checked)
-    const int mtiles = ucols * row_tiles;     // <= kTilesPerWave * waves (host checked)
-    // col = mt / row_tiles as (mt * magic) >> 16, exact for mt < 32768.
-    const int tile_magic = (65536 + row_tiles - 1) / row_tiles;
+    const int row_tiles = (urows + 15) / 16;  // <= kR (host checked)
     int t_low = 0;
     int t_high = 0;
     temporal_range(d, af, t_low, t_high);
     const int nt = t_high - t_low + 1;
     const size_t plane = static_cast<size_t>(d.width) * d.height;
 
-    const int n_lane = lane & 15;  // anchor (B column) / candidate row (A row)
-    const int kb = lane >> 4;      // 8-sample block of the 32-wide k step
-    const int my_anchor = s_index[n_lane];
-    const int my_ax = my_anchor >= 0 ? s_ax[n_lane] : 0;
-    const int my_ay = my_anchor >= 0 ? s_ay[n_lane] : 0;
+    const int n_lane = lane & 15;  // anchor slot (A row) / candidate of the tile (B column)
+    const int kb = lane >> 4;      // 8-sample block of the 32-wide k step / grid row of the result
+    // Union columns of this wavefront (wave, wave + 4, ...) and row tiles.
+    const int wave_cols = __builtin_amdgcn_readfirstlane(ucols > wave ? (ucols - wave + 3) / 4 : 0);
+    const int wave_rows = __builtin_amdgcn_readfirstlane(row_tiles);
+    const auto tile_ok = [&](int i) {
+        return i / kR < wave_cols && i % kR < wave_rows;
+    };
 
-    // Planes (candidate frame ti, channel c, patch frame t) in order, the
-    // region of plane p + 1 loaded into registers while plane p is on the
-    // matrix cores, then stored into the other LDS buffer.
+    // Planes (candidate frame ti, channel c, patch frame t) in order.
     const int planes_per_frame = match_channels * d.patch_time;
     const int total_planes = nt * planes_per_frame;
-    const int groups4 = S.pitch / 4;
-    const int fill_items = S.region_rows * groups4;  // <= kFillPerThread * threads (host checked)
-    float pre[kFillPerThread][kFillSpan];
 
-    const auto plane_of = [&](int p, int& frame, int& c, int& t) {
+    // Staged items: (row r, group g) holds columns 4 g .. 4 g + 6 of the
+    // region for the four copies.  Rows past the union's patches and groups
+    // past its columns are not staged (only tiles past the union read them).
+    // Each thread also stages one 16-byte piece of the anchors' samples:
+    // slot tid % 16, patch row tid / 32, columns 4 ((tid / 16) % 2) .. + 3.
+    const int groups_used = (ucols - 1) / 4 + 2;
+    const int rows_used = urows + kFastPatch - 1;
+    int item_src[kItems];  // sample offset in a plane of column 4 g, or -1
+    int item_lds[kItems];  // byte offset in a buffer
+    int item_x[kItems];    // frame column of 4 g
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const int item = tid + k * kFastThreads;
+        const int r = item / kGroups;
+        const int g = item - r * kGroups;
+        const bool ok = r < rows_used && g < groups_used;
+        item_x[k] = ux0 + 4 * g;
+        item_src[k] = ok ? (uy0 + r) * d.width + item_x[k] : -1;
+        item_lds[k] = r * kRowBytes + 8 * kCopies * g;
+    }
+    const int piece_slot = tid & 15;
+    const int piece_src = s_index[piece_slot] >= 0
+                              ? (s_ay[piece_slot] + (tid >> 5)) * d.width + s_ax[piece_slot] + 4 * ((tid >> 4) & 1)
+                              : -1;
+    uint32_t pre[kItems][7];
+    uint4 pre_piece;
+    const auto prefetch = [&](int p) {
         const int ti = p / planes_per_frame;
         const int rem = p - ti * planes_per_frame;
-        c = rem / d.patch_time;
-        t = rem - c * d.patch_time;
-        frame = scheduled_frame(af, t_low, t_high, ti);
-    };
-    const auto prefetch = [&](int p) {
-        int frame = 0;
-        int c = 0;
-        int t = 0;
-        plane_of(p, frame, c, t);
-        const float mu = offsets[c];
-        const float* src = source + (static_cast<size_t>(frame + t) * d.channels + c) * plane;
+        const int c = rem / d.patch_time;
+        const int t = rem - c * d.patch_time;
+        const int frame = scheduled_frame(af, t_low, t_high, ti);
+        const uint32_t* src = split + (static_cast<size_t>(frame + t) * match_channels + c) * plane;
 #pragma unroll
-        for (int k = 0; k < kFillPerThread; ++k) {
-            const int item = tid + k * kFastThreads;
-            const int ry = item / groups4;
-            const int x0 = ux0 + 4 * (item - ry * groups4);
-            const int y = uy0 + ry;
-            const bool row_ok = item < fill_items && y < d.height;
-            const float* row_src = src + static_cast<size_t>(y) * d.width + x0;
-            if (row_ok && x0 + 8 <= d.width) {
-                // Interior: two 16-byte loads (4-byte aligned) cover the span.
-                float4 a;
-                float4 b;
-                __builtin_memcpy(&a, row_src, sizeof(a));
-                __builtin_memcpy(&b, row_src + 4, sizeof(b));
-                const float span[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        for (int k = 0; k < kItems; ++k) {
+            if (item_src[k] >= 0) {
+                if (item_x[k] + 7 <= d.width) {
+                    uint4 a;
+                    uint2 b;
+                    __builtin_memcpy(&a, src + item_src[k], sizeof(a));
+                    __builtin_memcpy(&b, src + item_src[k] + 4, sizeof(b));
+                    pre[k][0] = a.x;
+                    pre[k][1] = a.y;
+                    pre[k][2] = a.z;
+                    pre[k][3] = a.w;
+                    pre[k][4] = b.x;
+                    pre[k][5] = b.y;
+                    pre[k][6] = src[item_src[k] + 6];
+                } else {
+                    // Right border: columns past the frame stage as zeros.
 #pragma unroll
-                for (int v = 0; v < kFillSpan; ++v) {
-                    pre[k][v] = span[v] - mu;
-                }
-            } else {
-#pragma unroll
-                for (int v = 0; v < kFillSpan; ++v) {
-                    float value = 0.0F;
-                    if (row_ok && x0 + v < d.width) {
-                        value = row_src[v] - mu;
+                    for (int e = 0; e < 7; ++e) {
+                        pre[k][e] = item_x[k] + e < d.width ? src[item_src[k] + e] : 0u;
                     }
-                    pre[k][v] = value;
                 }
             }
         }
+        // The anchor patch lies inside the frame.
+        const uint32_t* anchor_src = split + (static_cast<size_t>(af + t) * match_channels + c) * plane;
+        pre_piece = uint4{0u, 0u, 0u, 0u};
+        if (piece_src >= 0) {
+            __builtin_memcpy(&pre_piece, anchor_src + piece_src, sizeof(pre_piece));
+        }
     };
-    const auto store = [&](int buffer) {
-        __bf16* base = s_copy + static_cast<size_t>(buffer) * 2 * kCopies * S.copy_elems;
+    const auto store = [&](int p) {
+        unsigned char* base = s_bytes + static_cast<size_t>(p & 1) * L.buffer_bytes;
 #pragma unroll
-        for (int k = 0; k < kFillPerThread; ++k) {
-            const int item = tid + k * kFastThreads;
-            if (item < fill_items) {
-                const int ry = item / groups4;
-                const int j = 4 * (item - ry * groups4);
-                __bf16 hi[kFillSpan];
-                __bf16 lo[kFillSpan];
-#pragma unroll
-                for (int v = 0; v < kFillSpan; ++v) {
-                    hi[v] = static_cast<__bf16>(pre[k][v]);
-                    lo[v] = static_cast<__bf16>(pre[k][v] - static_cast<float>(hi[v]));
-                }
+        for (int k = 0; k < kItems; ++k) {
+            if (item_src[k] >= 0) {
+                // Copies 0..3 of group g (copy q: columns 4 g + q .. 4 g + q + 3),
+                // hi, then the same for lo in the row's second half.
+                const uint32_t* s = pre[k];
+                uint2* at = reinterpret_cast<uint2*>(base + item_lds[k]);
 #pragma unroll
                 for (int q = 0; q < kCopies; ++q) {
-                    // copy q holds plane[ry][j + q .. j + q + 3] at columns j .. j + 3
-                    const int at = q * S.copy_elems + ry * S.pitch + j;
-                    *reinterpret_cast<uint2*>(base + at) = pack4(hi[q], hi[q + 1], hi[q + 2], hi[q + 3]);
-                    *reinterpret_cast<uint2*>(base + kCopies * S.copy_elems + at) =
-                        pack4(lo[q], lo[q + 1], lo[q + 2], lo[q + 3]);
+                    at[q] = uint2{hi_pair(s[q], s[q + 1]), hi_pair(s[q + 2], s[q + 3])};
+                    at[kCopies * kGroups + q] = uint2{lo_pair(s[q], s[q + 1]), lo_pair(s[q + 2], s[q + 3])};
                 }
             }
+        }
+        reinterpret_cast<uint4*>(base + kRowsBytes)[tid] = pre_piece;
+    };
+
+    // Anchor operands of plane p: slot n_lane, patch rows 4 ks + kb, split
+    // into hi / lo.
+    bf16x8 a_hi[2];
+    bf16x8 a_lo[2];
+    const auto load_anchor = [&](int p) {
+        const uint4* piece = reinterpret_cast<const uint4*>(s_bytes + static_cast<size_t>(p & 1) * L.buffer_bytes +
+                                                            kRowsBytes) + n_lane;
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            const uint4 a = piece[32 * (4 * ks + kb)];
+            const uint4 b = piece[32 * (4 * ks + kb) + 16];
+            const u32x4 hi = {hi_pair(a.x, a.y), hi_pair(a.z, a.w), hi_pair(b.x, b.y), hi_pair(b.z, b.w)};
+            const u32x4 lo = {lo_pair(a.x, a.y), lo_pair(a.z, a.w), lo_pair(b.x, b.y), lo_pair(b.z, b.w)};
+            a_hi[ks] = __builtin_bit_cast(bf16x8, hi);
+            a_lo[ks] = __builtin_bit_cast(bf16x8, lo);
         }
     };
 
-    // Anchor operands (anchor n_lane, patch rows 4 ks + kb of plane
-    // (af + t, c), offset by the channel mean) for the next plane.
-    float pre_anchor[2][8];
-    const auto prefetch_anchor = [&](int p) {
-        int frame = 0;
-        int c = 0;
-        int t = 0;
-        plane_of(p, frame, c, t);
-        const float mu = offsets[c];
-        const float* anchor_plane = source + (static_cast<size_t>(af + t) * d.channels + c) * plane;
-#pragma unroll
-        for (int ks = 0; ks < 2; ++ks) {
-            const int py = 4 * ks + kb;
-            // The anchor patch lies inside the frame: 8 samples, two 16-byte loads.
-            float4 a = {0.0F, 0.0F, 0.0F, 0.0F};
-            float4 b = {0.0F, 0.0F, 0.0F, 0.0F};
-            if (my_anchor >= 0) {
-                const float* row = anchor_plane + static_cast<size_t>(my_ay + py) * d.width + my_ax;
-                __builtin_memcpy(&a, row, sizeof(a));
-                __builtin_memcpy(&b, row + 4, sizeof(b));
-            }
-            const float span[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-#pragma unroll
-            for (int v = 0; v < 8; ++v) {
-                pre_anchor[ks][v] = my_anchor >= 0 ? span[v] - mu : 0.0F;
-            }
-        }
+    // Candidate operands of tile i, patch rows 4 ks .. 4 ks + 3: row
+    // 16 rt + n_lane + 4 ks + kb, copy `wave` of groups a and a + 1
+    // (a = i / kR), hi and lo: the group offsets are immediates of the LDS
+    // reads.  Reads of tiles past the union stay inside the rows (groups
+    // kA + 1, rows 16 kR + 7).
+    const unsigned lane_bytes = static_cast<unsigned>((n_lane + kb) * kRowBytes + 8 * wave);
+    const auto load_tile = [&](const unsigned (&base)[kR][2], int i, int ks, bf16x8& b_hi, bf16x8& b_lo) {
+        const uint2* at = reinterpret_cast<const uint2*>(s_bytes + base[i % kR][ks] + 8 * kCopies * (i / kR));
+        const uint2 h0 = at[0];
+        const uint2 h1 = at[kCopies];
+        const uint2 l0 = at[kCopies * kGroups];
+        const uint2 l1 = at[kCopies * kGroups + kCopies];
+        const u32x4 hi = {h0.x, h0.y, h1.x, h1.y};
+        const u32x4 lo = {l0.x, l0.y, l1.x, l1.y};
+        b_hi = __builtin_bit_cast(bf16x8, hi);
+        b_lo = __builtin_bit_cast(bf16x8, lo);
     };
-    // Energies of a candidate frame over the union box, kEnergyPerThread
-    // items per thread (host checked), for the epilogue.
+
+    // Epilogue space (both plane buffers once a frame's products are done):
+    // the energy box, the window distances per anchor slot, the candidate
+    // lists per anchor slot.
+    float* s_ebox = reinterpret_cast<float*>(s_bytes);
+    float* s_stage = reinterpret_cast<float*>(s_bytes + L.ebox_bytes);
+    uint16_t* s_list = reinterpret_cast<uint16_t*>(s_bytes + L.ebox_bytes + L.stage_bytes);
     const int box_items = urows * ucols;
-    float pre_energy[kEnergyPerThread];
-    const auto prefetch_energy = [&](int frame) {
-        const float* frame_energy = energy + static_cast<size_t>(frame) * plane;
+
+    // Keys at or below tau (any scan position) may be selected whatever d_K is.
+    const uint64_t tau_key = (static_cast<uint64_t>(distance_key(d.tau)) << 32) | 0xFFFFFFFFull;
+    if (tid < kTileAnchors) {
+        s_valid[tid] = 0;
+    }
+    prefetch(0);
+    store(0);
+    __syncthreads();
+    for (int ti = 0; ti < nt; ++ti) {
+        const int frame = scheduled_frame(af, t_low, t_high, ti);
+        floatx4 acc[kTiles];
+#pragma unroll
+        for (int i = 0; i < kTiles; ++i) {
+            acc[i] = floatx4{0.0F, 0.0F, 0.0F, 0.0F};
+        }
+        const int p_last = (ti + 1) * planes_per_frame - 1;
+        for (int p = ti * planes_per_frame; p <= p_last; ++p) {
+            // The next plane of the frame goes into registers while this one
+            // is on the matrix cores, then into the other buffer.
+            if (p < p_last) {
+                prefetch(p + 1);
+            }
+            load_anchor(p);
+            unsigned base[kR][2];
+            const unsigned buffer_base = static_cast<unsigned>((p & 1) * L.buffer_bytes) + lane_bytes;
+#pragma unroll
+            for (int rt = 0; rt < kR; ++rt) {
+#pragma unroll
+                for (int ks = 0; ks < 2; ++ks) {
+                    base[rt][ks] = buffer_base + static_cast<unsigned>((16 * rt + 4 * ks) * kRowBytes);
+                }
+            }
+            // Pairs of tiles, their products alternating so consecutive MFMAs
+            // are independent; the operands of the pair's second half (ks = 1)
+            // and of the next pair's first half are read while the products run.
+            bf16x8 cur_hi[2][2];
+            bf16x8 cur_lo[2][2];
+            load_tile(base, 0, 0, cur_hi[0][0], cur_lo[0][0]);
+            load_tile(base, 1, 0, cur_hi[1][0], cur_lo[1][0]);
+#pragma unroll
+            for (int i = 0; i < kTiles; i += 2) {
+                const int j = i + 1;
+                const bool ok = tile_ok(i) || tile_ok(j);
+                load_tile(base, i, 1, cur_hi[0][1], cur_lo[0][1]);
+                load_tile(base, j, 1, cur_hi[1][1], cur_lo[1][1]);
+                if (ok) {
+                    // (A pair with one tile past the union computes it anyway:
+                    // its products are never read.)
+                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi[0], cur_hi[0][0], acc[i], 0, 0, 0);
+                    acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi[0], cur_hi[1][0], acc[j], 0, 0, 0);
+                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_lo[0], cur_hi[0][0], acc[i], 0, 0, 0);
+                    acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_lo[0], cur_hi[1][0], acc[j], 0, 0, 0);
+                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi[0], cur_lo[0][0], acc[i], 0, 0, 0);
+                    acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi[0], cur_lo[1][0], acc[j], 0, 0, 0);
+                }
+                if (i + 2 < kTiles) {
+                    load_tile(base, i + 2, 0, cur_hi[0][0], cur_lo[0][0]);
+                    load_tile(base, i + 3, 0, cur_hi[1][0], cur_lo[1][0]);
+                }
+                if (ok) {
+                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi[1], cur_hi[0][1], acc[i], 0, 0, 0);
+                    acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi[1], cur_hi[1][1], acc[j], 0, 0, 0);
+                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_lo[1], cur_hi[0][1], acc[i], 0, 0, 0);
+                    acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_lo[1], cur_hi[1][1], acc[j], 0, 0, 0);
+                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi[1], cur_lo[0][1], acc[i], 0, 0, 0);
+                    acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi[1], cur_lo[1][1], acc[j], 0, 0, 0);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            if (p < p_last) {
+                store(p + 1);
+            }
+            __syncthreads();
+        }
+
+        // Epilogue of the candidate frame in the space of both buffers.
 #pragma unroll
         for (int k = 0; k < kEnergyPerThread; ++k) {
             int item = tid + k * kFastThreads;
             // Opaque: the box offsets are recomputed per frame instead of
             // being held in registers across the plane loop.
             asm volatile("" : "+v"(item));
-            item = item < box_items ? item : box_items - 1;  // a cell of the box
-            const int ry = item / ucols;
-            const int cx = item - ry * ucols;
-            pre_energy[k] = frame_energy[(uy0 + ry) * d.width + ux0 + cx];
-        }
-    };
-
-    // Operands of tile i, patch rows 4 ks .. 4 ks + 3: mt clamped to a valid
-    // tile so every LDS read stays in the staged region (accumulators of
-    // tiles past mtiles are never used).
-    const auto load_tile = [&](const __bf16* copies, int i, int ks, bf16x8& a_hi, bf16x8& a_lo) {
-        int mt = wave + kFastWaves * i;
-        // Opaque to loop-invariant motion: the few scalar address operations
-        // are redone per plane instead of holding every tile's addresses.
-        asm volatile("" : "+s"(mt));
-        mt = mt < mtiles ? mt : mtiles - 1;
-        const int col = (mt * tile_magic) >> 16;
-        const int rt = mt - col * row_tiles;
-        const int shift = col & (kCopies - 1);
-        const int j = col - shift;
-        const int ry = 16 * rt + n_lane + 4 * ks + kb;
-        const int at = shift * S.copy_elems + ry * S.pitch + j;
-        a_hi = load_bf16x8(copies + at);
-        a_lo = load_bf16x8(copies + kCopies * S.copy_elems + at);
-    };
-
-    floatx4 acc[kTilesPerWave];
-    uint64_t best[4] = {kNoKey, kNoKey, kNoKey, kNoKey};  // kFused: anchors 4 wave + q
-    // Keys at or below tau (any scan position) may be selected whatever d_K is.
-    const uint64_t tau_key = (static_cast<uint64_t>(distance_key(d.tau)) << 32) | 0xFFFFFFFFull;
-    int valid_count[4] = {0, 0, 0, 0};
-    prefetch(0);
-    prefetch_anchor(0);
-    store(0);
-    float cur_anchor[2][8];
-#pragma unroll
-    for (int ks = 0; ks < 2; ++ks) {
-#pragma unroll
-        for (int v = 0; v < 8; ++v) {
-            cur_anchor[ks][v] = pre_anchor[ks][v];
-        }
-    }
-    __syncthreads();
-    for (int p = 0; p < total_planes; ++p) {
-        int frame = 0;
-        int c = 0;
-        int t = 0;
-        plane_of(p, frame, c, t);
-        const bool frame_done = c == match_channels - 1 && t == d.patch_time - 1;
-        if (c == 0 && t == 0) {
-#pragma unroll
-            for (int i = 0; i < kTilesPerWave; ++i) {
-                acc[i] = floatx4{0.0F, 0.0F, 0.0F, 0.0F};
+            if (item < box_items) {
+                const int ry = item / ucols;
+                const int cx = item - ry * ucols;
+                s_ebox[cx * L.ebox_pitch + ry] =
+                    energy[(static_cast<size_t>(frame) * d.height + uy0 + ry) * d.width + ux0 + cx];
             }
         }
-        if (p + 1 < total_planes) {
-            prefetch(p + 1);
-            prefetch_anchor(p + 1);
-        }
-        if (frame_done) {
-            prefetch_energy(frame);
-        }
-        bf16x8 b_hi[2];
-        bf16x8 b_lo[2];
-#pragma unroll
-        for (int ks = 0; ks < 2; ++ks) {
-#pragma unroll
-            for (int v = 0; v < 8; ++v) {
-                const float x = cur_anchor[ks][v];
-                const __bf16 hi = static_cast<__bf16>(x);
-                b_hi[ks][v] = hi;
-                b_lo[ks][v] = static_cast<__bf16>(x - static_cast<float>(hi));
+        if (wave == 0) {
+            const bool open = lane < kTileAnchors && s_index[lane] >= 0 && s_bound[lane] != s_bound[lane];
+            const unsigned long long any = __ballot(open);
+            if (lane == 0) {
+                s_full = any != 0ull ? 1 : 0;
             }
         }
-        const __bf16* copies = s_copy + static_cast<size_t>(p & 1) * 2 * kCopies * S.copy_elems;
-        // Pairs of tiles, their products alternating so consecutive MFMAs
-        // are independent; the LDS reads of the next pair's half (ks) go out
-        // as soon as the current pair's products of that half are issued.
-        bf16x8 cur_hi[2][2];
-        bf16x8 cur_lo[2][2];
+        __syncthreads();
+        const bool anchor_frame = ti == 0;  // the schedule starts with the anchor frame
+        const bool full = fused == 0 || anchor_frame || !finite || s_full != 0;
+        {
+            // Result lanes: candidate n_lane against slots 4 kb + v (grid row
+            // kb, one y range; an empty slot has an empty x range).
+            float ea[4];
+            int exl[4];
+            int enx[4];
+            int eyl = 0;
+            int eny = 0;
 #pragma unroll
-        for (int ks = 0; ks < 2; ++ks) {
-            load_tile(copies, 0, ks, cur_hi[0][ks], cur_lo[0][ks]);
-            load_tile(copies, 1, ks, cur_hi[1][ks], cur_lo[1][ks]);
-        }
-#pragma unroll
-        for (int i = 0; i < kTilesPerWave; i += 2) {
-            int mt = wave + kFastWaves * i;
-            asm volatile("" : "+s"(mt));
-            if (mt >= mtiles) {
-                break;  // uniform: later pairs are past the tile count too
-            }
-#pragma unroll
-            for (int ks = 0; ks < 2; ++ks) {
-                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur_hi[0][ks], b_hi[ks], acc[i], 0, 0, 0);
-                acc[i + 1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur_hi[1][ks], b_hi[ks], acc[i + 1], 0, 0, 0);
-                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur_hi[0][ks], b_lo[ks], acc[i], 0, 0, 0);
-                acc[i + 1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur_hi[1][ks], b_lo[ks], acc[i + 1], 0, 0, 0);
-                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur_lo[0][ks], b_hi[ks], acc[i], 0, 0, 0);
-                acc[i + 1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur_lo[1][ks], b_hi[ks], acc[i + 1], 0, 0, 0);
-                if (i + 2 < kTilesPerWave) {
-                    load_tile(copies, i + 2, ks, cur_hi[0][ks], cur_lo[0][ks]);
-                    load_tile(copies, i + 3, ks, cur_hi[1][ks], cur_lo[1][ks]);
+            for (int v = 0; v < 4; ++v) {
+                const int slot = 4 * kb + v;
+                ea[v] = s_energy[slot];
+                exl[v] = s_xl[slot];
+                enx[v] = s_nx[slot];
+                if (s_index[slot] >= 0) {
+                    eyl = s_yl[slot];
+                    eny = s_ny[slot];
                 }
             }
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        if (p + 1 < total_planes) {
-            store((p + 1) & 1);
+            if (full) {
+                // Every window distance of the frame into s_stage (scan
+                // order per anchor slot; the owners set the anchors' own
+                // distances).
 #pragma unroll
-            for (int ks = 0; ks < 2; ++ks) {
+                for (int i = 0; i < kTiles; ++i) {
+                    if (!tile_ok(i)) {
+                        continue;  // uniform
+                    }
+                    const int col = wave + 4 * (i / kR);
+                    const int rel_x0 = ux0 + col;
+                    const int ry = 16 * (i % kR) + n_lane;
+                    const float ec = s_ebox[col * L.ebox_pitch + ry];
+                    const int rel_y = uy0 + ry - eyl;
+                    const bool y_ok = static_cast<unsigned>(rel_y) < static_cast<unsigned>(eny);
 #pragma unroll
-                for (int v = 0; v < 8; ++v) {
-                    cur_anchor[ks][v] = pre_anchor[ks][v];
-                }
-            }
-        }
-        if (frame_done) {
-            // Epilogue: this candidate frame's distances into LDS per anchor
-            // (window scan order), then out to global memory.  The energy
-            // box takes the buffer of plane p once every wave is past it.
-            float* s_ebox = reinterpret_cast<float*>(s_copy + static_cast<size_t>(p & 1) * 2 * kCopies * S.copy_elems);
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < kEnergyPerThread; ++k) {
-                int item = tid + k * kFastThreads;
-                asm volatile("" : "+v"(item));
-                if (item < box_items) {
-                    const int ry = item / ucols;
-                    s_ebox[(item - ry * ucols) * S.ebox_pitch + ry] = pre_energy[k];
-                }
-            }
-            __syncthreads();
-            const int ti = p / planes_per_frame;
-            const bool anchor_frame = frame == af;
-            // The lane's anchor window in registers for the loop (an empty
-            // window for an empty slot).
-            const float my_energy = my_anchor >= 0 ? s_energy[n_lane] : 0.0F;
-            const int my_xl = my_anchor >= 0 ? s_xl[n_lane] : 0;
-            const int my_yl = my_anchor >= 0 ? s_yl[n_lane] : 0;
-            const int my_nx = my_anchor >= 0 ? s_nx[n_lane] : 0;
-            const int my_ny = my_anchor >= 0 ? s_ny[n_lane] : 0;
-            // The anchor against itself: exactly 0 (NaN for a patch holding
-            // NaN, as the exact sum gives).
-            const float my_self = my_energy == my_energy ? 0.0F : my_energy;
-            // Lane (n_lane, kb) holds candidates 16 rt + 4 kb + v of column
-            // col against anchor n_lane; their energies are consecutive in
-            // the column-major box (one broadcast 16-byte read).
-#pragma unroll
-            for (int i = 0; i < kTilesPerWave; ++i) {
-                int mt = wave + kFastWaves * i;
-                asm volatile("" : "+s"(mt));
-                if (mt >= mtiles) {
-                    break;  // uniform
-                }
-                const int col = (mt * tile_magic) >> 16;
-                const int rt = mt - col * row_tiles;
-                const int ry0 = 16 * rt + 4 * kb;
-                const float4 ec = *reinterpret_cast<const float4*>(s_ebox + col * S.ebox_pitch + ry0);
-                const float ecv[4] = {ec.x, ec.y, ec.z, ec.w};
-                const int rel_x = ux0 + col - my_xl;
-                const int rel_y0 = uy0 + ry0 - my_yl;
-                const bool x_ok = static_cast<unsigned>(rel_x) < static_cast<unsigned>(my_nx);
-                const bool self_x = anchor_frame && ux0 + col == my_ax;
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int rel_y = rel_y0 + v;
-                    float dist = (my_energy + ecv[v]) - 2.0F * acc[i][v];
-                    dist = dist < 0.0F ? 0.0F : dist;  // rounding below 0; NaN stays NaN
-                    dist = self_x && uy0 + ry0 + v == my_ay ? my_self : dist;
-                    if (x_ok && static_cast<unsigned>(rel_y) < static_cast<unsigned>(my_ny)) {
-                        s_stage[n_lane * S.per_frame + __mul24(rel_y, my_nx) + rel_x] = dist;
+                    for (int v = 0; v < 4; ++v) {
+                        const int rel_x = rel_x0 - exl[v];
+                        float dist = (ea[v] + ec) - 2.0F * acc[i][v];
+                        dist = dist < 0.0F ? 0.0F : dist;  // rounding below 0; NaN stays NaN
+                        if (y_ok && static_cast<unsigned>(rel_x) < static_cast<unsigned>(enx[v])) {
+                            s_stage[(4 * kb + v) * L.per_frame + __mul24(rel_y, enx[v]) + rel_x] = dist;
+                        }
                     }
                 }
-                __builtin_amdgcn_sched_barrier(0);
+            } else {
+                // Only candidates within the anchor's bound: their distance
+                // into s_stage and their window position onto the list of
+                // the anchor.  Every wavefront fills its own part of each
+                // list, counted in registers (the lanes of a grid row keep
+                // the same counts).  (Finite data: rows outside the y range
+                // get an infinite energy, which no finite bound admits.)
+                const float4 bound4 = *reinterpret_cast<const float4*>(s_bound + 4 * kb);
+                const float bound[4] = {bound4.x, bound4.y, bound4.z, bound4.w};
+                int vx0[4];
+                int vx1[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    vx0[v] = __builtin_amdgcn_readfirstlane(s_vx[v]);
+                    vx1[v] = __builtin_amdgcn_readfirstlane(s_vx[4 + v]);
+                }
+                const uint64_t row_lanes = 0xFFFFull << (16 * kb);
+                const uint64_t row_below = row_lanes & ((1ull << lane) - 1ull);
+                int appended[4] = {0, 0, 0, 0};
+#pragma unroll
+                for (int i = 0; i < kTiles; ++i) {
+                    if (!tile_ok(i)) {
+                        continue;  // uniform
+                    }
+                    const int col = wave + 4 * (i / kR);
+                    const int x = ux0 + col;
+                    const int ry = 16 * (i % kR) + n_lane;
+                    const int rel_y = uy0 + ry - eyl;
+                    float ec = s_ebox[col * L.ebox_pitch + ry];
+                    ec = static_cast<unsigned>(rel_y) < static_cast<unsigned>(eny) ? ec : __builtin_inff();
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        if (x < vx0[v] || x > vx1[v]) {
+                            continue;  // uniform: outside every window of the column
+                        }
+                        const int rel_x = x - exl[v];
+                        // (ea + ec) - 2 acc with one rounding of the sum and
+                        // one of the difference, as in the exhaustive path.
+                        const float dist = fmaf(-2.0F, acc[i][v], ea[v] + ec);
+                        const bool pass =
+                            static_cast<unsigned>(rel_x) < static_cast<unsigned>(enx[v]) && dist <= bound[v];
+                        const uint64_t mask = __ballot(pass);
+                        if (mask == 0ull) {
+                            continue;  // uniform
+                        }
+                        if (pass) {
+                            const int slot = 4 * kb + v;
+                            const int at = __mul24(rel_y, enx[v]) + rel_x;
+                            s_stage[slot * L.per_frame + at] = dist < 0.0F ? 0.0F : dist;
+                            s_list[(slot * kFastWaves + wave) * L.sub_keys + appended[v] + __popcll(mask & row_below)] =
+                                static_cast<uint16_t>(at);
+                        }
+                        appended[v] += __popcll(mask & row_lanes);
+                    }
+                }
+                if (n_lane == 0) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        s_parts[(4 * kb + v) * kFastWaves + wave] = appended[v];
+                    }
+                }
             }
-            __syncthreads();
-            if constexpr (kFused) {
-                // Compacted candidates (window positions of the keys below the
-                // anchor's 64th), one list per anchor, in the idle plane
-                // buffer after the energy box.
-                uint16_t* scratch = reinterpret_cast<uint16_t*>(s_ebox + S.ebox_words) + wave * 4 * S.scratch_keys;
+        }
+        __syncthreads();
+        if (fused != 0) {
+            // Keys of the frame per anchor (4 wave + q): m[q] window
+            // positions, in one list of the owner (full frames) or in the
+            // four wavefronts' parts of it (filtered frames; part_end[q][w]:
+            // end of part w in the list order).
+            uint16_t* scratch = s_list + wave * 4 * L.scratch_keys;
+            uint64_t best[4];
+            int valid_count[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                best[q] = anchor_frame ? kNoKey : s_best[wave][q][lane];
+                valid_count[q] = s_valid[4 * wave + q];
+            }
+            int m[4] = {0, 0, 0, 0};
+            int part_end[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+            if (full) {
+                if (anchor_frame && lane < 4) {
+                    // The anchor against itself: exactly 0 (NaN for a patch
+                    // holding NaN, as the exact sum gives).
+                    const int n = 4 * wave + lane;
+                    if (s_index[n] >= 0) {
+                        const float e = s_energy[n];
+                        s_stage[n * L.per_frame + (s_ay[n] - s_yl[n]) * s_nx[n] + s_ax[n] - s_xl[n]] =
+                            e == e ? 0.0F : e;
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 // First candidate frame: the lists are still empty, so every
                 // key would be admitted.  Each lane's smallest key (over the
                 // candidates i = lane mod 64) goes in first, which bounds the
@@ -748,7 +858,7 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
                         const int count = s_nx[n] * s_ny[n];
                         for (int i = lane; i < count; i += 64) {
                             const uint64_t key =
-                                (static_cast<uint64_t>(distance_key(s_stage[n * S.per_frame + i])) << 32) |
+                                (static_cast<uint64_t>(distance_key(s_stage[n * L.per_frame + i])) << 32) |
                                 static_cast<uint32_t>(i);
                             own_min[q] = key < own_min[q] ? key : own_min[q];
                         }
@@ -760,7 +870,8 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
                         best[q] = first[q];  // the lists were empty
                     }
                 }
-                int m[4] = {0, 0, 0, 0};
+                // Compaction: window positions of the keys below each anchor's
+                // bound, one list per anchor.
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int n = 4 * wave + q;
@@ -770,16 +881,15 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
                     const int count = s_nx[n] * s_ny[n];
                     const uint32_t base_index = static_cast<uint32_t>(ti * count);
                     const uint64_t threshold = admit_bound(best[q], d.similar, tau_key);
-                    uint16_t* list = scratch + q * S.scratch_keys;
+                    uint16_t* list = scratch + q * L.scratch_keys;
                     for (int c0 = 0; c0 < count; c0 += 64) {
                         const int i = c0 + lane;
                         bool pass = false;
                         bool valid = false;
-                        uint64_t key = kNoKey;
                         if (i < count) {
-                            const float dist = s_stage[n * S.per_frame + i];
+                            const float dist = s_stage[n * L.per_frame + i];
                             valid = dist == dist;
-                            key = (static_cast<uint64_t>(distance_key(dist)) << 32) | (base_index + i);
+                            const uint64_t key = (static_cast<uint64_t>(distance_key(dist)) << 32) | (base_index + i);
                             pass = key < threshold && key != own_min[q];
                         }
                         valid_count[q] += __popcll(__ballot(valid));
@@ -793,77 +903,135 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                const int most = max(max(m[0], m[1]), max(m[2], m[3]));
-                if (most <= kInsertKeys) {
-                    // Few keys (the usual case once the lists hold a frame's
-                    // worth): each goes into its sorted list directly, a ballot
-                    // finding its place and one lane shift making room; the
-                    // lists end as the sort-and-merge rounds below leave them.
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int n = 4 * wave + q;
-                        const uint32_t base_index = static_cast<uint32_t>(ti * s_nx[n] * s_ny[n]);
-                        for (int j = 0; j < m[q]; ++j) {
-                            const int i = scratch[q * S.scratch_keys + j];
-                            const uint64_t key =
-                                (static_cast<uint64_t>(distance_key(s_stage[n * S.per_frame + i])) << 32) |
-                                (base_index + i);
-                            best[q] = wave_insert(best[q], key, lane);
-                        }
-                    }
-                }
-                // Otherwise rounds of 64 compacted keys, the four anchors side
-                // by side; keys no longer below an anchor's 64th are dropped.
-                const int rounds = most <= kInsertKeys ? 0 : (most + 63) / 64;
-                for (int r = 0; r < rounds; ++r) {
-                    uint64_t key[4];
-                    bool any = false;
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int at = 64 * r + lane;
-                        const uint64_t current = admit_bound(best[q], d.similar, tau_key);
-                        uint64_t v = kNoKey;
-                        if (at < m[q]) {
-                            const int n = 4 * wave + q;
-                            const int i = scratch[q * S.scratch_keys + at];
-                            const uint32_t base_index = static_cast<uint32_t>(ti * s_nx[n] * s_ny[n]);
-                            v = (static_cast<uint64_t>(distance_key(s_stage[n * S.per_frame + i])) << 32) |
-                                (base_index + i);
-                        }
-                        v = v < current ? v : kNoKey;
-                        key[q] = v;
-                        any = any || __ballot(v != kNoKey) != 0;
-                    }
-                    if (!any) {
-                        continue;
-                    }
-                    wave_sort64x4(key, lane);
-                    wave_merge_lower64x4(best, key, lane);
-                }
             } else {
-                for (int n = 0; n < kTileAnchors; ++n) {
-                    const int index = s_index[n];
-                    if (index < 0) {
-                        continue;
-                    }
-                    const int count = s_nx[n] * s_ny[n];
-                    float* dst = out + static_cast<size_t>(index - anchor_base) * S.stride +
-                                 static_cast<size_t>(ti) * count;
-                    for (int i = tid; i < count; i += kFastThreads) {
-                        dst[i] = s_stage[n * S.per_frame + i];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int n = 4 * wave + q;
+                    if (s_index[n] >= 0) {
+                        int end = 0;
+#pragma unroll
+                        for (int w = 0; w < kFastWaves; ++w) {
+                            end += __builtin_amdgcn_readfirstlane(s_parts[n * kFastWaves + w]);
+                            if (w < kFastWaves - 1) {
+                                part_end[q][w] = end;
+                            }
+                        }
+                        m[q] = end;
+                        valid_count[q] += s_nx[n] * s_ny[n];  // finite data: every distance is valid
                     }
                 }
             }
+            // Window position of key j of anchor q.
+            const auto position = [&](int q, int j) -> int {
+                if (full) {
+                    return scratch[q * L.scratch_keys + j];
+                }
+                const int n = 4 * wave + q;
+                const int w = (j >= part_end[q][0] ? 1 : 0) + (j >= part_end[q][1] ? 1 : 0) +
+                              (j >= part_end[q][2] ? 1 : 0);
+                const int start = j >= part_end[q][2] ? part_end[q][2]
+                                  : j >= part_end[q][1] ? part_end[q][1]
+                                  : j >= part_end[q][0] ? part_end[q][0]
+                                                        : 0;
+                return s_list[(n * kFastWaves + w) * L.sub_keys + j - start];
+            };
+            const auto key_of = [&](int q, int i) -> uint64_t {
+                const int n = 4 * wave + q;
+                const uint32_t base_index = static_cast<uint32_t>(ti * s_nx[n] * s_ny[n]);
+                return (static_cast<uint64_t>(distance_key(s_stage[n * L.per_frame + i])) << 32) | (base_index + i);
+            };
+            const int most = max(max(m[0], m[1]), max(m[2], m[3]));
+            if (most <= kInsertKeys) {
+                // Few keys (the usual case once the lists hold a frame's
+                // worth): lane j holds key j of each anchor; the keys go into
+                // the four sorted lists one at a time, side by side, a ballot
+                // finding each one's place and one lane shift making room.
+                // Keys no longer below a list's bound change nothing.
+                uint64_t mine[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    mine[q] = lane < m[q] ? key_of(q, position(q, lane)) : kNoKey;
+                }
+                for (int j = 0; j < most; ++j) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        if (j < m[q]) {
+                            best[q] = wave_insert(best[q], readlane64(mine[q], j), lane);
+                        }
+                    }
+                }
+            }
+            // Otherwise rounds of 64 listed keys, the four anchors side by
+            // side; keys no longer below an anchor's bound are dropped.
+            const int rounds = most <= kInsertKeys ? 0 : (most + 63) / 64;
+            for (int r = 0; r < rounds; ++r) {
+                uint64_t key[4];
+                bool any = false;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int at = 64 * r + lane;
+                    const uint64_t current = admit_bound(best[q], d.similar, tau_key);
+                    uint64_t v = at < m[q] ? key_of(q, position(q, at)) : kNoKey;
+                    v = v < current ? v : kNoKey;
+                    key[q] = v;
+                    any = any || __ballot(v != kNoKey) != 0;
+                }
+                if (!any) {
+                    continue;
+                }
+                wave_sort64x4(key, lane);
+                wave_merge_lower64x4(best, key, lane);
+            }
+            // Bounds for the next frame: the largest distance a key below the
+            // admit bound can have (NaN where that is not finite).
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int n = 4 * wave + q;
+                const uint32_t top = static_cast<uint32_t>(admit_bound(best[q], d.similar, tau_key) >> 32);
+                if (lane == 0 && s_index[n] >= 0) {
+                    s_bound[n] = top < 0x7F800000u ? __uint_as_float(top) : __builtin_nanf("");
+                    s_valid[n] = valid_count[q];
+                }
+                s_best[wave][q][lane] = best[q];
+            }
+        } else {
+            for (int n = 0; n < kTileAnchors; ++n) {
+                const int index = s_index[n];
+                if (index < 0) {
+                    continue;
+                }
+                const int count = s_nx[n] * s_ny[n];
+                float* dst = out + static_cast<size_t>(index - anchor_base) * L.stride +
+                             static_cast<size_t>(ti) * count;
+                // The anchor against itself: exactly 0 (NaN for a patch
+                // holding NaN, as the exact sum gives).
+                const int self = anchor_frame ? (s_ay[n] - s_yl[n]) * s_nx[n] + s_ax[n] - s_xl[n] : -1;
+                const float e = s_energy[n];
+                for (int i = tid; i < count; i += kFastThreads) {
+                    dst[i] = i == self ? (e == e ? 0.0F : e) : s_stage[n * L.per_frame + i];
+                }
+            }
         }
+        // The next frame's first plane once every wave is past the epilogue.
         __syncthreads();
+        if (p_last + 1 < total_planes) {
+            prefetch(p_last + 1);
+            store(p_last + 1);
+            __syncthreads();
+        }
     }
-    if constexpr (kFused) {
+    if (fused != 0) {
         // select_and_write over the retained keys: k = min(K, valid), the
         // threshold max(tau, d_k), the prefix within it (at most the buffer
         // capacity of find_similar_patches, core.cpp:1279-1287), the K
         // nearest first, the rest in scan order.
         const long long window_cap = static_cast<long long>(d.search_window) * d.search_window * nt;
         const int buffer_cap = static_cast<int>(window_cap < d.soft_cap ? window_cap : d.soft_cap);
+        int valid_count[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            valid_count[q] = s_valid[4 * wave + q];
+        }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int n = 4 * wave + q;
@@ -878,7 +1046,7 @@ __global__ __launch_bounds__(kFastThreads) void match_fast_kernel(
                 continue;
             }
             const int k = d.similar < valid_count[q] ? d.similar : valid_count[q];
-            const uint64_t key = best[q];
+            const uint64_t key = s_best[wave][q][lane];
             const uint32_t dist_key = static_cast<uint32_t>(key >> 32);
             const uint32_t index = static_cast<uint32_t>(key & 0xFFFFFFFFu);
             const float kth = __uint_as_float(static_cast<uint32_t>(readlane64(key, k - 1) >> 32));
@@ -967,111 +1135,120 @@ __global__ __launch_bounds__(256) void select_fast_kernel(
                      distances);
 }
 
-FastShape fast_shape(const StageDesc& d, int max_cols, int max_rows) {
-    FastShape S{};
-    S.row_tiles = (max_rows + 15) / 16;
-    S.region_rows = 16 * S.row_tiles + kFastPatch - 1;
-    // Copy rows cover samples j .. j + 7 for j < max_cols rounded to 4.
-    S.pitch = (max_cols + kFastPatch - 1 + 3) / 4 * 4;
-    if ((S.pitch / 4) % 2 == 0) {
-        S.pitch += 4;  // odd number of 8-byte units: rows spread over banks
+// Instantiations: row tiles x union columns per wavefront, by cost.
+struct Variant {
+    int rows;
+    int cols;
+};
+constexpr Variant kVariants[] = {{1, 6}, {1, 10}, {2, 6}, {2, 8}, {2, 9}, {2, 10}, {3, 10}};
+constexpr int kVariantCount = static_cast<int>(sizeof(kVariants) / sizeof(kVariants[0]));
+
+int pick_variant(int max_cols, int max_rows) {
+    const int rows = (max_rows + 15) / 16;
+    const int cols = (max_cols + 3) / 4;
+    for (int v = 0; v < kVariantCount; ++v) {
+        if (kVariants[v].rows >= rows && kVariants[v].cols >= cols) {
+            return v;
+        }
     }
-    S.copy_elems = S.region_rows * S.pitch;
-    S.per_frame = d.search_window * d.search_window;
-    S.stride = S.per_frame * (d.search_bwd + d.search_fwd + 1);
-    S.ebox_pitch = 16 * S.row_tiles + 4;
-    S.ebox_words = max_cols * S.ebox_pitch;
-    S.scratch_keys = (S.per_frame + 1) / 2 * 2;
-    return S;
+    return -1;
 }
 
-constexpr int kMaxTilesPerWave = 24;
+TileLayout tile_layout(const StageDesc& d, int max_cols, int variant) {
+    const Variant V = kVariants[variant];
+    TileLayout L{};
+    L.groups = V.cols + 1;
+    L.row_bytes = staged_row_bytes(L.groups);
+    L.rows_bytes = (16 * V.rows + kFastPatch - 1) * L.row_bytes;
+    L.buffer_bytes = L.rows_bytes + kAnchorBytes;
+    L.per_frame = d.search_window * d.search_window;
+    L.stride = L.per_frame * (d.search_bwd + d.search_fwd + 1);
+    L.ebox_pitch = 16 * V.rows + 1;
+    L.ebox_bytes = (max_cols * L.ebox_pitch * 4 + 15) / 16 * 16;
+    L.stage_bytes = kTileAnchors * L.per_frame * 4;
+    L.scratch_keys = (L.per_frame + 1) / 2 * 2;
+    L.sub_keys = (d.search_window + 3) / 4 * d.search_window;
+    const int lists = 2 * kTileAnchors * std::max(L.scratch_keys, kFastWaves * L.sub_keys);
+    const int epilogue = L.ebox_bytes + L.stage_bytes + lists;
+    L.shared_bytes = std::max(2 * L.buffer_bytes, epilogue);
+    return L;
+}
 
 }  // namespace
 
 bool fast_match_supported(const StageDesc& d, int max_cols, int max_rows) {
-    if (d.patch != kFastPatch || d.similar <= 1) {
+    if (d.patch != kFastPatch || d.similar <= 1 || max_cols < 1 || max_rows < 1) {
         return false;
     }
-    const int row_tiles = (max_rows + 15) / 16;
-    const FastShape S = fast_shape(d, max_cols, max_rows);
-    return static_cast<long long>(max_cols) * row_tiles <= static_cast<long long>(kMaxTilesPerWave) * kFastWaves &&
-           S.region_rows * (S.pitch / 4) <= kFillPerThread * kFastThreads &&
-           static_cast<long long>(max_cols) * max_rows <= static_cast<long long>(kEnergyPerThread) * kFastThreads &&
-           static_cast<long long>(S.ebox_words) * 4 <= static_cast<long long>(2) * kCopies * S.copy_elems * 2 &&
-           fast_match_shared_bytes(d, max_cols, max_rows) <= 150 * 1024;  // of 160 KB per CU
+    const int variant = pick_variant(max_cols, max_rows);
+    if (variant < 0) {
+        return false;
+    }
+    const TileLayout L = tile_layout(d, max_cols, variant);
+    return static_cast<long long>(max_cols) * max_rows <= static_cast<long long>(kEnergyPerThread) * kFastThreads &&
+           L.shared_bytes <= 150 * 1024;  // of 160 KB per CU
 }
 
 bool fast_match_fused(const StageDesc& d, int max_cols, int max_rows) {
-    // The running lists hold 64 keys per anchor; the compaction scratch of
-    // four wavefronts shares the idle plane buffer with the energy box.
-    const FastShape S = fast_shape(d, max_cols, max_rows);
-    const long long scratch = static_cast<long long>(S.ebox_words) * 4 + 2LL * kTileAnchors * S.scratch_keys;
-    return d.cap <= 64 && d.similar <= 64 && scratch <= static_cast<long long>(2) * kCopies * S.copy_elems * 2;
+    // The running lists hold 64 keys per anchor.
+    return fast_match_supported(d, max_cols, max_rows) && d.cap <= 64 && d.similar <= 64;
 }
 
 size_t fast_match_shared_bytes(const StageDesc& d, int max_cols, int max_rows) {
-    const FastShape S = fast_shape(d, max_cols, max_rows);
-    const size_t buffers = static_cast<size_t>(4) * kCopies * S.copy_elems * sizeof(__bf16);
-    const size_t staging = static_cast<size_t>(kTileAnchors) * S.per_frame * sizeof(float);
-    return buffers + staging;
+    const int variant = pick_variant(max_cols, max_rows);
+    return variant < 0 ? 0 : static_cast<size_t>(tile_layout(d, max_cols, variant).shared_bytes);
 }
 
 size_t fast_match_stride(const StageDesc& d) {
     return static_cast<size_t>(d.search_window) * d.search_window * (d.search_bwd + d.search_fwd + 1);
 }
 
-hipError_t launch_fast_energy(const StageDesc& d, const float* source, float* offsets, float* rows,
-                              float* energy, hipStream_t stream) {
+hipError_t launch_fast_energy(const StageDesc& d, const float* source, float* offsets, unsigned* flags,
+                              float* energy, uint32_t* split, hipStream_t stream) {
     const int match_channels = d.stage == kFinal ? d.channels : 1;
     const size_t plane = static_cast<size_t>(d.width) * d.height;
     hipLaunchKernelGGL(channel_offset_kernel, dim3(static_cast<unsigned>(match_channels)), dim3(256), 0,
-                       stream, source, d.channels, plane, d.frames, offsets);
+                       stream, source, d.channels, plane, d.frames, offsets, flags);
+    // The row sums use the split planes' buffer, which is written last.
+    float* rows = reinterpret_cast<float*>(split);
     const size_t row_total = plane * match_channels * d.frames;
     hipLaunchKernelGGL(row_energy_kernel, dim3(static_cast<unsigned>((row_total + 255) / 256)), dim3(256), 0,
                        stream, d, source, offsets, match_channels, rows);
     const size_t total = plane * d.frames;
     hipLaunchKernelGGL(patch_energy_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0,
-                       stream, d, rows, match_channels, energy);
+                       stream, d, rows, match_channels, energy, flags);
+    hipLaunchKernelGGL(split_planes_kernel, dim3(static_cast<unsigned>((row_total + 255) / 256)), dim3(256), 0,
+                       stream, d, source, offsets, match_channels, split);
     return hipGetLastError();
 }
 
-hipError_t launch_fast_match(const StageDesc& d, int max_cols, int max_rows, const float* source,
-                             const float* offsets, const float* energy, const uint32_t* anchors,
+hipError_t launch_fast_match(const StageDesc& d, int max_cols, int max_rows, const uint32_t* split,
+                             const unsigned* flags, const float* energy, const uint32_t* anchors,
                              const int* tiles, int tile_count, int anchor_base, float* staged,
                              bool fused, int* counts, uint32_t* positions, float* distances,
                              hipStream_t stream) {
     if (tile_count <= 0) {
         return hipSuccess;
     }
-    const FastShape S = fast_shape(d, max_cols, max_rows);
-    const size_t shared = fast_match_shared_bytes(d, max_cols, max_rows);
-    const int mtiles = max_cols * S.row_tiles;
-    const int per_wave = (mtiles + kFastWaves - 1) / kFastWaves;
-#define VNLBHIP_FAST(T)                                                                            \
-    do {                                                                                           \
-        if (fused) {                                                                               \
-            hipLaunchKernelGGL((match_fast_kernel<T, true>), dim3(static_cast<unsigned>(tile_count)), \
-                               dim3(kFastThreads), shared, stream, d, S, source, offsets, energy,  \
-                               anchors, tiles, anchor_base, staged, counts, positions, distances); \
-        } else {                                                                                   \
-            hipLaunchKernelGGL((match_fast_kernel<T, false>), dim3(static_cast<unsigned>(tile_count)), \
-                               dim3(kFastThreads), shared, stream, d, S, source, offsets, energy,  \
-                               anchors, tiles, anchor_base, staged, counts, positions, distances); \
-        }                                                                                          \
-    } while (0)
-    if (per_wave <= 8) {
-        VNLBHIP_FAST(8);
-    } else if (per_wave <= 12) {
-        VNLBHIP_FAST(12);
-    } else if (per_wave <= 16) {
-        VNLBHIP_FAST(16);
-    } else if (per_wave <= 18) {
-        VNLBHIP_FAST(18);
-    } else if (per_wave <= 20) {
-        VNLBHIP_FAST(20);
-    } else {
-        VNLBHIP_FAST(24);
+    const int variant = pick_variant(max_cols, max_rows);
+    if (variant < 0) {
+        return hipErrorInvalidValue;
+    }
+    const TileLayout L = tile_layout(d, max_cols, variant);
+    const size_t shared = static_cast<size_t>(L.shared_bytes);
+    const int fused_flag = fused ? 1 : 0;
+#define VNLBHIP_FAST(R, A)                                                                              \
+    hipLaunchKernelGGL((match_fast_kernel<R, A>), dim3(static_cast<unsigned>(tile_count)), dim3(kFastThreads), \
+                       shared, stream, d, L, split, flags, energy, anchors, tiles, anchor_base, fused_flag, \
+                       staged, counts, positions, distances)
+    switch (variant) {
+        case 0: VNLBHIP_FAST(1, 6); break;
+        case 1: VNLBHIP_FAST(1, 10); break;
+        case 2: VNLBHIP_FAST(2, 6); break;
+        case 3: VNLBHIP_FAST(2, 8); break;
+        case 4: VNLBHIP_FAST(2, 9); break;
+        case 5: VNLBHIP_FAST(2, 10); break;
+        default: VNLBHIP_FAST(3, 10); break;
     }
 #undef VNLBHIP_FAST
     return hipGetLastError();

@@ -312,12 +312,13 @@ void append_grid_frame(const StageDesc& d, int anchor_frame, Grid& grid) {
 }
 
 // Anchor tiles of the MFMA matcher: `rows` consecutive grid rows x `cols`
-// consecutive anchors of each row, of one frame (rows * cols <= 16; slot
-// rr * cols + ii, -1 where a row is shorter; the matrix products take the
-// 16 slots as their anchor columns whatever the shape), and the largest
+// consecutive anchors of each row, of one frame (rows, cols <= 4; slot
+// 4 * rr + ii, -1 where a row is shorter or the shape narrower: the matrix
+// products take the 16 slots as their anchor rows and the matcher's
+// epilogue takes each group of 4 slots as one grid row), and the largest
 // union of the tiles' search windows.  The union per anchor is what the
-// shape trades: 4 x 4 is smallest for 19-wide windows; wider windows need
-// flatter tiles to stay within the kernel's limits.
+// shape trades: 4 x 4 is smallest for 19-wide windows; wider windows may
+// need fewer rows to stay within the kernel's limits.
 struct Tiles {
     std::vector<int> slots;
     int max_cols = 0;
@@ -330,7 +331,7 @@ struct TileShape {
 };
 
 // The shapes the autotuner tries, the default first.
-constexpr TileShape kTileShapes[] = {{4, 4}, {2, 8}, {8, 2}, {3, 5}, {2, 6}, {2, 5}, {4, 2}, {2, 4}};
+constexpr TileShape kTileShapes[] = {{4, 4}, {3, 4}, {2, 4}, {4, 3}, {4, 2}, {1, 4}};
 constexpr int kTileShapeCount = static_cast<int>(sizeof(kTileShapes) / sizeof(kTileShapes[0]));
 
 Tiles build_tiles(const StageDesc& d, const Grid& grid, int height, TileShape shape) {
@@ -359,11 +360,11 @@ Tiles build_tiles(const StageDesc& d, const Grid& grid, int height, TileShape sh
                 int y0 = INT_MAX;
                 int y1 = INT_MIN;
                 for (int slot = 0; slot < 16; ++slot) {
-                    const int rr = slot / cols;
-                    const int ii = slot - rr * cols;
+                    const int rr = slot / 4;
+                    const int ii = slot - rr * 4;
                     int index = -1;
                     const size_t row = r0 + static_cast<size_t>(rr);
-                    if (rr < rows && row < ys.size() && i0 + ii < count[ys[row]]) {
+                    if (rr < rows && ii < cols && row < ys.size() && i0 + ii < count[ys[row]]) {
                         index = frame_begin + first[ys[row]] + i0 + ii;
                         const uint32_t p = grid.anchors[static_cast<size_t>(index)];
                         int lo = 0;
@@ -587,7 +588,7 @@ struct vnlbhip_context {
     size_t fast_stage_budget = static_cast<size_t>(1) << 30;  // anchors * window slots
     int form = VNLBHIP_FORM_PLAIN;
     DeviceBuffer<float> fast_offsets{&generation};
-    DeviceBuffer<float> fast_rows{&generation};
+    DeviceBuffer<float> fast_split{&generation};  // packed bf16 hi / lo planes (row sums on the way)
     DeviceBuffer<float> fast_energy{&generation};
     DeviceBuffer<int> fast_tiles{&generation};
     DeviceBuffer<float> fast_staged{&generation};
@@ -888,7 +889,9 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
     const float* source = stage == VNLBHIP_STAGE_FINAL ? ref : noisy;
     // MFMA matcher: patch energies of the matched source, once per stage.
     float* fast_offsets = nullptr;
+    unsigned* fast_flags = nullptr;
     float* fast_energy = nullptr;
+    uint32_t* fast_split = nullptr;
     const bool guided = ctx.search_offsets != nullptr;
     if (guided) {
         require(static_cast<long long>(ctx.search_offsets_temporal) ==
@@ -916,11 +919,14 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
     // matcher and filter.
     if (ctx.form == VNLBHIP_FORM_FAST && d.patch == 8 && d.similar > 1 && !guided && !learned) {
         const int match_channels = stage == VNLBHIP_STAGE_FINAL ? g.channels : 1;
-        fast_offsets = ctx.fast_offsets.reserve(4);
-        float* rows = ctx.fast_rows.reserve(plane * match_channels * g.frames);
+        // Channel offsets [0, 4), the energy flag at [4].
+        fast_offsets = ctx.fast_offsets.reserve(8);
+        fast_flags = reinterpret_cast<unsigned*>(fast_offsets + 4);
+        fast_split = reinterpret_cast<uint32_t*>(ctx.fast_split.reserve(plane * match_channels * g.frames));
         fast_energy = ctx.fast_energy.reserve(plane * g.frames);
         timed(ctx, VNLBHIP_KERNEL_ENERGY, stream, "energy kernels", [&]() {
-            return vnlbhip::launch_fast_energy(d, source, fast_offsets, rows, fast_energy, stream);
+            return vnlbhip::launch_fast_energy(d, source, fast_offsets, fast_flags, fast_energy, fast_split,
+                                               stream);
         });
     }
     const int cap = d.cap;
@@ -1097,8 +1103,8 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
                 const int tb = tile_at(ab);
                 const int te = tile_at(ae);
                 timed(ctx, VNLBHIP_KERNEL_MATCH, stream, "fast match kernel", [&]() {
-                    return vnlbhip::launch_fast_match(d, tiles->max_cols, tiles->max_rows, source,
-                                                      fast_offsets, fast_energy, d_anchors,
+                    return vnlbhip::launch_fast_match(d, tiles->max_cols, tiles->max_rows, fast_split,
+                                                      fast_flags, fast_energy, d_anchors,
                                                       d_tiles + static_cast<size_t>(tb) * 16, te - tb, 0,
                                                       nullptr, true, d_counts, d_positions, d_distances,
                                                       stream);
@@ -1124,8 +1130,8 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
                     const int tb = tile_at(anchor_begin);
                     const int te = tile_at(anchor_end);
                     timed(ctx, VNLBHIP_KERNEL_MATCH, stream, "fast match kernel", [&]() {
-                        return vnlbhip::launch_fast_match(d, tiles->max_cols, tiles->max_rows, source,
-                                                          fast_offsets, fast_energy, d_anchors,
+                        return vnlbhip::launch_fast_match(d, tiles->max_cols, tiles->max_rows, fast_split,
+                                                          fast_flags, fast_energy, d_anchors,
                                                           d_tiles + static_cast<size_t>(tb) * 16, te - tb,
                                                           anchor_begin, staged, false, nullptr, nullptr,
                                                           nullptr, stream);
@@ -1736,7 +1742,7 @@ int vnlbhip_set_autotune(vnlbhip_context* context, int enable) {
 }
 
 int vnlbhip_set_tile_shape(vnlbhip_context* context, int rows, int cols) {
-    if (context == nullptr || rows < 1 || cols < 1 || rows * cols > 16) {
+    if (context == nullptr || rows < 1 || cols < 1 || rows > 4 || cols > 4) {
         return VNLBHIP_ERROR_INVALID;
     }
     context->fixed_tile_shape = TileShape{rows, cols};

@@ -152,10 +152,10 @@ class Context:
 
     def set_tile_shape(self, rows: int, cols: int) -> None:
         """Fast form: anchor tiles of the MFMA matcher (rows x cols anchors,
-        at most 16; default 4 x 4) for calls the autotuner does not tune."""
+        1 to 4 each; default 4 x 4) for calls the autotuner does not tune."""
         rc = self._lib.vnlbhip_set_tile_shape(self._handle, int(rows), int(cols))
         if rc != 0:
-            raise VNLBError("VNLB HIP: a tile holds 1 .. 16 anchors (rows x cols)")
+            raise VNLBError("VNLB HIP: a tile has 1 .. 4 grid rows of 1 .. 4 anchors")
 
     def set_jacobi(self, max_sweeps: int = 64, fail_unconverged: bool = False) -> None:
         """Jacobi sweep limit of every eigen solve (default 64) and whether a
