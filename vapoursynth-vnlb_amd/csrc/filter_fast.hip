@@ -472,27 +472,24 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                     for (int s = 0; s < kFastModel; ++s) {
                         crow[s] = s < B ? s_gram[s * kFastRows + lane] : 0.0F;
                     }
-                    for (int j = 0; j < rank_used; ++j) {
+                    // Unrolled and branch-free (U is zero-padded), so the
+                    // rows' LDS reads are issued ahead of the products;
+                    // components past rank_used add exact zeros.
+#pragma unroll
+                    for (int j = 0; j < kFastModel; ++j) {
                         const float inv_j = __builtin_bit_cast(float, __builtin_amdgcn_readlane(
                                                                           __builtin_bit_cast(int, inv), j));
                         const float lh = __builtin_bit_cast(float, __builtin_amdgcn_readlane(
                                                                        __builtin_bit_cast(int, lhat), j));
-                        float p = 0.0F;
-                        if (inv_j != 0.0F) {
-                            p = dot16(crow, s_u + j * 16);
-                            p = p * inv_j;
-                        }
-                        if (estimate_is_member && j < kFastModel) {
-#pragma unroll
-                            for (int jj = 0; jj < kFastModel; ++jj) {
-                                if (jj == j) {
-                                    p_est[jj] = p;
-                                }
-                            }
+                        const bool used = j < rank_used;
+                        const float dot = dot16(crow, s_u + j * 16);
+                        const float p = used && inv_j != 0.0F ? dot * inv_j : 0.0F;
+                        if (estimate_is_member) {
+                            p_est[j] = p;
                         }
                         const float p2 = p * p;
-                        distance = distance + p2 / (lh + membership_noise2);
-                        projected = projected + p2;
+                        distance = distance + (used ? p2 / (lh + membership_noise2) : 0.0F);
+                        projected = projected + (used ? p2 : 0.0F);
                     }
                     const float norm2 = s_norm[lane];
                     const float residual = norm2 - projected > 0.0F ? norm2 - projected : 0.0F;
@@ -550,12 +547,8 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                     for (int j = 0; j < kFastModel; ++j) {
                         const float inv_j = __builtin_bit_cast(float, __builtin_amdgcn_readlane(
                                                                           __builtin_bit_cast(int, inv), j));
-                        float p = 0.0F;
-                        if (j < R && inv_j != 0.0F) {
-                            p = dot16(crow, s_u + j * 16);
-                            p = p * inv_j;
-                        }
-                        p_est[j] = p;
+                        const float dot = dot16(crow, s_u + j * 16);  // branch-free, as above
+                        p_est[j] = j < R && inv_j != 0.0F ? dot * inv_j : 0.0F;
                     }
                 }
             }

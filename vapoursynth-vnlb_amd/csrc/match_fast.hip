@@ -79,13 +79,24 @@ __global__ __launch_bounds__(256) void channel_offset_kernel(const float* __rest
     const size_t stride = per_channel / 65536 + 1;
     double sum = 0.0;
     size_t count = 0;
-    for (size_t i = threadIdx.x * stride; i < per_channel; i += blockDim.x * stride) {
-        const size_t f = i / plane;
-        const size_t rem = i - f * plane;
-        const float v = source[(f * channels + c) * plane + rem];
-        if (v == v && fabsf(v) <= FLT_MAX) {
-            sum += static_cast<double>(v);
-            ++count;
+    // Sixteen samples' loads in flight per thread, added in sample order.
+    constexpr int kBatch = 16;
+    const size_t step = blockDim.x * stride;
+    for (size_t i0 = threadIdx.x * stride; i0 < per_channel; i0 += kBatch * step) {
+        float v[kBatch];
+#pragma unroll
+        for (int b = 0; b < kBatch; ++b) {
+            const size_t i = i0 + b * step;
+            const size_t f = i / plane;
+            const size_t rem = i - f * plane;
+            v[b] = i < per_channel ? source[(f * channels + c) * plane + rem] : 0.0F;
+        }
+#pragma unroll
+        for (int b = 0; b < kBatch; ++b) {
+            if (i0 + b * step < per_channel && v[b] == v[b] && fabsf(v[b]) <= FLT_MAX) {
+                sum += static_cast<double>(v[b]);
+                ++count;
+            }
         }
     }
     sum = wave_sum_double(sum);
@@ -106,32 +117,33 @@ __global__ __launch_bounds__(256) void channel_offset_kernel(const float* __rest
     }
 }
 
+// Grid of the per-sample kernels below: x over the pixels of a plane
+// (32-bit, a plane holds at most 4096 x 4096), y over frame x channel.
+
 // Horizontal sums of squares: rows[f][c][y][x] = sum_{px<P} (v - offset)^2.
 __global__ __launch_bounds__(256) void row_energy_kernel(StageDesc d, const float* __restrict__ source,
                                                          const float* __restrict__ offsets,
                                                          int match_channels, float* __restrict__ rows) {
-    const size_t plane = static_cast<size_t>(d.width) * d.height;
-    const size_t total = plane * match_channels * d.frames;
-    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= total) {
+    const int plane = d.width * d.height;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= plane) {
         return;
     }
-    const size_t f = i / (plane * match_channels);
-    const size_t rem = i - f * plane * match_channels;
-    const int c = static_cast<int>(rem / plane);
-    const size_t p = rem - static_cast<size_t>(c) * plane;
-    const int y = static_cast<int>(p / d.width);
-    const int x = static_cast<int>(p - static_cast<size_t>(y) * d.width);
+    const int fc = blockIdx.y;  // f * match_channels + c
+    const int f = fc / match_channels;
+    const int c = fc - f * match_channels;
+    const int y = p / d.width;
+    const int x = p - y * d.width;
     float sum = 0.0F;
     if (x <= d.width - d.patch) {
         const float mu = offsets[c];
-        const float* src = source + ((f * d.channels + c) * d.height + y) * d.width + x;
+        const float* src = source + (static_cast<size_t>(f) * d.channels + c) * plane + p;
         for (int px = 0; px < d.patch; ++px) {
             const float v = src[px] - mu;
             sum = fmaf(v, v, sum);
         }
     }
-    rows[i] = sum;
+    rows[static_cast<size_t>(fc) * plane + p] = sum;
 }
 
 // Patch energies: energy[f][y][x] = sum over channels, frames and rows of
@@ -140,21 +152,20 @@ __global__ __launch_bounds__(256) void row_energy_kernel(StageDesc d, const floa
 __global__ __launch_bounds__(256) void patch_energy_kernel(StageDesc d, const float* __restrict__ rows,
                                                            int match_channels, float* __restrict__ energy,
                                                            unsigned* __restrict__ flags) {
-    const size_t plane = static_cast<size_t>(d.width) * d.height;
-    const size_t total = plane * d.frames;
-    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= total) {
+    const int plane = d.width * d.height;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= plane) {
         return;
     }
-    const int f = static_cast<int>(i / plane);
-    const size_t p = i - static_cast<size_t>(f) * plane;
-    const int y = static_cast<int>(p / d.width);
-    const int x = static_cast<int>(p - static_cast<size_t>(y) * d.width);
+    const int f = blockIdx.y;
+    const size_t i = static_cast<size_t>(f) * plane + p;
+    const int y = p / d.width;
+    const int x = p - y * d.width;
     float sum = 0.0F;
     if (x <= d.width - d.patch && y <= d.height - d.patch && f <= d.frames - d.patch_time) {
         for (int t = 0; t < d.patch_time; ++t) {
             for (int c = 0; c < match_channels; ++c) {
-                const float* r = rows + ((static_cast<size_t>(f + t) * match_channels + c) * d.height + y) * d.width + x;
+                const float* r = rows + (static_cast<size_t>(f + t) * match_channels + c) * plane + p;
                 for (int py = 0; py < d.patch; ++py) {
                     sum += r[static_cast<size_t>(py) * d.width];
                 }
@@ -171,17 +182,16 @@ __global__ __launch_bounds__(256) void patch_energy_kernel(StageDesc d, const fl
 __global__ __launch_bounds__(256) void split_planes_kernel(StageDesc d, const float* __restrict__ source,
                                                            const float* __restrict__ offsets,
                                                            int match_channels, uint32_t* __restrict__ split) {
-    const size_t plane = static_cast<size_t>(d.width) * d.height;
-    const size_t total = plane * match_channels * d.frames;
-    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= total) {
+    const int plane = d.width * d.height;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= plane) {
         return;
     }
-    const size_t f = i / (plane * match_channels);
-    const size_t rem = i - f * plane * match_channels;
-    const int c = static_cast<int>(rem / plane);
-    const size_t p = rem - static_cast<size_t>(c) * plane;
-    const float v = source[(f * d.channels + c) * plane + p] - offsets[c];
+    const int fc = blockIdx.y;  // f * match_channels + c
+    const int f = fc / match_channels;
+    const int c = fc - f * match_channels;
+    const size_t i = static_cast<size_t>(fc) * plane + p;
+    const float v = source[(static_cast<size_t>(f) * d.channels + c) * plane + p] - offsets[c];
     const __bf16 hi = static_cast<__bf16>(v);
     const __bf16 lo = static_cast<__bf16>(v - static_cast<float>(hi));
     split[i] = (static_cast<uint32_t>(__builtin_bit_cast(unsigned short, hi)) << 16) |
@@ -1211,14 +1221,13 @@ hipError_t launch_fast_energy(const StageDesc& d, const float* source, float* of
                        stream, source, d.channels, plane, d.frames, offsets, flags);
     // The row sums use the split planes' buffer, which is written last.
     float* rows = reinterpret_cast<float*>(split);
-    const size_t row_total = plane * match_channels * d.frames;
-    hipLaunchKernelGGL(row_energy_kernel, dim3(static_cast<unsigned>((row_total + 255) / 256)), dim3(256), 0,
-                       stream, d, source, offsets, match_channels, rows);
-    const size_t total = plane * d.frames;
-    hipLaunchKernelGGL(patch_energy_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0,
+    const unsigned plane_blocks = static_cast<unsigned>((plane + 255) / 256);
+    hipLaunchKernelGGL(row_energy_kernel, dim3(plane_blocks, static_cast<unsigned>(d.frames * match_channels)),
+                       dim3(256), 0, stream, d, source, offsets, match_channels, rows);
+    hipLaunchKernelGGL(patch_energy_kernel, dim3(plane_blocks, static_cast<unsigned>(d.frames)), dim3(256), 0,
                        stream, d, rows, match_channels, energy, flags);
-    hipLaunchKernelGGL(split_planes_kernel, dim3(static_cast<unsigned>((row_total + 255) / 256)), dim3(256), 0,
-                       stream, d, source, offsets, match_channels, split);
+    hipLaunchKernelGGL(split_planes_kernel, dim3(plane_blocks, static_cast<unsigned>(d.frames * match_channels)),
+                       dim3(256), 0, stream, d, source, offsets, match_channels, split);
     return hipGetLastError();
 }
 

@@ -239,7 +239,9 @@ __global__ __launch_bounds__(128) void jacobi_kernel(
 // ---------------------------------------------------------------------------
 
 // Column pair (j, j ^ k) of the relative row: column x = r ^ j pairs with
-// x ^ k and takes the rotation of lane x.
+// x ^ k and takes the rotation of lane x.  Both lanes of a pair hold the
+// same c and opposite sigma, so lane x ^ k's values are c and -sigma of lane
+// x: one exchange per column pair instead of two.
 template <int kK, int kJ>
 __device__ inline void rotate_columns(float (&a)[16], float (&v)[16], const float (&b)[16], float c,
                                       float sigma) {
@@ -248,8 +250,8 @@ __device__ inline void rotate_columns(float (&a)[16], float (&v)[16], const floa
         constexpr int kJJ = kJ ^ kK;
         const float c0 = xor16<kJ>(c);
         const float s0 = xor16<kJ>(sigma);
-        const float c1 = xor16<kJJ>(c);
-        const float s1 = xor16<kJJ>(sigma);
+        const float c1 = c0;
+        const float s1 = -s0;
         const float v0 = v[kJ];
         const float v1 = v[kJJ];
         a[kJ] = fmaf(s0, b[kJJ], c0 * b[kJ]);
