@@ -116,34 +116,61 @@ def match_roofline(form, times, bp, fp, clip, sb, sf) -> dict:
 
 
 def filter_roofline(form, times, bp, fp, clip, sb, sf) -> dict:
-    """Roofline of the filter + aggregation kernel against HBM: the
-    algorithmic bytes of the scatter design per stage, sum over groups of
-    S * D * 4 (filtered rows read) + S * Pt * P^2 * (C + 1) * 8 (accumulator
-    read-modify-write, fp32), S the retained samples of the group, over the
-    kernel's average launch time from hipEvents (a stage launches it once per
-    part of its anchor frames, so per launch is the stage total over the
-    launches)."""
+    """Roofline of the filter + aggregation kernel by the max rule: the
+    bound is whichever of its algorithmic flops (at the fp32 peak) and its
+    algorithmic HBM bytes (at the HBM peak) takes longer, over the kernel's
+    average launch time from hipEvents (a stage launches it once per part of
+    its anchor frames, so per launch is the stage total over the launches).
+      flops, per group of S retained samples, B model rows, rank R and
+      dimension D: projection + reconstruction 4 S R D and the dual map
+      2 R B D;
+      minimum bytes, per stage: the fp32 accumulators written and read once,
+      frames * (C + 1) * pixels * 4 * 2, and one read of the clip (and of the
+      Basic estimate in Final), frames * C * pixels * 4;
+      scatter-design bytes, reported beside it: S * D * 4 (filtered rows
+      read) + S * Pt * P^2 * (C + 1) * 8 (accumulator read-modify-write)."""
     channels = clip["channels"]
-    total = 0.0
-    for params, stats in ((bp, sb), (fp, sf)):
+    pixels = clip["width"] * clip["height"]
+    frames = clip["frames"]
+    flops = 0.0
+    minimum = 0.0
+    scatter = 0.0
+    for stage, params, stats in ((0, bp, sb), (1, fp, sf)):
         dim = params.patch_size ** 2 * params.patch_time * channels
+        rows = max(1, int(params.similar * params.model_cap_factor))
+        rank = min(params.rank, rows)
+        flops += 4.0 * rank * dim * stats["retained"] + 2.0 * rank * rows * dim * stats["groups"]
+        minimum += 2.0 * frames * (channels + 1) * pixels * 4.0 + frames * channels * pixels * 4.0 * (1 + stage)
         contributions = params.patch_time * params.patch_size ** 2 * (channels + 1)
-        total += stats["retained"] * (dim * 4.0 + contributions * 8.0)
+        scatter += stats["retained"] * (dim * 4.0 + contributions * 8.0)
     filt = times["filter"]
     launches = max(1, filt["launches"])
     avg_s = filt["ms"] / launches / 1000.0
-    achieved = total / launches / avg_s / 1e9 if avg_s > 0 else 0.0
     kernel = "filter_fast_kernel" if form == "fast" else "filter_kernel"
+
+    def gbs(total):
+        achieved = total / launches / avg_s / 1e9 if avg_s > 0 else 0.0
+        return {"bytes_per_launch": round(total / launches, 1), "achieved_gbs": round(achieved, 2),
+                "frac": round(achieved / MI355X_PEAK["hbm_gbs"], 5)}
+
+    flop_time = flops / (MI355X_PEAK["fp32_tflops"] * 1e12)
+    byte_time = minimum / (MI355X_PEAK["hbm_gbs"] * 1e9)
+    if flop_time >= byte_time:
+        achieved = flops / launches / avg_s / 1e12 if avg_s > 0 else 0.0
+        head = {"bound": "mfma", "achieved": round(achieved, 3), "peak": MI355X_PEAK["fp32_tflops"],
+                "unit": "TFLOP/s", "frac": round(achieved / MI355X_PEAK["fp32_tflops"], 5),
+                "algorithmic_per_launch": round(flops / launches, 1)}
+    else:
+        g = gbs(minimum)
+        head = {"bound": "hbm", "achieved": g["achieved_gbs"], "peak": MI355X_PEAK["hbm_gbs"], "unit": "GB/s",
+                "frac": g["frac"], "algorithmic_per_launch": g["bytes_per_launch"]}
     return {
         "kernel": kernel,
-        "bound": "hbm",
-        "achieved": round(achieved, 2),
-        "peak": MI355X_PEAK["hbm_gbs"],
-        "unit": "GB/s",
-        "frac": round(achieved / MI355X_PEAK["hbm_gbs"], 5),
+        **head,
         "traffic": measured_traffic(kernel, launches) if form == "fast" else None,
         "launch_ms": round(avg_s * 1000.0, 4),
-        "algorithmic_per_launch": round(total / launches, 1),
+        "hbm_minimum": gbs(minimum),
+        "scatter_design": gbs(scatter),
     }
 
 

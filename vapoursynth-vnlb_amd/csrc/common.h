@@ -30,6 +30,11 @@ constexpr int kMaxChannels = 4;
 constexpr int kMaxTemporal = 64;
 constexpr int kMaxFrames = 256;
 constexpr int kMaxSide = 4096;
+// XCDs of the GPU and the layout of a per-XCD work queue: one counter per XCD,
+// 16 ints (a 64-byte line) apart.
+constexpr int kXcds = 8;
+constexpr int kXcdCounterStride = 16;
+constexpr int kXcdCounterWords = kXcds * kXcdCounterStride;
 
 enum Stage : int {
     kBasic = 0,

@@ -7,6 +7,41 @@
 
 namespace vnlbhip {
 
+// XCD-aware work distribution (kXcds, counters: common.h).  The GPU
+// dispatches the workgroups of a launch round-robin over its 8 XCDs (block b
+// on XCD b % 8), and each XCD has its own L2: work that shares data
+// (neighbouring anchors read overlapping patches) should run on one XCD.
+
+// Remap of block b of n: the blocks of XCD x take the contiguous range
+// [x * (n / 8) + min(x, n % 8), ...) of the work items, in order.
+__device__ inline int xcd_block(int b, int n) {
+    const int x = b % kXcds;
+    const int per = n / kXcds;
+    const int rem = n % kXcds;
+    return x * per + (x < rem ? x : rem) + b / kXcds;
+}
+
+// Group queue over kXcdCounterWords zeroed ints: XCD x hands out the items
+// of its contiguous eighth [count x / 8, count (x + 1) / 8) of [0, count) in
+// order, then a caller whose eighth is exhausted takes from the following
+// XCDs' eighths.  `visited` (0 on the first call) is the caller's own state.
+// Returns count when every eighth is exhausted.  Called by one lane.
+__device__ inline int xcd_next(int* counters, int count, int home, int& visited) {
+    while (visited < kXcds) {
+        const int x = (home + visited) % kXcds;
+        const int begin = static_cast<int>(static_cast<long long>(count) * x / kXcds);
+        const int end = static_cast<int>(static_cast<long long>(count) * (x + 1) / kXcds);
+        if (begin < end) {
+            const int g = begin + atomicAdd(counters + x * kXcdCounterStride, 1);
+            if (g < end) {
+                return g;
+            }
+        }
+        ++visited;
+    }
+    return count;
+}
+
 __device__ inline float video_at(const float* __restrict__ video,
                                  const StageDesc& d, int x, int y, int frame,
                                  int channel) {

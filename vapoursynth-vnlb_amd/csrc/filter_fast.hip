@@ -303,17 +303,21 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
     const float* model_video = final_stage ? basic : noisy;
 
     const int group_count = *group_count_dev;
-    // Groups are taken in order from the work counter, one request per
-    // wavefront; the next index is requested while the current group runs.
+    // Groups are taken one request per wavefront from this XCD's contiguous
+    // eighth of the list (then from the others'), so the groups in flight on
+    // an XCD are neighbours that share its L2; the next index is requested
+    // while the current group runs.
+    const int home = blockIdx.x % kXcds;
+    int visited = 0;
     int request = 0;
     if (lane == 0) {
-        request = atomicAdd(work_counter, 1);
+        request = xcd_next(work_counter, group_count, home, visited);
     }
     int following = 0;
     for (int g = __builtin_amdgcn_readlane(request, 0); g < group_count;
          g = __builtin_amdgcn_readlane(following, 0)) {
         if (lane == 0) {
-            following = atomicAdd(work_counter, 1);
+            following = xcd_next(work_counter, group_count, home, visited);
         }
         wave_sync();  // the previous group is done with this wave's LDS
         const int a = refs.group_list[g];

@@ -94,10 +94,12 @@ hipError_t launch_compact(const unsigned char* active, int anchor_begin, int anc
 
 size_t prepare_shared_bytes(const StageDesc& d);
 size_t filter_shared_bytes(const StageDesc& d);
-// prepare, filter and fast filter: persistent grids that take groups in
-// order from *work_counter (zero on entry), so workgroups that become
+// prepare, filter and fast filter: persistent grids that take groups from
+// a work queue at work_counter (zero on entry), so workgroups that become
 // resident late (beside other kernels) take fewer groups instead of
-// stretching the launch.  The model kernels read the group count from
+// stretching the launch.  prepare and the fast filter use per-XCD queues
+// (kXcdCounterWords ints: each XCD's contiguous eighth of the list first);
+// the filter uses work_counter[0].  The model kernels read the group count from
 // *group_count_dev (written by the compaction on the device); grids and
 // buffers are sized by the capacity, an upper bound (the part's anchors).
 hipError_t launch_prepare(const StageDesc& d, const float* noisy, const float* basic,

@@ -400,7 +400,9 @@ void match_fast_kernel(
     // Uniform per wavefront: tile indices derived from it stay scalar.
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lane = tid & 63;
-    const int* tile = tiles + static_cast<size_t>(blockIdx.x) * kTileAnchors;
+    // Each XCD takes a contiguous run of tiles (neighbouring anchors, whose
+    // window unions overlap in its L2).
+    const int* tile = tiles + static_cast<size_t>(xcd_block(blockIdx.x, gridDim.x)) * kTileAnchors;
     const bool final_stage = d.stage == kFinal;
     const int match_channels = final_stage ? d.channels : 1;
     const bool finite = flags[0] == 0u;

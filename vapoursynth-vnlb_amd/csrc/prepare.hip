@@ -212,10 +212,14 @@ __global__ __launch_bounds__(kModelThreads, 4) void prepare_kernel(
     }
 
     const int group_count = *group_count_dev;
-    // Groups are taken in order from the work counter; the next index is
-    // requested while the current group is processed.
+    // Groups come from this XCD's contiguous eighth of the list (then from
+    // the others'), so the groups in flight on an XCD are neighbours that
+    // share its L2; the next index is requested while the current group is
+    // processed.
+    const int home = blockIdx.x % kXcds;
+    int visited = 0;
     if (tid == 0) {
-        s_next = atomicAdd(work_counter, 1);
+        s_next = xcd_next(work_counter, group_count, home, visited);
     }
     int following = 0;
     const auto advance = [&]() {
@@ -229,7 +233,7 @@ __global__ __launch_bounds__(kModelThreads, 4) void prepare_kernel(
     __syncthreads();
     for (int g = s_next; g < group_count; g = advance()) {
         if (tid == 0) {
-            following = atomicAdd(work_counter, 1);
+            following = xcd_next(work_counter, group_count, home, visited);
         }
         const int a = refs.group_list[g];
         const int similar = refs.counts[a];

@@ -557,7 +557,7 @@ struct vnlbhip_context {
     DeviceBuffer<float> vectors{&generation};
     DeviceBuffer<int> status{&generation};
     DeviceBuffer<int> refine_list{&generation};
-    DeviceBuffer<int> work_counters{&generation};  // prepare, filter: next group to take
+    DeviceBuffer<int> work_counters{&generation};  // prepare, filter: per-XCD group queues
     // Search-centre offsets (device, [frames][temporal][H][W][2] int16) or
     // null: every window centred on its anchor.
     const int16_t* search_offsets = nullptr;
@@ -1204,8 +1204,11 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
             int* d_status = ctx.status.reserve(static_cast<size_t>(part_capacity) * d.models);
             d_dims_last = d_dims;
             d_problems_last = d_problems;
-            int* d_work = ctx.work_counters.reserve(2);
-            untimed(ctx, "clear work counters", [&]() { return vnlbhip::launch_fill_zero_int(d_work, 2, stream); });
+            // Per-XCD group queues: prepare's, then the filter's.
+            int* d_work = ctx.work_counters.reserve(2 * vnlbhip::kXcdCounterWords);
+            untimed(ctx, "clear work counters", [&]() {
+                return vnlbhip::launch_fill_zero_int(d_work, 2 * vnlbhip::kXcdCounterWords, stream);
+            });
             timed(ctx, VNLBHIP_KERNEL_PREPARE, stream, "prepare kernel", [&]() {
                 return vnlbhip::launch_prepare(d, noisy, basic, d_anchors, d_counts, d_positions,
                                                d_part_list, d_count, capacity, n_stride, d_flags, d_dims,
@@ -1239,12 +1242,12 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
                     return vnlbhip::launch_fast_filter(d, noisy, basic, d_anchors, d_counts, d_positions,
                                                        d_part_list, d_count, capacity, n_stride, d_flags,
                                                        d_means, d_eigen, d_vectors, d_problems, acc32,
-                                                       d_weights, d_work + 1, stream);
+                                                       d_weights, d_work + vnlbhip::kXcdCounterWords, stream);
                 }
                 return vnlbhip::launch_filter(d, noisy, basic, d_anchors, d_counts, d_positions,
                                               d_part_list, d_count, capacity, n_stride, d_flags, d_dims,
-                                              d_means, d_eigen, d_vectors, acc, d_weights, d_work + 1,
-                                              stream);
+                                              d_means, d_eigen, d_vectors, acc, d_weights,
+                                              d_work + vnlbhip::kXcdCounterWords, stream);
             });
             untimed(ctx, "summarize kernel", [&]() { return vnlbhip::launch_summarize(d_flags, d.rank > 0 ? d_status : nullptr, d_counts,
                                             d_part_list, d_count, capacity, d.models, summary, stream); });
