@@ -32,6 +32,7 @@ struct PrepareLayout {
     int tab_c;     // [models][row] channel of the element (-1 outside the row)
     int rows;      // [max_rows][width] model rows
     int pos;       // [cap] packed positions
+    int gram;      // [waves][4][64] partial Gram tiles of the matrix-core form
     int words;
 };
 
@@ -52,6 +53,7 @@ PrepareLayout prepare_layout(const StageDesc& d) {
     L.tab_c = take(L.width);
     L.rows = take(d.max_rows * L.width);
     L.pos = take(d.cap);
+    L.gram = take(kModelThreads / 64 * 4 * 64);  // per-wave partial Grams (matrix-core form)
     L.words = at;
     return L;
 }
@@ -472,17 +474,20 @@ __global__ __launch_bounds__(kModelThreads, 4) void prepare_kernel(
         for (int m = 0; m < d.models; ++m) {
             float* out = problems + (static_cast<size_t>(g) * d.models + m) * problem_stride * problem_stride;
             if (kMfma && dual) {
-                // G = X X^T / rows with X the centered model rows (at most 16),
-                // one wavefront: v_mfma_f32_16x16x32_bf16 over the elements in
-                // blocks of 8 slots of one lane column (any element order gives
-                // the same sum; A and B share the lane's registers since
-                // B = A^T), three products hi.hi + hi.lo + lo.hi.
-                if (tid < 64) {
+                // G = X X^T / rows with X the centered model rows (at most 16):
+                // v_mfma_f32_16x16x32_bf16 over the elements in blocks of 8
+                // slots of one lane column (any element order gives the same
+                // sum; A and B share the lane's registers since B = A^T),
+                // three products hi.hi + hi.lo + lo.hi.  Wavefront w takes the
+                // k steps q = w mod 4; the four partial tiles are summed in
+                // wavefront order through LDS.
+                {
+                    const int wave = tid >> 6;
                     const int col = tid & 15;
-                    const int kb = tid >> 4;
+                    const int kb = (tid & 63) >> 4;
                     const int blocks_per_lane = kS / 8;
                     floatx4 acc = {0.0F, 0.0F, 0.0F, 0.0F};
-                    for (int q = 0; q < kS / 2; ++q) {
+                    for (int q = wave; q < kS / 2; q += kModelThreads / 64) {
                         const int b = 4 * q + kb;
                         const int src_lane = b / blocks_per_lane;
                         const int k0 = 8 * (b - src_lane * blocks_per_lane);
@@ -508,14 +513,27 @@ __global__ __launch_bounds__(kModelThreads, 4) void prepare_kernel(
                         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, a_lo, acc, 0, 0, 0);
                         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_lo, a_hi, acc, 0, 0, 0);
                     }
-                    // acc[v] = G[4 kb + v][col]
+                    // acc[v] = partial G[4 kb + v][col]
+                    float* s_gram = s_dyn + L.gram;
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
-                        const int r = 4 * kb + v;
-                        if (r < rows && col < rows) {
-                            out[r * problem_stride + col] = scale * acc[v];
+                        s_gram[(wave * 4 + v) * 64 + (tid & 63)] = acc[v];
+                    }
+                    __syncthreads();
+                    if (tid < 64) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            float total = s_gram[v * 64 + tid];
+                            for (int w = 1; w < kModelThreads / 64; ++w) {
+                                total = total + s_gram[(w * 4 + v) * 64 + tid];
+                            }
+                            const int r = 4 * kb + v;
+                            if (r < rows && col < rows) {
+                                out[r * problem_stride + col] = scale * total;
+                            }
                         }
                     }
+                    __syncthreads();  // the partials are rewritten by the next model
                 }
             } else if (dual) {
                 // compute_gram_centered (linalg.cpp:326-354): scale * dot in
