@@ -83,6 +83,18 @@ __host__ __device__ inline int region_floats(int window, int patch, int temporal
     return (guided ? temporal * patch_time : temporal + patch_time - 1) * side * side + 8;
 }
 
+// Candidate distances of a full window: window^2 per scheduled frame.
+__host__ __device__ inline int window_candidates(const StageDesc& d) {
+    return d.search_window * d.search_window * (d.search_bwd + d.search_fwd + 1);
+}
+
+// Entries of each of the three selection arrays after the distances: the
+// selected list (at most the cap) and the short path's candidate list (at
+// most one per thread).
+__host__ __device__ inline int selection_entries(const StageDesc& d) {
+    return d.cap > kMatchThreads ? d.cap : kMatchThreads;
+}
+
 // Distances (scan order) with the search region staged in LDS one channel
 // at a time; each thread owns 4 horizontally adjacent candidates of one
 // row so every loaded sample feeds 4 distances.  The per-candidate order of
@@ -101,9 +113,6 @@ __global__ __launch_bounds__(kMatchThreads) void match_kernel(
     __shared__ unsigned s_hist[256];
     __shared__ int s_int[16];
     __shared__ int s_scan[8];
-    __shared__ float s_sel_dist[kMaxCap];
-    __shared__ int s_sel_index[kMaxCap];
-    __shared__ int s_sel_rank[kMaxCap];
 
     const int a = blockIdx.x;
     if (a >= anchor_count) {
@@ -176,6 +185,10 @@ __global__ __launch_bounds__(kMatchThreads) void match_kernel(
     float* s_region = s_dyn;
     float* s_dist = s_dyn + region_floats(d.search_window, P, d.search_bwd + d.search_fwd + 1,
                                           d.patch_time, guided);
+    // Selection scratch after the candidate distances.
+    float* s_sel_dist = s_dist + window_candidates(d);
+    int* s_sel_index = reinterpret_cast<int*>(s_sel_dist + selection_entries(d));
+    int* s_sel_rank = s_sel_index + selection_entries(d);
 
     // Reference patch [c][t][y][x] (load_reference_patch_* core.cpp:1152-1198).
     const int ref_len = match_channels * d.patch_time * area;
@@ -318,9 +331,6 @@ __global__ __launch_bounds__(kMatchThreads) void learned_match_kernel(
     __shared__ unsigned s_hist[256];
     __shared__ int s_int[16];
     __shared__ int s_scan[8];
-    __shared__ float s_sel_dist[kMaxCap];
-    __shared__ int s_sel_index[kMaxCap];
-    __shared__ int s_sel_rank[kMaxCap];
     __shared__ __attribute__((aligned(16))) __bf16 s_x[kPropTile * kPropXPitch];
     __shared__ __attribute__((aligned(16))) __bf16 s_h[kPropTile * kPropHPitch];
     __shared__ float s_cbox[kPropTile * kPropBoxes];
@@ -365,6 +375,9 @@ __global__ __launch_bounds__(kMatchThreads) void learned_match_kernel(
     const int region_size = rframes * plane;
     float* s_region = s_dyn;
     float* s_dist = s_dyn + region_floats(d.search_window, P, d.search_bwd + d.search_fwd + 1, d.patch_time, false);
+    float* s_sel_dist = s_dist + window_candidates(d);
+    int* s_sel_index = reinterpret_cast<int*>(s_sel_dist + selection_entries(d));
+    int* s_sel_rank = s_sel_index + selection_entries(d);
 
     const int ref_len = match_channels * d.patch_time * area;
     for (int i = tid; i < ref_len; i += blockDim.x) {
@@ -621,9 +634,8 @@ hipError_t launch_learned_match(const StageDesc& d, const float* source, const u
 
 size_t match_shared_bytes(const StageDesc& d, bool guided) {
     const int temporal = d.search_bwd + d.search_fwd + 1;
-    const long long candidates = static_cast<long long>(d.search_window) * d.search_window * temporal;
     return (static_cast<size_t>(region_floats(d.search_window, d.patch, temporal, d.patch_time, guided)) +
-            static_cast<size_t>(candidates)) *
+            static_cast<size_t>(window_candidates(d)) + 3 * static_cast<size_t>(selection_entries(d))) *
            sizeof(float);
 }
 

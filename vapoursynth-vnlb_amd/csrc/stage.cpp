@@ -255,7 +255,8 @@ StageDesc make_desc(int stage, const vnlbhip_params& p, const Geometry& g, bool 
             "model rows times patch dimension exceed the prepare kernel's LDS budget");
     require(matching_only || vnlbhip::filter_shared_bytes(d) <= 112 * 1024,
             "rank and model rows times patch dimension exceed the filter kernel's LDS budget");
-    require(vnlbhip::match_shared_bytes(d) <= 96 * 1024,
+    // (the budget includes the selection scratch, 3 x max(cap, 256) words)
+    require(vnlbhip::match_shared_bytes(d) <= 108 * 1024,
             "search window too large for the matcher's LDS budget");
     aggregation_window(p, d.window);
     return d;
@@ -901,7 +902,7 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
         require(ctx.search_offsets_frames == g.frames && ctx.search_offsets_height == g.height &&
                     ctx.search_offsets_width == g.width,
                 "search offsets were set for a clip of another shape");
-        require(vnlbhip::match_shared_bytes(d, true) <= 140 * 1024,
+        require(vnlbhip::match_shared_bytes(d, true) <= 152 * 1024,
                 "search window too large for guided matching's LDS budget");
     }
     // Learned candidate proposal: the proposing matcher replaces the
