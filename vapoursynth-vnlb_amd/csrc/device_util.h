@@ -58,7 +58,7 @@ __device__ inline float row_rotate(float v) {
     static_assert(kShift >= 1 && kShift <= 15, "row_ror takes 1..15");
     return __builtin_bit_cast(
         float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x120 + kShift, 0xF, 0xF,
-                                           false));
+                                           true));
 }
 
 // Halving-tree sum over a 16-lane segment (the ReduceSum order of the CPU
@@ -162,6 +162,9 @@ __device__ inline double wave_sum_double(double v) {
 
 // v from lane (lane ^ kMask) of the same 16-lane row, with DPP moves only:
 // row_mirror is xor 15, row_half_mirror xor 7 and quad_perm any xor below 4.
+// Every lane has a source in these patterns, so the moves are issued with
+// bound_ctrl set: the "old" operand is then dead and no zero needs to be
+// materialised before each move.
 template <int kMask>
 __device__ inline int xor16_bits(int x) {
     static_assert(kMask >= 0 && kMask < 16, "xor within a row of 16 lanes");
@@ -169,13 +172,13 @@ __device__ inline int xor16_bits(int x) {
     constexpr int kQuad = (kAfterMirror & 4) != 0 ? (kAfterMirror ^ 7) : kAfterMirror;
     constexpr int kQuadCtrl = (0 ^ kQuad) | ((1 ^ kQuad) << 2) | ((2 ^ kQuad) << 4) | ((3 ^ kQuad) << 6);
     if constexpr ((kMask & 8) != 0) {
-        x = __builtin_amdgcn_update_dpp(0, x, 0x140, 0xF, 0xF, false);  // row_mirror
+        x = __builtin_amdgcn_update_dpp(0, x, 0x140, 0xF, 0xF, true);  // row_mirror
     }
     if constexpr ((kAfterMirror & 4) != 0) {
-        x = __builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, false);  // row_half_mirror
+        x = __builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, true);  // row_half_mirror
     }
     if constexpr (kQuad != 0) {
-        x = __builtin_amdgcn_update_dpp(0, x, kQuadCtrl, 0xF, 0xF, false);
+        x = __builtin_amdgcn_update_dpp(0, x, kQuadCtrl, 0xF, 0xF, true);
     }
     return x;
 }

@@ -29,6 +29,8 @@
 #include <float.h>
 #include <hip/hip_bf16.h>
 
+#include <type_traits>
+
 namespace vnlbhip {
 namespace {
 
@@ -667,27 +669,78 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                         }
                         continue;
                     }
-                    for (int k = __builtin_amdgcn_readfirstlane(k_begin); k < __builtin_amdgcn_readfirstlane(k_end);
-                         ++k) {
-                        const int cell = __builtin_amdgcn_readlane(kcell_l, k) + lane_cell;
-                        const float cw =
-                            __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, kw_l), k)) *
-                            lane_window;
-                        float value = center[tt];
-                        if (modelled) {
-                            const float4* w4 = reinterpret_cast<const float4*>(s_gram + k * kFastModel);
-                            float sum = 0.0F;
-#pragma unroll
-                            for (int h = 0; h < kFastModel / 4; ++h) {
-                                const float4 wv = w4[h];
-                                sum = fmaf(wv.x, x[tt][4 * h + 0], sum);
-                                sum = fmaf(wv.y, x[tt][4 * h + 1], sum);
-                                sum = fmaf(wv.z, x[tt][4 * h + 2], sum);
-                                sum = fmaf(wv.w, x[tt][4 * h + 3], sum);
+                    const int kb0 = __builtin_amdgcn_readfirstlane(k_begin);
+                    const int ke0 = __builtin_amdgcn_readfirstlane(k_end);
+                    // Modelled groups and the weight plane as compile-time
+                    // cases, so the two estimates of a pair have no branch
+                    // between them.
+                    const auto run = [&](auto model_case, auto weight_case) {
+                        constexpr bool kModelled = decltype(model_case)::value;
+                        constexpr bool kWeight = decltype(weight_case)::value;
+                        const auto estimate = [&](int k) -> float {
+                            float value = center[tt];
+                            if constexpr (kModelled) {
+                                const float4* w4 = reinterpret_cast<const float4*>(s_gram + k * kFastModel);
+                                const float4 wa = w4[0];
+                                const float4 wb = w4[1];
+                                const float4 wc = w4[2];
+                                const float4 wd = w4[3];
+                                float s0 = wa.x * x[tt][0];
+                                float s1 = wc.x * x[tt][8];
+                                s0 = fmaf(wa.y, x[tt][1], s0);
+                                s1 = fmaf(wc.y, x[tt][9], s1);
+                                s0 = fmaf(wa.z, x[tt][2], s0);
+                                s1 = fmaf(wc.z, x[tt][10], s1);
+                                s0 = fmaf(wa.w, x[tt][3], s0);
+                                s1 = fmaf(wc.w, x[tt][11], s1);
+                                s0 = fmaf(wb.x, x[tt][4], s0);
+                                s1 = fmaf(wd.x, x[tt][12], s1);
+                                s0 = fmaf(wb.y, x[tt][5], s0);
+                                s1 = fmaf(wd.y, x[tt][13], s1);
+                                s0 = fmaf(wb.z, x[tt][6], s0);
+                                s1 = fmaf(wd.z, x[tt][14], s1);
+                                s0 = fmaf(wb.w, x[tt][7], s0);
+                                s1 = fmaf(wd.w, x[tt][15], s1);
+                                value = value + (s0 + s1);
                             }
-                            value = value + sum;
+                            return value;
+                        };
+                        int k = kb0;
+                        for (; k + 1 < ke0; k += 2) {
+                            const int cell0 = __builtin_amdgcn_readlane(kcell_l, k) + lane_cell;
+                            const int cell1 = __builtin_amdgcn_readlane(kcell_l, k + 1) + lane_cell;
+                            const float cw0 = __builtin_bit_cast(
+                                                  float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, kw_l), k)) *
+                                              lane_window;
+                            const float cw1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(
+                                                                            __builtin_bit_cast(int, kw_l), k + 1)) *
+                                              lane_window;
+                            const float v0 = estimate(k);
+                            const float v1 = estimate(k + 1);
+                            tile_add(s_tile, cell0, cw0 * v0, cw0, kWeight);
+                            tile_add(s_tile, cell1, cw1 * v1, cw1, kWeight);
                         }
-                        tile_add(s_tile, cell, cw * value, cw, c == 0);
+                        if (k < ke0) {
+                            const int cell = __builtin_amdgcn_readlane(kcell_l, k) + lane_cell;
+                            const float cw = __builtin_bit_cast(
+                                                 float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, kw_l), k)) *
+                                             lane_window;
+                            const float value = estimate(k);
+                            tile_add(s_tile, cell, cw * value, cw, kWeight);
+                        }
+                    };
+                    if (modelled) {
+                        if (c == 0) {
+                            run(std::true_type{}, std::true_type{});
+                        } else {
+                            run(std::true_type{}, std::false_type{});
+                        }
+                    } else {
+                        if (c == 0) {
+                            run(std::false_type{}, std::true_type{});
+                        } else {
+                            run(std::false_type{}, std::false_type{});
+                        }
                     }
                 }
                 wave_sync();

@@ -281,10 +281,11 @@ constexpr uint64_t kNoKey = ~static_cast<uint64_t>(0);
 // rather than sorted and merged.
 constexpr int kInsertKeys = 24;
 
-// The value of lane l - 1 in lane l (DPP wave_shr:1; lane 0 keeps 0).
+// The value of lane l - 1 in lane l (DPP wave_shr:1; bound_ctrl gives lane
+// 0, which has no source, a 0).
 __device__ inline uint64_t from_lane_below(uint64_t v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(v & 0xFFFFFFFFu), 0x138, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(v >> 32), 0x138, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(v & 0xFFFFFFFFu), 0x138, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(v >> 32), 0x138, 0xF, 0xF, true);
     return (static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) | static_cast<uint32_t>(lo);
 }
 
