@@ -497,6 +497,7 @@ void match_fast_kernel(
     // past its columns are not staged (only tiles past the union read them).
     // Each thread also stages one 16-byte piece of the anchors' samples:
     // slot tid % 16, patch row tid / 32, columns 4 ((tid / 16) % 2) .. + 3.
+    const int box_items = urows * ucols;
     const int groups_used = (ucols - 1) / 4 + 2;
     const int rows_used = urows + kFastPatch - 1;
     int item_src[kItems];  // sample offset in a plane of column 4 g, or -1
@@ -617,7 +618,6 @@ void match_fast_kernel(
     float* s_ebox = reinterpret_cast<float*>(s_bytes);
     float* s_stage = reinterpret_cast<float*>(s_bytes + L.ebox_bytes);
     uint16_t* s_list = reinterpret_cast<uint16_t*>(s_bytes + L.ebox_bytes + L.stage_bytes);
-    const int box_items = urows * ucols;
 
     // Keys at or below tau (any scan position) may be selected whatever d_K is.
     const uint64_t tau_key = (static_cast<uint64_t>(distance_key(d.tau)) << 32) | 0xFFFFFFFFull;
@@ -635,12 +635,8 @@ void match_fast_kernel(
             acc[i] = floatx4{0.0F, 0.0F, 0.0F, 0.0F};
         }
         const int p_last = (ti + 1) * planes_per_frame - 1;
-        for (int p = ti * planes_per_frame; p <= p_last; ++p) {
-            // The next plane of the frame goes into registers while this one
-            // is on the matrix cores, then into the other buffer.
-            if (p < p_last) {
-                prefetch(p + 1);
-            }
+        // Products of plane p on the matrix cores (accumulated into acc).
+        const auto products = [&](int p) {
             load_anchor(p);
             unsigned base[kR][2];
             const unsigned buffer_base = static_cast<unsigned>((p & 1) * L.buffer_bytes) + lane_bytes;
@@ -688,24 +684,39 @@ void match_fast_kernel(
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
-            if (p < p_last) {
-                store(p + 1);
-            }
+        };
+        for (int p = ti * planes_per_frame; p < p_last; ++p) {
+            // The next plane of the frame goes into registers while this one
+            // is on the matrix cores, then into the other buffer.
+            prefetch(p + 1);
+            products(p);
+            store(p + 1);
             __syncthreads();
         }
+        // The frame's energy box goes into registers while its last plane
+        // is on the matrix cores (its LDS space is free only after them).
+        float ebox_pre[kEnergyPerThread];
+#pragma unroll
+        for (int k = 0; k < kEnergyPerThread; ++k) {
+            const int item = tid + k * kFastThreads;
+            ebox_pre[k] = 0.0F;
+            if (item < box_items) {
+                const int ry = item / ucols;
+                const int cx = item - ry * ucols;
+                ebox_pre[k] = energy[(static_cast<size_t>(frame) * d.height + uy0 + ry) * d.width + ux0 + cx];
+            }
+        }
+        products(p_last);
+        __syncthreads();
 
         // Epilogue of the candidate frame in the space of both buffers.
 #pragma unroll
         for (int k = 0; k < kEnergyPerThread; ++k) {
-            int item = tid + k * kFastThreads;
-            // Opaque: the box offsets are recomputed per frame instead of
-            // being held in registers across the plane loop.
-            asm volatile("" : "+v"(item));
+            const int item = tid + k * kFastThreads;
             if (item < box_items) {
                 const int ry = item / ucols;
                 const int cx = item - ry * ucols;
-                s_ebox[cx * L.ebox_pitch + ry] =
-                    energy[(static_cast<size_t>(frame) * d.height + uy0 + ry) * d.width + ux0 + cx];
+                s_ebox[cx * L.ebox_pitch + ry] = ebox_pre[k];
             }
         }
         if (wave == 0) {
