@@ -316,17 +316,44 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
         request = xcd_next(work_counter, group_count, home, visited);
     }
     int following = 0;
-    for (int g = __builtin_amdgcn_readlane(request, 0); g < group_count;
-         g = __builtin_amdgcn_readlane(following, 0)) {
+    // A group's anchor, flags, retained count and positions (sample `lane`)
+    // are loaded while the wave's previous group is processed, as vector
+    // loads that nothing waits for until they are used.
+    int next_a = 0;
+    int next_flags = 0;
+    int next_similar = 0;
+    uint32_t next_anchor = 0;
+    uint32_t next_pos = 0;
+    const auto fetch_group = [&](int index) {
+        int v = 0;
+        asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "s"(index));
+        next_a = refs.group_list[v];
+        next_flags = group_flags[v];
+    };
+    const auto fetch_members = [&]() {
+        next_anchor = refs.anchors[next_a];
+        next_similar = refs.counts[next_a];
+        if (lane < d.cap) {
+            next_pos = refs.positions[static_cast<size_t>(next_a) * d.cap + lane];
+        }
+    };
+    int g = __builtin_amdgcn_readlane(request, 0);
+    if (g < group_count) {
+        fetch_group(g);
+        fetch_members();
+    }
+    while (g < group_count) {
         if (lane == 0) {
             following = xcd_next(work_counter, group_count, home, visited);
         }
         wave_sync();  // the previous group is done with this wave's LDS
-        const int a = refs.group_list[g];
-        const int af = position_frame(refs.anchors[a]);
-        const int similar = refs.counts[a];
+        const int a = __builtin_amdgcn_readfirstlane(next_a);
+        const uint32_t anchor_p = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(next_anchor)));
+        const int af = position_frame(anchor_p);
+        const int similar = __builtin_amdgcn_readfirstlane(next_similar);
         const size_t list = static_cast<size_t>(a) * d.cap;
-        const int flags = group_flags[g];
+        const int flags = __builtin_amdgcn_readfirstlane(next_flags);
+        const uint32_t group_pos = next_pos;
         const bool equal = (flags & kFlagEqual) != 0;
         const bool flat = (flags & kFlagFlat) != 0;
         const bool modelled = (flags & kFlagModelled) != 0;
@@ -335,7 +362,7 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
 
         uint32_t my_pos = 0;
         if (lane < similar) {
-            my_pos = refs.positions[list + lane];
+            my_pos = group_pos;
             s_pos[lane] = my_pos;
         }
         // Samples sorted by source slot q = frame - af + search_bwd over
@@ -343,7 +370,6 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
         // frame t, and output slots outside [0, slots) are dropped (scatter).
         // Lane r keeps its sorted index; the sorted tables hold the position
         // and the tile cell of the patch origin.
-        const uint32_t anchor_p = refs.anchors[a];
         int x_low = 0;
         int x_high = 0;
         int y_low = 0;
@@ -431,6 +457,11 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
             volume_md = volume / static_cast<float>(md);
         }
         wave_sync();
+        // The wave's next group: its index, anchor and flags are requested now.
+        const int g_next = __builtin_amdgcn_readlane(following, 0);
+        if (g_next < group_count) {
+            fetch_group(g_next);
+        }
 
         // Aggregation weights (finalize_aggregation_weights core.cpp:276-294).
         const bool weights_need_scores = !equal && d.weight_beta != 0.0F;
@@ -592,6 +623,9 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
             }
         }
         wave_sync();
+        if (g_next < group_count) {
+            fetch_members();
+        }
         // The sorted tables in registers, sample k in lane k, so the loops
         // below take them with v_readlane instead of LDS broadcasts.
         const int sorted_count = __builtin_amdgcn_readfirstlane(s_begin[sorted_slots]);
@@ -786,6 +820,7 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                 wave_sync();
             }
         }
+        g = g_next;
     }
 }
 
