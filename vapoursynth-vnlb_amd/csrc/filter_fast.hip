@@ -644,15 +644,19 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
         // the group's window, contributions are combined in the tile (value
         // plane; weight plane for channel 0) and flushed once per cell.
         const size_t plane = static_cast<size_t>(d.width) * d.height;
-        for (int c = 0; c < d.channels; ++c) {
-            // The lane's centered model-row samples of this channel, every
-            // load issued at once (rows beyond B read row 0 and are zeroed).
-            float x[kPt][kFastModel];
-            float center[kPt];
+        // The lane's model-row samples of a channel, every load issued at
+        // once (rows beyond B read row 0 and are zeroed when centered); the
+        // next channel's are issued after the last estimates of this one,
+        // so they arrive during its last flush.
+        float x[kPt][kFastModel];
+        float center[kPt];
+        float xmean[kPt];
+        const auto issue_loads = [&](int c) {
 #pragma unroll
             for (int tt = 0; tt < kPt; ++tt) {
                 const int e = c * d.patch_dim + 64 * tt + lane;
                 center[tt] = center_of[e];
+                xmean[tt] = 0.0F;
 #pragma unroll
                 for (int s2 = 0; s2 < kFastModel; ++s2) {
                     x[tt][s2] = 0.0F;
@@ -663,18 +667,31 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                 for (int tt = 0; tt < kPt; ++tt) {
                     const int e = c * d.patch_dim + 64 * tt + lane;
                     const int in_off = tab_in[e];
+                    xmean[tt] = xmean_of[e];
 #pragma unroll
                     for (int s2 = 0; s2 < kFastModel; ++s2) {
                         const uint32_t ps = s_pos[s2 < B ? s2 : 0];
                         x[tt][s2] = model_video[patch_base(d, ps) + in_off];
                     }
                 }
+            }
+        };
+        // The last output frame that receives samples (its flush is the
+        // one the next channel's loads overlap).
+        int last_active = -1;
+        for (int slot = d.slots - 1; slot >= 0 && last_active < 0; --slot) {
+            if (__builtin_amdgcn_readfirstlane(s_begin[slot]) != __builtin_amdgcn_readfirstlane(s_begin[slot + kPt])) {
+                last_active = slot;
+            }
+        }
+        issue_loads(0);
+        for (int c = 0; c < d.channels; ++c) {
+            if (fit) {
 #pragma unroll
                 for (int tt = 0; tt < kPt; ++tt) {
-                    const float mean = xmean_of[c * d.patch_dim + 64 * tt + lane];
 #pragma unroll
                     for (int s2 = 0; s2 < kFastModel; ++s2) {
-                        x[tt][s2] = s2 < B ? x[tt][s2] - mean : 0.0F;
+                        x[tt][s2] = s2 < B ? x[tt][s2] - xmean[tt] : 0.0F;
                     }
                 }
             }
@@ -776,6 +793,9 @@ __global__ __launch_bounds__(kModelThreads, 3) void filter_fast_kernel(
                             run(std::false_type{}, std::false_type{});
                         }
                     }
+                }
+                if (slot == last_active && c + 1 < d.channels) {
+                    issue_loads(c + 1);
                 }
                 wave_sync();
                 // Flush.  A global atomic costs one memory-side operation per
