@@ -44,16 +44,18 @@ ARGS = dict(sigma=15, block_size=8, block_step=4, group_size=16, bm_range=9, pat
             radius=2, rank=16, tau=200, cap_factor=4.0, model_cap_factor=1.0, chroma=1)
 
 
+# 93 x 69: the energy kernels' strips of 8 origins end inside the frame.
+@pytest.mark.parametrize("width,height", [(96, 72), (93, 69)])
 @pytest.mark.parametrize("stage", [BASIC, FINAL])
-def test_fast_distances_track_exact(contexts, stage):
+def test_fast_distances_track_exact(contexts, stage, width, height):
     plain, fast = contexts
-    noisy, _ = synth.make_clip(width=96, height=72, frames=6, channels=3, sigma8=15, seed=11,
+    noisy, _ = synth.make_clip(width=width, height=height, frames=6, channels=3, sigma8=15, seed=11,
                                motion="subpixel")
     params = parse_stage_parameters(stage, ARGS)
     ref = None
     if stage == FINAL:
         ref = plain.stage(BASIC, parse_stage_parameters(BASIC, ARGS), _dev(noisy))
-    print(f"[fast] distances stage={stage}", flush=True)
+    print(f"[fast] distances stage={stage} {width}x{height}", flush=True)
     want = plain.debug_frame(stage, params, _dev(noisy), ref, 2)
     got = fast.debug_frame(stage, params, _dev(noisy), ref, 2)
     np.testing.assert_array_equal(got["anchor_x"], want["anchor_x"])

@@ -54,9 +54,11 @@ size_t fast_match_stride(const StageDesc& d);
 // offsets [4] channel offsets, flags [1] (set where an energy is not finite
 // or too large for the bounded epilogue), energy [frames][H][W], split
 // [frames][match channels][H][W] packed bf16 hi << 16 | lo of the offset
-// samples (its buffer holds the row sums on the way).
+// samples (its buffer holds the row sums on the way), partials
+// [kFastPartialDoubles] (8-byte aligned) for the offsets' partial sums.
+constexpr int kFastPartialDoubles = 2 * kMaxChannels * 16;
 hipError_t launch_fast_energy(const StageDesc& d, const float* source, float* offsets, unsigned* flags,
-                              float* energy, uint32_t* split, hipStream_t stream);
+                              float* energy, uint32_t* split, double* partials, hipStream_t stream);
 // Distances of anchor index i land at staged[(i - anchor_base) * stride].
 // fused: k-nearest selection inside the matcher (fast_match_fused), results
 // straight into counts / positions / distances; otherwise distances are

@@ -63,36 +63,35 @@ constexpr float kEnergyBound = 8.0e37F;
 
 // Per-channel offset: the mean of the finite values of a strided sample of
 // the channel (any finite offset leaves the distances unchanged; non-finite
-// samples only poison the patches that hold them).  Block 0 also clears the
-// stage's energy flag.
-__global__ __launch_bounds__(256) void channel_offset_kernel(const float* __restrict__ source,
-                                                             int channels, size_t plane,
-                                                             int frames, float* __restrict__ offsets,
-                                                             unsigned* __restrict__ flags) {
+// samples only poison the patches that hold them).  The sample is split
+// over kOffsetBlocks workgroups per channel, each holding sixteen loads per
+// thread in flight; their (sum, count) partials are added in block order by
+// channel_offset_kernel, which also clears the stage's energy flag.
+constexpr int kOffsetBlocks = 16;
+constexpr int kOffsetBatch = 16;
+__global__ __launch_bounds__(256) void channel_partial_kernel(const float* __restrict__ source, int channels,
+                                                              size_t plane, int frames,
+                                                              double* __restrict__ partials) {
     __shared__ double s_part[4];
     __shared__ double s_count[4];
-    const int c = blockIdx.x;
-    if (c == 0 && threadIdx.x == 0) {
-        flags[0] = 0u;
-    }
+    const int c = blockIdx.y;
     const size_t per_channel = plane * frames;
     const size_t stride = per_channel / 65536 + 1;
+    const size_t step = static_cast<size_t>(kOffsetBlocks) * blockDim.x * stride;
     double sum = 0.0;
     size_t count = 0;
-    // Sixteen samples' loads in flight per thread, added in sample order.
-    constexpr int kBatch = 16;
-    const size_t step = blockDim.x * stride;
-    for (size_t i0 = threadIdx.x * stride; i0 < per_channel; i0 += kBatch * step) {
-        float v[kBatch];
+    for (size_t i0 = (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x) * stride; i0 < per_channel;
+         i0 += kOffsetBatch * step) {
+        float v[kOffsetBatch];
 #pragma unroll
-        for (int b = 0; b < kBatch; ++b) {
+        for (int b = 0; b < kOffsetBatch; ++b) {
             const size_t i = i0 + b * step;
             const size_t f = i / plane;
             const size_t rem = i - f * plane;
             v[b] = i < per_channel ? source[(f * channels + c) * plane + rem] : 0.0F;
         }
 #pragma unroll
-        for (int b = 0; b < kBatch; ++b) {
+        for (int b = 0; b < kOffsetBatch; ++b) {
             if (i0 + b * step < per_channel && v[b] == v[b] && fabsf(v[b]) <= FLT_MAX) {
                 sum += static_cast<double>(v[b]);
                 ++count;
@@ -113,69 +112,137 @@ __global__ __launch_bounds__(256) void channel_offset_kernel(const float* __rest
             total += s_part[w];
             total_n += s_count[w];
         }
+        double* out = partials + 2 * (static_cast<size_t>(c) * kOffsetBlocks + blockIdx.x);
+        out[0] = total;
+        out[1] = total_n;
+    }
+}
+
+__global__ __launch_bounds__(64) void channel_offset_kernel(const double* __restrict__ partials, int channels,
+                                                            float* __restrict__ offsets,
+                                                            unsigned* __restrict__ flags) {
+    const int c = threadIdx.x;
+    if (c == 0) {
+        flags[0] = 0u;
+    }
+    if (c < channels) {
+        double total = 0.0;
+        double total_n = 0.0;
+        for (int b = 0; b < kOffsetBlocks; ++b) {
+            total += partials[2 * (c * kOffsetBlocks + b)];
+            total_n += partials[2 * (c * kOffsetBlocks + b) + 1];
+        }
         offsets[c] = total_n > 0.0 ? static_cast<float>(total / total_n) : 0.0F;
     }
 }
 
-// Grid of the per-sample kernels below: x over the pixels of a plane
-// (32-bit, a plane holds at most 4096 x 4096), y over frame x channel.
+// Grids of the kernels below: x over the items of a plane (pixels or strips
+// of patch origins; 32-bit, a plane holds at most 4096 x 4096), y over
+// frame (x channel).
 
-// Horizontal sums of squares: rows[f][c][y][x] = sum_{px<P} (v - offset)^2.
+// Horizontal sums of squares: rows[f][c][y][x] = sum_{px<8} (v - offset)^2
+// (0 where no 8-wide patch starts).  One thread per row strip of
+// kRowStrip origins: each sample of the strip is loaded once, and each
+// origin's sum takes its 8 terms in order.
+constexpr int kRowStrip = 8;
 __global__ __launch_bounds__(256) void row_energy_kernel(StageDesc d, const float* __restrict__ source,
                                                          const float* __restrict__ offsets,
                                                          int match_channels, float* __restrict__ rows) {
-    const int plane = d.width * d.height;
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= plane) {
+    const int strips = (d.width + kRowStrip - 1) / kRowStrip;
+    const int item = blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= strips * d.height) {
         return;
     }
+    const int plane = d.width * d.height;
     const int fc = blockIdx.y;  // f * match_channels + c
     const int f = fc / match_channels;
     const int c = fc - f * match_channels;
-    const int y = p / d.width;
-    const int x = p - y * d.width;
-    float sum = 0.0F;
-    if (x <= d.width - d.patch) {
-        const float mu = offsets[c];
-        const float* src = source + (static_cast<size_t>(f) * d.channels + c) * plane + p;
-        for (int px = 0; px < d.patch; ++px) {
-            const float v = src[px] - mu;
-            sum = fmaf(v, v, sum);
+    const int y = item / strips;
+    const int x0 = (item - y * strips) * kRowStrip;
+    const float mu = offsets[c];
+    const float* src = source + (static_cast<size_t>(f) * d.channels + c) * plane + static_cast<size_t>(y) * d.width;
+    float v[kRowStrip + kFastPatch - 1];
+#pragma unroll
+    for (int k = 0; k < kRowStrip + kFastPatch - 1; ++k) {
+        const int x = x0 + k;
+        v[k] = x < d.width ? src[x] - mu : 0.0F;
+    }
+    float* out = rows + static_cast<size_t>(fc) * plane + static_cast<size_t>(y) * d.width;
+#pragma unroll
+    for (int j = 0; j < kRowStrip; ++j) {
+        const int x = x0 + j;
+        if (x < d.width) {
+            float sum = 0.0F;
+            if (x <= d.width - kFastPatch) {
+#pragma unroll
+                for (int px = 0; px < kFastPatch; ++px) {
+                    sum = fmaf(v[j + px], v[j + px], sum);
+                }
+            }
+            out[x] = sum;
         }
     }
-    rows[static_cast<size_t>(fc) * plane + p] = sum;
 }
 
-// Patch energies: energy[f][y][x] = sum over channels, frames and rows of
-// the row sums, for every patch origin.  flags[0] becomes 1 where an energy
-// is not finite or above kEnergyBound.
+// Patch energies: energy[f][y][x] = sum over patch frames t, channels c and
+// rows py (in that order) of the row sums, for every patch origin (0
+// elsewhere).  flags[0] becomes 1 where an energy is not finite or above
+// kEnergyBound.  One thread per column and strip of kEnergyStrip origins: it
+// reads the strip's rows once per (t, c) and keeps one running sum per
+// origin, so each origin's sum has the terms and order of a sum over its own
+// 8 rows with an eighth of the loads.  8x8 patches (the matcher's).
+constexpr int kEnergyStrip = 8;
 __global__ __launch_bounds__(256) void patch_energy_kernel(StageDesc d, const float* __restrict__ rows,
                                                            int match_channels, float* __restrict__ energy,
                                                            unsigned* __restrict__ flags) {
-    const int plane = d.width * d.height;
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= plane) {
+    const int strips = (d.height + kEnergyStrip - 1) / kEnergyStrip;
+    const int item = blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= d.width * strips) {
         return;
     }
     const int f = blockIdx.y;
-    const size_t i = static_cast<size_t>(f) * plane + p;
-    const int y = p / d.width;
-    const int x = p - y * d.width;
-    float sum = 0.0F;
-    if (x <= d.width - d.patch && y <= d.height - d.patch && f <= d.frames - d.patch_time) {
+    const int strip = item / d.width;
+    const int x = item - strip * d.width;
+    const int y0 = strip * kEnergyStrip;
+    const int plane = d.width * d.height;
+    const bool column_ok = x <= d.width - kFastPatch && f <= d.frames - d.patch_time;
+    float sum[kEnergyStrip];
+#pragma unroll
+    for (int j = 0; j < kEnergyStrip; ++j) {
+        sum[j] = 0.0F;
+    }
+    if (column_ok) {
         for (int t = 0; t < d.patch_time; ++t) {
             for (int c = 0; c < match_channels; ++c) {
-                const float* r = rows + (static_cast<size_t>(f + t) * match_channels + c) * plane + p;
-                for (int py = 0; py < d.patch; ++py) {
-                    sum += r[static_cast<size_t>(py) * d.width];
+                const float* r = rows + (static_cast<size_t>(f + t) * match_channels + c) * plane + x;
+                float v[kEnergyStrip + kFastPatch - 1];
+#pragma unroll
+                for (int k = 0; k < kEnergyStrip + kFastPatch - 1; ++k) {
+                    const int y = y0 + k;
+                    v[k] = y < d.height ? r[static_cast<size_t>(y) * d.width] : 0.0F;
+                }
+#pragma unroll
+                for (int j = 0; j < kEnergyStrip; ++j) {
+#pragma unroll
+                    for (int py = 0; py < kFastPatch; ++py) {
+                        sum[j] += v[j + py];
+                    }
                 }
             }
         }
-        if (!(sum <= kEnergyBound)) {
-            atomicOr(flags, 1u);
+    }
+#pragma unroll
+    for (int j = 0; j < kEnergyStrip; ++j) {
+        const int y = y0 + j;
+        if (y < d.height) {
+            const bool origin = column_ok && y <= d.height - kFastPatch;
+            const float value = origin ? sum[j] : 0.0F;
+            if (origin && !(value <= kEnergyBound)) {
+                atomicOr(flags, 1u);
+            }
+            energy[(static_cast<size_t>(f) * d.height + y) * d.width + x] = value;
         }
     }
-    energy[i] = sum;
 }
 
 // split[f][c][y][x] = bf16(v) << 16 | bf16(v - bf16(v)) of v = sample - offset.
@@ -1228,18 +1295,28 @@ size_t fast_match_stride(const StageDesc& d) {
 }
 
 hipError_t launch_fast_energy(const StageDesc& d, const float* source, float* offsets, unsigned* flags,
-                              float* energy, uint32_t* split, hipStream_t stream) {
+                              float* energy, uint32_t* split, double* partials, hipStream_t stream) {
+    if (d.patch != kFastPatch) {
+        return hipErrorInvalidValue;  // the patch energies are summed over 8 rows
+    }
     const int match_channels = d.stage == kFinal ? d.channels : 1;
     const size_t plane = static_cast<size_t>(d.width) * d.height;
-    hipLaunchKernelGGL(channel_offset_kernel, dim3(static_cast<unsigned>(match_channels)), dim3(256), 0,
-                       stream, source, d.channels, plane, d.frames, offsets, flags);
+    static_assert(2 * kMaxChannels * kOffsetBlocks <= kFastPartialDoubles, "offset partials");
+    hipLaunchKernelGGL(channel_partial_kernel, dim3(kOffsetBlocks, static_cast<unsigned>(match_channels)),
+                       dim3(256), 0, stream, source, d.channels, plane, d.frames, partials);
+    hipLaunchKernelGGL(channel_offset_kernel, dim3(1), dim3(64), 0, stream, partials, match_channels, offsets,
+                       flags);
     // The row sums use the split planes' buffer, which is written last.
     float* rows = reinterpret_cast<float*>(split);
     const unsigned plane_blocks = static_cast<unsigned>((plane + 255) / 256);
-    hipLaunchKernelGGL(row_energy_kernel, dim3(plane_blocks, static_cast<unsigned>(d.frames * match_channels)),
+    const size_t row_items = static_cast<size_t>(d.height) * ((d.width + kRowStrip - 1) / kRowStrip);
+    hipLaunchKernelGGL(row_energy_kernel, dim3(static_cast<unsigned>((row_items + 255) / 256),
+                                               static_cast<unsigned>(d.frames * match_channels)),
                        dim3(256), 0, stream, d, source, offsets, match_channels, rows);
-    hipLaunchKernelGGL(patch_energy_kernel, dim3(plane_blocks, static_cast<unsigned>(d.frames)), dim3(256), 0,
-                       stream, d, rows, match_channels, energy, flags);
+    const size_t strip_items = static_cast<size_t>(d.width) * ((d.height + kEnergyStrip - 1) / kEnergyStrip);
+    hipLaunchKernelGGL(patch_energy_kernel, dim3(static_cast<unsigned>((strip_items + 255) / 256),
+                                                 static_cast<unsigned>(d.frames)),
+                       dim3(256), 0, stream, d, rows, match_channels, energy, flags);
     hipLaunchKernelGGL(split_planes_kernel, dim3(plane_blocks, static_cast<unsigned>(d.frames * match_channels)),
                        dim3(256), 0, stream, d, source, offsets, match_channels, split);
     return hipGetLastError();

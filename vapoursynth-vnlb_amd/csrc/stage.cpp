@@ -920,14 +920,16 @@ void run_stage(vnlbhip_context& ctx, int stage, const vnlbhip_params& p, const f
     // matcher and filter.
     if (ctx.form == VNLBHIP_FORM_FAST && d.patch == 8 && d.similar > 1 && !guided && !learned) {
         const int match_channels = stage == VNLBHIP_STAGE_FINAL ? g.channels : 1;
-        // Channel offsets [0, 4), the energy flag at [4].
-        fast_offsets = ctx.fast_offsets.reserve(8);
+        // Channel offsets [0, 4), the energy flag at [4], the offsets'
+        // partial sums (doubles) from [8].
+        fast_offsets = ctx.fast_offsets.reserve(8 + 2 * vnlbhip::kFastPartialDoubles);
         fast_flags = reinterpret_cast<unsigned*>(fast_offsets + 4);
+        double* fast_partials = reinterpret_cast<double*>(fast_offsets + 8);
         fast_split = reinterpret_cast<uint32_t*>(ctx.fast_split.reserve(plane * match_channels * g.frames));
         fast_energy = ctx.fast_energy.reserve(plane * g.frames);
         timed(ctx, VNLBHIP_KERNEL_ENERGY, stream, "energy kernels", [&]() {
             return vnlbhip::launch_fast_energy(d, source, fast_offsets, fast_flags, fast_energy, fast_split,
-                                               stream);
+                                               fast_partials, stream);
         });
     }
     const int cap = d.cap;
