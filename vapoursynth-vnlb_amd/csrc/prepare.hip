@@ -157,7 +157,7 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
 // the 16-lane fp32 team dots.  At least four workgroups per compute unit
 // (at most 128 VGPRs): a group is a chain of dependent memory round trips,
 // so groups in flight set the rate.
-template <int kS, bool kExact, bool kMfma>
+template <int kS, bool kExact, bool kMfma, bool kFlat>
 __global__ __launch_bounds__(kModelThreads, 4) void prepare_kernel(
     StageDesc d, PrepareLayout L, const float* __restrict__ noisy,
     const float* __restrict__ basic, GroupRefs refs, const int* __restrict__ group_count_dev,
@@ -185,7 +185,9 @@ __global__ __launch_bounds__(kModelThreads, 4) void prepare_kernel(
     const int tail = kExact ? 0 : L.tail;
     const int md = d.model_dim;
     const bool final_stage = d.stage == kFinal;
-    const bool want_flat = final_stage && d.flat_areas != 0;
+    // kFlat: the flat-area test runs (Final with flat_areas): the sums it
+    // takes over every retained row are compiled only there.
+    constexpr bool want_flat = kFlat;
     // Slots per channel when one model covers whole 16-element slots of each
     // channel (coupled, patch_dim a multiple of 16), else 0.
     const int channel_runs = (d.models == 1 && d.patch_dim % kLanes == 0) ? d.patch_dim / kLanes : 0;
@@ -646,11 +648,12 @@ hipError_t launch_prepare(const StageDesc& d, const float* noisy, const float* b
     const bool exact = d.model_dim == kLanes * slots;
     // The matrix-core Gram needs whole 8-slot blocks and at most 16 rows.
     const bool mfma = mfma_gram && exact && slots % 8 == 0 && d.max_rows <= 16;
-#define VNLBHIP_PREPARE3(KS, EXACT, MFMA)                                                          \
+    const bool flat = d.stage == kFinal && d.flat_areas != 0;
+#define VNLBHIP_PREPARE4(KS, EXACT, MFMA, FLAT)                                                    \
     do {                                                                                           \
         int per_unit = 0;                                                                          \
         status = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_unit,                           \
-                                                              prepare_kernel<KS, EXACT, MFMA>,     \
+                                                              prepare_kernel<KS, EXACT, MFMA, FLAT>, \
                                                               kModelThreads, shared);              \
         if (status != hipSuccess) {                                                                \
             return status;                                                                         \
@@ -658,10 +661,16 @@ hipError_t launch_prepare(const StageDesc& d, const float* noisy, const float* b
         const long long resident = static_cast<long long>(per_unit > 0 ? per_unit : 1) * units;    \
         const unsigned blocks =                                                                    \
             static_cast<unsigned>(resident < group_capacity ? resident : group_capacity);          \
-        hipLaunchKernelGGL((prepare_kernel<KS, EXACT, MFMA>), dim3(blocks), dim3(kModelThreads), shared, \
+        hipLaunchKernelGGL((prepare_kernel<KS, EXACT, MFMA, FLAT>), dim3(blocks), dim3(kModelThreads), shared, \
                            stream, d, L, noisy, basic, refs, group_count_dev, problem_stride,      \
                            group_flags, problem_dims, means, problems, work_counter);              \
     } while (0)
+#define VNLBHIP_PREPARE3(KS, EXACT, MFMA)                                                          \
+    if (flat) {                                                                                    \
+        VNLBHIP_PREPARE4(KS, EXACT, MFMA, true);                                                   \
+    } else {                                                                                       \
+        VNLBHIP_PREPARE4(KS, EXACT, MFMA, false);                                                  \
+    }
 #define VNLBHIP_PREPARE(KS, EXACT)                                                                 \
     if (EXACT && mfma) {                                                                           \
         VNLBHIP_PREPARE3(KS, EXACT, true);                                                         \
@@ -685,6 +694,7 @@ hipError_t launch_prepare(const StageDesc& d, const float* noisy, const float* b
 #undef VNLBHIP_PREPARE_SLOTS
 #undef VNLBHIP_PREPARE
 #undef VNLBHIP_PREPARE3
+#undef VNLBHIP_PREPARE4
     return hipGetLastError();
 }
 
