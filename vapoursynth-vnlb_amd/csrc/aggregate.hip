@@ -119,6 +119,8 @@ __global__ __launch_bounds__(256) void render_frame_kernel(StageDesc d, StackSet
 // convergence test, summary[2..3] retained samples over the groups (64-bit),
 // summary[4] problems re-solved in fp64, summary[5] groups.  The group count
 // of the list is read on the device (grids are sized by the capacity).
+// One problem per thread; each wavefront adds its counts with one atomic
+// per counter (a single accumulator word takes every wavefront's add).
 __global__ __launch_bounds__(256) void summarize_kernel(const int* __restrict__ flags,
                                                         const int* __restrict__ status,
                                                         const int* __restrict__ counts,
@@ -130,20 +132,34 @@ __global__ __launch_bounds__(256) void summarize_kernel(const int* __restrict__ 
     if (problem == 0) {
         atomicAdd(&summary[5], groups);
     }
-    if (problem >= groups * models) {
-        return;
+    unsigned long long retained = 0;
+    bool short_rows = false;
+    bool unconverged = false;
+    if (problem < groups * models) {
+        const int group = problem / models;
+        const int flag = flags[group];
+        if ((problem % models) == 0) {
+            short_rows = (flag & 8) != 0;
+            retained = static_cast<unsigned long long>(counts[group_list[group]]);
+        }
+        unconverged = status != nullptr && (flag & kFlagModelled) != 0 && (status[problem] >> 16) != 0;
     }
-    const int group = problem / models;
-    const int flag = flags[group];
-    if ((problem % models) == 0 && (flag & 8) != 0) {
-        atomicAdd(&summary[0], 1);
+    const int short_count = __popcll(__ballot(short_rows));
+    const int unconverged_count = __popcll(__ballot(unconverged));
+#pragma unroll
+    for (int offset = 32; offset > 0; offset >>= 1) {
+        retained += __shfl_xor(retained, offset, 64);
     }
-    if ((problem % models) == 0) {
-        atomicAdd(reinterpret_cast<unsigned long long*>(summary + 2),
-                  static_cast<unsigned long long>(counts[group_list[group]]));
-    }
-    if (status != nullptr && (flag & kFlagModelled) != 0 && (status[problem] >> 16) != 0) {
-        atomicAdd(&summary[1], 1);
+    if ((threadIdx.x & 63) == 0) {
+        if (short_count != 0) {
+            atomicAdd(&summary[0], short_count);
+        }
+        if (unconverged_count != 0) {
+            atomicAdd(&summary[1], unconverged_count);
+        }
+        if (retained != 0) {
+            atomicAdd(reinterpret_cast<unsigned long long*>(summary + 2), retained);
+        }
     }
 }
 
