@@ -119,8 +119,10 @@ __global__ __launch_bounds__(256) void render_frame_kernel(StageDesc d, StackSet
 // convergence test, summary[2..3] retained samples over the groups (64-bit),
 // summary[4] problems re-solved in fp64, summary[5] groups.  The group count
 // of the list is read on the device (grids are sized by the capacity).
-// One problem per thread; each wavefront adds its counts with one atomic
-// per counter (a single accumulator word takes every wavefront's add).
+// A grid-stride loop over the problems (a few hundred workgroups: one
+// problem per thread would launch a workgroup per 256 problems, whose
+// dispatch costs more than their work); each wavefront adds its counts with
+// one atomic per counter.
 __global__ __launch_bounds__(256) void summarize_kernel(const int* __restrict__ flags,
                                                         const int* __restrict__ status,
                                                         const int* __restrict__ counts,
@@ -128,24 +130,30 @@ __global__ __launch_bounds__(256) void summarize_kernel(const int* __restrict__ 
                                                         const int* __restrict__ groups_dev, int models,
                                                         int* __restrict__ summary) {
     const int groups = *groups_dev;
-    const int problem = blockIdx.x * blockDim.x + threadIdx.x;
-    if (problem == 0) {
+    const int first = blockIdx.x * blockDim.x + threadIdx.x;
+    if (first == 0) {
         atomicAdd(&summary[5], groups);
     }
     unsigned long long retained = 0;
-    bool short_rows = false;
-    bool unconverged = false;
-    if (problem < groups * models) {
-        const int group = problem / models;
-        const int flag = flags[group];
-        if ((problem % models) == 0) {
-            short_rows = (flag & 8) != 0;
-            retained = static_cast<unsigned long long>(counts[group_list[group]]);
+    int short_count = 0;
+    int unconverged_count = 0;
+    const int problems = groups * models;
+    for (int base = blockIdx.x * blockDim.x; base < problems; base += gridDim.x * blockDim.x) {
+        const int problem = base + threadIdx.x;
+        bool short_rows = false;
+        bool unconverged = false;
+        if (problem < problems) {
+            const int group = problem / models;
+            const int flag = flags[group];
+            if ((problem % models) == 0) {
+                short_rows = (flag & 8) != 0;
+                retained += static_cast<unsigned long long>(counts[group_list[group]]);
+            }
+            unconverged = status != nullptr && (flag & kFlagModelled) != 0 && (status[problem] >> 16) != 0;
         }
-        unconverged = status != nullptr && (flag & kFlagModelled) != 0 && (status[problem] >> 16) != 0;
+        short_count += __popcll(__ballot(short_rows));
+        unconverged_count += __popcll(__ballot(unconverged));
     }
-    const int short_count = __popcll(__ballot(short_rows));
-    const int unconverged_count = __popcll(__ballot(unconverged));
 #pragma unroll
     for (int offset = 32; offset > 0; offset >>= 1) {
         retained += __shfl_xor(retained, offset, 64);
@@ -180,7 +188,8 @@ hipError_t launch_summarize(const int* flags, const int* status, const int* coun
     if (problems <= 0) {
         return hipSuccess;
     }
-    hipLaunchKernelGGL(summarize_kernel, dim3(static_cast<unsigned>((problems + 255) / 256)),
+    const int blocks = (problems + 255) / 256 < 512 ? (problems + 255) / 256 : 512;
+    hipLaunchKernelGGL(summarize_kernel, dim3(static_cast<unsigned>(blocks)),
                        dim3(256), 0, stream, flags, status, counts, group_list, groups_dev, models,
                        summary);
     return hipGetLastError();
