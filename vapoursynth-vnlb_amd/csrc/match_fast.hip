@@ -140,48 +140,32 @@ __global__ __launch_bounds__(64) void channel_offset_kernel(const double* __rest
 // of patch origins; 32-bit, a plane holds at most 4096 x 4096), y over
 // frame (x channel).
 
-// Horizontal sums of squares: rows[f][c][y][x] = sum_{px<8} (v - offset)^2
-// (0 where no 8-wide patch starts).  One thread per row strip of
-// kRowStrip origins: each sample of the strip is loaded once, and each
-// origin's sum takes its 8 terms in order.
-constexpr int kRowStrip = 8;
+// Horizontal sums of squares: rows[f][c][y][x] = sum_{px<P} (v - offset)^2,
+// one origin per thread (consecutive lanes on consecutive columns: each
+// load instruction reads a few cache lines).
 __global__ __launch_bounds__(256) void row_energy_kernel(StageDesc d, const float* __restrict__ source,
                                                          const float* __restrict__ offsets,
                                                          int match_channels, float* __restrict__ rows) {
-    const int strips = (d.width + kRowStrip - 1) / kRowStrip;
-    const int item = blockIdx.x * blockDim.x + threadIdx.x;
-    if (item >= strips * d.height) {
+    const int plane = d.width * d.height;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= plane) {
         return;
     }
-    const int plane = d.width * d.height;
     const int fc = blockIdx.y;  // f * match_channels + c
     const int f = fc / match_channels;
     const int c = fc - f * match_channels;
-    const int y = item / strips;
-    const int x0 = (item - y * strips) * kRowStrip;
-    const float mu = offsets[c];
-    const float* src = source + (static_cast<size_t>(f) * d.channels + c) * plane + static_cast<size_t>(y) * d.width;
-    float v[kRowStrip + kFastPatch - 1];
-#pragma unroll
-    for (int k = 0; k < kRowStrip + kFastPatch - 1; ++k) {
-        const int x = x0 + k;
-        v[k] = x < d.width ? src[x] - mu : 0.0F;
-    }
-    float* out = rows + static_cast<size_t>(fc) * plane + static_cast<size_t>(y) * d.width;
-#pragma unroll
-    for (int j = 0; j < kRowStrip; ++j) {
-        const int x = x0 + j;
-        if (x < d.width) {
-            float sum = 0.0F;
-            if (x <= d.width - kFastPatch) {
-#pragma unroll
-                for (int px = 0; px < kFastPatch; ++px) {
-                    sum = fmaf(v[j + px], v[j + px], sum);
-                }
-            }
-            out[x] = sum;
+    const int y = p / d.width;
+    const int x = p - y * d.width;
+    float sum = 0.0F;
+    if (x <= d.width - d.patch) {
+        const float mu = offsets[c];
+        const float* src = source + (static_cast<size_t>(f) * d.channels + c) * plane + p;
+        for (int px = 0; px < d.patch; ++px) {
+            const float v = src[px] - mu;
+            sum = fmaf(v, v, sum);
         }
     }
+    rows[static_cast<size_t>(fc) * plane + p] = sum;
 }
 
 // Patch energies: energy[f][y][x] = sum over patch frames t, channels c and
@@ -752,38 +736,70 @@ void match_fast_kernel(
                 __builtin_amdgcn_sched_barrier(0);
             }
         };
-        for (int p = ti * planes_per_frame; p < p_last; ++p) {
-            // The next plane of the frame goes into registers while this one
-            // is on the matrix cores, then into the other buffer.
-            prefetch(p + 1);
-            products(p);
-            store(p + 1);
-            __syncthreads();
-        }
         // The frame's energy box goes into registers while its last plane
-        // is on the matrix cores (its LDS space is free only after them).
-        float ebox_pre[kEnergyPerThread];
+        // is on the matrix cores (its LDS space is free only after them);
+        // tile shapes above 18 products per wave have no registers to spare
+        // for it and load it after the products.
+        constexpr bool kEarlyBox = kTiles <= 18;
+        float ebox_pre[kEarlyBox ? kEnergyPerThread : 1];
+        const auto load_box = [&]() {
 #pragma unroll
-        for (int k = 0; k < kEnergyPerThread; ++k) {
-            const int item = tid + k * kFastThreads;
-            ebox_pre[k] = 0.0F;
-            if (item < box_items) {
-                const int ry = item / ucols;
-                const int cx = item - ry * ucols;
-                ebox_pre[k] = energy[(static_cast<size_t>(frame) * d.height + uy0 + ry) * d.width + ux0 + cx];
+            for (int k = 0; k < kEnergyPerThread; ++k) {
+                const int item = tid + k * kFastThreads;
+                ebox_pre[k % (kEarlyBox ? kEnergyPerThread : 1)] = 0.0F;
+                if (item < box_items) {
+                    const int ry = item / ucols;
+                    const int cx = item - ry * ucols;
+                    ebox_pre[k % (kEarlyBox ? kEnergyPerThread : 1)] =
+                        energy[(static_cast<size_t>(frame) * d.height + uy0 + ry) * d.width + ux0 + cx];
+                }
+            }
+        };
+        if constexpr (kEarlyBox) {
+            for (int p = ti * planes_per_frame; p < p_last; ++p) {
+                // The next plane of the frame goes into registers while this
+                // one is on the matrix cores, then into the other buffer.
+                prefetch(p + 1);
+                products(p);
+                store(p + 1);
+                __syncthreads();
+            }
+            load_box();
+            products(p_last);
+            __syncthreads();
+        } else {
+            for (int p = ti * planes_per_frame; p <= p_last; ++p) {
+                if (p < p_last) {
+                    prefetch(p + 1);
+                }
+                products(p);
+                if (p < p_last) {
+                    store(p + 1);
+                }
+                __syncthreads();
             }
         }
-        products(p_last);
-        __syncthreads();
 
         // Epilogue of the candidate frame in the space of both buffers.
 #pragma unroll
         for (int k = 0; k < kEnergyPerThread; ++k) {
-            const int item = tid + k * kFastThreads;
-            if (item < box_items) {
-                const int ry = item / ucols;
-                const int cx = item - ry * ucols;
-                s_ebox[cx * L.ebox_pitch + ry] = ebox_pre[k];
+            int item = tid + k * kFastThreads;
+            if constexpr (kEarlyBox) {
+                if (item < box_items) {
+                    const int ry = item / ucols;
+                    const int cx = item - ry * ucols;
+                    s_ebox[cx * L.ebox_pitch + ry] = ebox_pre[k % (kEarlyBox ? kEnergyPerThread : 1)];
+                }
+            } else {
+                // Opaque: the box offsets are recomputed per frame instead of
+                // being held in registers across the plane loop.
+                asm volatile("" : "+v"(item));
+                if (item < box_items) {
+                    const int ry = item / ucols;
+                    const int cx = item - ry * ucols;
+                    s_ebox[cx * L.ebox_pitch + ry] =
+                        energy[(static_cast<size_t>(frame) * d.height + uy0 + ry) * d.width + ux0 + cx];
+                }
             }
         }
         if (wave == 0) {
@@ -1309,9 +1325,7 @@ hipError_t launch_fast_energy(const StageDesc& d, const float* source, float* of
     // The row sums use the split planes' buffer, which is written last.
     float* rows = reinterpret_cast<float*>(split);
     const unsigned plane_blocks = static_cast<unsigned>((plane + 255) / 256);
-    const size_t row_items = static_cast<size_t>(d.height) * ((d.width + kRowStrip - 1) / kRowStrip);
-    hipLaunchKernelGGL(row_energy_kernel, dim3(static_cast<unsigned>((row_items + 255) / 256),
-                                               static_cast<unsigned>(d.frames * match_channels)),
+    hipLaunchKernelGGL(row_energy_kernel, dim3(plane_blocks, static_cast<unsigned>(d.frames * match_channels)),
                        dim3(256), 0, stream, d, source, offsets, match_channels, rows);
     const size_t strip_items = static_cast<size_t>(d.width) * ((d.height + kEnergyStrip - 1) / kEnergyStrip);
     hipLaunchKernelGGL(patch_energy_kernel, dim3(static_cast<unsigned>((strip_items + 255) / 256),
