@@ -329,8 +329,13 @@ __device__ inline void parallel_sweep(float (&a)[16], float (&v)[16], int r, boo
     (parallel_round<kKs>(a, v, r, active, threshold), ...);
 }
 
-// Same stopping rule, ordering and canonical vectors as jacobi_kernel, with
-// the parallel ordering above: 16 lanes per problem, four per wavefront.
+// Same sweep structure and canonical vectors as jacobi_kernel, with the
+// parallel ordering above: 16 lanes per problem, four per wavefront.  The
+// sweeps stop once every off-diagonal entry is at most 2^-14 of
+// max(1, max |A|) (the exact form stops at 64 float epsilons, about 2^-17):
+// the sweeps this saves (7 % of the solver's time on 16-row models) only
+// refine eigenpairs already within the fast form's tolerance (eigenvalues
+// within 1e-3 of the largest, checked against the fp64 oracle).
 __global__ __launch_bounds__(64) void jacobi_parallel_kernel(
     StageDesc d, const int* __restrict__ group_count_dev, int problem_stride,
     const int* __restrict__ group_flags, const int* __restrict__ problem_dims,
@@ -374,7 +379,7 @@ __global__ __launch_bounds__(64) void jacobi_parallel_kernel(
     row_max = fmaxf(row_max, xor16<2>(row_max));
     row_max = fmaxf(row_max, xor16<1>(row_max));
     const float matrix_scale = row_max > 1.0F ? row_max : 1.0F;
-    const float threshold = (64.0F * FLT_EPSILON) * matrix_scale;
+    const float threshold = 0x1p-14F * matrix_scale;
 
     const auto offdiag_max = [&]() {
         float value = 0.0F;
