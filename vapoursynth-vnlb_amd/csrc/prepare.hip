@@ -49,11 +49,19 @@ PrepareLayout prepare_layout(const StageDesc& d) {
         at += (words + 3) / 4 * 4;
         return start;
     };
+    // The channel table is read only by the flat-area test (Final) and by
+    // rows that do not fill whole 16-element slots; without either it is
+    // left out (-1), which lets one more workgroup fit on a compute unit.
+    const bool exact = d.model_dim == kLanes * template_slots(d.model_dim);
+    const bool flat = d.stage == kFinal && d.flat_areas != 0;
     L.tab_in = take(L.width);
-    L.tab_c = take(L.width);
+    L.tab_c = exact && !flat ? -1 : take(L.width);
     L.rows = take(d.max_rows * L.width);
     L.pos = take(d.cap);
-    L.gram = take(kModelThreads / 64 * 4 * 64);  // per-wave partial Grams (matrix-core form)
+    // Per-wave partial Grams of the matrix-core form: over the model rows
+    // (after a barrier: the products have read them) when there is one model.
+    constexpr int kGramWords = kModelThreads / 64 * 4 * 64;
+    L.gram = d.models == 1 && d.max_rows * L.width >= kGramWords ? L.rows : take(kGramWords);
     L.words = at;
     return L;
 }
@@ -158,7 +166,7 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
 // (at most 128 VGPRs): a group is a chain of dependent memory round trips,
 // so groups in flight set the rate.
 template <int kS, bool kExact, bool kMfma, bool kFlat>
-__global__ __launch_bounds__(kModelThreads, 4) void prepare_kernel(
+__global__ __launch_bounds__(kModelThreads, kFlat ? 4 : 5) void prepare_kernel(
     StageDesc d, PrepareLayout L, const float* __restrict__ noisy,
     const float* __restrict__ basic, GroupRefs refs, const int* __restrict__ group_count_dev,
     int problem_stride,
@@ -171,7 +179,7 @@ __global__ __launch_bounds__(kModelThreads, 4) void prepare_kernel(
     __shared__ double s_sum[kModelThreads / 64][kMaxChannels][2];
 
     int* tab_in = reinterpret_cast<int*>(s_dyn + L.tab_in);
-    int* tab_c = reinterpret_cast<int*>(s_dyn + L.tab_c);
+    int* tab_c = L.tab_c >= 0 ? reinterpret_cast<int*>(s_dyn + L.tab_c) : nullptr;
     float* s_rows = s_dyn + L.rows;
     uint32_t* s_pos = reinterpret_cast<uint32_t*>(s_dyn + L.pos);
 
@@ -212,7 +220,9 @@ __global__ __launch_bounds__(kModelThreads, 4) void prepare_kernel(
             channel = c;
         }
         tab_in[i] = in_off;
-        tab_c[i] = channel;
+        if (tab_c != nullptr) {
+            tab_c[i] = channel;
+        }
     }
 
     const int group_count = *group_count_dev;
@@ -517,6 +527,9 @@ __global__ __launch_bounds__(kModelThreads, 4) void prepare_kernel(
                     }
                     // acc[v] = partial G[4 kb + v][col]
                     float* s_gram = s_dyn + L.gram;
+                    if (L.gram == L.rows) {
+                        __syncthreads();  // every wave is done reading the rows
+                    }
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         s_gram[(wave * 4 + v) * 64 + (tid & 63)] = acc[v];
